@@ -1,0 +1,236 @@
+// common.hpp -- device-side definitions shared by the strand kernels.
+//
+// The arithmetic helpers reproduce WGSL semantics exactly: f32 only, no FMA
+// contraction (every translation unit is built with -ffp-contract=off),
+// correctly rounded division, ties-to-even round, saturating f32 -> int
+// conversions, clamp(e, lo, hi) = min(max(e, lo), hi).  Binning results
+// (counts, offsets, per-cell reference lists) are therefore bit-identical
+// to any implementation that follows the same rules on the same inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/strand_rt.h"
+
+#define SR_HD __host__ __device__ __forceinline__
+
+namespace srk {
+
+constexpr uint32_t kExt = SR_COARSE_FINE_TILE_EXTENT;   // 4 fine tiles per coarse tile edge
+constexpr uint32_t kSlices = SR_COARSE_DEPTH_SLICES;    // 16 coarse and 16 fine depth slices
+constexpr uint32_t kPage = SR_COARSE_COUNT_PAGE_SIZE;   // 256 cells per count page
+constexpr uint32_t kQuant = SR_DEPTH_QUANT_MAX;
+constexpr uint32_t kInvalid = SR_INVALID_PTR;
+constexpr uint32_t kMarked = 0xFFFFFFFEu;
+
+// ------------------------------------------------------------- records
+// Per-frustum record built on the host for every frame.
+struct FrustumRec {
+  sr_frustum_desc d;
+  float clip[16];
+  float eye[4];
+  uint32_t light_layer;    // kInvalid unless a light with a bound light slot
+  uint32_t cam_index;      // kInvalid unless a camera
+  uint32_t enabled;        // produces binning tasks
+  uint32_t fine_tiles_x;
+  uint32_t fine_tiles_y;
+  uint32_t stamp_w;
+  uint32_t stamp_h;
+  uint32_t pad;
+  uint64_t dom_depth_off;    // floats
+  uint64_t dom_opacity_off;  // floats
+  uint64_t image_off;        // pixels
+  uint64_t stamp_off;        // floats
+};
+
+// Per-instance record: the instance's slab pointers resolved from its
+// DevicePtr ids, plus its transform.
+struct InstRec {
+  const float4* vertices;
+  const uint32_t* indices;
+  const sr_strand_meta* meta;
+  const sr_strand_material* materials;
+  const uint32_t* pos_strand;  // index position -> strand, kInvalid if not a segment start
+  uint32_t vertex_count;
+  uint32_t index_count;
+  uint32_t meta_count;
+  uint32_t material_count;
+  uint32_t geo_strand_count;
+  uint32_t pos_base;  // first slot of this instance inside a frustum's slot range
+  uint32_t valid;
+  uint32_t pad;
+  sr_aabb aabb;
+  float world_from_local[16];
+};
+
+// Device counters of one frame (uint32 words).
+enum Counter : uint32_t {
+  C_TASKS = 0,
+  C_CAM_TASKS = 1,
+  C_PAGES = 2,
+  C_REFS = 3,
+  C_ACTIVE = 4,
+  C_WORK = 5,
+  C_CAM_RUNS = 6,
+  C_SHADOW_RUNS = 7,
+  C_OVERFLOW = 8,
+  C_MARKED = 9,       // marked pages before the page capacity is applied
+  C_PAGE_DROPS = 10,  // marked pages dropped by the page capacity
+  C_COUNT = 16
+};
+
+struct FrameParams {
+  const FrustumRec* fr;
+  const InstRec* inst;
+  const sr_directional_light* lights;
+  uint32_t F, NI, NL, cams;
+  uint32_t slots;         // slot count per frustum (multiple of 64)
+  uint32_t coarse_total;  // coarse tiles over all frusta
+  uint32_t task_cap, ref_cap, work_cap, atlas_texels;
+  uint32_t dom_layers, page_cap;
+  sr_frame_settings st;
+
+  uint32_t* counters;  // Counter words
+  // broad
+  sr_broad_instance_meta* broad;
+  float* keep;                       // [NI * F]
+  sr_coarse_asset_range* ranges;     // [NI * F]
+  // projection
+  unsigned long long* slot_masks;    // [F * slots / 64]
+  uint32_t* wave_offsets;            // [F * slots / 64]
+  uint32_t* frustum_task_base;       // [F + 1]
+  uint32_t* frustum_cam_base;        // [F]
+  uint8_t* shaded;                   // [cams * slots] camera segment starts shaded this frame
+  sr_binning_task* tasks;            // [task_cap]
+  sr_projected_segment* proj;        // [task_cap]
+  uint32_t* shading_queue;           // [task_cap]
+  // binning
+  sr_coarse_depth_lut_entry* lut;    // [coarse_total * 16]
+  uint32_t* page_table;              // [coarse_total * 16]
+  sr_fine_page_meta* page_meta;      // [coarse_total * 16]
+  uint32_t* page_totals;             // [coarse_total * 16]
+  uint32_t* cell_counts;             // [pages * 256]
+  uint32_t* cell_offsets;            // [pages * 256]
+  uint32_t* cell_cursors;            // [pages * 256]
+  sr_fine_seg_ref* refs;             // [ref_cap]
+  uint32_t* active_flags;            // [coarse_total * 16]
+  uint32_t* active_queue;            // [coarse_total * 16]
+  uint4* tile_work;                  // [coarse_total * 16]: (work count, cam run, shadow run, load)
+  uint4* tile_bases;                 // [coarse_total * 16]: exclusive scans of tile_work
+  sr_raster_work_item* work;         // [work_cap]
+  sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
+  sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
+  // shadow / shading / raster
+  float* dom_depth;
+  float* dom_opacity;
+  float* stamp;
+  float4* atlas;      // [cams * NI * atlas_texels]
+  float2* history;    // [cams * NI * atlas_texels] (visibility, current visibility)
+  const float2* history_prev;
+  float4* image;
+  float* depth;
+  float4* composite;
+};
+
+// ------------------------------------------------------------- vectors
+struct V2 { float x, y; };
+struct V3 { float x, y, z; };
+struct V4 { float x, y, z, w; };
+
+SR_HD float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
+SR_HD float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+SR_HD int32_t iclamp(int32_t e, int32_t lo, int32_t hi) { return min(max(e, lo), hi); }
+SR_HD uint32_t ceil_div(uint32_t x, uint32_t y) { return (x + y - 1u) / y; }
+
+SR_HD uint32_t f2u(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+SR_HD int32_t f2i(float x) {
+  if (x != x) return 0;
+  if (x <= -2147483648.0f) return INT32_MIN;
+  if (x >= 2147483648.0f) return INT32_MAX;
+  return (int32_t)x;
+}
+
+// mat4 (column-major) x vec4 summed as ((c0*x + c1*y) + c2*z) + c3*w
+SR_HD V4 mat_vec(const float* m, V4 v) {
+  V4 r;
+  r.x = ((m[0] * v.x + m[4] * v.y) + m[8] * v.z) + m[12] * v.w;
+  r.y = ((m[1] * v.x + m[5] * v.y) + m[9] * v.z) + m[13] * v.w;
+  r.z = ((m[2] * v.x + m[6] * v.y) + m[10] * v.z) + m[14] * v.w;
+  r.w = ((m[3] * v.x + m[7] * v.y) + m[11] * v.z) + m[15] * v.w;
+  return r;
+}
+
+SR_HD uint32_t pack2x16unorm(float a, float b) {
+  uint32_t lo = (uint32_t)floorf(0.5f + 65535.0f * wclamp(a, 0.0f, 1.0f));
+  uint32_t hi = (uint32_t)floorf(0.5f + 65535.0f * wclamp(b, 0.0f, 1.0f));
+  return lo | (hi << 16);
+}
+SR_HD V2 unpack2x16unorm(uint32_t v) {
+  return V2{(float)(v & 0xFFFFu) / 65535.0f, (float)(v >> 16) / 65535.0f};
+}
+
+SR_HD uint32_t wang_hash(uint32_t seed) {
+  uint32_t x = seed;
+  x = (x ^ 61u) ^ (x >> 16u);
+  x = x + (x << 3u);
+  x = x ^ (x >> 4u);
+  x = x * 0x27d4eb2du;
+  x = x ^ (x >> 15u);
+  return x;
+}
+SR_HD float hash_to_unit_float(uint32_t x) { return (float)x / 4294967296.0f; }
+
+SR_HD float normalize_depth01(float z) {
+  if (z >= 0.0f && z <= 1.0f) return z;
+  return wclamp(z * 0.5f + 0.5f, 0.0f, 1.0f);
+}
+SR_HD uint32_t quantize_depth01(float z) { return f2u(rintf(wclamp(z, 0.0f, 1.0f) * (float)kQuant)); }
+SR_HD float dequantize_depth01(uint32_t q) { return (float)q / (float)kQuant; }
+
+SR_HD V4 world_to_screen_raw(V4 pos, const float* clip_from_world, float vw, float vh) {
+  V4 clip = mat_vec(clip_from_world, pos);
+  if (clip.w <= 0.0f) return V4{-1.0f, -1.0f, -1.0f, -1.0f};
+  float nx = clip.x / clip.w, ny = clip.y / clip.w, nz = clip.z / clip.w;
+  float sx = 0.0f + (nx * 0.5f + 0.5f) * vw;
+  float sy = 0.0f + (ny * -0.5f + 0.5f) * vh;
+  return V4{sx, sy, nz, clip.w};
+}
+
+SR_HD int32_t sgn_i32(float f) { return f > 1e-6f ? 1 : (f < -1e-6f ? -1 : 0); }
+
+struct Clipped { bool ok; V3 p0, p1; };
+
+SR_HD bool clip_axis(float p, float d, float mn, float mx, float& tmin, float& tmax) {
+  if (fabsf(d) < 1e-6f) return !(p < mn || p > mx);
+  float inv = 1.0f / d;
+  float a = (mn - p) * inv, b = (mx - p) * inv;
+  if (a > b) { float t = a; a = b; b = t; }
+  float omin = fmaxf(tmin, a), omax = fminf(tmax, b);
+  tmin = omin; tmax = omax;
+  return !(omin > omax);
+}
+
+SR_HD Clipped clip_segment_to_box(V3 p0, V3 p1, V3 bmin, V3 bmax) {
+  V3 d{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+  float tmin = 0.0f, tmax = 1.0f;
+  Clipped c;
+  c.ok = false; c.p0 = p0; c.p1 = p1;
+  if (!clip_axis(p0.x, d.x, bmin.x, bmax.x, tmin, tmax)) return c;
+  if (!clip_axis(p0.y, d.y, bmin.y, bmax.y, tmin, tmax)) return c;
+  if (!clip_axis(p0.z, d.z, bmin.z, bmax.z, tmin, tmax)) return c;
+  c.ok = true;
+  c.p0 = V3{p0.x + d.x * tmin, p0.y + d.y * tmin, p0.z + d.z * tmin};
+  c.p1 = V3{p0.x + d.x * tmax, p0.y + d.y * tmax, p0.z + d.z * tmax};
+  return c;
+}
+
+SR_HD uint32_t pack_binning_field(uint32_t is_shadow, uint32_t frustum) {
+  return (is_shadow & 1u) << SR_BINNING_LEVEL_BITS | (frustum << SR_BINNING_FRUSTUM_SHIFT);
+}
+
+}  // namespace srk

@@ -1,0 +1,49 @@
+// kernels.hpp -- host-side launch interface of the strand kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace srk {
+
+// Named HIP events recorded between pipeline stages when profiling is on.
+struct StageTimer {
+  bool enabled = false;
+  std::vector<hipEvent_t> events;
+  std::vector<std::string> names;
+  size_t used = 0;
+  void begin() { used = 0; names.clear(); }
+  void mark(hipStream_t s, const char* name) {
+    if (!enabled) return;
+    if (used == events.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      events.push_back(e);
+    }
+    if (hipEventRecord(events[used], s) != hipSuccess) return;
+    names.emplace_back(name);
+    ++used;
+  }
+};
+
+// Scratch owned by the runtime for one frame's prepass.
+struct PrepassScratch {
+  sr_projected_segment* slot_proj;  // [F * slots]
+  uint32_t* partials;               // scan tile partials
+  uint4* partials4;                 // scan tile partials (4-wide)
+  uint32_t* page_xy_mask;           // [coarse_total * 16]
+};
+
+void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer);
+void launch_shadow(const FrameParams& fp, hipStream_t s);
+void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, hipStream_t s);
+void launch_shading(const FrameParams& fp, hipStream_t s);
+void launch_camera_raster(const FrameParams& fp, hipStream_t s);
+void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
+void launch_fill_u32(uint32_t* dst, uint64_t count, uint32_t value, hipStream_t s);
+
+}  // namespace srk

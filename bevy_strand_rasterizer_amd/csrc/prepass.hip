@@ -1,0 +1,849 @@
+// prepass.hip -- culling, projection, froxel binning, scans, compaction and
+// raster-work emission on MI355X (gfx950, wave64).
+//
+// Reference kernels restated here (paths relative to the reference root):
+//   broad_prepass            src/shaders/strand_prepass.wgsl:684-760
+//   broad_strand_prepass     :762-782   (fused into the projection)
+//   fine_prepass             :1016-1140
+//   coarse_interval_pass     :835-882   (fused into the LUT build)
+//   build_depth_warp_lut     :884-1014
+//   mark / allocate pages    :1928-1969
+//   binning_queue_pass       :1986-1990 (count)
+//   prefix_fine_pages        :2051-2143
+//   fill_fine_seg_refs       :2190-2194
+//   prefix_active_tile_blocks / compact_active_tiles  :2581-2676
+//   emit_raster_work         :2678-2834
+//
+// Differences of design (results are the canonical ones the oracle defines):
+//   * every queue position comes from a device-wide scan instead of an
+//     atomic tail, so task order, page ids, page bases, the active-tile queue,
+//     work bases and run indices are deterministic;
+//   * projection runs over a dense (frustum, index position) slot space:
+//     pass 1 writes a wave64 ballot mask and the packed endpoints per slot,
+//     a scan of mask popcounts gives each wave its queue base, pass 2
+//     compacts with ballot ranks;
+//   * the fill pass appends with cell cursors (as the reference) and a
+//     per-cell sort puts every cell in (inst, seg) order afterwards.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "scan.hpp"
+
+namespace srk {
+
+// ----------------------------------------------------------- K2 broad
+struct CornerBounds { V2 smin, smax; float zmin, zmax; bool any, near_clip; };
+
+__device__ CornerBounds project_aabb(const FrustumRec& fr, const sr_aabb& bb, const float* w) {
+  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  CornerBounds cb{{1e30f, 1e30f}, {-1e30f, -1e30f}, 1.0f, 0.0f, false, false};
+  for (int i = 0; i < 8; ++i) {
+    V4 c{(i & 4) ? bb.max[0] : bb.min[0], (i & 2) ? bb.max[1] : bb.min[1], (i & 1) ? bb.max[2] : bb.min[2], 1.0f};
+    V4 wc = mat_vec(w, c);
+    V4 clip = mat_vec(fr.clip, wc);
+    if (clip.w <= 1e-6f) { cb.near_clip = true; continue; }
+    float nx = clip.x / clip.w, ny = clip.y / clip.w, nz = clip.z / clip.w;
+    if (nz < 0.0f) cb.near_clip = true;
+    float rx = 0.0f + (nx * 0.5f + 0.5f) * vw;
+    float ry = 0.0f + (ny * -0.5f + 0.5f) * vh;
+    cb.any = true;
+    cb.smin.x = fminf(cb.smin.x, rx); cb.smin.y = fminf(cb.smin.y, ry);
+    cb.smax.x = fmaxf(cb.smax.x, rx); cb.smax.y = fmaxf(cb.smax.y, ry);
+    float z = normalize_depth01(nz);
+    cb.zmin = fminf(cb.zmin, z); cb.zmax = fmaxf(cb.zmax, z);
+  }
+  return cb;
+}
+
+__device__ float screen_area(const FrustumRec& fr, const CornerBounds& cb) {
+  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  if (cb.near_clip) return vw * vh;
+  if (!cb.any) return -1.0f;
+  if (cb.smax.x < 0.0f || cb.smax.y < 0.0f || cb.smin.x >= vw || cb.smin.y >= vh) return -1.0f;
+  if (cb.zmax < 0.0f || cb.zmin > 1.0f) return -1.0f;
+  float ex = fmaxf(wclamp(cb.smax.x, 0.0f, vw - 1.0f) - wclamp(cb.smin.x, 0.0f, vw - 1.0f), 1.0f);
+  float ey = fmaxf(wclamp(cb.smax.y, 0.0f, vh - 1.0f) - wclamp(cb.smin.y, 0.0f, vh - 1.0f), 1.0f);
+  return ex * ey;
+}
+
+__global__ void k_broad(FrameParams fp) {
+  uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ii >= fp.NI) return;
+  const InstRec& in = fp.inst[ii];
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    sr_coarse_asset_range none{};
+    none.geo_id = kInvalid;
+    fp.ranges[(size_t)ii * fp.F + fi] = none;
+    fp.keep[(size_t)ii * fp.F + fi] = 0.0f;
+  }
+  if (!in.valid || in.meta_count == 0u || in.geo_strand_count == 0u) {
+    fp.broad[ii] = sr_broad_instance_meta{0u, 0u, 0.0f, 0.0f};
+    return;
+  }
+  const sr_frame_settings& st = fp.st;
+  uint32_t strand_count = min(in.meta_count, in.geo_strand_count);
+  bool visible = false;
+  float cam = 0.0f, shd = 0.0f;
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    const FrustumRec& fr = fp.fr[fi];
+    float k = 0.0f;
+    if (fr.d.kind > 1u) { fp.keep[(size_t)ii * fp.F + fi] = 0.0f; continue; }
+    // emit_coarse_asset_range_for_aabb
+    bool vis = false;
+    CornerBounds cb = project_aabb(fr, in.aabb, in.world_from_local);
+    float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+    if (fr.d.coarse_depth_tile_count != 0u && !(fr.d.kind == 1u && fr.light_layer == kInvalid)) {
+      CornerBounds r = cb;
+      if (r.near_clip) {
+        r.smin = V2{0.0f, 0.0f};
+        r.smax = V2{vw - 1.0f, vh - 1.0f};
+        r.zmax = 1.0f;
+        if (!r.any) r.zmin = 0.0f;
+        r.any = true;
+      }
+      if (r.any && !(r.smax.x < 0.0f || r.smax.y < 0.0f || r.smin.x >= vw || r.smin.y >= vh) &&
+          !(r.zmax < 0.0f || r.zmin > 1.0f)) {
+        uint32_t cpx = max(1u, fr.d.froxel_size_x * kExt), cpy = max(1u, fr.d.froxel_size_y * kExt);
+        sr_coarse_asset_range rg;
+        rg.geo_id = ii;
+        rg.frustum_id = fi;
+        rg.min_x = min(f2u(wclamp(r.smin.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
+        rg.max_x = min(f2u(wclamp(r.smax.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
+        rg.min_y = min(f2u(wclamp(r.smin.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
+        rg.max_y = min(f2u(wclamp(r.smax.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
+        rg.z_min_q = quantize_depth01(r.zmin);
+        rg.z_max_q = quantize_depth01(r.zmax);
+        fp.ranges[(size_t)ii * fp.F + fi] = rg;
+        vis = true;
+      }
+    }
+    visible = vis || visible;
+    if (vis) {
+      float area = screen_area(fr, cb);
+      float target = fmaxf(st.target_strands_per_pixel, 1e-5f);
+      float min_keep = wclamp(st.min_keep_probability, 0.0f, 1.0f);
+      if (st.stochastic_cull_enabled == 0u) {
+        k = 1.0f;
+      } else if (area < 0.0f) {
+        k = 0.0f;
+      } else {
+        float area_keep = wclamp(area * target / fmaxf((float)strand_count, 1.0f), min_keep, 1.0f);
+        if (fr.d.kind == 0u) {
+          k = area_keep;
+        } else {
+          float cap = wclamp(st.shadow_keep_probability, 0.0f, 1.0f);
+          float scale = 1.0f / exp2f((float)fr.d.cascade_index);
+          k = wclamp(area_keep * area_keep * scale * cap, min_keep, 1.0f);
+        }
+      }
+      if (fr.d.kind == 0u) cam = fmaxf(cam, k); else shd = fmaxf(shd, k);
+    }
+    fp.keep[(size_t)ii * fp.F + fi] = k;
+  }
+  fp.broad[ii] = sr_broad_instance_meta{strand_count, visible ? 1u : 0u, cam, shd};
+}
+
+// -------------------------------------------------- projection (K3 + K5)
+__device__ __forceinline__ uint32_t find_instance(const FrameParams& fp, uint32_t pos) {
+  // instances own consecutive position ranges starting at pos_base
+  uint32_t lo = 0, hi = fp.NI;
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (fp.inst[mid].pos_base <= pos) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Evaluates one (frustum, index position) slot.  Returns true and the
+// packed endpoints when the segment survives culling and clipping.
+__device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, uint32_t* ii_out, uint32_t* strand_out,
+                             uint32_t* seg_out, sr_projected_segment* out) {
+  if (fp.NI == 0u) return false;
+  const FrustumRec& fr = fp.fr[fi];
+  if (!fr.enabled) return false;
+  uint32_t ii = find_instance(fp, pos);
+  const InstRec& in = fp.inst[ii];
+  if (!in.valid) return false;
+  uint32_t p = pos - in.pos_base;
+  if (p >= in.index_count) return false;
+  const sr_broad_instance_meta b = fp.broad[ii];
+  if (b.visible == 0u) return false;
+  uint32_t strand = in.pos_strand[p];
+  if (strand == kInvalid || strand >= b.strand_count || strand >= in.meta_count) return false;
+  float rnd = hash_to_unit_float(wang_hash((ii * 16777619u) ^ strand));
+  if (rnd > fmaxf(b.camera_keep_probability, b.shadow_keep_probability)) return false;
+  if (rnd > fp.keep[(size_t)ii * fp.F + fi]) return false;
+  if (p + 1u >= in.index_count) return false;
+  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  float4 a = in.vertices[in.indices[p]];
+  float4 bq = in.vertices[in.indices[p + 1u]];
+  V4 p0w = mat_vec(in.world_from_local, V4{a.x, a.y, a.z, 1.0f});
+  V4 p1w = mat_vec(in.world_from_local, V4{bq.x, bq.y, bq.z, 1.0f});
+  V4 r0 = world_to_screen_raw(p0w, fr.clip, vw, vh);
+  V4 r1 = world_to_screen_raw(p1w, fr.clip, vw, vh);
+  V3 q0{r0.x, r0.y, normalize_depth01(r0.z)};
+  V3 q1{r1.x, r1.y, normalize_depth01(r1.z)};
+  Clipped c = clip_segment_to_box(q0, q1, V3{0.0f, 0.0f, 0.0f},
+                                  V3{fmaxf(0.0f, vw - 1e-3f), fmaxf(0.0f, vh - 1e-3f), 1.0f});
+  if (!c.ok) return false;
+  float ssx = fmaxf(1.0f, vw), ssy = fmaxf(1.0f, vh);
+  out->p0_xy = pack2x16unorm(wclamp(c.p0.x / ssx, 0.0f, 1.0f), wclamp(c.p0.y / ssy, 0.0f, 1.0f));
+  out->p1_xy = pack2x16unorm(wclamp(c.p1.x / ssx, 0.0f, 1.0f), wclamp(c.p1.y / ssy, 0.0f, 1.0f));
+  out->depths = pack2x16unorm(wclamp(c.p0.z, 0.0f, 1.0f), wclamp(c.p1.z, 0.0f, 1.0f));
+  *ii_out = ii;
+  *strand_out = strand;
+  *seg_out = p;
+  return true;
+}
+
+// Pass 1: per slot validity ballot + packed endpoints in slot layout.
+__global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
+  const uint64_t total = (uint64_t)fp.F * fp.slots;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+    uint32_t ii, strand, seg;
+    sr_projected_segment ps;
+    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
+    if (ok) slot_proj[s] = ps;
+    unsigned long long m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
+    const FrustumRec& fr = fp.fr[fi];
+    if (fr.cam_index != kInvalid) fp.shaded[(size_t)fr.cam_index * fp.slots + pos] = ok ? 1 : 0;
+  }
+}
+
+struct MaskSrc {
+  const unsigned long long* masks;
+  __device__ uint32_t load(uint32_t i) const { return (uint32_t)__popcll(masks[i]); }
+};
+struct WaveOffsetDst {
+  uint32_t* offsets;
+  uint32_t* counters;
+  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { offsets[i] = ex; }
+  __device__ void total(uint32_t t) const { counters[C_TASKS] = t; }
+};
+
+__global__ void k_frustum_bases(FrameParams fp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t waves_per_frustum = fp.slots / 64u;
+  uint32_t cam = 0;
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    uint32_t base = fp.wave_offsets[(size_t)fi * waves_per_frustum];
+    fp.frustum_task_base[fi] = base;
+  }
+  fp.frustum_task_base[fp.F] = fp.counters[C_TASKS];
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    fp.frustum_cam_base[fi] = cam;
+    if (fp.fr[fi].d.kind == 0u) cam += fp.frustum_task_base[fi + 1] - fp.frustum_task_base[fi];
+  }
+  fp.counters[C_CAM_TASKS] = cam;
+}
+
+// Pass 2: compaction with ballot ranks; the queue order is slot order.
+__global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_projected_segment* slot_proj) {
+  const uint64_t total = (uint64_t)fp.F * fp.slots;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long m = fp.slot_masks[s >> 6];
+    if (!((m >> lane) & 1ull)) continue;
+    uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint32_t t = fp.wave_offsets[s >> 6] + rank;
+    if (t >= fp.task_cap) continue;
+    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+    uint32_t ii = find_instance(fp, pos);
+    const InstRec& in = fp.inst[ii];
+    uint32_t p = pos - in.pos_base;
+    const FrustumRec& fr = fp.fr[fi];
+    fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
+    fp.proj[t] = slot_proj[s];
+    if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
+  }
+}
+
+// ------------------------------------------------------- K7 + K8 LUT
+// Interval merge of build_depth_warp_lut, fed one interval at a time in
+// linked-list walk order.
+struct LutBuilder {
+  uint32_t mins[kSlices], maxs[kSlices];
+  uint32_t count;
+  __device__ void add(uint32_t a, uint32_t b) {
+    uint32_t zmin = min(a, b), zmax = max(a, b);
+    bool merged = false;
+    for (uint32_t i = 0; i < count; ++i) {
+      if (zmin <= maxs[i] + 1u && zmax + 1u >= mins[i]) {
+        mins[i] = min(mins[i], zmin);
+        maxs[i] = max(maxs[i], zmax);
+        merged = true;
+      }
+    }
+    if (merged) return;
+    if (count < kSlices) {
+      mins[count] = zmin; maxs[count] = zmax; ++count;
+      return;
+    }
+    uint32_t best = 0, best_gap = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < kSlices; ++i) {
+      uint32_t gap = maxs[i] < zmin ? zmin - maxs[i] : (zmax < mins[i] ? mins[i] - zmax : 0u);
+      if (gap < best_gap) { best_gap = gap; best = i; }
+    }
+    mins[best] = min(mins[best], zmin);
+    maxs[best] = max(maxs[best], zmax);
+  }
+  __device__ void finish(sr_coarse_depth_lut_entry* out) {
+    if (count == 0u) {
+      for (uint32_t i = 0; i < kSlices; ++i) out[i] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+      return;
+    }
+    for (uint32_t i = 1; i < count; ++i) {
+      uint32_t j = i, kmin = mins[i], kmax = maxs[i];
+      while (j != 0u && mins[j - 1u] > kmin) { mins[j] = mins[j - 1u]; maxs[j] = maxs[j - 1u]; --j; }
+      mins[j] = kmin; maxs[j] = kmax;
+    }
+    uint32_t out_slice = 0;
+    for (uint32_t iv = 0; iv < count; ++iv) {
+      uint32_t slice_count = max(1u, kSlices / count) + (iv < kSlices % count ? 1u : 0u);
+      uint32_t zmin = mins[iv], zmax = maxs[iv];
+      uint32_t span = max(1u, zmax - zmin + 1u);
+      for (uint32_t ls = 0; ls < slice_count && out_slice < kSlices; ++ls) {
+        uint32_t start = zmin + (span * ls) / slice_count;
+        uint32_t ex_end = zmin + (span * (ls + 1u)) / slice_count;
+        uint32_t end = min(ex_end > start ? ex_end - 1u : start, zmax);
+        out[out_slice++] = sr_coarse_depth_lut_entry{start, end, start, max(1u, end - start + 1u)};
+      }
+    }
+    while (out_slice < kSlices) out[out_slice++] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+  }
+};
+
+__device__ __forceinline__ uint32_t frustum_for_coarse_tile(const FrameParams& fp, uint32_t tile) {
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    const sr_frustum_desc& d = fp.fr[fi].d;
+    if (tile >= d.coarse_depth_tile_base && tile < d.coarse_depth_tile_base + d.coarse_depth_tile_count) return fi;
+  }
+  return kInvalid;
+}
+
+// One thread per coarse tile: walk the intervals of every asset range that
+// covers it (newest-first over (instance, frustum) insertion order), build
+// the 16-slice depth-warp LUT and clear the tile's page-table and active
+// words for this frame.
+__global__ void k_lut(FrameParams fp) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fp.coarse_total) return;
+  for (uint32_t z = 0; z < kSlices; ++z) fp.page_table[t * kSlices + z] = 0u;
+  for (uint32_t z = 0; z < kExt * kExt; ++z) fp.active_flags[t * kExt * kExt + z] = 0u;
+  uint32_t fi = frustum_for_coarse_tile(fp, t);
+  LutBuilder b;
+  b.count = 0;
+  if (fi != kInvalid) {
+    const sr_frustum_desc& d = fp.fr[fi].d;
+    uint32_t local = t - d.coarse_depth_tile_base;
+    uint32_t cx = local % d.coarse_tiles_x, cy = local / d.coarse_tiles_x;
+    for (int32_t ii = (int32_t)fp.NI - 1; ii >= 0; --ii) {
+      const sr_coarse_asset_range r = fp.ranges[(size_t)ii * fp.F + fi];
+      if (r.geo_id == kInvalid || r.max_x < r.min_x || r.max_y < r.min_y) continue;
+      if (cx < r.min_x || cx > r.max_x || cy < r.min_y || cy > r.max_y) continue;
+      b.add(r.z_min_q, r.z_max_q);
+    }
+  }
+  sr_coarse_depth_lut_entry out[kSlices];
+  b.finish(out);
+  for (uint32_t z = 0; z < kSlices; ++z) fp.lut[(size_t)t * kSlices + z] = out[z];
+}
+
+// ------------------------------------------------ binning traversal
+__device__ __forceinline__ uint32_t lut_slice_for_depth(const sr_coarse_depth_lut_entry* lut, float z) {
+  uint32_t zq = quantize_depth01(z);
+  uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
+  for (uint32_t i = 0; i < kSlices; ++i) {
+    const sr_coarse_depth_lut_entry e = lut[i];
+    if (e.virtual_count_q == 0u) continue;
+    if (zq >= e.z_min_q && zq <= e.z_max_q) return i;
+    uint32_t d = zq > e.z_max_q ? zq - e.z_max_q : e.z_min_q - zq;
+    if (d < best_d) { best_d = d; best = i; }
+  }
+  return best;
+}
+
+enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2 };
+
+template <int MODE>
+__device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
+                                  uint32_t local, const sr_fine_seg_ref& ref) {
+  uint32_t cx = local % fd.coarse_tiles_x, cy = local / fd.coarse_tiles_x;
+  uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
+  V2 origin{(float)(cx * cpx), (float)(cy * cpy)};
+  float tmx = fminf(origin.x + (float)cpx, (float)fd.screen_width) - 1e-3f;
+  float tmy = fminf(origin.y + (float)cpy, (float)fd.screen_height) - 1e-3f;
+  V3 bmax{fmaxf(origin.x, tmx), fmaxf(origin.y, tmy), 1.0f};
+  Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
+  if (!c.ok) return;
+  const sr_coarse_depth_lut_entry* lut = fp.lut + (size_t)tile * kSlices;
+  uint32_t z0s = lut_slice_for_depth(lut, c.p0.z), z1s = lut_slice_for_depth(lut, c.p1.z);
+  if (z0s == kInvalid || z1s == kInvalid) return;
+  uint32_t zlo = min(z0s, z1s), zhi = max(z0s, z1s);
+  for (uint32_t cz = zlo; cz <= zhi; ++cz) {
+    const sr_coarse_depth_lut_entry e = lut[cz];
+    if (e.virtual_count_q == 0u) continue;
+    float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
+    Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
+    if (!dc.ok) continue;
+    uint32_t pti = tile * kSlices + cz;
+    if (MODE == MODE_MARK) {
+      fp.page_table[pti] = kMarked;
+      continue;
+    }
+    uint32_t h = fp.page_table[pti];
+    if (h == 0u || h == kMarked) continue;
+    uint32_t page = h - 1u;
+    uint32_t ref_base = 0, ref_count = 0;
+    if (MODE == MODE_FILL) {
+      ref_base = fp.page_meta[page].seg_ref_base;
+      ref_count = fp.page_meta[page].seg_ref_count;
+      if (ref_count == 0u) continue;
+    }
+    float zspan = fmaxf(zmax - zmin, 1.0f / (float)kQuant);
+    float fxs = (float)fd.froxel_size_x, fys = (float)fd.froxel_size_y;
+    float pa[3] = {(dc.p0.x - origin.x) / fxs, (dc.p0.y - origin.y) / fys,
+                   wclamp((dc.p0.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
+    float pb[3] = {(dc.p1.x - origin.x) / fxs, (dc.p1.y - origin.y) / fys,
+                   wclamp((dc.p1.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
+    const int32_t hi[3] = {(int32_t)kExt - 1, (int32_t)kExt - 1, (int32_t)kSlices - 1};
+    int32_t f[3], f1[3], step[3];
+    float delta[3], tmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      f[k] = iclamp(f2i(floorf(pa[k])), 0, hi[k]);
+      f1[k] = iclamp(f2i(floorf(pb[k])), 0, hi[k]);
+      float d = pb[k] - pa[k];
+      step[k] = sgn_i32(d);
+      bool tiny = fabsf(d) < 1e-6f;
+      float safe = tiny ? 1e-6f : d;
+      delta[k] = fabsf(1.0f / safe);
+      float fl = floorf(pa[k]);
+      tmax[k] = step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
+      if (tiny) tmax[k] = 1e38f;
+    }
+    const uint32_t page_cells = page * kPage;
+    for (uint32_t safety = 0; safety <= kExt + kExt + kSlices + 3u; ++safety) {
+      uint32_t cell = ((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0];
+      if (MODE == MODE_COUNT) {
+        atomicAdd(&fp.cell_counts[page_cells + cell], 1u);
+      } else {
+        uint32_t w = atomicAdd(&fp.cell_cursors[page_cells + cell], 1u);
+        uint64_t dst = (uint64_t)ref_base + fp.cell_offsets[page_cells + cell] + w;
+        if (dst < fp.ref_cap) fp.refs[dst] = ref;
+      }
+      if (f[0] == f1[0] && f[1] == f1[1] && f[2] == f1[2]) break;
+      float mn = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
+      int k = tmax[0] == mn ? 0 : (tmax[1] == mn ? 1 : 2);
+      f[k] += step[k];
+      tmax[k] += delta[k];
+      if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] > hi[0] || f[1] > hi[1] || f[2] > hi[2]) break;
+    }
+  }
+}
+
+template <int MODE>
+__device__ void visit_task(const FrameParams& fp, uint32_t t) {
+  const sr_binning_task tk = fp.tasks[t];
+  const sr_projected_segment ps = fp.proj[t];
+  uint32_t fi = tk.packed_field >> SR_BINNING_FRUSTUM_SHIFT;
+  if (fi >= fp.F) return;
+  const sr_frustum_desc& fd = fp.fr[fi].d;
+  uint32_t ii = tk.id_info;
+  const sr_coarse_asset_range r = fp.ranges[(size_t)ii * fp.F + fi];
+  if (r.geo_id != ii || r.frustum_id != fi) return;
+  float W = (float)fd.screen_width, H = (float)fd.screen_height;
+  V2 a = unpack2x16unorm(ps.p0_xy), b = unpack2x16unorm(ps.p1_xy), dz = unpack2x16unorm(ps.depths);
+  V3 p0{a.x * W, a.y * H, dz.x}, p1{b.x * W, b.y * H, dz.y};
+  sr_fine_seg_ref ref{ii, tk.seg_idx, 0u};
+  if (MODE == MODE_FILL) {
+    const InstRec& in = fp.inst[ii];
+    if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
+  }
+  uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
+  uint32_t min_cx = max(r.min_x, min(f2u(fmaxf(fminf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
+  uint32_t max_cx = min(r.max_x, min(f2u(fmaxf(fmaxf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
+  uint32_t min_cy = max(r.min_y, min(f2u(fmaxf(fminf(p0.y, p1.y), 0.0f)) / cpy, fd.coarse_tiles_y - 1u));
+  uint32_t max_cy = min(r.max_y, min(f2u(fmaxf(fmaxf(p0.y, p1.y), 0.0f)) / cpy, fd.coarse_tiles_y - 1u));
+  if (max_cx < min_cx || max_cy < min_cy) return;
+  V3 bmin{(float)(min_cx * cpx), (float)(min_cy * cpy), 0.0f};
+  V3 bmax{fminf((float)((max_cx + 1u) * cpx), W) - 1e-3f, fminf((float)((max_cy + 1u) * cpy), H) - 1e-3f, 1.0f};
+  Clipped c = clip_segment_to_box(p0, p1, bmin, bmax);
+  if (!c.ok) return;
+  float pa[2] = {c.p0.x / (float)cpx, c.p0.y / (float)cpy};
+  float pb[2] = {c.p1.x / (float)cpx, c.p1.y / (float)cpy};
+  const int32_t lo[2] = {(int32_t)min_cx, (int32_t)min_cy}, hi[2] = {(int32_t)max_cx, (int32_t)max_cy};
+  int32_t f[2], f1[2], step[2];
+  float delta[2], tmax[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    f[k] = iclamp(f2i(floorf(pa[k])), lo[k], hi[k]);
+    f1[k] = iclamp(f2i(floorf(pb[k])), lo[k], hi[k]);
+    float d = pb[k] - pa[k];
+    step[k] = sgn_i32(d);
+    bool tiny = fabsf(d) < 1e-6f;
+    float safe = tiny ? 1e-6f : d;
+    delta[k] = fabsf(1.0f / safe);
+    float fl = floorf(pa[k]);
+    tmax[k] = step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
+    if (tiny) tmax[k] = 1e38f;
+  }
+  uint32_t max_steps = (max_cx - min_cx + 1u) + (max_cy - min_cy + 1u) + 2u;
+  for (uint32_t safety = 0; safety <= max_steps; ++safety) {
+    if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
+    uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
+    if (local < fd.coarse_depth_tile_count)
+      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref);
+    if (f[0] == f1[0] && f[1] == f1[1]) break;
+    if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
+    else { f[1] += step[1]; tmax[1] += delta[1]; }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
+  const uint32_t n = min(fp.counters[C_TASKS], fp.task_cap);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) visit_task<MODE>(fp, t);
+}
+
+// ------------------------------------------------------ K11 allocate
+struct PageMarkSrc {
+  const uint32_t* table;
+  __device__ uint32_t load(uint32_t i) const { return table[i] == kMarked ? 1u : 0u; }
+};
+struct PageAllocDst {
+  uint32_t* table;
+  sr_fine_page_meta* meta;
+  uint32_t* counters;
+  uint32_t page_cap;
+  __device__ void store(uint32_t e, uint32_t ex, uint32_t v) const {
+    if (!v) return;
+    if (ex >= page_cap) {  // the reference drops pages past its capacity (entry 0)
+      table[e] = 0u;
+      atomicAdd(&counters[C_PAGE_DROPS], 1u);
+      return;
+    }
+    table[e] = ex + 1u;
+    meta[ex] = sr_fine_page_meta{0u, 0u, 0u, 0u, e / kSlices, e % kSlices, 0u, 0u};
+  }
+  __device__ void total(uint32_t t) const {
+    counters[C_MARKED] = t;
+    counters[C_PAGES] = min(t, page_cap);
+  }
+};
+
+__global__ __launch_bounds__(256) void k_zero_cells(FrameParams fp) {
+  const uint32_t n = fp.counters[C_PAGES] * (kPage / 4u);
+  uint4* c = reinterpret_cast<uint4*>(fp.cell_counts);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    c[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// ------------------------------------------------- K14 page prefix
+// One 256-lane workgroup per page: page-local exclusive scan of the cell
+// counts (wave64 shuffles + one LDS hop across the four waves), cursor reset,
+// page total and the 16-bit mask of fine tiles with work.
+__global__ __launch_bounds__(256) void k_page_prefix(FrameParams fp, uint32_t* page_xy_mask) {
+  __shared__ uint32_t lds4[4];
+  __shared__ uint32_t colmask;
+  const uint32_t P = fp.counters[C_PAGES];
+  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+    uint32_t c = threadIdx.x;
+    uint32_t cnt = fp.cell_counts[page * kPage + c];
+    if (c == 0) colmask = 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive<uint32_t>(cnt, &total, lds4);
+    fp.cell_offsets[page * kPage + c] = ex;
+    fp.cell_cursors[page * kPage + c] = 0u;
+    if (cnt) atomicOr(&colmask, 1u << (c & 15u));
+    __syncthreads();
+    if (c == 0) {
+      fp.page_totals[page] = total;
+      page_xy_mask[page] = colmask;
+    }
+    __syncthreads();
+  }
+}
+
+struct PageTotalSrc {
+  const uint32_t* totals;
+  __device__ uint32_t load(uint32_t i) const { return totals[i]; }
+};
+struct PageBaseDst {
+  FrameParams fp;
+  const uint32_t* xy_mask;
+  __device__ void store(uint32_t p, uint32_t ex, uint32_t v) const {
+    sr_fine_page_meta& pm = fp.page_meta[p];
+    uint32_t flags = (v > 0u && (uint64_t)ex + v > fp.ref_cap) ? 1u : 0u;
+    pm.seg_ref_base = v > 0u ? ex : 0u;
+    pm.seg_ref_count = flags ? 0u : v;
+    pm.work_base = 0u; pm.work_count = 0u; pm.first_depth_key = 0u;
+    pm.flags = flags;
+    if (flags) atomicAdd(&fp.counters[C_OVERFLOW], 1u);
+    uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
+    if (flags || fi == kInvalid) return;
+    const FrustumRec& fr = fp.fr[fi];
+    uint32_t lc = pm.coarse_tile_id - fr.d.coarse_depth_tile_base;
+    uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
+    uint32_t m = xy_mask[p];
+    while (m) {
+      uint32_t xy = __ffs(m) - 1u;
+      m &= m - 1u;
+      uint32_t fx = cx * kExt + xy % kExt, fy = cy * kExt + xy / kExt;
+      if (fx >= fr.fine_tiles_x || fy >= fr.fine_tiles_y) continue;
+      fp.active_flags[pm.coarse_tile_id * kExt * kExt + xy] = 1u;
+    }
+  }
+  __device__ void total(uint32_t t) const { fp.counters[C_REFS] = t; }
+};
+
+// ------------------------------------------------ canonical cell order
+// Sorts every cell's references by (inst, seg).  The mirror-bitonic network
+// compares i with i ^ (k - 1) and then i with i ^ j, always ascending, so a
+// non-power-of-two length behaves as if padded with +inf at the end.
+__device__ __forceinline__ unsigned long long ref_key(const sr_fine_seg_ref& r) {
+  return ((unsigned long long)r.inst_id << 32) | r.seg_id;
+}
+
+__device__ void sort_cell_wave(sr_fine_seg_ref* refs, uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63u;
+  unsigned long long key = ~0ull;
+  uint32_t mat = 0;
+  if (lane < n) { sr_fine_seg_ref r = refs[lane]; key = ref_key(r); mat = r.material_idx; }
+#pragma unroll
+  for (uint32_t k = 2; k <= 64; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      uint32_t partner = (j == (k >> 1)) ? (lane ^ (k - 1u)) : (lane ^ j);
+      unsigned long long ok = __shfl(key, (int)partner, 64);
+      uint32_t om = __shfl(mat, (int)partner, 64);
+      bool lower = lane < partner;
+      bool take = lower ? (ok < key) : (ok > key);
+      if (take) { key = ok; mat = om; }
+    }
+  }
+  if (lane < n) refs[lane] = sr_fine_seg_ref{(uint32_t)(key >> 32), (uint32_t)key, mat};
+}
+
+constexpr uint32_t kSortLds = 4096;
+
+__device__ void sort_cell_block(sr_fine_seg_ref* refs, uint32_t n, unsigned long long* skey, uint32_t* smat) {
+  const bool in_lds = n <= kSortLds;
+  if (in_lds) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      sr_fine_seg_ref r = refs[i];
+      skey[i] = ref_key(r);
+      smat[i] = r.material_idx;
+    }
+    __syncthreads();
+  }
+  uint32_t m = 1;
+  while (m < n) m <<= 1;
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t partner = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
+        if (partner <= i || partner >= n) continue;
+        if (in_lds) {
+          if (skey[partner] < skey[i]) {
+            unsigned long long tk = skey[i]; skey[i] = skey[partner]; skey[partner] = tk;
+            uint32_t tm = smat[i]; smat[i] = smat[partner]; smat[partner] = tm;
+          }
+        } else {
+          sr_fine_seg_ref a = refs[i], b = refs[partner];
+          if (ref_key(b) < ref_key(a)) { refs[i] = b; refs[partner] = a; }
+        }
+      }
+      if (!in_lds) __threadfence_block();
+      __syncthreads();
+    }
+  }
+  if (in_lds) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+      refs[i] = sr_fine_seg_ref{(uint32_t)(skey[i] >> 32), (uint32_t)skey[i], smat[i]};
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sort_cells(FrameParams fp) {
+  __shared__ unsigned long long skey[kSortLds];
+  __shared__ uint32_t smat[kSortLds];
+  const uint32_t P = fp.counters[C_PAGES];
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+    const sr_fine_page_meta pm = fp.page_meta[page];
+    if (pm.seg_ref_count == 0u) continue;
+    for (uint32_t c = wave; c < kPage; c += 4u) {
+      uint32_t n = fp.cell_counts[page * kPage + c];
+      if (n < 2u || n > 64u) continue;
+      sort_cell_wave(fp.refs + pm.seg_ref_base + fp.cell_offsets[page * kPage + c], n);
+    }
+    for (uint32_t c = 0; c < kPage; ++c) {
+      uint32_t n = fp.cell_counts[page * kPage + c];
+      if (n <= 64u) continue;
+      sort_cell_block(fp.refs + pm.seg_ref_base + fp.cell_offsets[page * kPage + c], n, skey, smat);
+    }
+  }
+}
+
+// ---------------------------------------------- K20 + K21 active tiles
+struct FlagSrc {
+  const uint32_t* flags;
+  __device__ uint32_t load(uint32_t i) const { return flags[i]; }
+};
+struct ActiveDst {
+  uint32_t* queue;
+  uint32_t* counters;
+  __device__ void store(uint32_t i, uint32_t ex, uint32_t v) const { if (v) queue[ex] = i; }
+  __device__ void total(uint32_t t) const { counters[C_ACTIVE] = t; }
+};
+
+// ------------------------------------------------------ K23 emission
+__device__ __forceinline__ uint32_t raster_depth_key(const FrameParams& fp, uint32_t pti, uint32_t fz) {
+  const sr_coarse_depth_lut_entry e = fp.lut[pti];
+  if (e.virtual_count_q == 0u) return fz;
+  return e.virtual_start_q + (e.virtual_count_q * fz) / kSlices;
+}
+
+struct TileLane {
+  uint32_t count, base, page, cell, pti, fz;
+};
+
+// Lane l of a 256-lane workgroup serves depth lane 255 - l, so lanes run
+// near (large keys) to far.
+__device__ __forceinline__ TileLane tile_lane(const FrameParams& fp, uint32_t tile, uint32_t lx, uint32_t ly) {
+  TileLane tl{0u, 0u, kInvalid, 0u, 0u, 0u};
+  uint32_t dl = (kPage - 1u) - threadIdx.x;
+  uint32_t cz = dl / kSlices, fz = dl % kSlices;
+  uint32_t pti = tile * kSlices + cz;
+  tl.pti = pti; tl.fz = fz;
+  uint32_t h = fp.page_table[pti];
+  if (h == 0u || h == kMarked) return tl;
+  uint32_t page = h - 1u;
+  const sr_fine_page_meta& pm = fp.page_meta[page];
+  uint32_t cell = (fz * kExt + ly) * kExt + lx;
+  tl.page = page; tl.cell = cell;
+  if (!(pm.seg_ref_count > 0u && pm.flags == 0u)) return tl;
+  uint32_t n = fp.cell_counts[page * kPage + cell];
+  if (n == 0u) return tl;
+  tl.count = n;
+  tl.base = pm.seg_ref_base + fp.cell_offsets[page * kPage + cell];
+  return tl;
+}
+
+__global__ __launch_bounds__(256) void k_emit_count(FrameParams fp) {
+  __shared__ uint32_t lds4[4];
+  __shared__ uint32_t lds4b[4];
+  const uint32_t A = fp.counters[C_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
+    uint32_t at = fp.active_queue[a];
+    uint32_t tile = at / (kExt * kExt), lf = at % (kExt * kExt);
+    TileLane tl = tile_lane(fp, tile, lf % kExt, lf / kExt);
+    uint32_t works, load;
+    (void)block_exclusive<uint32_t>(tl.count ? 1u : 0u, &works, lds4);
+    (void)block_exclusive<uint32_t>(tl.count, &load, lds4b);
+    if (threadIdx.x == 0) {
+      uint32_t fi = frustum_for_coarse_tile(fp, tile);
+      uint32_t kind = fi == kInvalid ? 2u : fp.fr[fi].d.kind;
+      bool has = works > 0u && fi != kInvalid;
+      fp.tile_work[a] = make_uint4(has ? works : 0u, has && kind == 0u ? 1u : 0u, has && kind == 1u ? 1u : 0u, load);
+    }
+  }
+}
+
+struct TileWorkSrc {
+  const uint4* w;
+  __device__ U4 load(uint32_t i) const { uint4 v = w[i]; return U4{v.x, v.y, v.z, 0u}; }
+};
+struct TileBaseDst {
+  uint4* bases;
+  uint32_t* counters;
+  __device__ void store(uint32_t i, U4 ex, U4) const { bases[i] = make_uint4(ex.x, ex.y, ex.z, 0u); }
+  __device__ void total(U4 t) const {
+    counters[C_WORK] = t.x;
+    counters[C_CAM_RUNS] = t.y;
+    counters[C_SHADOW_RUNS] = t.z;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_emit_write(FrameParams fp) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t A = fp.counters[C_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
+    uint4 tw = fp.tile_work[a];
+    if (tw.x == 0u) continue;  // uniform per block
+    uint4 tb = fp.tile_bases[a];
+    uint32_t at = fp.active_queue[a];
+    uint32_t tile = at / (kExt * kExt), lf = at % (kExt * kExt);
+    uint32_t lx = lf % kExt, ly = lf / kExt;
+    uint32_t fi = frustum_for_coarse_tile(fp, tile);
+    const FrustumRec& fr = fp.fr[fi];
+    uint32_t lc = tile - fr.d.coarse_depth_tile_base;
+    uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
+    uint32_t screen_tile = (cy * kExt + ly) * fr.fine_tiles_x + (cx * kExt + lx);
+    TileLane tl = tile_lane(fp, tile, lx, ly);
+    uint32_t total;
+    uint32_t rank = block_exclusive<uint32_t>(tl.count ? 1u : 0u, &total, lds4);
+    if (tl.count) {
+      uint32_t w = tb.x + rank;
+      if (w < fp.work_cap)
+        fp.work[w] = sr_raster_work_item{tl.base, tl.count, fi, screen_tile, raster_depth_key(fp, tl.pti, tl.fz),
+                                         tl.count, tl.page, tl.cell};
+    }
+    if (threadIdx.x == 0) {
+      sr_raster_tile_run run{tb.x, tw.x, fi, screen_tile, tw.w, 0u, 0u, 0u};
+      if (fr.d.kind == 1u) fp.shadow_runs[tb.z] = run; else fp.cam_runs[tb.y] = run;
+    }
+  }
+}
+
+// ------------------------------------------------------------ sequencer
+static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t max_blocks) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return (uint32_t)g;
+}
+
+void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer) {
+  const uint32_t kMaxGrid = 256u * 16u;
+  const uint64_t slots_total = (uint64_t)fp.F * fp.slots;
+  const uint32_t ptab = fp.coarse_total * kSlices;
+
+  timer->mark(s, "broad");
+  hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 64, 1024)), dim3(64), 0, s, fp);
+  timer->mark(s, "project");
+  hipLaunchKernelGGL(k_project_mark, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  launch_scan<uint32_t>(MaskSrc{fp.slot_masks}, WaveOffsetDst{fp.wave_offsets, fp.counters},
+                        (uint32_t)(slots_total / 64u), nullptr, ps.partials, s);
+  hipLaunchKernelGGL(k_frustum_bases, dim3(1), dim3(64), 0, s, fp);
+  hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  timer->mark(s, "depth_lut");
+  hipLaunchKernelGGL(k_lut, dim3(grid_for(fp.coarse_total, 64, 1u << 20)), dim3(64), 0, s, fp);
+  timer->mark(s, "mark");
+  hipLaunchKernelGGL((k_visit<MODE_MARK>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+  timer->mark(s, "allocate");
+  launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap}, ptab,
+                        nullptr, ps.partials, s);
+  hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp);
+  timer->mark(s, "count");
+  hipLaunchKernelGGL((k_visit<MODE_COUNT>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+  timer->mark(s, "prefix");
+  hipLaunchKernelGGL(k_page_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp, ps.page_xy_mask);
+  launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, ps.page_xy_mask}, fp.page_cap, fp.counters + C_PAGES,
+                        ps.partials, s);
+  timer->mark(s, "fill");
+  hipLaunchKernelGGL((k_visit<MODE_FILL>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+  timer->mark(s, "sort_cells");
+  hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  timer->mark(s, "compact");
+  launch_scan<uint32_t>(FlagSrc{fp.active_flags}, ActiveDst{fp.active_queue, fp.counters},
+                        fp.coarse_total * kExt * kExt, nullptr, ps.partials, s);
+  timer->mark(s, "emit");
+  hipLaunchKernelGGL(k_emit_count, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
+                  fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
+  hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+}
+
+}  // namespace srk

@@ -1,0 +1,682 @@
+// runtime.hip -- host side of the strand runtime behind the C ABI.
+//
+// Replaces the reference's plugin orchestration (src/plugin.rs:264-1209
+// use_prepass_buffers: frustum and instance tables, capacity planning, queue
+// resets) and its four run_* sequencers (src/pipelines/prepass.rs:818-1397,
+// shadows.rs:621-747, shading.rs:470-567, raster.rs:540-641).
+//
+// Assets live in device paging slabs addressed by DevicePtr{slab, offset,
+// size} (src/shaders/types.wgsl:23-27).  Every frame resolves each instance's
+// five slab ids to raw device pointers once on the host, so kernels never
+// chase page tables.  Per-frame buffers grow and never shrink.  A frame is
+// one stream-ordered sequence of launches with every data-dependent size kept
+// on the device; nothing is read back until sr_frame_wait / sr_frame_dump.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "scan.hpp"
+
+using namespace srk;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& e) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = e;
+}
+
+inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------------ slabs
+struct SlabAllocator {
+  struct Slab {
+    char* base = nullptr;
+    uint64_t bytes = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> free;  // (offset, size), sorted by offset
+  };
+  uint64_t slab_bytes = 256ull << 20;
+  std::vector<Slab> slabs;
+
+  ~SlabAllocator() {
+    for (auto& s : slabs)
+      if (s.base) (void)hipFree(s.base);
+  }
+  // returns false on device OOM
+  bool alloc(uint64_t size, sr_device_ptr* out) {
+    size = align_up(std::max<uint64_t>(size, 1), 256);
+    for (uint32_t si = 0; si < slabs.size(); ++si) {
+      auto& fl = slabs[si].free;
+      for (size_t k = 0; k < fl.size(); ++k) {
+        if (fl[k].second >= size) {
+          uint64_t off = fl[k].first;
+          fl[k].first += size;
+          fl[k].second -= size;
+          if (fl[k].second == 0) fl.erase(fl.begin() + k);
+          if (off > 0xFFFFFFFFull || size > 0xFFFFFFFFull) return false;
+          *out = sr_device_ptr{si, (uint32_t)off, (uint32_t)size};
+          return true;
+        }
+      }
+    }
+    uint64_t bytes = std::max(slab_bytes, size);
+    Slab s;
+    if (hipMalloc((void**)&s.base, bytes) != hipSuccess) return false;
+    s.bytes = bytes;
+    if (bytes > size) s.free.push_back({size, bytes - size});
+    slabs.push_back(std::move(s));
+    *out = sr_device_ptr{(uint32_t)(slabs.size() - 1), 0u, (uint32_t)size};
+    return true;
+  }
+  void release(const sr_device_ptr& p) {
+    if (p.slab >= slabs.size() || p.size == 0) return;
+    auto& fl = slabs[p.slab].free;
+    fl.push_back({p.offset, p.size});
+    std::sort(fl.begin(), fl.end());
+    std::vector<std::pair<uint64_t, uint64_t>> merged;
+    for (auto& f : fl) {
+      if (!merged.empty() && merged.back().first + merged.back().second == f.first) merged.back().second += f.second;
+      else merged.push_back(f);
+    }
+    fl.swap(merged);
+  }
+  char* ptr(const sr_device_ptr& p) const { return slabs[p.slab].base + p.offset; }
+  uint64_t capacity() const {
+    uint64_t c = 0;
+    for (auto& s : slabs) c += s.bytes;
+    return c;
+  }
+};
+
+struct AssetRec {
+  bool live = false;
+  sr_asset_info info{};
+  sr_device_ptr pos_strand{};
+  sr_strand_geo geo{};
+  uint32_t vertex_count = 0, material_count = 0;
+};
+
+// grow-only device buffer
+struct DBuf {
+  void* p = nullptr;
+  uint64_t bytes = 0;
+  bool ensure(uint64_t need) {
+    need = std::max<uint64_t>(need, 256);
+    if (need <= bytes) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    uint64_t b = align_up(need + need / 4, 4096);
+    if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; return false; }
+    bytes = b;
+    return true;
+  }
+  void free_() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+enum BufId {
+  B_COUNTERS, B_BROAD, B_KEEP, B_RANGES, B_SLOT_MASKS, B_WAVE_OFF, B_FR_TASK_BASE, B_FR_CAM_BASE, B_SHADED,
+  B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
+  B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
+  B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_FRUSTA, B_INSTS, B_LIGHTS, B_COUNT
+};
+
+// Shape of the last submitted frame, for dumps.
+struct FrameShape {
+  bool valid = false;
+  uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
+  uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, atlas_texels = 0, dom_layers = 0;
+  uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
+  int hist_cur = 0;
+};
+
+}  // namespace
+
+struct sr_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  SlabAllocator slabs;
+  std::vector<AssetRec> assets;
+  DBuf buf[B_COUNT];
+  void* staging = nullptr;  // pinned host staging for per-frame tables
+  uint64_t staging_bytes = 0;
+  FrameShape shape;
+  StageTimer timer;
+  hipEvent_t done = nullptr;
+  bool submitted = false;
+  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0;
+  int hist_cur = 0;
+  uint64_t hbm_budget = 0;
+};
+
+namespace {
+
+int fail(sr_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  set_global_error(msg);
+  return code;
+}
+
+int hip_fail(sr_context* ctx, hipError_t e, const char* what) {
+  return fail(ctx, SR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_OK(expr, what)                        \
+  do {                                            \
+    hipError_t _e = (expr);                       \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+  } while (0)
+
+bool ensure_staging(sr_context* ctx, uint64_t bytes) {
+  if (bytes <= ctx->staging_bytes) return true;
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
+  ctx->staging = nullptr;
+  ctx->staging_bytes = 0;
+  uint64_t b = align_up(bytes * 2, 4096);
+  if (hipHostMalloc(&ctx->staging, b, hipHostMallocDefault) != hipSuccess) return false;
+  ctx->staging_bytes = b;
+  return true;
+}
+
+inline uint32_t next_pow2(uint32_t x) {
+  uint32_t p = 1;
+  while (p < x && p < 0x80000000u) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sr_version(void) { return "bevy_strand_rasterizer_amd 0.1.0 (gfx950)"; }
+
+const char* sr_last_error(sr_context* ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  return g_err.c_str();
+}
+
+int sr_create(const sr_device_config* cfg, sr_context** out) {
+  if (!out) return fail(nullptr, SR_ERR_INVALID_ARGUMENT, "sr_create: out is NULL");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(nullptr, SR_ERR_DEVICE, "sr_create: no HIP device available");
+  int dev = cfg ? cfg->device_index : 0;
+  if (dev < 0 || dev >= count) return fail(nullptr, SR_ERR_INVALID_ARGUMENT, "sr_create: device index out of range");
+  if ((e = hipSetDevice(dev)) != hipSuccess) return fail(nullptr, SR_ERR_DEVICE, hipGetErrorString(e));
+  sr_context* ctx = new sr_context();
+  ctx->device = dev;
+  if (cfg && cfg->slab_bytes) ctx->slabs.slab_bytes = cfg->slab_bytes;
+  if (cfg) ctx->hbm_budget = cfg->hbm_budget_bytes;
+  if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming)) != hipSuccess) {
+    std::string m = std::string("sr_create: ") + hipGetErrorString(e);
+    delete ctx;
+    return fail(nullptr, SR_ERR_DEVICE, m);
+  }
+  *out = ctx;
+  return SR_OK;
+}
+
+int sr_destroy(sr_context* ctx) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->buf) b.free_();
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
+  for (auto e : ctx->timer.events) (void)hipEventDestroy(e);
+  if (ctx->done) (void)hipEventDestroy(ctx->done);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SR_OK;
+}
+
+// validate_cache-style checks (src/strand_cache.rs:174-209) plus the slot
+// layout requirement: strands occupy disjoint index ranges.
+int sr_asset_upload(sr_context* ctx, const sr_asset_slabs* a, sr_asset_info* out) {
+  if (!ctx || !a || !out) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_asset_upload: NULL argument");
+  if (!a->vertices || !a->indices || !a->meta || !a->geo || !a->materials || a->vertex_count == 0 ||
+      a->index_count == 0 || a->strand_count == 0 || a->material_count == 0)
+    return fail(ctx, SR_ERR_INVALID_ASSET, "sr_asset_upload: empty slab");
+  for (uint32_t i = 0; i < a->index_count; ++i)
+    if (a->indices[i] >= a->vertex_count) return fail(ctx, SR_ERR_INVALID_ASSET, "index points past the vertex slab");
+  std::vector<std::pair<uint32_t, uint32_t>> spans;
+  spans.reserve(a->strand_count);
+  uint32_t max_verts = 0;
+  for (uint32_t s = 0; s < a->strand_count; ++s) {
+    const sr_strand_meta& m = a->meta[s];
+    if ((uint64_t)m.offset + m.count > a->index_count)
+      return fail(ctx, SR_ERR_INVALID_ASSET, "strand range outside the index slab");
+    if (m.material_idx >= a->material_count) return fail(ctx, SR_ERR_INVALID_ASSET, "material index out of range");
+    if (m.count > 0) spans.push_back({m.offset, m.count});
+    max_verts = std::max(max_verts, m.count);
+  }
+  std::sort(spans.begin(), spans.end());
+  for (size_t i = 1; i < spans.size(); ++i)
+    if (spans[i - 1].first + spans[i - 1].second > spans[i].first)
+      return fail(ctx, SR_ERR_INVALID_ASSET, "strands overlap in the index slab");
+  std::vector<uint32_t> pos_strand(a->index_count, kInvalid);
+  for (uint32_t s = 0; s < a->strand_count; ++s) {
+    const sr_strand_meta& m = a->meta[s];
+    if (m.count < 2) continue;
+    for (uint32_t p = m.offset; p + 1 < m.offset + m.count; ++p) pos_strand[p] = s;
+  }
+  (void)hipSetDevice(ctx->device);
+  AssetRec rec;
+  rec.live = true;
+  rec.geo = *a->geo;
+  rec.vertex_count = a->vertex_count;
+  rec.material_count = a->material_count;
+  struct Up { const void* src; uint64_t bytes; sr_device_ptr* dst; };
+  sr_asset_info& info = rec.info;
+  Up ups[6] = {{a->vertices, 16ull * a->vertex_count, &info.vertices},
+               {a->indices, 4ull * a->index_count, &info.indices},
+               {a->meta, 16ull * a->strand_count, &info.meta},
+               {a->geo, 64ull, &info.geo},
+               {a->materials, 64ull * a->material_count, &info.materials},
+               {pos_strand.data(), 4ull * a->index_count, &rec.pos_strand}};
+  for (auto& u : ups) {
+    if (!ctx->slabs.alloc(u.bytes, u.dst)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "sr_asset_upload: slab allocation failed");
+    u.dst->size = (uint32_t)u.bytes;  // DevicePtr.size carries the payload size
+    HIP_OK(hipMemcpy(ctx->slabs.ptr(*u.dst), u.src, u.bytes, hipMemcpyHostToDevice), "sr_asset_upload: copy");
+  }
+  uint32_t id = 0;
+  while (id < ctx->assets.size() && ctx->assets[id].live) ++id;
+  info.vertex_id = info.index_id = info.meta_id = info.geo_id = info.material_id = id;
+  info.strand_count = a->strand_count;
+  info.index_count = a->index_count;
+  info.max_segments_in_strand = max_verts;
+  if (id == ctx->assets.size()) ctx->assets.push_back(rec); else ctx->assets[id] = rec;
+  *out = info;
+  return SR_OK;
+}
+
+int sr_asset_release(sr_context* ctx, uint32_t id) {
+  if (!ctx || id >= ctx->assets.size() || !ctx->assets[id].live)
+    return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_asset_release: unknown asset");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  AssetRec& r = ctx->assets[id];
+  const sr_device_ptr* owned[6] = {&r.info.vertices, &r.info.indices, &r.info.meta,
+                                   &r.info.geo,      &r.info.materials, &r.pos_strand};
+  for (const sr_device_ptr* p : owned) {
+    sr_device_ptr q = *p;
+    q.size = (uint32_t)align_up(q.size, 256);
+    ctx->slabs.release(q);
+  }
+  r.live = false;
+  return SR_OK;
+}
+
+int sr_set_profiling(sr_context* ctx, int32_t enabled) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  ctx->timer.enabled = enabled != 0;
+  return SR_OK;
+}
+
+int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) {
+  if (!ctx || !fd) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: NULL argument");
+  if (fd->frustum_count == 0 || !fd->frusta || !fd->views)
+    return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no frusta");
+  if (fd->instance_count && !fd->instances) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
+  if (fd->light_count && !fd->lights) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no lights");
+  const sr_frame_settings& st = fd->settings;
+  if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
+    return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
+  if (st.fine_binning_backend > 1) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = stream_ptr ? (hipStream_t)stream_ptr : ctx->stream;
+
+  const uint32_t F = fd->frustum_count, NI = fd->instance_count, NL = fd->light_count;
+  // ---- frustum records
+  std::vector<FrustumRec> fr(F);
+  uint32_t lights_seen = 0, cams = 0, coarse_total = 0;
+  uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
+  for (uint32_t i = 0; i < F; ++i) {
+    const sr_frustum_desc& d = fd->frusta[i];
+    if (d.screen_width == 0 || d.screen_height == 0 || d.froxel_size_x == 0 || d.froxel_size_y == 0)
+      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "frustum with zero extent");
+    if (d.froxel_size_x * d.froxel_size_y > 256)
+      return fail(ctx, SR_ERR_UNSUPPORTED, "froxel tiles above 256 pixels are not supported");
+    uint32_t ftx = (d.screen_width + d.froxel_size_x - 1) / d.froxel_size_x;
+    uint32_t fty = (d.screen_height + d.froxel_size_y - 1) / d.froxel_size_y;
+    if (d.coarse_tiles_x * kExt < ftx || d.coarse_tiles_y * kExt < fty ||
+        d.coarse_depth_tile_count < d.coarse_tiles_x * d.coarse_tiles_y)
+      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "frustum coarse tiling does not cover its fine tiles");
+    FrustumRec& r = fr[i];
+    std::memset(&r, 0, sizeof(r));
+    r.d = d;
+    std::memcpy(r.clip, fd->views[i].clip_from_world, sizeof(r.clip));
+    std::memcpy(r.eye, fd->views[i].world_position, sizeof(r.eye));
+    r.fine_tiles_x = ftx;
+    r.fine_tiles_y = fty;
+    r.light_layer = kInvalid;
+    r.cam_index = kInvalid;
+    if (d.kind == 1u) {
+      if (lights_seen < NL) r.light_layer = lights_seen;
+      ++lights_seen;
+      r.dom_depth_off = dom_px;
+      r.dom_opacity_off = dom_px * st.dom_layers;
+      dom_px += (uint64_t)d.screen_width * d.screen_height;
+      r.stamp_w = st.stamp_target_width ? st.stamp_target_width : d.screen_width;
+      r.stamp_h = st.stamp_target_height ? st.stamp_target_height : d.screen_height;
+      r.stamp_off = stamp_px;
+      stamp_px += (uint64_t)r.stamp_w * r.stamp_h;
+    } else if (d.kind == 0u) {
+      r.cam_index = cams++;
+      r.image_off = img_px;
+      img_px += (uint64_t)d.screen_width * d.screen_height;
+    }
+    r.enabled = (d.kind == 0u) || (d.kind == 1u && r.light_layer != kInvalid);
+    coarse_total = std::max(coarse_total, d.coarse_depth_tile_base + d.coarse_depth_tile_count);
+  }
+  // ---- instance records
+  std::vector<InstRec> ir(NI);
+  uint64_t pos = 0;
+  uint32_t max_idx = 1;
+  for (uint32_t i = 0; i < NI; ++i) {
+    const sr_strand_instance& in = fd->instances[i];
+    InstRec& r = ir[i];
+    std::memset(&r, 0, sizeof(r));
+    std::memcpy(r.world_from_local, in.world_from_local, sizeof(r.world_from_local));
+    r.pos_base = (uint32_t)pos;
+    auto live = [&](uint32_t id) { return id < ctx->assets.size() && ctx->assets[id].live; };
+    if (!live(in.vertex_id) || !live(in.index_id) || !live(in.meta_id) || !live(in.geo_id) || !live(in.material_id))
+      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "instance references an unknown asset id");
+    const AssetRec& va = ctx->assets[in.vertex_id];
+    const AssetRec& ia = ctx->assets[in.index_id];
+    const AssetRec& ma = ctx->assets[in.meta_id];
+    const AssetRec& ga = ctx->assets[in.geo_id];
+    const AssetRec& ta = ctx->assets[in.material_id];
+    r.vertices = reinterpret_cast<const float4*>(ctx->slabs.ptr(va.info.vertices));
+    r.indices = reinterpret_cast<const uint32_t*>(ctx->slabs.ptr(ia.info.indices));
+    r.pos_strand = reinterpret_cast<const uint32_t*>(ctx->slabs.ptr(ia.pos_strand));
+    r.meta = reinterpret_cast<const sr_strand_meta*>(ctx->slabs.ptr(ma.info.meta));
+    r.materials = reinterpret_cast<const sr_strand_material*>(ctx->slabs.ptr(ta.info.materials));
+    r.vertex_count = va.vertex_count;
+    r.index_count = ia.info.index_count;
+    r.meta_count = ma.info.strand_count;
+    r.material_count = ta.material_count;
+    r.geo_strand_count = ga.geo.strand_count;
+    r.aabb = ga.geo.aabb;
+    // kernels index vertices/materials through the meta/index slabs of the
+    // same instance; mixing assets requires matching slab shapes
+    if (in.index_id != in.meta_id || in.vertex_id != in.index_id || in.material_id != in.meta_id)
+      return fail(ctx, SR_ERR_UNSUPPORTED, "an instance must take all slabs from one asset");
+    r.valid = 1u;
+    pos += r.index_count;
+    max_idx = std::max(max_idx, r.index_count);
+  }
+  const uint32_t slots = (uint32_t)align_up(std::max<uint64_t>(pos, 64), 64);
+  if ((uint64_t)F * slots > 0x7FFFFFFFull) return fail(ctx, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
+  const uint32_t task_cap = F * slots;
+  const uint32_t ptab = coarse_total * kSlices;
+  uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, std::max(4096u, ptab / 2)), ctx->page_cap_hint);
+  page_cap = std::min(page_cap, ptab);
+  // reference slab: 8 references per task slot (the reference sizes it at 4x
+  // its binning queue, src/plugin.rs:540-546); work items: half of that.
+  // Both grow when a frame reports an overflow.
+  uint64_t ref_cap64 = std::max<uint64_t>({(uint64_t)task_cap * 8ull, 1ull << 16, (uint64_t)ctx->ref_cap_hint});
+  const uint32_t ref_cap = (uint32_t)std::min<uint64_t>(ref_cap64, 0x7FFFFFFFull);
+  const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, ctx->work_cap_hint});
+  const uint32_t atlas_w = std::min(std::max(max_idx, 1u), SR_MAX_TEXTURE_EXTENT);
+  const uint32_t atlas_h = std::min(std::max((max_idx + atlas_w - 1) / atlas_w, 1u), SR_MAX_TEXTURE_EXTENT);
+  const uint32_t atlas_texels = atlas_w * atlas_h;
+  const uint64_t atlas_n = (uint64_t)cams * NI * atlas_texels;
+  const uint32_t tiles16 = coarse_total * kExt * kExt;
+  const uint32_t scan_tiles = (std::max({task_cap / 64u, ptab, tiles16, 1u}) + kScanTile - 1) / kScanTile + 1;
+
+  // history: keep the previous buffer when the atlas layout is unchanged
+  const bool hist_valid = st.shadow_history_valid && ctx->shape.valid && ctx->shape.cams == cams &&
+                          ctx->shape.NI == NI && ctx->shape.atlas_texels == atlas_texels;
+  const int hist_next = hist_valid ? (ctx->hist_cur ^ 1) : 0;
+
+  struct Need { BufId id; uint64_t bytes; };
+  const Need needs[] = {
+      {B_COUNTERS, 4ull * C_COUNT}, {B_BROAD, 16ull * NI + 16}, {B_KEEP, 4ull * NI * F + 4},
+      {B_RANGES, 32ull * NI * F + 32}, {B_SLOT_MASKS, 8ull * task_cap / 64}, {B_WAVE_OFF, 4ull * task_cap / 64},
+      {B_FR_TASK_BASE, 4ull * (F + 1)}, {B_FR_CAM_BASE, 4ull * F}, {B_SHADED, (uint64_t)std::max(cams, 1u) * slots},
+      {B_TASKS, 16ull * task_cap}, {B_PROJ, 12ull * task_cap}, {B_SHADING_Q, 4ull * task_cap},
+      {B_LUT, 16ull * ptab}, {B_PAGE_TABLE, 4ull * ptab}, {B_PAGE_META, 32ull * page_cap},
+      {B_PAGE_TOTALS, 4ull * page_cap}, {B_CELL_COUNTS, 4ull * page_cap * kPage},
+      {B_CELL_OFFSETS, 4ull * page_cap * kPage}, {B_CELL_CURSORS, 4ull * page_cap * kPage},
+      {B_REFS, 12ull * ref_cap}, {B_ACTIVE_FLAGS, 4ull * tiles16}, {B_ACTIVE_Q, 4ull * tiles16},
+      {B_TILE_WORK, 16ull * tiles16}, {B_TILE_BASES, 16ull * tiles16}, {B_WORK, 32ull * work_cap},
+      {B_CAM_RUNS, 32ull * tiles16}, {B_SHADOW_RUNS, 32ull * tiles16}, {B_DOM_DEPTH, 4ull * dom_px},
+      {B_DOM_OPACITY, 4ull * dom_px * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
+      {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
+      {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
+      {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_FRUSTA, sizeof(FrustumRec) * F},
+      {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
+  if (ctx->submitted) (void)hipStreamSynchronize(s);  // buffers may be reallocated below
+  for (const Need& n : needs) {
+    if (!ctx->buf[n.id].ensure(n.bytes)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
+  }
+  // ---- upload per-frame tables through pinned staging
+  const uint64_t tbytes = sizeof(FrustumRec) * F + sizeof(InstRec) * NI + sizeof(sr_directional_light) * NL;
+  if (!ensure_staging(ctx, tbytes + 256)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
+  char* stg = (char*)ctx->staging;
+  std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
+  if (NI) std::memcpy(stg + sizeof(FrustumRec) * F, ir.data(), sizeof(InstRec) * NI);
+  if (NL) std::memcpy(stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI, fd->lights, sizeof(sr_directional_light) * NL);
+  HIP_OK(hipMemcpyAsync(ctx->buf[B_FRUSTA].p, stg, sizeof(FrustumRec) * F, hipMemcpyHostToDevice, s), "upload frusta");
+  if (NI)
+    HIP_OK(hipMemcpyAsync(ctx->buf[B_INSTS].p, stg + sizeof(FrustumRec) * F, sizeof(InstRec) * NI,
+                          hipMemcpyHostToDevice, s), "upload instances");
+  if (NL)
+    HIP_OK(hipMemcpyAsync(ctx->buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
+                          sizeof(sr_directional_light) * NL, hipMemcpyHostToDevice, s), "upload lights");
+
+  FrameParams fp;
+  std::memset(&fp, 0, sizeof(fp));
+  fp.fr = ctx->buf[B_FRUSTA].as<FrustumRec>();
+  fp.inst = ctx->buf[B_INSTS].as<InstRec>();
+  fp.lights = ctx->buf[B_LIGHTS].as<sr_directional_light>();
+  fp.F = F; fp.NI = NI; fp.NL = NL; fp.cams = cams;
+  fp.slots = slots; fp.coarse_total = coarse_total;
+  fp.task_cap = task_cap; fp.ref_cap = ref_cap; fp.work_cap = work_cap; fp.atlas_texels = atlas_texels;
+  fp.dom_layers = st.dom_layers;
+  fp.st = st;
+  fp.st.shadow_history_valid = hist_valid ? 1u : 0u;
+  fp.counters = ctx->buf[B_COUNTERS].as<uint32_t>();
+  fp.broad = ctx->buf[B_BROAD].as<sr_broad_instance_meta>();
+  fp.keep = ctx->buf[B_KEEP].as<float>();
+  fp.ranges = ctx->buf[B_RANGES].as<sr_coarse_asset_range>();
+  fp.slot_masks = ctx->buf[B_SLOT_MASKS].as<unsigned long long>();
+  fp.wave_offsets = ctx->buf[B_WAVE_OFF].as<uint32_t>();
+  fp.frustum_task_base = ctx->buf[B_FR_TASK_BASE].as<uint32_t>();
+  fp.frustum_cam_base = ctx->buf[B_FR_CAM_BASE].as<uint32_t>();
+  fp.shaded = ctx->buf[B_SHADED].as<uint8_t>();
+  fp.tasks = ctx->buf[B_TASKS].as<sr_binning_task>();
+  fp.proj = ctx->buf[B_PROJ].as<sr_projected_segment>();
+  fp.shading_queue = ctx->buf[B_SHADING_Q].as<uint32_t>();
+  fp.lut = ctx->buf[B_LUT].as<sr_coarse_depth_lut_entry>();
+  fp.page_table = ctx->buf[B_PAGE_TABLE].as<uint32_t>();
+  fp.page_meta = ctx->buf[B_PAGE_META].as<sr_fine_page_meta>();
+  fp.page_totals = ctx->buf[B_PAGE_TOTALS].as<uint32_t>();
+  fp.cell_counts = ctx->buf[B_CELL_COUNTS].as<uint32_t>();
+  fp.cell_offsets = ctx->buf[B_CELL_OFFSETS].as<uint32_t>();
+  fp.cell_cursors = ctx->buf[B_CELL_CURSORS].as<uint32_t>();
+  fp.refs = ctx->buf[B_REFS].as<sr_fine_seg_ref>();
+  fp.active_flags = ctx->buf[B_ACTIVE_FLAGS].as<uint32_t>();
+  fp.active_queue = ctx->buf[B_ACTIVE_Q].as<uint32_t>();
+  fp.tile_work = ctx->buf[B_TILE_WORK].as<uint4>();
+  fp.tile_bases = ctx->buf[B_TILE_BASES].as<uint4>();
+  fp.work = ctx->buf[B_WORK].as<sr_raster_work_item>();
+  fp.cam_runs = ctx->buf[B_CAM_RUNS].as<sr_raster_tile_run>();
+  fp.shadow_runs = ctx->buf[B_SHADOW_RUNS].as<sr_raster_tile_run>();
+  fp.dom_depth = ctx->buf[B_DOM_DEPTH].as<float>();
+  fp.dom_opacity = ctx->buf[B_DOM_OPACITY].as<float>();
+  fp.stamp = ctx->buf[B_STAMP].as<float>();
+  fp.atlas = ctx->buf[B_ATLAS].as<float4>();
+  fp.history = ctx->buf[hist_next ? B_HIST1 : B_HIST0].as<float2>();
+  fp.history_prev = hist_valid ? ctx->buf[hist_next ? B_HIST0 : B_HIST1].as<float2>() : nullptr;
+  fp.image = ctx->buf[B_IMAGE].as<float4>();
+  fp.depth = ctx->buf[B_DEPTH].as<float>();
+  fp.composite = ctx->buf[B_COMPOSITE].as<float4>();
+  // page_cap bounds the page arrays; the allocation scan drops pages past it
+  PrepassScratch ps{ctx->buf[B_SLOT_PROJ].as<sr_projected_segment>(), ctx->buf[B_PARTIALS].as<uint32_t>(),
+                    ctx->buf[B_PARTIALS4].as<uint4>(), ctx->buf[B_XY_MASK].as<uint32_t>()};
+  fp.page_cap = page_cap;
+
+  ctx->timer.begin();
+  ctx->timer.mark(s, "clear");
+  launch_fill_u32(fp.counters, C_COUNT, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.image), img_px * 4, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.depth), img_px, 0u, s);
+  launch_prepass(fp, ps, s, &ctx->timer);
+  ctx->timer.mark(s, "shadow");
+  launch_shadow(fp, s);
+  ctx->timer.mark(s, "stampback");
+  for (uint32_t i = 0; i < F; ++i)
+    if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
+  ctx->timer.mark(s, "shading");
+  launch_shading(fp, s);
+  ctx->timer.mark(s, "raster");
+  launch_camera_raster(fp, s);
+  ctx->timer.mark(s, "composite");
+  launch_composite(fp, img_px, s);
+  ctx->timer.mark(s, "end");
+  HIP_OK(hipGetLastError(), "sr_frame_submit: launch");
+  HIP_OK(hipEventRecord(ctx->done, s), "sr_frame_submit: event");
+
+  FrameShape& sh = ctx->shape;
+  sh.valid = true;
+  sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
+  sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
+  sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
+  sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
+  sh.hist_cur = hist_next;
+  ctx->hist_cur = hist_next;
+  ctx->submitted = true;
+  return SR_OK;
+}
+
+int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  if (!ctx->submitted) return fail(ctx, SR_ERR_NOT_READY, "sr_frame_wait: no frame submitted");
+  (void)hipSetDevice(ctx->device);
+  HIP_OK(hipEventSynchronize(ctx->done), "sr_frame_wait");
+  uint32_t c[C_COUNT];
+  HIP_OK(hipMemcpy(c, ctx->buf[B_COUNTERS].p, sizeof(c), hipMemcpyDeviceToHost), "sr_frame_wait: counters");
+  const FrameShape& sh = ctx->shape;
+  sr_frame_counters fc;
+  std::memset(&fc, 0, sizeof(fc));
+  fc.binning_tasks = c[C_TASKS];
+  fc.camera_tasks = c[C_CAM_TASKS];
+  fc.allocated_pages = c[C_PAGES];
+  fc.fine_refs = c[C_REFS];
+  fc.active_tiles = c[C_ACTIVE];
+  fc.work_items = c[C_WORK];
+  fc.camera_runs = c[C_CAM_RUNS];
+  fc.shadow_runs = c[C_SHADOW_RUNS];
+  fc.overflow_pages = c[C_OVERFLOW];
+  fc.coarse_tiles = sh.coarse_total;
+  fc.task_capacity = sh.task_cap;
+  fc.ref_capacity = sh.ref_cap;
+  // grow-only capacity planning for the next frame
+  if (c[C_REFS] > sh.ref_cap) ctx->ref_cap_hint = next_pow2(c[C_REFS]);
+  if (c[C_PAGE_DROPS] > 0) ctx->page_cap_hint = next_pow2(c[C_MARKED]);
+  if (c[C_WORK] > sh.work_cap) ctx->work_cap_hint = next_pow2(c[C_WORK]);
+  fc.pad[0] = c[C_PAGE_DROPS];
+  if (out) *out = fc;
+  return SR_OK;
+}
+
+int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, uint64_t* out_bytes) {
+  if (!ctx || !out_bytes) return SR_ERR_INVALID_ARGUMENT;
+  sr_frame_counters fc;
+  int rc = sr_frame_wait(ctx, &fc);
+  if (rc != SR_OK) return rc;
+  const FrameShape& sh = ctx->shape;
+  const void* src = nullptr;
+  uint64_t bytes = 0;
+  uint32_t P = std::min(fc.allocated_pages, sh.page_cap);
+  switch (kind) {
+    case SR_DUMP_COUNTERS:
+      *out_bytes = sizeof(fc);
+      if (dst) {
+        if (capacity < sizeof(fc)) return fail(ctx, SR_ERR_CAPACITY, "dump buffer too small");
+        std::memcpy(dst, &fc, sizeof(fc));
+      }
+      return SR_OK;
+    case SR_DUMP_BROAD_META: src = ctx->buf[B_BROAD].p; bytes = 16ull * sh.NI; break;
+    case SR_DUMP_KEEP: src = ctx->buf[B_KEEP].p; bytes = 4ull * sh.NI * sh.F; break;
+    case SR_DUMP_RANGES: src = ctx->buf[B_RANGES].p; bytes = 32ull * sh.NI * sh.F; break;
+    case SR_DUMP_TASKS: src = ctx->buf[B_TASKS].p; bytes = 16ull * std::min(fc.binning_tasks, sh.task_cap); break;
+    case SR_DUMP_PROJECTED: src = ctx->buf[B_PROJ].p; bytes = 12ull * std::min(fc.binning_tasks, sh.task_cap); break;
+    case SR_DUMP_SHADING_QUEUE: src = ctx->buf[B_SHADING_Q].p; bytes = 4ull * fc.camera_tasks; break;
+    case SR_DUMP_LUT: src = ctx->buf[B_LUT].p; bytes = 16ull * sh.coarse_total * kSlices; break;
+    case SR_DUMP_PAGE_TABLE: src = ctx->buf[B_PAGE_TABLE].p; bytes = 4ull * sh.coarse_total * kSlices; break;
+    case SR_DUMP_PAGE_META: src = ctx->buf[B_PAGE_META].p; bytes = 32ull * P; break;
+    case SR_DUMP_CELL_COUNTS: src = ctx->buf[B_CELL_COUNTS].p; bytes = 4ull * P * kPage; break;
+    case SR_DUMP_CELL_OFFSETS: src = ctx->buf[B_CELL_OFFSETS].p; bytes = 4ull * P * kPage; break;
+    case SR_DUMP_REFS: src = ctx->buf[B_REFS].p; bytes = 12ull * std::min(fc.fine_refs, sh.ref_cap); break;
+    case SR_DUMP_ACTIVE_TILES: src = ctx->buf[B_ACTIVE_Q].p; bytes = 4ull * fc.active_tiles; break;
+    case SR_DUMP_WORK_ITEMS: src = ctx->buf[B_WORK].p; bytes = 32ull * std::min(fc.work_items, sh.work_cap); break;
+    case SR_DUMP_CAMERA_RUNS: src = ctx->buf[B_CAM_RUNS].p; bytes = 32ull * fc.camera_runs; break;
+    case SR_DUMP_SHADOW_RUNS: src = ctx->buf[B_SHADOW_RUNS].p; bytes = 32ull * fc.shadow_runs; break;
+    case SR_DUMP_DOM_DEPTH: src = ctx->buf[B_DOM_DEPTH].p; bytes = 4ull * sh.dom_px; break;
+    case SR_DUMP_DOM_OPACITY: src = ctx->buf[B_DOM_OPACITY].p; bytes = 4ull * sh.dom_px * sh.dom_layers; break;
+    case SR_DUMP_ATLAS: src = ctx->buf[B_ATLAS].p; bytes = 16ull * sh.cams * sh.NI * sh.atlas_texels; break;
+    case SR_DUMP_HISTORY:
+      src = ctx->buf[sh.hist_cur ? B_HIST1 : B_HIST0].p;
+      bytes = 8ull * sh.cams * sh.NI * sh.atlas_texels;
+      break;
+    case SR_DUMP_IMAGE: src = ctx->buf[B_IMAGE].p; bytes = 16ull * sh.img_px; break;
+    case SR_DUMP_DEPTH: src = ctx->buf[B_DEPTH].p; bytes = 4ull * sh.img_px; break;
+    case SR_DUMP_COMPOSITE: src = ctx->buf[B_COMPOSITE].p; bytes = 16ull * sh.img_px; break;
+    case SR_DUMP_STAMP: src = ctx->buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
+    default: return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_dump: unknown kind");
+  }
+  *out_bytes = bytes;
+  if (!dst) return SR_OK;
+  if (capacity < bytes) return fail(ctx, SR_ERR_CAPACITY, "dump buffer too small");
+  if (bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "sr_frame_dump: copy");
+  return SR_OK;
+}
+
+int sr_stage_times(sr_context* ctx, float* dst_ms, int32_t capacity, int32_t* out_count, char* names,
+                   int32_t names_capacity) {
+  if (!ctx || !out_count) return SR_ERR_INVALID_ARGUMENT;
+  StageTimer& t = ctx->timer;
+  if (!t.enabled || t.used < 2) { *out_count = 0; return SR_OK; }
+  HIP_OK(hipEventSynchronize(t.events[t.used - 1]), "sr_stage_times");
+  int n = (int)t.used - 1;
+  std::string joined;
+  for (int i = 0; i < n; ++i) {
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, t.events[i], t.events[i + 1]), "sr_stage_times: elapsed");
+    if (dst_ms && i < capacity) dst_ms[i] = ms;
+    joined += t.names[i];
+    joined += '\n';
+  }
+  if (names && names_capacity > 0) {
+    std::strncpy(names, joined.c_str(), (size_t)names_capacity - 1);
+    names[names_capacity - 1] = 0;
+  }
+  *out_count = n;
+  return SR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// scan.hpp -- device-wide exclusive scan for wave64 / CDNA4.
+//
+// Reduce-then-scan in three launches, no inter-workgroup spin waits:
+//   1. scan_reduce_k : each 2048-item tile reduces to one partial;
+//   2. scan_spine_k  : one workgroup scans the tile partials in place and
+//                      publishes the grand total;
+//   3. scan_down_k   : each tile rescans with its offset and calls the
+//                      consumer's store(i, exclusive, value).
+// Items are read striped (item k of lane t at tile + k*256 + t) so every
+// load is coalesced; the element count may live in device memory so the
+// launch shape is fixed by a capacity and never needs a host round trip.
+#pragma once
+
+#include "common.hpp"
+
+namespace srk {
+
+constexpr int kScanWg = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanWg * kScanItems;
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+__device__ __forceinline__ U4 operator+(U4 a, U4 b) { return U4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+__device__ __forceinline__ uint32_t shfl_up_v(uint32_t v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ U4 shfl_up_v(U4 v, int d) {
+  return U4{__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64), __shfl_up(v.z, d, 64), __shfl_up(v.w, d, 64)};
+}
+template <class T> __device__ __forceinline__ T zero_v() { return T{}; }
+
+// Inclusive scan across one wave64.
+template <class T>
+__device__ __forceinline__ T wave_inclusive(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T o = shfl_up_v(v, d);
+    if (lane >= d) v = v + o;
+  }
+  return v;
+}
+
+// Exclusive scan across the 256-thread block; returns the block total too.
+template <class T>
+__device__ __forceinline__ T block_exclusive(T v, T* total, T* lds4) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T inc = wave_inclusive(v);
+  if (lane == 63) lds4[wave] = inc;
+  __syncthreads();
+  T wave_base = zero_v<T>(), tot = zero_v<T>();
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    T s = lds4[w];
+    if (w < wave) wave_base = wave_base + s;
+    tot = tot + s;
+  }
+  __syncthreads();
+  *total = tot;
+  T exc = inc;
+  // exclusive = inclusive - v, expressed without subtraction for U4 too
+  exc = shfl_up_v(inc, 1);
+  if (lane == 0) exc = zero_v<T>();
+  return wave_base + exc;
+}
+
+__device__ __forceinline__ uint32_t scan_count(uint32_t cap, const uint32_t* n_dev) {
+  return n_dev ? min(cap, *n_dev) : cap;
+}
+
+template <class T, class Src>
+__global__ __launch_bounds__(kScanWg) void scan_reduce_k(Src src, uint32_t cap, const uint32_t* n_dev, T* partials) {
+  __shared__ T lds4[4];
+  const uint32_t n = scan_count(cap, n_dev);
+  const uint32_t tile = blockIdx.x * (uint32_t)kScanTile;
+  if (tile >= n) return;
+  T acc = zero_v<T>();
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = tile + k * kScanWg + threadIdx.x;
+    if (i < n) acc = acc + src.load(i);
+  }
+  T total;
+  (void)block_exclusive(acc, &total, lds4);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+template <class T, class Dst>
+__global__ __launch_bounds__(kScanWg) void scan_spine_k(T* partials, uint32_t cap, const uint32_t* n_dev, Dst dst) {
+  __shared__ T lds4[4];
+  const uint32_t n = scan_count(cap, n_dev);
+  const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+  T carry = zero_v<T>();
+  for (uint32_t b = 0; b < tiles; b += kScanWg) {
+    uint32_t i = b + threadIdx.x;
+    T v = i < tiles ? partials[i] : zero_v<T>();
+    T total;
+    T ex = block_exclusive(v, &total, lds4);
+    if (i < tiles) partials[i] = carry + ex;
+    carry = carry + total;
+  }
+  if (threadIdx.x == 0) dst.total(carry);
+}
+
+template <class T, class Src, class Dst>
+__global__ __launch_bounds__(kScanWg) void scan_down_k(Src src, Dst dst, uint32_t cap, const uint32_t* n_dev,
+                                                       const T* partials) {
+  __shared__ T lds4[4];
+  const uint32_t n = scan_count(cap, n_dev);
+  const uint32_t tile = blockIdx.x * (uint32_t)kScanTile;
+  if (tile >= n) return;
+  T carry = partials[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    uint32_t i = tile + k * kScanWg + threadIdx.x;
+    T v = i < n ? src.load(i) : zero_v<T>();
+    T total;
+    T ex = block_exclusive(v, &total, lds4);
+    if (i < n) dst.store(i, carry + ex, v);
+    carry = carry + total;
+  }
+}
+
+// Host-side launcher: partials must hold ceil(cap / 2048) items.
+template <class T, class Src, class Dst>
+inline void launch_scan(Src src, Dst dst, uint32_t cap, const uint32_t* n_dev, T* partials, hipStream_t s) {
+  if (cap == 0) {
+    hipLaunchKernelGGL((scan_spine_k<T, Dst>), dim3(1), dim3(kScanWg), 0, s, partials, 0u, nullptr, dst);
+    return;
+  }
+  uint32_t tiles = (cap + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((scan_reduce_k<T, Src>), dim3(tiles), dim3(kScanWg), 0, s, src, cap, n_dev, partials);
+  hipLaunchKernelGGL((scan_spine_k<T, Dst>), dim3(1), dim3(kScanWg), 0, s, partials, cap, n_dev, dst);
+  hipLaunchKernelGGL((scan_down_k<T, Src, Dst>), dim3(tiles), dim3(kScanWg), 0, s, src, dst, cap, n_dev, partials);
+}
+
+}  // namespace srk

@@ -1,0 +1,327 @@
+"""Synthetic strand workloads (the benchmark configurations) and frame tables.
+
+An asset is a set of host slabs in the reference layout (vertices float4 with
+w = 1, identity index slab, StrandMeta, one StrandGeo, StrandMaterial per
+strand).  A workload adds instances, frusta in frustum-table order (cameras
+are kind 0, directional-light cascades kind 1), one view per frustum, the
+directional lights and the frame settings.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class Asset:
+    vertices: np.ndarray   # float32 (V, 4)
+    indices: np.ndarray    # uint32 (I,)
+    meta: np.ndarray       # strand_meta_dtype (N,)
+    geo: np.ndarray        # strand_geo_dtype (1,)
+    materials: np.ndarray  # strand_material_dtype (M,)
+
+    @property
+    def strand_count(self) -> int:
+        return int(self.meta.shape[0])
+
+    @property
+    def segment_count(self) -> int:
+        return int(np.maximum(self.meta["count"].astype(np.int64) - 1, 0).sum())
+
+    def slabs(self) -> abi.AssetSlabs:
+        s = abi.AssetSlabs()
+        s.vertices = abi.ptr(self.vertices)
+        s.vertex_count = self.vertices.shape[0]
+        s.indices = abi.ptr(self.indices)
+        s.index_count = self.indices.shape[0]
+        s.meta = abi.ptr(self.meta)
+        s.strand_count = self.meta.shape[0]
+        s.geo = abi.ptr(self.geo)
+        s.materials = abi.ptr(self.materials)
+        s.material_count = self.materials.shape[0]
+        return s
+
+
+@dataclass
+class Workload:
+    name: str
+    assets: list
+    instances: np.ndarray
+    frusta: np.ndarray
+    views: np.ndarray
+    lights: np.ndarray
+    settings: dict = field(default_factory=dict)
+
+    @property
+    def camera_count(self) -> int:
+        return int((self.frusta["kind"] == 0).sum())
+
+    @property
+    def input_segments(self) -> int:
+        return sum(self.assets[int(i["geo_id"])].segment_count for i in self.instances)
+
+    def frame_settings(self) -> abi.FrameSettings:
+        st = abi.FrameSettings()
+        d = default_settings()
+        d.update(self.settings)
+        st.stochastic_cull_enabled = int(d["stochastic_cull_enabled"])
+        st.target_strands_per_pixel = d["target_strands_per_pixel"]
+        st.min_keep_probability = d["min_keep_probability"]
+        st.shadow_keep_probability = d["shadow_keep_probability"]
+        st.telemetry_enabled = int(d["telemetry_enabled"])
+        st.fine_binning_backend = int(d["fine_binning_backend"])
+        st.dom_layers = int(d["dom_layers"])
+        st.shadow_history_valid = int(d["shadow_history_valid"])
+        for i in range(4):
+            st.ambient_color[i] = d["ambient_color"][i]
+            st.scene_clear_color[i] = d["scene_clear_color"][i]
+        st.scene_clear_depth = d["scene_clear_depth"]
+        st.stamp_target_width = int(d["stamp_target_width"])
+        st.stamp_target_height = int(d["stamp_target_height"])
+        return st
+
+    def frame_desc(self, keep: list | None = None) -> abi.FrameDesc:
+        """Builds a sr_frame_desc pointing at this workload's arrays."""
+        for a in (self.instances, self.frusta, self.views, self.lights):
+            assert a.flags["C_CONTIGUOUS"]
+        fd = abi.FrameDesc()
+        fd.instances = abi.ptr(self.instances) if len(self.instances) else None
+        fd.instance_count = len(self.instances)
+        fd.frustum_count = len(self.frusta)
+        fd.frusta = abi.ptr(self.frusta)
+        fd.views = abi.ptr(self.views)
+        fd.lights = abi.ptr(self.lights) if len(self.lights) else None
+        fd.light_count = len(self.lights)
+        fd.settings = self.frame_settings()
+        return fd
+
+
+def default_settings() -> dict:
+    """StochasticCullSettings defaults (src/resources.rs:95-104) and the
+    headless scene parameters used by the benchmark workloads."""
+    return {
+        "stochastic_cull_enabled": 1,
+        "target_strands_per_pixel": 2.0,
+        "min_keep_probability": 0.02,
+        "shadow_keep_probability": 1.0,
+        "telemetry_enabled": 0,
+        "fine_binning_backend": 0,
+        "dom_layers": 4,
+        "shadow_history_valid": 0,
+        "ambient_color": (80.0, 80.0, 80.0, 1.0),
+        # ClearColor srgb(0.025, 0.025, 0.03) in linear RGB
+        "scene_clear_color": (0.0019353, 0.0019353, 0.0025950, 1.0),
+        "scene_clear_depth": 0.0,
+        "stamp_target_width": 0,
+        "stamp_target_height": 0,
+    }
+
+
+# ----------------------------------------------------------- generators
+def scalp_asset(strands: int, points: int, seed: int, width_m: float, radius_px: float,
+                curl_max: float = 0.01, scalp_radius: float = 0.1, length=(0.05, 0.25),
+                color=(0.3, 0.05)) -> Asset:
+    """Seeded scalp hair: roots uniform on a hemisphere, gravity-drooping
+    Catmull-Rom strands with helical curl, per-strand colour."""
+    lib = abi.load_library()
+    p = abi.ScalpParams()
+    p.strand_count, p.points_per_strand, p.seed = strands, points, seed
+    p.scalp_radius, p.length_min, p.length_max = scalp_radius, length[0], length[1]
+    p.curl_radius_max, p.width = curl_max, width_m
+    p.color_mean, p.color_sigma, p.radius_pixels = color[0], color[1], radius_px
+    nv, ni, ns, nm = (C.c_uint32() for _ in range(4))
+    abi.check(lib, lib.sr_scalp_sizes(C.byref(p), C.byref(nv), C.byref(ni), C.byref(ns), C.byref(nm)),
+              None, "sr_scalp_sizes")
+    a = Asset(vertices=np.zeros((nv.value, 4), np.float32), indices=np.zeros(ni.value, np.uint32),
+              meta=np.zeros(ns.value, abi.strand_meta_dtype), geo=np.zeros(1, abi.strand_geo_dtype),
+              materials=np.zeros(nm.value, abi.strand_material_dtype))
+    abi.check(lib, lib.sr_scalp_generate(C.byref(p), abi.ptr(a.vertices), abi.ptr(a.indices), abi.ptr(a.meta),
+                                         abi.ptr(a.geo), abi.ptr(a.materials)), None, "sr_scalp_generate")
+    return a
+
+
+def asset_from_strands(strands: list, material: dict | None = None, radius_px: float = 1.0) -> Asset:
+    """Packs explicit polylines like pack_strands (src/plugin.rs:1315-1355):
+    strands with fewer than 2 points are dropped, vertices become (x, y, z, 1),
+    StrandMeta{count, offset, material}."""
+    pts, meta = [], []
+    for s in strands:
+        s = np.asarray(s, np.float32)
+        if s.shape[0] < 2:
+            continue
+        meta.append((s.shape[0], sum(len(p) for p in pts), len(meta), 0))
+        pts.append(s)
+    v = np.concatenate(pts).astype(np.float32)
+    verts = np.concatenate([v, np.ones((v.shape[0], 1), np.float32)], axis=1)
+    a = Asset(vertices=np.ascontiguousarray(verts), indices=np.arange(v.shape[0], dtype=np.uint32),
+              meta=np.array(meta, dtype=abi.strand_meta_dtype), geo=np.zeros(1, abi.strand_geo_dtype),
+              materials=np.zeros(len(meta), abi.strand_material_dtype))
+    g = a.geo[0]
+    g["strand_count"] = len(meta)
+    g["max_segments_in_strand"] = max(m[0] for m in meta)
+    g["aabb"]["min"] = v.min(axis=0)
+    g["aabb"]["max"] = v.max(axis=0)
+    a.geo[0] = g
+    m = default_material()
+    if material:
+        m.update(material)
+    m["min_radius_pixels"] = m.get("min_radius_pixels", radius_px)
+    m["max_radius_pixels"] = m.get("max_radius_pixels", radius_px)
+    for k, val in m.items():
+        a.materials[k] = val
+    return a
+
+
+def default_material() -> dict:
+    """benchmark_material of the reference bench (examples/strand_bench.rs:489-502)."""
+    return {"absorption_color": (0.56, 0.25, 0.14, 0.45), "specular_color": (0.9, 0.72, 0.52, 0.2),
+            "ambient_factor": 0.15, "ao_factor": 0.1, "eta": 1.55, "beta": 0.85, "alpha": 0.35, "shift": 0.1}
+
+
+# ----------------------------------------------------------- frame tables
+def identity_instance(asset_id: int) -> np.ndarray:
+    inst = np.zeros(1, abi.strand_instance_dtype)
+    for k in ("vertex_id", "index_id", "meta_id", "geo_id", "material_id"):
+        inst[k] = asset_id
+    eye = np.eye(4, dtype=np.float32).reshape(-1)
+    inst["world_from_local"] = eye
+    inst["local_from_world"] = eye
+    return inst
+
+
+def frustum_table(configs: list, kinds: list, cascades: list | None = None) -> np.ndarray:
+    """FroxelConfig list -> GpuFrustumDesc table (src/plugin.rs:386-458)."""
+    lib = abi.load_library()
+    cfg = np.array([tuple(c) for c in configs], dtype=abi.froxel_config_dtype)
+    kinds_a = np.array(kinds, np.uint32)
+    casc = np.array(cascades if cascades else [0] * len(configs), np.uint32)
+    out = np.zeros(len(configs), abi.frustum_desc_dtype)
+    abi.check(lib, lib.sr_build_frustum_table(abi.ptr(cfg), abi.ptr(kinds_a), abi.ptr(casc), len(configs),
+                                              abi.ptr(out)), None, "sr_build_frustum_table")
+    return out
+
+
+def camera_view(eye, target, fov_y_deg: float, width: int, height: int, near: float = 0.1) -> np.ndarray:
+    lib = abi.load_library()
+    v = np.zeros(1, abi.view_dtype)
+    e = np.array(eye, np.float32)
+    t = np.array(target, np.float32)
+    up = np.array((0.0, 1.0, 0.0), np.float32)
+    abi.check(lib, lib.sr_camera_view(abi.ptr(e), abi.ptr(t), abi.ptr(up), math.radians(fov_y_deg),
+                                      width / height, near, abi.ptr(v)), None, "sr_camera_view")
+    return v
+
+
+def light_view(direction_to_light, center, radius: float) -> np.ndarray:
+    lib = abi.load_library()
+    v = np.zeros(1, abi.view_dtype)
+    d = np.array(direction_to_light, np.float32)
+    c = np.array(center, np.float32)
+    abi.check(lib, lib.sr_light_view(abi.ptr(d), abi.ptr(c), radius, abi.ptr(v)), None, "sr_light_view")
+    return v
+
+
+def make_light(direction_to_light, color=(255.0, 245.0, 230.0)) -> np.ndarray:
+    l = np.zeros(1, abi.light_dtype)
+    d = np.array(direction_to_light, np.float64)
+    d = d / np.linalg.norm(d)
+    l["direction_to_light"] = (d[0], d[1], d[2], 0.0)
+    l["color"] = (*color, 1.0)
+    return l
+
+
+LIGHT_DIRS = [(0.35, 0.85, 0.4), (-0.6, 0.7, 0.3), (0.2, 0.6, -0.75), (-0.3, 0.9, -0.3)]
+HEAD_CENTER = (0.0, -0.06, 0.0)
+FOV_Y = 45.0
+
+
+def radius_px_for(width_m: float, height_px: int, distance: float, fov_y_deg: float = FOV_Y) -> float:
+    focal = (height_px / 2.0) / math.tan(math.radians(fov_y_deg) / 2.0)
+    return 0.5 * width_m * focal / distance
+
+
+def bounding_sphere(asset: Asset):
+    g = asset.geo[0]["aabb"]
+    mn, mx = np.asarray(g["min"], np.float64), np.asarray(g["max"], np.float64)
+    return tuple((mn + mx) / 2.0), float(np.linalg.norm(mx - mn) / 2.0) * 1.05
+
+
+def orbit_eye(i: int, n: int, distance: float, center=HEAD_CENTER, elev_deg: float = 15.0):
+    az = 2.0 * math.pi * i / max(n, 1)
+    el = math.radians(elev_deg * math.sin(3.0 * az))
+    return (center[0] + distance * math.cos(el) * math.sin(az), center[1] + distance * math.sin(el),
+            center[2] + distance * math.cos(el) * math.cos(az))
+
+
+def build_workload(name: str, asset: Asset, cameras: list, light_count: int, light_res: int, dom_layers: int,
+                   froxel: int = 8, light_froxel: int = 8, settings: dict | None = None) -> Workload:
+    """cameras: list of (eye, width, height).  Frusta are ordered cameras first,
+    then one cascade per light."""
+    center, radius = bounding_sphere(asset)
+    cfgs, kinds, views = [], [], []
+    for eye, w, h in cameras:
+        cfgs.append((w, h, froxel, froxel, 16))
+        kinds.append(0)
+        views.append(camera_view(eye, HEAD_CENTER, FOV_Y, w, h))
+    lights = []
+    for li in range(light_count):
+        d = LIGHT_DIRS[li % len(LIGHT_DIRS)]
+        cfgs.append((light_res, light_res, light_froxel, light_froxel, 16))
+        kinds.append(1)
+        views.append(light_view(d, center, radius))
+        lights.append(make_light(d))
+    st = {"dom_layers": dom_layers}
+    if settings:
+        st.update(settings)
+    return Workload(name=name, assets=[asset], instances=identity_instance(0), frusta=frustum_table(cfgs, kinds),
+                    views=np.ascontiguousarray(np.concatenate(views)),
+                    lights=np.ascontiguousarray(np.concatenate(lights)) if lights else np.zeros(0, abi.light_dtype),
+                    settings=st)
+
+
+def config_small(froxel: int = 16, seed: int = 0x5EEDCAFE) -> Workload:
+    """2,000 strands x 16 points, width 0.0008 m, 512x512 camera at 0.6 m,
+    one directional light with a 512^2 deep shadow map of 4 layers."""
+    dist = 0.6
+    asset = scalp_asset(2000, 16, seed, 0.0008, radius_px_for(0.0008, 512, dist))
+    eye = (HEAD_CENTER[0], HEAD_CENTER[1] + 0.05, HEAD_CENTER[2] + dist)
+    return build_workload(f"scalp-2k-x16-512sq-f{froxel}", asset, [(eye, 512, 512)], 1, 512, 4, froxel, froxel)
+
+
+def config_named(seed: int = 0x5EEDCAFE, strands: int = 100_000, points: int = 32) -> Workload:
+    """100k strands x 32 points, width 0.0006 m, 1920x1080 camera, one
+    directional light with a 2048^2 deep shadow map of 4 layers, 8x8 froxels."""
+    dist = 0.6
+    asset = scalp_asset(strands, points, seed, 0.0006, radius_px_for(0.0006, 1080, dist))
+    eye = (HEAD_CENTER[0], HEAD_CENTER[1] + 0.05, HEAD_CENTER[2] + dist)
+    return build_workload(f"scalp-{strands // 1000}k-x{points}-1080p", asset, [(eye, 1920, 1080)], 1, 2048, 4)
+
+
+def config_batched(views: int = 64, seed: int = 0x5EEDCAFE, strands: int = 100_000, points: int = 32,
+                   lights: int = 4) -> Workload:
+    """The named asset from `views` orbit cameras with resolutions drawn from
+    {1280x720, 1920x1080, 2560x1440} and `lights` 2048^2 x 4-layer lights."""
+    dist = 0.6
+    asset = scalp_asset(strands, points, seed, 0.0006, radius_px_for(0.0006, 1080, dist))
+    rng = np.random.default_rng(seed)
+    res = [(1280, 720), (1920, 1080), (2560, 1440)]
+    cams = []
+    for i in range(views):
+        w, h = res[int(rng.integers(0, 3))]
+        cams.append((orbit_eye(i, views, dist), w, h))
+    return build_workload(f"scalp-{strands // 1000}k-x{points}-{views}views", asset, cams, lights, 2048, 4)
+
+
+def config_tiny(strands: int = 24, points: int = 6, res: int = 64, froxel: int = 8, seed: int = 7,
+                light_res: int = 64, dom_layers: int = 4, lights: int = 1) -> Workload:
+    """Tiny scalp for exhaustive parity checks."""
+    dist = 0.6
+    asset = scalp_asset(strands, points, seed, 0.004, radius_px_for(0.004, res, dist))
+    eye = (HEAD_CENTER[0], HEAD_CENTER[1] + 0.05, HEAD_CENTER[2] + dist)
+    return build_workload(f"tiny-{strands}x{points}-{res}", asset, [(eye, res, res)], lights, light_res,
+                          dom_layers, froxel, froxel)

@@ -1,0 +1,389 @@
+"""Pins of the CPU restatement (oracle/strand_oracle.cpp) against what the
+mathematics and the reference's own documents fix, independently of the
+restatement's code:
+
+* closed forms: pack2x16unorm per the WGSL spec, depth quantisation end
+  points, integer depth-warp LUT slicing, Fresnel at normal incidence,
+  the truncated Bessel series vs scipy's I0, the Gauss-Legendre table vs
+  numpy.leggauss, the wrapped-Gaussian detector's unit integral;
+* brute force on tiny inputs: Liang-Barsky clipping vs dense sampling, the
+  3D DDA's cells vs densely sampled segment/cell intersections, exclusive
+  scans, stable compaction, stampback max-reduce;
+* invariants: count/offset/reference consistency, per-cell (inst, seg) order,
+  DOM monotonicity, alpha range, single-segment coverage against the
+  analytic distance-to-segment formula.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from bevy_strand_rasterizer_amd import abi, scene
+from oracle import oracle
+
+Q = abi.DEPTH_QUANT_MAX
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return oracle.load()
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    w = scene.config_tiny()
+    return w, oracle.render(w, threads=2)
+
+
+@pytest.fixture(scope="module")
+def small16():
+    w = scene.config_small(froxel=16)
+    return w, oracle.render(w)
+
+
+# --------------------------------------------------------------- primitives
+def test_wang_hash_is_the_published_mix(lib):
+    # Thomas Wang's 32-bit mix as used at common.wgsl:229-237; x + (x << 3)
+    # is written here as a multiply by 9 and the steps are checked to be a
+    # bijection on a sample (every step is invertible mod 2^32).
+    rng = np.random.default_rng(1)
+    xs = rng.integers(0, 2**32, 4096, dtype=np.uint64)
+    seen = set()
+    for x in xs:
+        x = int(x)
+        h = ((x ^ 61) ^ (x >> 16)) & 0xFFFFFFFF
+        h = (h * 9) & 0xFFFFFFFF
+        h ^= h >> 4
+        h = (h * 0x27D4EB2D) & 0xFFFFFFFF
+        h ^= h >> 15
+        assert lib.oracle_wang_hash(x) == h
+        seen.add(h)
+    assert len(seen) == len(set(int(x) for x in xs))
+    # hash_to_unit_float: x / 2^32 in [0, 1)
+    assert lib.oracle_hash_to_unit_float(0) == 0.0
+    assert lib.oracle_hash_to_unit_float(2**31) == 0.5
+    assert lib.oracle_hash_to_unit_float(2**32 - 1) <= 1.0
+
+
+def test_pack2x16unorm_spec(lib):
+    # WGSL spec: u16 = floor(0.5 + 65535 * clamp(e, 0, 1)), low half first
+    assert lib.oracle_pack2x16unorm(0.0, 1.0) == 0xFFFF0000
+    assert lib.oracle_pack2x16unorm(1.0, 0.0) == 0x0000FFFF
+    assert lib.oracle_pack2x16unorm(0.5, -3.0) == 32768
+    assert lib.oracle_pack2x16unorm(2.0, 0.25) == 0xFFFF | (16384 << 16)
+    rng = np.random.default_rng(2)
+    for a in rng.random(2000, dtype=np.float32):
+        v = lib.oracle_pack2x16unorm(float(a), 0.0) & 0xFFFF
+        assert abs(v / 65535.0 - float(a)) <= 0.5 / 65535.0 + 1e-7
+
+
+def test_depth_quantisation(lib):
+    assert lib.oracle_quantize_depth01(0.0) == 0
+    assert lib.oracle_quantize_depth01(1.0) == Q
+    assert lib.oracle_quantize_depth01(-5.0) == 0
+    assert lib.oracle_quantize_depth01(7.0) == Q
+    zs = np.sort(np.random.default_rng(3).random(500, dtype=np.float32))
+    qs = [lib.oracle_quantize_depth01(float(z)) for z in zs]
+    assert all(a <= b for a, b in zip(qs, qs[1:]))
+    for z, q in zip(zs, qs):
+        assert abs(q - float(z) * Q) <= 0.5 + 2.0
+    # normalize_depth01: identity on [0, 1], z*0.5+0.5 clamped otherwise
+    assert lib.oracle_normalize_depth01(0.3) == pytest.approx(0.3)
+    assert lib.oracle_normalize_depth01(-0.5) == pytest.approx(0.25)
+    assert lib.oracle_normalize_depth01(-3.0) == 0.0
+
+
+def lut(lib, intervals):
+    arr = np.array(intervals, np.uint32).reshape(-1)
+    out = np.zeros(16, abi.coarse_depth_lut_entry_dtype)
+    lib.oracle_build_lut(abi.ptr(arr) if arr.size else None, len(intervals), abi.ptr(out))
+    return out
+
+
+def test_depth_warp_lut_closed_forms(lib):
+    # one interval of span 160 -> 16 equal slices of 10
+    e = lut(lib, [(0, 159)])
+    assert list(e["z_min_q"]) == [10 * i for i in range(16)]
+    assert list(e["z_max_q"]) == [10 * i + 9 for i in range(16)]
+    assert list(e["virtual_count_q"]) == [10] * 16
+    # two disjoint intervals of span 100 -> 8 slices each, floor division
+    e = lut(lib, [(200, 299), (0, 99)])
+    starts = [(100 * k) // 8 for k in range(8)]
+    assert list(e["z_min_q"][:8]) == starts
+    assert list(e["z_min_q"][8:]) == [200 + s for s in starts]
+    # overlapping intervals merge; three intervals -> 6/5/5 slices
+    e = lut(lib, [(0, 10), (5, 20)])
+    assert e["z_min_q"][0] == 0 and e["z_max_q"][15] == 20
+    e = lut(lib, [(0, 29), (100, 129), (200, 229)])
+    assert list(e["z_min_q"][:6]) == [(30 * k) // 6 for k in range(6)]
+    assert e["z_min_q"][6] == 100 and e["z_min_q"][11] == 200
+    # no intervals -> all empty
+    assert (lut(lib, [])["virtual_count_q"] == 0).all()
+    # more than 16 disjoint intervals: each one is covered by some slice
+    ivs = [(1000 * i, 1000 * i + 10) for i in range(20)]
+    e = lut(lib, ivs)
+    for a, b in ivs:
+        assert any(lo <= a and b <= hi for lo, hi in zip(e["z_min_q"], e["z_max_q"]))
+
+
+def clip(lib, p0, p1, bmin, bmax):
+    a = [np.array(x, np.float32) for x in (p0, p1, bmin, bmax)]
+    out = np.zeros(6, np.float32)
+    ok = lib.oracle_clip_segment(*[abi.ptr(x) for x in a], abi.ptr(out))
+    return ok, out[:3], out[3:]
+
+
+def test_clip_against_sampling(lib):
+    rng = np.random.default_rng(4)
+    bmin, bmax = np.array([0, 0, 0], np.float32), np.array([10, 8, 1], np.float32)
+    for _ in range(300):
+        p0 = rng.uniform([-5, -5, -0.5], [15, 13, 1.5]).astype(np.float32)
+        p1 = rng.uniform([-5, -5, -0.5], [15, 13, 1.5]).astype(np.float32)
+        ok, a, b = clip(lib, p0, p1, bmin, bmax)
+        t = np.linspace(0, 1, 4001)
+        pts = p0[None, :] + (p1 - p0)[None, :] * t[:, None]
+        inside = np.all((pts >= bmin) & (pts <= bmax), axis=1)
+        if not ok:
+            assert not np.all((pts >= bmin + 1e-3) & (pts <= bmax - 1e-3), axis=1).any()
+            continue
+        assert np.all(a >= bmin - 1e-3) and np.all(a <= bmax + 1e-3)
+        assert np.all(b >= bmin - 1e-3) and np.all(b <= bmax + 1e-3)
+        ts = t[inside]
+        if ts.size:
+            step = np.linalg.norm(p1 - p0) / 4000.0
+            assert np.linalg.norm(a - pts[inside][0]) <= 1.01 * step + 1e-4
+            assert np.linalg.norm(b - pts[inside][-1]) <= 1.01 * step + 1e-4
+
+
+def test_gauss_legendre_table():
+    # shading_LUTs.wgsl:3-4 are the 10-point Gauss-Legendre nodes/weights
+    x, wts = np.polynomial.legendre.leggauss(10)
+    roots = [-0.9739065285171717, -0.8650633666889844, -0.6794095682990244, -0.4333953941292472,
+             -0.14887433898163116, 0.14887433898163116, 0.4333953941292472, 0.6794095682990244,
+             0.8650633666889844, 0.9739065285171717]
+    np.testing.assert_allclose(roots, x, atol=1e-12)
+    assert wts.sum() == pytest.approx(2.0)
+
+
+def test_fresnel_normal_incidence(lib):
+    for eta in (1.33, 1.55, 2.0):
+        assert lib.oracle_fresnel(1.0, eta, 1.0) == pytest.approx(((1 - eta) / (1 + eta)) ** 2, rel=1e-5)
+    # grazing incidence (cos -> 0) reflects everything
+    assert lib.oracle_fresnel(1.0, 1.55, 0.0) == pytest.approx(1.0, abs=1e-6)
+
+
+def test_bessel_series_is_truncated_i0(lib):
+    from scipy.special import i0
+
+    assert lib.oracle_bessel(0.0, 5) == 1.0
+    for x in (0.1, 0.5, 1.0, 2.0):
+        # sum_{k<5} (x/2)^(2k) / (k!)^2, the first five terms of I0
+        assert lib.oracle_bessel(x, 5) == pytest.approx(i0(x), rel=2e-5 if x <= 1 else 2e-3)
+
+
+def test_wrapped_gaussian_integrates_to_one(lib):
+    for beta in (0.3, 0.85, 1.5):
+        phis = np.linspace(-math.pi, math.pi, 20001)
+        vals = np.array([lib.oracle_gaussian_detector(beta, float(p), 10) for p in phis])
+        assert np.trapezoid(vals, phis) == pytest.approx(1.0, rel=2e-4)
+
+
+# -------------------------------------------------------------- binning
+def frustum_of_tile(w, tile):
+    for fi, d in enumerate(w.frusta):
+        if d["coarse_depth_tile_base"] <= tile < d["coarse_depth_tile_base"] + d["coarse_depth_tile_count"]:
+            return fi
+    return -1
+
+
+def cells_by_segment(w, f):
+    """(frustum, inst, seg) -> set of (page, cell) holding it."""
+    counts = f.get("cell_counts").reshape(-1, 256)
+    offs = f.get("cell_offsets").reshape(-1, 256)
+    meta = f.get("page_meta")
+    refs = f.get("refs")
+    out = {}
+    for p in range(counts.shape[0]):
+        fi = frustum_of_tile(w, int(meta[p]["coarse_tile_id"]))
+        for c in np.nonzero(counts[p])[0]:
+            base = meta[p]["seg_ref_base"] + offs[p, c]
+            for r in refs[base:base + counts[p, c]]:
+                out.setdefault((fi, int(r["inst_id"]), int(r["seg_id"])), set()).add((p, int(c)))
+    return out
+
+
+def test_scans_and_consistency(small16):
+    w, f = small16
+    counts = f.get("cell_counts").reshape(-1, 256).astype(np.int64)
+    offs = f.get("cell_offsets").reshape(-1, 256).astype(np.int64)
+    meta = f.get("page_meta")
+    ctr = f.counters()
+    assert counts.sum() == ctr["fine_refs"]
+    ex = np.cumsum(counts, axis=1) - counts
+    np.testing.assert_array_equal(offs, ex)
+    totals = counts.sum(axis=1)
+    bases = np.cumsum(totals) - totals
+    nz = totals > 0
+    np.testing.assert_array_equal(meta["seg_ref_base"][nz], bases[nz])
+    np.testing.assert_array_equal(meta["seg_ref_count"], totals)
+    # page ids are ranks of marked entries in page-table order
+    pt = f.get("page_table")
+    used = np.nonzero(pt)[0]
+    np.testing.assert_array_equal(pt[used], np.arange(1, used.size + 1))
+    np.testing.assert_array_equal(meta["coarse_tile_id"], used // 16)
+    np.testing.assert_array_equal(meta["coarse_z"], used % 16)
+    # references inside a cell are sorted by (inst, seg)
+    refs = f.get("refs")
+    for p in range(counts.shape[0]):
+        for c in np.nonzero(counts[p] > 1)[0]:
+            b = meta[p]["seg_ref_base"] + offs[p, c]
+            key = refs[b:b + counts[p, c]]["inst_id"].astype(np.int64) << 32 | refs[b:b + counts[p, c]]["seg_id"]
+            assert np.all(np.diff(key) > 0)
+    # active queue == sorted set flags, work items cover every non-empty cell
+    act = f.get("active_tiles")
+    assert np.all(np.diff(act.astype(np.int64)) > 0)
+    work = f.get("work_items")
+    assert work["seg_ref_count"].sum() == ctr["fine_refs"]
+    runs = np.concatenate([f.get("camera_runs"), f.get("shadow_runs")])
+    assert runs["work_count"].sum() == len(work)
+    # within a run, depth keys are non-increasing (near to far)
+    for r in runs:
+        dk = work["depth_key"][r["work_base"]:r["work_base"] + r["work_count"]].astype(np.int64)
+        assert np.all(np.diff(dk) <= 0)
+
+
+def test_dda_cells_match_sampled_intersections(tiny):
+    """Every cell a densely sampled projected segment passes through (away from
+    cell boundaries) is among the cells the 3D DDA assigned to it, and the
+    DDA assigns no cell the segment does not come close to."""
+    w, f = tiny
+    tasks, proj, lut_e = f.get("tasks"), f.get("projected"), f.get("lut")
+    fr = w.frusta
+    got = cells_by_segment(w, f)
+    pt = f.get("page_table")
+    checked = 0
+    for t, (tk, ps) in enumerate(zip(tasks, proj)):
+        fi = int(tk["packed_field"]) >> 3
+        d = fr[fi]
+        W, H = float(d["screen_width"]), float(d["screen_height"])
+        unpack = lambda v: (np.float64(v & 0xFFFF) / 65535.0, np.float64(v >> 16) / 65535.0)
+        x0, y0 = unpack(int(ps["p0_xy"]))
+        x1, y1 = unpack(int(ps["p1_xy"]))
+        z0, z1 = unpack(int(ps["depths"]))
+        p0 = np.array([x0 * W, y0 * H, z0])
+        p1 = np.array([x1 * W, y1 * H, z1])
+        fx, fy = int(d["froxel_size_x"]), int(d["froxel_size_y"])
+        cpx, cpy = 4 * fx, 4 * fy
+        sampled, near = set(), set()
+        for s in np.linspace(0.0, 1.0, 3001):
+            p = p0 + (p1 - p0) * s
+            cx, cy = int(p[0] // cpx), int(p[1] // cpy)
+            if not (0 <= cx < d["coarse_tiles_x"] and 0 <= cy < d["coarse_tiles_y"]):
+                continue
+            tile = int(d["coarse_depth_tile_base"]) + cy * int(d["coarse_tiles_x"]) + cx
+            zq = p[2] * Q
+            for cz in range(16):
+                e = lut_e[tile * 16 + cz]
+                if e["virtual_count_q"] == 0 or not (e["z_min_q"] - 0.5 <= zq <= e["z_max_q"] + 0.5):
+                    continue
+                zmin, zmax = e["z_min_q"] / Q, e["z_max_q"] / Q
+                span = max(zmax - zmin, 1.0 / Q)
+                lx = (p[0] - cx * cpx) / fx
+                ly = (p[1] - cy * cpy) / fy
+                lz = min(max((p[2] - zmin) / span, 0.0), 0.999999) * 16
+                page = int(pt[tile * 16 + cz]) - 1
+                if page < 0:
+                    continue
+                cell = (int(lz) * 4 + min(int(ly), 3)) * 4 + min(int(lx), 3)
+                margin = min(abs(lx - round(lx)), abs(ly - round(ly)), abs(lz - round(lz)))
+                near.add((page, cell))
+                if margin > 0.02:
+                    sampled.add((page, cell))
+        dda = got.get((fi, int(tk["id_info"]), int(tk["seg_idx"])), set())
+        missing = sampled - dda
+        assert not missing, f"task {t}: DDA misses sampled cells {sorted(missing)[:4]}"
+        # every DDA cell is touched by the segment up to boundary tolerance
+        extra = dda - near
+        assert len(extra) <= 2, f"task {t}: DDA visits cells the segment does not touch: {sorted(extra)[:4]}"
+        checked += 1
+    assert checked == len(tasks) > 50
+
+
+# ------------------------------------------------------------ render pins
+def test_dom_invariants(small16):
+    w, f = small16
+    L = 4
+    depth = f.get("dom_depth")
+    op = f.get("dom_opacity").reshape(L, -1)
+    assert np.all((op >= 0.0) & (op <= 1.0))
+    assert np.all(np.diff(op, axis=0) >= -1e-7)  # cumulative across layers
+    assert np.all(op[:, depth <= 0.0] == 0.0)
+    assert (depth > 0).sum() > 1000
+
+
+def test_stampback_is_footprint_max(small16):
+    w, f = small16
+    res = int(w.frusta[1]["screen_width"])
+    depth = f.get("dom_depth").reshape(res, res)
+    stamp = f.get("stamp").reshape(res, res)
+    np.testing.assert_array_equal(stamp, depth)  # same-size target: identity
+    # half-size target: 2x2 block max
+    w2 = scene.config_small(froxel=16)
+    w2.settings["stamp_target_width"] = res // 2
+    w2.settings["stamp_target_height"] = res // 2
+    f2 = oracle.render(w2)
+    s2 = f2.get("stamp").reshape(res // 2, res // 2)
+    np.testing.assert_array_equal(s2, depth.reshape(res // 2, 2, res // 2, 2).max(axis=(1, 3)))
+
+
+def test_image_alpha_range_and_composite(small16):
+    w, f = small16
+    img = f.get("image").reshape(-1, 4)
+    comp = f.get("composite").reshape(-1, 4)
+    assert np.all((img >= 0) & (img <= 1))
+    sc = np.array(scene.default_settings()["scene_clear_color"], np.float32)
+    empty = img[:, 3] == 0
+    np.testing.assert_allclose(comp[empty], np.broadcast_to(sc, comp[empty].shape), atol=1e-7)
+    opaque = img[:, 3] == 1
+    np.testing.assert_allclose(comp[opaque, :3], img[opaque, :3], atol=1e-6)
+    assert empty.sum() > 0 and (img[:, 3] > 0).sum() > 1000
+
+
+def test_single_segment_coverage_is_analytic():
+    """One horizontal segment in front of the camera: inside the fine tiles its
+    centre line is binned to, pixel alpha equals absorption.w x (1 - d/r) with
+    d the distance from the pixel centre to the projected segment wherever the
+    projection parameter lies in [0, 1].  Binning uses the centre line only
+    (strand_prepass.wgsl:1591-1696), so pixels of other tiles stay empty."""
+    asset = scene.asset_from_strands([[(-0.05, 0.0, 0.0), (0.05, 0.0, 0.0)]], radius_px=2.5)
+    cams = [((0.0, 0.0, 0.5), 64, 64)]
+    w = scene.build_workload("one-seg", asset, cams, 0, 64, 4)
+    w.views[0] = scene.camera_view((0.0, 0.0, 0.5), (0.0, 0.0, 0.0), 45.0, 64, 64)[0]
+    f = oracle.render(w)
+    img = f.get("image").reshape(64, 64, 4)
+    # projected endpoints from the camera matrix (float64, independent of the kernel)
+    m = w.views[0]["clip_from_world"].astype(np.float64).reshape(4, 4).T
+    def proj(p):
+        c = m @ np.array([*p, 1.0])
+        return np.array([(c[0] / c[3] * 0.5 + 0.5) * 64, (c[1] / c[3] * -0.5 + 0.5) * 64])
+    a, b = proj((-0.05, 0, 0)), proj((0.05, 0, 0))
+    alpha_w = float(asset.materials[0]["absorption_color"][3])
+    ab = b - a
+    tiles = set(int(t) for t in f.get("camera_runs")["screen_tile_id"])
+    assert len(tiles) >= 2
+    checked = 0
+    for y in range(64):
+        for x in range(64):
+            pc = np.array([x + 0.5, y + 0.5])
+            t = np.dot(pc - a, ab) / np.dot(ab, ab)
+            if (y // 8) * 8 + (x // 8) not in tiles:
+                assert img[y, x, 3] == 0.0
+                continue
+            if not (0 <= t <= 1):
+                continue
+            d = np.linalg.norm(pc - (a + ab * t))
+            cov = max(0.0, 1.0 - d / 2.5)
+            assert img[y, x, 3] == pytest.approx(alpha_w * cov, abs=2e-5)
+            checked += cov > 0
+    assert checked > 20
