@@ -1,0 +1,197 @@
+"""Benchmark of the MI355X strand rendering path.
+
+One step = one frame of the plain per-frame path (prepass/culling,
+projection, froxel binning, scans and compaction, raster-work emission,
+deep-shadow-map build and stampback, hair shading, strand raster,
+composite) over the named workload: 100k strands x 32 points (3.1M
+segments), 1920x1080 camera, one directional light with a 2048^2 x 4-layer
+deep shadow map, 8x8 froxels.  Inputs are resident in HBM before timing.
+
+Prints one JSON line.  value = camera frames per second over K pipelined
+frames (whole job, one GPU); Msegments/s, p50/p99 frame latency, per-stage
+times, the roofline of the dominant kernel and the CPU baseline ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def percentile(samples, f):
+    """Nearest rank on sorted samples, index round((n-1) f), as the reference
+    bench's summarize (examples/strand_bench.rs:931-947)."""
+    s = sorted(samples)
+    if not s:
+        return None
+    return s[int(round((len(s) - 1) * f))]
+
+
+def stage_bytes(ctr, w):
+    """Algorithmic bytes per stage: every ABI record read or written once.
+    T tasks, C = 256 P cells, H references, W work items, R runs."""
+    T, P, H = ctr["binning_tasks"], ctr["allocated_pages"], ctr["fine_refs"]
+    C = 256 * P
+    W = ctr["work_items"]
+    R = ctr["camera_runs"] + ctr["shadow_runs"]
+    tiles16 = 16 * ctr["coarse_tiles"]
+    return {
+        "mark": 28 * T,
+        "count": 28 * T + 4 * C,
+        "prefix": 8 * C + 32 * P,
+        "fill": 28 * T + 4 * C + 12 * H,
+        "sort_cells": 24 * H,
+        "compact": 8 * tiles16,
+        "emit": 4 * C + 32 * W + 32 * R,
+    }
+
+
+def roofline_for(stage, ms, ctr, w):
+    b = stage_bytes(ctr, w).get(stage)
+    if b is None or ms <= 0:
+        return None
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": stage,
+            "bytes_per_launch": int(b), "avg_ms": round(ms, 4)}
+
+
+def cpu_baseline(threads: int, fraction: float = 0.25):
+    """The oracle (C++ restatement) on a bounded sample: the same workload
+    with `fraction` of its strands, one frame; frames/s scaled by `fraction`
+    to the full workload (work is linear in segments)."""
+    from bevy_strand_rasterizer_amd import scene
+    from oracle import oracle
+
+    strands = int(100_000 * fraction)
+    w = scene.config_named(strands=strands)
+    oracle.render(scene.config_tiny(), threads=threads)  # load + warm
+    t0 = time.perf_counter()
+    f = oracle.render(w, threads=threads)
+    dt = time.perf_counter() - t0
+    f.close()
+    return {"value": round(fraction / dt, 5), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"one frame of the named workload with {strands} of 100000 strands "
+                      f"({w.input_segments} segments), {dt:.1f} s, scaled by {fraction}",
+            "sample_seconds": round(dt, 2), "msegments_per_s": round(w.input_segments / dt / 1e6, 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="named", choices=["named", "small", "batched"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent frames and views shard across GPUs with no exchange)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from bevy_strand_rasterizer_amd import scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    if args.config == "named":
+        w = scene.config_named()
+    elif args.config == "small":
+        w = scene.config_small()
+    else:
+        w = scene.config_batched()
+    views = w.camera_count
+    segs = w.input_segments
+
+    r = StrandRenderer(device=0)
+    r.upload_workload(w)
+    for _ in range(max(1, args.warmup)):
+        r.submit(w)
+    ctr = r.wait()
+
+    # stage timing (HIP events on the library stream), a few frames
+    r.set_profiling(True)
+    stage_acc = {}
+    nprof = 3
+    for _ in range(nprof):
+        r.submit(w)
+        r.wait()
+        for k, v in r.stage_times().items():
+            stage_acc[k] = stage_acc.get(k, 0.0) + v / nprof
+    r.set_profiling(False)
+
+    # latency: one frame at a time
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        r.submit(w)
+        r.wait()
+        lat.append((time.perf_counter() - t0) * 1e3)
+
+    # throughput: K pipelined frames
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.submit(w)
+    ctr = r.wait()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    fps = views * args.steps / dt
+
+    dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
+    roof = roofline_for(dom, stage_acc[dom], ctr, w)
+    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w) for k in ("count", "prefix", "fill")}
+    if roof is None:
+        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4),
+                "note": "dominant stage is LDS/VALU bound; binning rooflines in binning_roofline"}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(threads)
+        except Exception as e:  # the baseline never blocks the GPU line
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": "frames/s + Msegments/s (100k x 32 strands, 1080p), batched vs plain; p99 latency",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scalp generator)",
+        "config": {"workload": w.name, "mode": "plain", "strands": int(w.assets[0].strand_count),
+                   "points_per_strand": int(w.assets[0].meta[0]["count"]), "segments": segs, "views": views,
+                   "lights": int(len(w.lights)), "dom_layers": int(w.settings["dom_layers"]),
+                   "froxel": int(w.frusta[0]["froxel_size_x"])},
+        "msegments_per_s": round(segs * views * args.steps / dt / 1e6, 2),
+        "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_acc.items()},
+        "counters": {k: v for k, v in ctr.items() if not k.startswith("pad")},
+        "roofline": roof,
+        "binning_roofline": binning,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

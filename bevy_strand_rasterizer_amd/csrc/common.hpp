@@ -77,6 +77,8 @@ enum Counter : uint32_t {
   C_OVERFLOW = 8,
   C_MARKED = 9,       // marked pages before the page capacity is applied
   C_PAGE_DROPS = 10,  // marked pages dropped by the page capacity
+  C_CAMERA_HEAD = 11, // camera raster run queue head
+  C_SHADOW_HEAD = 12, // shadow raster run queue head
   C_COUNT = 16
 };
 

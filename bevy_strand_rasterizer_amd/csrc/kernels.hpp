@@ -39,10 +39,10 @@ struct PrepassScratch {
 };
 
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer);
-void launch_shadow(const FrameParams& fp, hipStream_t s);
+void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, hipStream_t s);
 void launch_shading(const FrameParams& fp, hipStream_t s);
-void launch_camera_raster(const FrameParams& fp, hipStream_t s);
+void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
 void launch_fill_u32(uint32_t* dst, uint64_t count, uint32_t value, hipStream_t s);
 

@@ -606,37 +606,9 @@ __device__ __forceinline__ unsigned long long ref_key(const sr_fine_seg_ref& r) 
   return ((unsigned long long)r.inst_id << 32) | r.seg_id;
 }
 
-__device__ void sort_cell_wave(sr_fine_seg_ref* refs, uint32_t n) {
-  const uint32_t lane = threadIdx.x & 63u;
-  unsigned long long key = ~0ull;
-  uint32_t mat = 0;
-  if (lane < n) { sr_fine_seg_ref r = refs[lane]; key = ref_key(r); mat = r.material_idx; }
-#pragma unroll
-  for (uint32_t k = 2; k <= 64; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      uint32_t partner = (j == (k >> 1)) ? (lane ^ (k - 1u)) : (lane ^ j);
-      unsigned long long ok = __shfl(key, (int)partner, 64);
-      uint32_t om = __shfl(mat, (int)partner, 64);
-      bool lower = lane < partner;
-      bool take = lower ? (ok < key) : (ok > key);
-      if (take) { key = ok; mat = om; }
-    }
-  }
-  if (lane < n) refs[lane] = sr_fine_seg_ref{(uint32_t)(key >> 32), (uint32_t)key, mat};
-}
-
-constexpr uint32_t kSortLds = 4096;
-
-__device__ void sort_cell_block(sr_fine_seg_ref* refs, uint32_t n, unsigned long long* skey, uint32_t* smat) {
-  const bool in_lds = n <= kSortLds;
-  if (in_lds) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      sr_fine_seg_ref r = refs[i];
-      skey[i] = ref_key(r);
-      smat[i] = r.material_idx;
-    }
-    __syncthreads();
-  }
+// Sorts refs[0, n) by (inst, seg) in global memory with the whole workgroup
+// (cells larger than an LDS chunk; rare).
+__device__ void sort_refs_global(sr_fine_seg_ref* refs, uint32_t n) {
   uint32_t m = 1;
   while (m < n) m <<= 1;
   for (uint32_t k = 2; k <= m; k <<= 1) {
@@ -644,44 +616,97 @@ __device__ void sort_cell_block(sr_fine_seg_ref* refs, uint32_t n, unsigned long
       for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         uint32_t partner = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
         if (partner <= i || partner >= n) continue;
-        if (in_lds) {
-          if (skey[partner] < skey[i]) {
-            unsigned long long tk = skey[i]; skey[i] = skey[partner]; skey[partner] = tk;
-            uint32_t tm = smat[i]; smat[i] = smat[partner]; smat[partner] = tm;
-          }
-        } else {
-          sr_fine_seg_ref a = refs[i], b = refs[partner];
-          if (ref_key(b) < ref_key(a)) { refs[i] = b; refs[partner] = a; }
-        }
+        sr_fine_seg_ref a = refs[i], b = refs[partner];
+        if (ref_key(b) < ref_key(a)) { refs[i] = b; refs[partner] = a; }
       }
-      if (!in_lds) __threadfence_block();
+      __threadfence_block();
       __syncthreads();
     }
   }
-  if (in_lds) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-      refs[i] = sr_fine_seg_ref{(uint32_t)(skey[i] >> 32), (uint32_t)skey[i], smat[i]};
-    __syncthreads();
-  }
 }
 
+constexpr uint32_t kSortChunk = 2048;
+
+// One workgroup per page.  The page's cells are cut into chunks of at most
+// kSortChunk references at cell boundaries; each chunk is loaded into LDS
+// with key (cell, inst, seg), sorted by one mirror-bitonic network and
+// written back, which orders every cell and keeps the cell layout.
 __global__ __launch_bounds__(256) void k_sort_cells(FrameParams fp) {
-  __shared__ unsigned long long skey[kSortLds];
-  __shared__ uint32_t smat[kSortLds];
+  __shared__ unsigned long long skey[kSortChunk];
+  __shared__ uint32_t smat[kSortChunk];
+  __shared__ uint32_t offs[kPage], cnts[kPage];
+  __shared__ uint32_t cstart[kPage + 1];
+  __shared__ uint32_t cneed[kPage];
+  __shared__ uint32_t nchunks;
   const uint32_t P = fp.counters[C_PAGES];
-  const uint32_t wave = threadIdx.x >> 6;
   for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
     const sr_fine_page_meta pm = fp.page_meta[page];
     if (pm.seg_ref_count == 0u) continue;
-    for (uint32_t c = wave; c < kPage; c += 4u) {
-      uint32_t n = fp.cell_counts[page * kPage + c];
-      if (n < 2u || n > 64u) continue;
-      sort_cell_wave(fp.refs + pm.seg_ref_base + fp.cell_offsets[page * kPage + c], n);
+    __syncthreads();
+    offs[threadIdx.x] = fp.cell_offsets[page * kPage + threadIdx.x];
+    cnts[threadIdx.x] = fp.cell_counts[page * kPage + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t nc = 0, acc = 0, need = 0;
+      cstart[0] = 0;
+      for (uint32_t c = 0; c < kPage; ++c) {
+        uint32_t n = cnts[c];
+        if (acc > 0 && acc + n > kSortChunk) {
+          cneed[nc] = need;
+          cstart[++nc] = c;
+          acc = 0;
+          need = 0;
+        }
+        acc += n;
+        need |= n > 1u ? 1u : 0u;
+      }
+      cneed[nc] = need;
+      cstart[++nc] = kPage;
+      nchunks = nc;
     }
-    for (uint32_t c = 0; c < kPage; ++c) {
-      uint32_t n = fp.cell_counts[page * kPage + c];
-      if (n <= 64u) continue;
-      sort_cell_block(fp.refs + pm.seg_ref_base + fp.cell_offsets[page * kPage + c], n, skey, smat);
+    __syncthreads();
+    sr_fine_seg_ref* page_refs = fp.refs + pm.seg_ref_base;
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+      if (!cneed[ch]) continue;
+      const uint32_t c0 = cstart[ch], c1 = cstart[ch + 1];
+      const uint32_t start = offs[c0];
+      const uint32_t n = offs[c1 - 1] + cnts[c1 - 1] - start;
+      if (n > kSortChunk) {  // a single cell above the chunk size
+        sort_refs_global(page_refs + start, n);
+        continue;
+      }
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t lo = c0, hi = c1;  // offs[lo] - start <= i
+        while (hi - lo > 1u) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (offs[mid] - start <= i) lo = mid; else hi = mid;
+        }
+        const sr_fine_seg_ref r = page_refs[start + i];
+        skey[i] = ((unsigned long long)lo << 56) | ((unsigned long long)(r.inst_id & 0xFFFFFFu) << 32) | r.seg_id;
+        smat[i] = r.material_idx;
+      }
+      __syncthreads();
+      uint32_t m = 1;
+      while (m < n) m <<= 1;
+      for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            uint32_t partner = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
+            if (partner <= i || partner >= n) continue;
+            unsigned long long a = skey[i], bk = skey[partner];
+            if (bk < a) {
+              skey[i] = bk; skey[partner] = a;
+              uint32_t tm = smat[i]; smat[i] = smat[partner]; smat[partner] = tm;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        unsigned long long k = skey[i];
+        page_refs[start + i] = sr_fine_seg_ref{(uint32_t)(k >> 32) & 0xFFFFFFu, (uint32_t)k, smat[i]};
+      }
+      __syncthreads();
     }
   }
 }

@@ -8,12 +8,14 @@
 //   rasterize_strands [LINEAR]    src/shaders/strand_rasterizer.wgsl:855-1044
 //   composite fragment            src/shaders/strand_composite.wgsl:12-33
 //
-// Design: one 256-lane workgroup per tile run; every lane owns one pixel of
-// the tile (8x8 or 16x16 froxels, all pixels rasterised) and all lanes stage
-// a 256-reference batch of projected segments in LDS, so each reference is
-// fetched and projected once per tile.  Work items arrive near to far; a
-// workgroup vote ends the walk once every pixel is opaque (camera) or has
-// its front depth (shadow pass 1).  Deep-shadow partial opacities live in
+// Design: one workgroup per tile run, sized to the tile (64 lanes for 8x8
+// froxels, 256 for 16x16); every lane owns one pixel (all pixels of a tile
+// are rasterised) and all lanes stage a batch of projected references in
+// LDS, so each reference is fetched and projected once per tile.  Runs come
+// from a device queue head for load balance; batches span work items, whose
+// boundaries each pixel tracks for the early-out rules.  Work items arrive
+// near to far; a workgroup vote ends the walk once every pixel is opaque
+// (camera) or has its front depth (shadow pass 1).  Deep-shadow partial opacities live in
 // LDS (pixel x layer), so no dynamic register indexing spills.  Shadow maps
 // are dense per light (no virtual paging), outputs stay FP32.
 #include "common.hpp"
@@ -41,24 +43,133 @@ __device__ __forceinline__ V3 world_v3(const InstRec& in, uint32_t vi) {
   return V3{w.x, w.y, w.z};
 }
 
-// --------------------------------------------------------- K25 shadow
-struct ShadowBatch {
-  float p0x[kBatch], p0y[kBatch], p0z[kBatch];
-  float p1x[kBatch], p1y[kBatch], p1z[kBatch];
-  float rmin[kBatch], rmax[kBatch], absorb[kBatch];
-  uint32_t valid[kBatch];
+// ------------------------------------------------ tile-run work tables
+// A workgroup owns one tile run at a time, taken from a device queue head
+// (dynamic balance: run loads vary by orders of magnitude).  The run's work
+// items (<= 256 cells, near to far) are staged as an exclusive prefix of
+// their reference counts, so reference batches of TPX entries span work-item
+// boundaries: one barrier pair per TPX references instead of per cell.
+template <int TPX>
+struct RunTable {
+  uint32_t pref[257];  // exclusive prefix of counts; pref[n] = total
+  uint32_t base[256];  // seg_ref_base per work item
+  uint32_t n;
+  uint32_t run;
 };
 
-__device__ __forceinline__ bool shadow_cover(const ShadowBatch& b, uint32_t k, float px, float py, float* pz,
-                                             float* cov) {
-  if (!b.valid[k]) return false;
-  float x0 = b.p0x[k], y0 = b.p0y[k], z0 = b.p0z[k];
-  float abx = b.p1x[k] - x0, aby = b.p1y[k] - y0;
+template <int TPX>
+__device__ bool next_run(uint32_t* head, uint32_t count, RunTable<TPX>& t) {
+  __syncthreads();
+  if (threadIdx.x == 0) t.run = atomicAdd(head, 1u);
+  __syncthreads();
+  return t.run < count;
+}
+
+// Stages the work table of `run`; items failing `keep` get count 0.
+template <int TPX, bool CHECK>
+__device__ void stage_run(const FrameParams& fp, const sr_raster_tile_run& run, uint32_t fi, RunTable<TPX>& t) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t n = min(run.work_count, 256u);
+  uint32_t cnt[256 / TPX];
+#pragma unroll
+  for (int k = 0; k < 256 / TPX; ++k) {
+    uint32_t w = threadIdx.x * (256 / TPX) + k;
+    cnt[k] = 0u;
+    if (w < n) {
+      const sr_raster_work_item wi = fp.work[run.work_base + w];
+      bool ok = !CHECK || (wi.frustum_id == fi && wi.screen_tile_id == run.screen_tile_id);
+      cnt[k] = ok ? wi.seg_ref_count : 0u;
+      t.base[w] = wi.seg_ref_base;
+    }
+  }
+  uint32_t local = 0;
+#pragma unroll
+  for (int k = 0; k < 256 / TPX; ++k) local += cnt[k];
+  // exclusive scan of per-thread sums across the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = local;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (TPX > 64) {
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    uint32_t wb = 0;
+    for (int w = 0; w < wave; ++w) wb += lds4[w];
+    inc += wb;
+  }
+  uint32_t ex = inc - local;
+#pragma unroll
+  for (int k = 0; k < 256 / TPX; ++k) {
+    uint32_t w = threadIdx.x * (256 / TPX) + k;
+    if (w <= n) t.pref[w] = ex;
+    ex += cnt[k];
+  }
+  if (threadIdx.x == 0) t.n = n;
+  __syncthreads();
+  if (threadIdx.x == TPX - 1) t.pref[n] = ex;  // total
+  __syncthreads();
+}
+
+// Work item holding concatenated reference i (pref[w] <= i < pref[w + 1]).
+template <int TPX>
+__device__ __forceinline__ uint32_t item_of(const RunTable<TPX>& t, uint32_t i) {
+  uint32_t lo = 0, hi = t.n;  // pref[lo] <= i < pref[hi]
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (t.pref[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// --------------------------------------------------------- K25 shadow
+template <int TPX>
+struct ShadowBatch {
+  float p0x[TPX], p0y[TPX], p0z[TPX];
+  float p1x[TPX], p1y[TPX], p1z[TPX], l2[TPX];
+  float rmin[TPX], rmax[TPX], absorb[TPX];
+  uint32_t item[TPX];
+};
+
+template <int TPX>
+__device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, const RunTable<TPX>& t, uint32_t i,
+                                ShadowBatch<TPX>& b) {
+  const uint32_t k = threadIdx.x;
+  const uint32_t w = item_of(t, i);
+  const sr_fine_seg_ref sr = fp.refs[t.base[w] + (i - t.pref[w])];
+  const InstRec& in = fp.inst[sr.inst_id];
+  V3 a = world_v3(in, in.indices[sr.seg_id]);
+  V3 c = world_v3(in, in.indices[sr.seg_id + 1u]);
+  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  V4 ra = world_to_screen_raw(V4{a.x, a.y, a.z, 1.0f}, fr.clip, vw, vh);
+  V4 rc = world_to_screen_raw(V4{c.x, c.y, c.z, 1.0f}, fr.clip, vw, vh);
+  const sr_strand_material& m = in.materials[sr.material_idx];
+  float minr = fmaxf(m.min_radius_pixels, 1e-4f);
+  float z0 = normalize_depth01(ra.z), z1 = normalize_depth01(rc.z);
+  float abx = rc.x - ra.x, aby = rc.y - ra.y;
   float l2 = abx * abx + aby * aby;
+  b.p0x[k] = ra.x; b.p0y[k] = ra.y; b.p0z[k] = z0;
+  b.p1x[k] = rc.x; b.p1y[k] = rc.y; b.p1z[k] = z1;
+  b.l2[k] = l2;  // divided per pixel: t = dot / l2 as the reference
+  b.rmin[k] = minr;
+  b.rmax[k] = fmaxf(m.max_radius_pixels, minr);
+  b.absorb[k] = m.absorption_color[3];
+  b.item[k] = w;
+}
+
+// fragment_position_line_relative + coverage (strand_rasterizer.wgsl:138-147,
+// 553-566); p = mix(p0, p1, t) evaluated as p0 * (1 - t) + p1 * t.
+template <int TPX>
+__device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t k, float px, float py, float* pz,
+                                             float* cov) {
+  float x0 = b.p0x[k], y0 = b.p0y[k], z0 = b.p0z[k];
+  float x1 = b.p1x[k], y1 = b.p1y[k], l2 = b.l2[k];
   float t = 0.0f;
-  if (!(l2 == 0.0f)) t = ((px - x0) * abx + (py - y0) * aby) / l2;
+  if (!(l2 == 0.0f)) t = ((px - x0) * (x1 - x0) + (py - y0) * (y1 - y0)) / l2;
   if (t < 0.0f || t > 1.0f) return false;
-  float qx = wmix(x0, b.p1x[k], t), qy = wmix(y0, b.p1y[k], t), qz = wmix(z0, b.p1z[k], t);
+  float qx = wmix(x0, x1, t), qy = wmix(y0, y1, t), qz = wmix(z0, b.p1z[k], t);
   float r = wmix(b.rmin[k], b.rmax[k], wclamp(qz, 0.0f, 1.0f));
   float dx = px - qx, dy = py - qy;
   float c = wclamp(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f, 1.0f);
@@ -67,105 +178,84 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch& b, uint32_t k, f
   return c > 0.0f;
 }
 
-__device__ void load_shadow_batch(const FrameParams& fp, const FrustumRec& fr, uint32_t fi, uint32_t screen_tile,
-                                  const sr_raster_work_item& wi, uint32_t base, ShadowBatch& b) {
-  uint32_t k = threadIdx.x;
-  uint32_t ref_idx = base + k;
-  uint32_t end = wi.seg_ref_base + wi.seg_ref_count;
-  b.valid[k] = 0u;
-  if (ref_idx < end && wi.frustum_id == fi && wi.screen_tile_id == screen_tile) {
-    const sr_fine_seg_ref sr = fp.refs[ref_idx];
-    const InstRec& in = fp.inst[sr.inst_id];
-    V3 a = world_v3(in, in.indices[sr.seg_id]);
-    V3 c = world_v3(in, in.indices[sr.seg_id + 1u]);
-    float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
-    V4 ra = world_to_screen_raw(V4{a.x, a.y, a.z, 1.0f}, fr.clip, vw, vh);
-    V4 rc = world_to_screen_raw(V4{c.x, c.y, c.z, 1.0f}, fr.clip, vw, vh);
-    const sr_strand_material& m = in.materials[sr.material_idx];
-    float minr = fmaxf(m.min_radius_pixels, 1e-4f);
-    b.p0x[k] = ra.x; b.p0y[k] = ra.y; b.p0z[k] = normalize_depth01(ra.z);
-    b.p1x[k] = rc.x; b.p1y[k] = rc.y; b.p1z[k] = normalize_depth01(rc.z);
-    b.rmin[k] = minr;
-    b.rmax[k] = fmaxf(m.max_radius_pixels, minr);
-    b.absorb[k] = m.absorption_color[3];
-    b.valid[k] = 1u;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_shadow_raster(FrameParams fp) {
-  __shared__ ShadowBatch b;
-  __shared__ float part[256 * SR_MAX_DOM_LAYERS];
+template <int TPX>
+__global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
+  __shared__ RunTable<TPX> t;
+  __shared__ ShadowBatch<TPX> b;
+  __shared__ float part[TPX * SR_MAX_DOM_LAYERS];
   const uint32_t R = fp.counters[C_SHADOW_RUNS];
   const uint32_t L = fp.dom_layers;
-  for (uint32_t ri = blockIdx.x; ri < R; ri += gridDim.x) {
-    const sr_raster_tile_run run = fp.shadow_runs[ri];
+  while (next_run<TPX>(&fp.counters[C_SHADOW_HEAD], R, t)) {
+    const sr_raster_tile_run run = fp.shadow_runs[t.run];
     const uint32_t fi = run.frustum_id;
     const FrustumRec& fr = fp.fr[fi];
     if (fr.light_layer == kInvalid || run.work_count == 0u) continue;
     if ((uint64_t)run.work_base + run.work_count > fp.work_cap) continue;  // emission overflowed
+    stage_run<TPX, true>(fp, run, fi, t);
+    const uint32_t total = t.pref[t.n];
     const uint32_t fx = fr.d.froxel_size_x, fy = fr.d.froxel_size_y;
     const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
     const uint32_t pidx = threadIdx.x;
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
     const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
     const float pfx = (float)pxu + 0.5f, pfy = (float)pyu + 0.5f;
-    // pass 1: front depth from the first (nearest) work item with a hit
+    // pass 1: front depth = max depth over the covering references of the
+    // first (nearest) work item with any hit
     float z0 = -1.0f;
-    bool found = false;
-    for (uint32_t w = 0; w < run.work_count; ++w) {
-      const sr_raster_work_item wi = fp.work[run.work_base + w];
-      bool hit = false;
-      for (uint32_t base = wi.seg_ref_base; base < wi.seg_ref_base + wi.seg_ref_count; base += kBatch) {
-        __syncthreads();
-        load_shadow_batch(fp, fr, fi, run.screen_tile_id, wi, base, b);
-        __syncthreads();
-        if (active && !found) {
-          uint32_t n = min(kBatch, wi.seg_ref_base + wi.seg_ref_count - base);
-          for (uint32_t k = 0; k < n; ++k) {
-            float pz, cov;
-            if (shadow_cover(b, k, pfx, pfy, &pz, &cov)) { z0 = fmaxf(z0, pz); hit = true; }
+    bool found = !active, hit = false;
+    uint32_t cur = 0;
+    for (uint32_t bs = 0; bs < total; bs += TPX) {
+      __syncthreads();
+      if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
+      __syncthreads();
+      if (!found) {
+        const uint32_t nb = min((uint32_t)TPX, total - bs);
+        for (uint32_t k = 0; k < nb; ++k) {
+          uint32_t w = b.item[k];
+          if (w != cur) {
+            if (hit) { found = true; break; }
+            cur = w;
           }
+          float pz, cov;
+          if (shadow_cover(b, k, pfx, pfy, &pz, &cov)) { z0 = fmaxf(z0, pz); hit = true; }
         }
       }
-      if (hit) found = true;
-      if (__syncthreads_count(active && !found) == 0) break;
+      if (__syncthreads_count(!found) == 0) break;
     }
-    const size_t plane = (size_t)fr.d.screen_width * fr.d.screen_height;
-    const size_t pix = (size_t)pyu * fr.d.screen_width + pxu;
+    if (hit) found = true;
+    found = found && active;
     float* part_px = part + threadIdx.x * SR_MAX_DOM_LAYERS;
     for (uint32_t i = 0; i < L; ++i) part_px[i] = 0.0f;
-    const bool alpha_active = active && found;
-    const float inv_span = 1.0f / fmaxf(1e-6f, alpha_active ? z0 : 1.0f);
-    if (__syncthreads_count(alpha_active) != 0) {
-      for (uint32_t w = 0; w < run.work_count; ++w) {
-        const sr_raster_work_item wi = fp.work[run.work_base + w];
-        for (uint32_t base = wi.seg_ref_base; base < wi.seg_ref_base + wi.seg_ref_count; base += kBatch) {
-          __syncthreads();
-          load_shadow_batch(fp, fr, fi, run.screen_tile_id, wi, base, b);
-          __syncthreads();
-          if (alpha_active) {
-            uint32_t n = min(kBatch, wi.seg_ref_base + wi.seg_ref_count - base);
-            for (uint32_t k = 0; k < n; ++k) {
-              float pz, cov;
-              if (!shadow_cover(b, k, pfx, pfy, &pz, &cov)) continue;
-              float dzp = fmaxf(0.0f, z0 - pz) * inv_span;
-              float u = wclamp(dzp, 0.0f, 1.0f);  // pow(u, DOM_GAMMA = 1)
-              float tL = u * (float)L;
-              uint32_t si = min(f2u(floorf(tL)), L - 1u);
-              float wgt = tL - floorf(tL);
-              float a = cov * b.absorb[k] * 0.5f;
-              float a0 = part_px[si];
-              part_px[si] = wclamp(a0 + (1.0f - a0) * (1.0f - wgt) * a, 0.0f, 1.0f);
-              if (si + 1u < L) {
-                float a1 = part_px[si + 1u];
-                part_px[si + 1u] = wclamp(a1 + (1.0f - a1) * wgt * a, 0.0f, 1.0f);
-              }
+    const float inv_span = 1.0f / fmaxf(1e-6f, found ? z0 : 1.0f);
+    if (__syncthreads_count(found) != 0) {
+      for (uint32_t bs = 0; bs < total; bs += TPX) {
+        __syncthreads();
+        if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
+        __syncthreads();
+        if (found) {
+          const uint32_t nb = min((uint32_t)TPX, total - bs);
+          for (uint32_t k = 0; k < nb; ++k) {
+            float pz, cov;
+            if (!shadow_cover(b, k, pfx, pfy, &pz, &cov)) continue;
+            float dzp = fmaxf(0.0f, z0 - pz) * inv_span;
+            float u = wclamp(dzp, 0.0f, 1.0f);  // pow(u, DOM_GAMMA = 1)
+            float tL = u * (float)L;
+            uint32_t si = min(f2u(floorf(tL)), L - 1u);
+            float wgt = tL - floorf(tL);
+            float a = cov * b.absorb[k] * 0.5f;
+            float a0 = part_px[si];
+            part_px[si] = wclamp(a0 + (1.0f - a0) * (1.0f - wgt) * a, 0.0f, 1.0f);
+            if (si + 1u < L) {
+              float a1 = part_px[si + 1u];
+              part_px[si + 1u] = wclamp(a1 + (1.0f - a1) * wgt * a, 0.0f, 1.0f);
             }
           }
         }
       }
     }
     if (active) {
+      const size_t plane = (size_t)fr.d.screen_width * fr.d.screen_height;
+      const size_t pix = (size_t)pyu * fr.d.screen_width + pxu;
       float* dd = fp.dom_depth + fr.dom_depth_off;
       float* op = fp.dom_opacity + fr.dom_opacity_off;
       if (!found) {
@@ -180,7 +270,6 @@ __global__ __launch_bounds__(256) void k_shadow_raster(FrameParams fp) {
         }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -241,14 +330,18 @@ __device__ float fresnel_reflectance(float eta_t, float c_in) {  // eta_i = 1
   return (rpar * rpar + rper * rper) * 0.5f;
 }
 
-__device__ float bessel5(float x) {
-  // sum_{i<5} x^(2i) / (4^i (i!)^2)
-  const float fact[5] = {1.0f, 1.0f, 2.0f, 6.0f, 24.0f};
-  float y = 0.0f;
-  for (int i = 0; i < 5; ++i) {
-    float num = powf(x, 2.0f * (float)i);
-    float den = powf(4.0f, (float)i) * fact[i] * fact[i];
-    y = y + num / den;
+// sum_{i<5} x^(2i) / (4^i (i!)^2), bessel_first_approx(x, 5)
+__device__ __forceinline__ float bessel5(float x) {
+  float x2 = x * x;
+  float y = 1.0f;
+  float term = 1.0f;
+  const float den[5] = {1.0f, 4.0f, 64.0f, 2304.0f, 147456.0f};  // 4^i (i!)^2
+  float xp = 1.0f;
+#pragma unroll
+  for (int i = 1; i < 5; ++i) {
+    xp = xp * x2;
+    term = xp / den[i];
+    y = y + term;
   }
   return y;
 }
@@ -263,53 +356,77 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
   return bessel5((cti * ctr) / v) * expf((sti * str) / v);
 }
 
-__device__ V3 Np(uint32_t p, float phi, float theta_i, float theta_r, float eta, V3 mu, float beta) {
+// Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)); terms whose
+// exponent is below -110 are zero in FP32 and are not evaluated.
+__device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, int kmax) {
+  float kc = rintf(pd / TWO_PI_F);
+  int k0 = max(-10, (int)kc - kmax), k1 = min(10, (int)kc + kmax);
+  float total = 0.0f;
+  for (int k = k0; k <= k1; ++k) {
+    float ang = pd - TWO_PI_F * (float)k;
+    total = total + __expf(ang * ang * neg_inv_2var);
+  }
+  return total;
+}
+
+// The three Np lobes of weta_strand_bsdf (strand_shading.wgsl:525-577) in
+// one pass over the 10 quadrature points: the per-point terms of A and Phi
+// (gamma_i, gamma_t, Fresnel, absorption) are shared by the three paths.
+__device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 mu, float beta, float scat, V3* out) {
   float theta_d = (theta_r - theta_i) * 0.5f;
   float sti = sinf(theta_i);
   float ctd = cosf(theta_d);
   float eta_prime = sqrtf(fmaxf(0.0f, eta * eta - sti * sti)) / ctd;
   V3 mup{mu.x / ctd, mu.y / ctd, mu.z / ctd};
   float var = beta * beta;
+  bool narrow = var < 1e-6f;
   float norm = 1.0f / (SQRT_2PI_F * beta);
-  float cos_td = cosf(theta_d);
-  V3 acc{0.0f, 0.0f, 0.0f};
+  float neg_inv_2var = -1.0f / (2.0f * var);
+  // |d - 2 pi k| above sqrt(110 * 2 var) contributes exactly zero
+  int kmax = (int)ceilf(sqrtf(220.0f * var) / TWO_PI_F) + 1;
+  V3 acc[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
   for (int i = 0; i < 10; ++i) {
     float h = wclamp(-1.0f + 1e-6f, 1.0f - 1e-6f, kLegRoots[i]);
-    // A(p, h)
     float gamma_t = asinf(h / eta_prime);
     float gamma_i = asinf(h);
-    float f = fresnel_reflectance(eta, sacos(cos_td * cosf(gamma_i)));
+    float f = fresnel_reflectance(eta, sacos(ctd * cosf(gamma_i)));
     float e = 1.0f + cosf(2.0f * gamma_t);
-    V3 T{expf(-2.0f * mup.x * e), expf(-2.0f * mup.y * e), expf(-2.0f * mup.z * e)};
-    V3 att;
-    if (p == 0u) {
-      att = V3{f * T.x * T.x, f * T.y * T.y, f * T.z * T.z};
-    } else {
-      float f_ = 1.0f - f;
-      float s = (f_ * f_) * powf(f, (float)(p - 1u));
-      att = V3{s * powf(T.x, (float)p), s * powf(T.y, (float)p), s * powf(T.z, (float)p)};
-    }
-    // Phi(p, h)
-    float gt2 = asinf(wclamp(h / eta_prime, -1.0f, 1.0f));
-    float shift = 2.0f * (float)p * gt2 - 2.0f * asinf(h) + (float)p * kPiF;
-    // gaussian_detector(beta, phi - shift, 10)
-    float det;
-    float pd = phi - shift;
-    if (var < 1e-6f) {
-      det = fabsf(fmodf(pd, TWO_PI_F)) < 1e-3f ? 0.0f : 1e6f;
-    } else {
-      float total = 0.0f;
-      for (int k = -10; k <= 10; ++k) {
-        float ang = pd - TWO_PI_F * (float)k;
-        total = total + expf(-ang * ang / (2.0f * var));
+    V3 T{__expf(-2.0f * mup.x * e), __expf(-2.0f * mup.y * e), __expf(-2.0f * mup.z * e)};
+    float gt_phi = asinf(wclamp(h / eta_prime, -1.0f, 1.0f));
+    float ai = 2.0f * asinf(h);
+    float f_ = 1.0f - f;
+    float wgt = kLegWeights[i];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      V3 att;
+      if (p == 0) {
+        att = V3{f * T.x * T.x, f * T.y * T.y, f * T.z * T.z};
+      } else if (p == 1) {
+        float s = f_ * f_;
+        att = V3{s * T.x, s * T.y, s * T.z};
+      } else {
+        float s = (f_ * f_) * f;
+        att = V3{s * T.x * T.x, s * T.y * T.y, s * T.z * T.z};
       }
-      det = norm * total;
+      float shift = 2.0f * (float)p * gt_phi - ai + (float)p * kPiF;
+      float pd = phi - shift;
+      float det;
+      if (narrow) det = fabsf(fmodf(pd, TWO_PI_F)) < 1e-3f ? 0.0f : 1e6f;
+      else det = norm * wrapped_gaussian(pd, neg_inv_2var, kmax);
+      acc[p].x = acc[p].x + att.x * det * wgt;
+      acc[p].y = acc[p].y + att.y * det * wgt;
+      acc[p].z = acc[p].z + att.z * det * wgt;
     }
-    acc.x = acc.x + att.x * det * kLegWeights[i];
-    acc.y = acc.y + att.y * det * kLegWeights[i];
-    acc.z = acc.z + att.z * det * kLegWeights[i];
   }
-  return v3scale(acc, 0.5f);
+  V3 total{0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    float s = ((float)(p + 1) * scat) / 3.0f;
+    total.x = total.x + (acc[p].x * 0.5f) * s;
+    total.y = total.y + (acc[p].y * 0.5f) * s;
+    total.z = total.z + (acc[p].z * 0.5f) * s;
+  }
+  *out = total;
 }
 
 __device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material& m, float occlusion) {
@@ -327,13 +444,11 @@ __device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material&
   float mp = Mp(v_long, theta_i, theta_r, m.shift);
   V3 total{mp * m.specular_color[0] * m.specular_color[3] * direct, mp * m.specular_color[1] * m.specular_color[3] * direct,
            mp * m.specular_color[2] * m.specular_color[3] * direct};
-  for (uint32_t p = 0; p < 3u; ++p) {
-    V3 n = Np(p, phi, theta_i, theta_r, m.eta, sigma, beta_azim);
-    float s = ((float)(p + 1u) * scat) / 3.0f;
-    total.x = total.x + n.x * s;
-    total.y = total.y + n.y * s;
-    total.z = total.z + n.z * s;
-  }
+  V3 lobes;
+  np_lobes(phi, theta_i, theta_r, m.eta, sigma, beta_azim, scat, &lobes);
+  total.x = total.x + lobes.x;
+  total.y = total.y + lobes.y;
+  total.z = total.z + lobes.z;
   return total;
 }
 
@@ -422,106 +537,117 @@ __global__ __launch_bounds__(128) void k_shading(FrameParams fp) {
 }
 
 // --------------------------------------------------- K28 camera raster
+template <int TPX>
 struct ColorBatch {
-  float p0x[kBatch], p0y[kBatch], abx[kBatch], aby[kBatch], inv_l2[kBatch];
-  float d0[kBatch], d1[kBatch], w0[kBatch], w1[kBatch];
-  float rmin[kBatch], rmax[kBatch];
-  float mnx[kBatch], mny[kBatch], mxx[kBatch], mxy[kBatch];
-  float4 s0[kBatch], s1[kBatch];
-  float vis[kBatch];
-  uint32_t valid[kBatch];
+  float p0x[TPX], p0y[TPX], abx[TPX], aby[TPX], inv_l2[TPX];
+  float d0[TPX], d1[TPX], aw0[TPX], aw1[TPX];
+  float rmin[TPX], rmax[TPX];
+  float mnx[TPX], mny[TPX], mxx[TPX], mxy[TPX];
+  float4 s0[TPX], s1[TPX];
+  float vis[TPX];
+  uint32_t item[TPX];  // work item, with bit 31 set when the reference is invalid
 };
 
-__global__ __launch_bounds__(256) void k_camera_raster(FrameParams fp) {
-  __shared__ ColorBatch b;
+template <int TPX>
+__device__ void load_color_ref(const FrameParams& fp, const FrustumRec& fr, const RunTable<TPX>& t, uint32_t i,
+                               ColorBatch<TPX>& b) {
+  const uint32_t k = threadIdx.x;
+  const uint32_t w = item_of(t, i);
+  const sr_fine_seg_ref sr = fp.refs[t.base[w] + (i - t.pref[w])];
+  const InstRec& in = fp.inst[sr.inst_id];
+  const float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  V3 a3 = world_v3(in, in.indices[sr.seg_id]);
+  V3 c3 = world_v3(in, in.indices[sr.seg_id + 1u]);
+  V4 a = world_to_screen_raw(V4{a3.x, a3.y, a3.z, 1.0f}, fr.clip, vw, vh);
+  V4 c = world_to_screen_raw(V4{c3.x, c3.y, c3.z, 1.0f}, fr.clip, vw, vh);
+  b.item[k] = w | ((a.x < 0.0f && c.x < 0.0f) ? 0x80000000u : 0u);
+  if (a.x < 0.0f && c.x < 0.0f) return;
+  const sr_strand_material& m = in.materials[sr.material_idx];
+  float minr = fmaxf(m.min_radius_pixels, 1e-4f);
+  float maxr = fmaxf(m.max_radius_pixels, minr);
+  const size_t lt = ((size_t)fr.cam_index * fp.NI + sr.inst_id) * fp.atlas_texels;
+  uint32_t i0 = sr.seg_id, i1 = min(i0 + 1u, fp.atlas_texels - 1u);
+  float abx = c.x - a.x, aby = c.y - a.y;
+  b.p0x[k] = a.x; b.p0y[k] = a.y; b.abx[k] = abx; b.aby[k] = aby;
+  b.inv_l2[k] = 1.0f / (abx * abx + aby * aby);
+  b.d0[k] = a.z; b.d1[k] = c.z;
+  b.aw0[k] = fmaxf(fabsf(a.w), 1e-6f); b.aw1[k] = fmaxf(fabsf(c.w), 1e-6f);
+  b.rmin[k] = minr; b.rmax[k] = maxr;
+  b.mnx[k] = fminf(a.x, c.x) - maxr; b.mny[k] = fminf(a.y, c.y) - maxr;
+  b.mxx[k] = fmaxf(a.x, c.x) + maxr; b.mxy[k] = fmaxf(a.y, c.y) + maxr;
+  b.s0[k] = fp.atlas[lt + i0];
+  b.s1[k] = fp.atlas[lt + i1];
+  b.vis[k] = fp.history[lt + i0].x;
+}
+
+template <int TPX>
+__global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
+  __shared__ RunTable<TPX> t;
+  __shared__ ColorBatch<TPX> b;
   const uint32_t R = fp.counters[C_CAM_RUNS];
-  for (uint32_t ri = blockIdx.x; ri < R; ri += gridDim.x) {
-    const sr_raster_tile_run run = fp.cam_runs[ri];
+  while (next_run<TPX>(&fp.counters[C_CAMERA_HEAD], R, t)) {
+    const sr_raster_tile_run run = fp.cam_runs[t.run];
     const uint32_t fi = run.frustum_id;
     const FrustumRec& fr = fp.fr[fi];
     if (fr.d.kind != 0u || run.work_count == 0u) continue;
     if ((uint64_t)run.work_base + run.work_count > fp.work_cap) continue;  // emission overflowed
+    stage_run<TPX, false>(fp, run, fi, t);
+    const uint32_t total = t.pref[t.n];
     const uint32_t fx = fr.d.froxel_size_x, fy = fr.d.froxel_size_y;
     const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
     const uint32_t pidx = threadIdx.x;
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
     const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
     const float cx = (float)pxu + 0.5f, cy = (float)pyu + 0.5f;
-    const float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
-    const size_t layer_base = (size_t)fr.cam_index * fp.NI;
     float fc0 = 0.0f, fc1 = 0.0f, fc2 = 0.0f, fc3 = 0.0f, gdepth = 0.0f;
     bool done = !active;
-    for (uint32_t w = 0; w < run.work_count; ++w) {
-      const sr_raster_work_item wi = fp.work[run.work_base + w];
-      const uint32_t end = wi.seg_ref_base + wi.seg_ref_count;
-      for (uint32_t base = wi.seg_ref_base; base < end; base += kBatch) {
-        __syncthreads();
-        {
-          uint32_t k = threadIdx.x, ref_idx = base + k;
-          b.valid[k] = 0u;
-          if (ref_idx < end) {
-            const sr_fine_seg_ref sr = fp.refs[ref_idx];
-            const InstRec& in = fp.inst[sr.inst_id];
-            V3 a3 = world_v3(in, in.indices[sr.seg_id]);
-            V3 c3 = world_v3(in, in.indices[sr.seg_id + 1u]);
-            V4 a = world_to_screen_raw(V4{a3.x, a3.y, a3.z, 1.0f}, fr.clip, vw, vh);
-            V4 c = world_to_screen_raw(V4{c3.x, c3.y, c3.z, 1.0f}, fr.clip, vw, vh);
-            if (!(a.x < 0.0f && c.x < 0.0f)) {
-              const sr_strand_material& m = in.materials[sr.material_idx];
-              float minr = fmaxf(m.min_radius_pixels, 1e-4f);
-              float maxr = fmaxf(m.max_radius_pixels, minr);
-              const size_t lt = (layer_base + sr.inst_id) * fp.atlas_texels;
-              uint32_t i0 = sr.seg_id, i1 = min(i0 + 1u, fp.atlas_texels - 1u);
-              float abx = c.x - a.x, aby = c.y - a.y;
-              b.p0x[k] = a.x; b.p0y[k] = a.y; b.abx[k] = abx; b.aby[k] = aby;
-              b.inv_l2[k] = 1.0f / (abx * abx + aby * aby);
-              b.d0[k] = a.z; b.d1[k] = c.z; b.w0[k] = a.w; b.w1[k] = c.w;
-              b.rmin[k] = minr; b.rmax[k] = maxr;
-              b.mnx[k] = fminf(a.x, c.x) - maxr; b.mny[k] = fminf(a.y, c.y) - maxr;
-              b.mxx[k] = fmaxf(a.x, c.x) + maxr; b.mxy[k] = fmaxf(a.y, c.y) + maxr;
-              b.s0[k] = fp.atlas[lt + i0];
-              b.s1[k] = fp.atlas[lt + i1];
-              b.vis[k] = fp.history[lt + i0].x;
-              b.valid[k] = 1u;
+    uint32_t cur = 0;
+    for (uint32_t bs = 0; bs < total; bs += TPX) {
+      __syncthreads();
+      if (bs + threadIdx.x < total) load_color_ref(fp, fr, t, bs + threadIdx.x, b);
+      __syncthreads();
+      if (!done) {
+        const uint32_t nb = min((uint32_t)TPX, total - bs);
+        for (uint32_t k = 0; k < nb; ++k) {
+          const uint32_t item = b.item[k];
+          const uint32_t w = item & 0x7FFFFFFFu;
+          if (w != cur) {  // end of a work item: early out above 0.98 alpha
+            cur = w;
+            if (fc3 > 0.98f) {
+              fc0 = fc0 / fc3; fc1 = fc1 / fc3; fc2 = fc2 / fc3; fc3 = 1.0f;
+              done = true;
+              break;
             }
           }
+          if (item & 0x80000000u) continue;
+          if (cx < b.mnx[k] || cy < b.mny[k] || cx > b.mxx[k] || cy > b.mxy[k]) continue;
+          float tt = ((cx - b.p0x[k]) * b.abx[k] + (cy - b.p0y[k]) * b.aby[k]) * b.inv_l2[k];
+          if (!(tt >= 0.0f && tt <= 1.0f)) continue;
+          float qx = b.p0x[k] + b.abx[k] * tt, qy = b.p0y[k] + b.aby[k] * tt;
+          float qz = wmix(b.d0[k], b.d1[k], tt);
+          float minr = fmaxf(b.rmin[k], 1e-4f);
+          float r = wmix(minr, fmaxf(b.rmax[k], minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
+          float dx = cx - qx, dy = cy - qy;
+          float d2 = dx * dx + dy * dy;
+          if (!(d2 < r * r)) continue;
+          float cov = wclamp(1.0f - sqrtf(d2) / r, 0.0f, 1.0f);
+          float aw0 = b.aw0[k], aw1 = b.aw1[k];
+          float tw = (tt * aw0) / ((1.0f - tt) * aw1 + tt * aw0);
+          float4 s0 = b.s0[k], s1 = b.s1[k];
+          float alpha = wmix(s0.w, s1.w, tw) * cov;
+          float vis = b.vis[k];
+          float om = 1.0f - fc3;
+          fc0 = fc0 + (wmix(s0.x, s1.x, tw) * vis * alpha) * om;
+          fc1 = fc1 + (wmix(s0.y, s1.y, tw) * vis * alpha) * om;
+          fc2 = fc2 + (wmix(s0.z, s1.z, tw) * vis * alpha) * om;
+          fc3 = fc3 + alpha * om;
+          gdepth = fmaxf(gdepth, qz);
         }
-        __syncthreads();
-        if (!done) {
-          uint32_t n = min(kBatch, end - base);
-          for (uint32_t k = 0; k < n; ++k) {
-            if (!b.valid[k]) continue;
-            if (cx < b.mnx[k] || cy < b.mny[k] || cx > b.mxx[k] || cy > b.mxy[k]) continue;
-            float t = ((cx - b.p0x[k]) * b.abx[k] + (cy - b.p0y[k]) * b.aby[k]) * b.inv_l2[k];
-            if (!(t >= 0.0f && t <= 1.0f)) continue;
-            float qx = b.p0x[k] + b.abx[k] * t, qy = b.p0y[k] + b.aby[k] * t;
-            float qz = wmix(b.d0[k], b.d1[k], t);
-            float minr = fmaxf(b.rmin[k], 1e-4f);
-            float r = wmix(minr, fmaxf(b.rmax[k], minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
-            float dx = cx - qx, dy = cy - qy;
-            float d2 = dx * dx + dy * dy;
-            if (!(d2 < r * r)) continue;
-            float cov = wclamp(1.0f - sqrtf(d2) / r, 0.0f, 1.0f);
-            float aw0 = fmaxf(fabsf(b.w0[k]), 1e-6f), aw1 = fmaxf(fabsf(b.w1[k]), 1e-6f);
-            float tw = (t * aw0) / ((1.0f - t) * aw1 + t * aw0);
-            float4 s0 = b.s0[k], s1 = b.s1[k];
-            float sa = wmix(s0.w, s1.w, tw);
-            float alpha = sa * cov;
-            float vis = b.vis[k];
-            float om = 1.0f - fc3;
-            fc0 = fc0 + (wmix(s0.x, s1.x, tw) * vis * alpha) * om;
-            fc1 = fc1 + (wmix(s0.y, s1.y, tw) * vis * alpha) * om;
-            fc2 = fc2 + (wmix(s0.z, s1.z, tw) * vis * alpha) * om;
-            fc3 = fc3 + alpha * om;
-            gdepth = fmaxf(gdepth, qz);
-          }
-        }
-      }
-      if (!done && fc3 > 0.98f) {
-        fc0 = fc0 / fc3; fc1 = fc1 / fc3; fc2 = fc2 / fc3; fc3 = 1.0f;
-        done = true;
       }
       if (__syncthreads_count(!done) == 0) break;
+    }
+    if (!done && fc3 > 0.98f) {  // end of the last work item
+      fc0 = fc0 / fc3; fc1 = fc1 / fc3; fc2 = fc2 / fc3; fc3 = 1.0f;
     }
     if (active) {
       float o0 = fc0, o1 = fc1, o2 = fc2, o3 = fc3;
@@ -548,16 +674,22 @@ __global__ void k_composite(FrameParams fp, uint64_t pixels) {
 }
 
 // ------------------------------------------------------------ launchers
-void launch_shadow(const FrameParams& fp, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_raster, dim3(256u * 8u), dim3(256), 0, s, fp);
+void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
+  if (max_tile_px <= 64)
+    hipLaunchKernelGGL(k_shadow_raster<64>, dim3(256u * 16u), dim3(64), 0, s, fp);
+  else
+    hipLaunchKernelGGL(k_shadow_raster<256>, dim3(256u * 4u), dim3(256), 0, s, fp);
 }
 
 void launch_shading(const FrameParams& fp, hipStream_t s) {
   hipLaunchKernelGGL(k_shading, dim3(256u * 16u), dim3(128), 0, s, fp);
 }
 
-void launch_camera_raster(const FrameParams& fp, hipStream_t s) {
-  hipLaunchKernelGGL(k_camera_raster, dim3(256u * 8u), dim3(256), 0, s, fp);
+void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
+  if (max_tile_px <= 64)
+    hipLaunchKernelGGL(k_camera_raster<64>, dim3(256u * 16u), dim3(64), 0, s, fp);
+  else
+    hipLaunchKernelGGL(k_camera_raster<256>, dim3(256u * 4u), dim3(256), 0, s, fp);
 }
 
 void launch_stampback(const FrameParams& fp, uint32_t fi, uint32_t texels, hipStream_t s) {

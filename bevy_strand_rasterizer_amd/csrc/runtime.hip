@@ -155,8 +155,14 @@ struct sr_context {
   SlabAllocator slabs;
   std::vector<AssetRec> assets;
   DBuf buf[B_COUNT];
-  void* staging = nullptr;  // pinned host staging for per-frame tables
-  uint64_t staging_bytes = 0;
+  // pinned host staging for per-frame tables: a ring, so the host can run
+  // several frames ahead of the device without overwriting pending copies
+  static constexpr int kRing = 8;
+  void* staging[kRing] = {};
+  uint64_t staging_bytes[kRing] = {};
+  hipEvent_t staging_ev[kRing] = {};
+  bool staging_used[kRing] = {};
+  int ring_pos = 0;
   FrameShape shape;
   StageTimer timer;
   hipEvent_t done = nullptr;
@@ -184,15 +190,24 @@ int hip_fail(sr_context* ctx, hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
   } while (0)
 
-bool ensure_staging(sr_context* ctx, uint64_t bytes) {
-  if (bytes <= ctx->staging_bytes) return true;
-  if (ctx->staging) (void)hipHostFree(ctx->staging);
-  ctx->staging = nullptr;
-  ctx->staging_bytes = 0;
-  uint64_t b = align_up(bytes * 2, 4096);
-  if (hipHostMalloc(&ctx->staging, b, hipHostMallocDefault) != hipSuccess) return false;
-  ctx->staging_bytes = b;
-  return true;
+// Returns the next ring slot with at least `bytes`, after the device has
+// consumed the copy that last used it.
+char* next_staging(sr_context* ctx, uint64_t bytes, int* slot_out) {
+  int k = ctx->ring_pos;
+  ctx->ring_pos = (k + 1) % sr_context::kRing;
+  if (ctx->staging_used[k]) (void)hipEventSynchronize(ctx->staging_ev[k]);
+  if (!ctx->staging_ev[k] && hipEventCreateWithFlags(&ctx->staging_ev[k], hipEventDisableTiming) != hipSuccess)
+    return nullptr;
+  if (bytes > ctx->staging_bytes[k]) {
+    if (ctx->staging[k]) (void)hipHostFree(ctx->staging[k]);
+    ctx->staging[k] = nullptr;
+    ctx->staging_bytes[k] = 0;
+    uint64_t b = align_up(bytes * 2, 4096);
+    if (hipHostMalloc(&ctx->staging[k], b, hipHostMallocDefault) != hipSuccess) return nullptr;
+    ctx->staging_bytes[k] = b;
+  }
+  *slot_out = k;
+  return (char*)ctx->staging[k];
 }
 
 inline uint32_t next_pow2(uint32_t x) {
@@ -240,7 +255,10 @@ int sr_destroy(sr_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& b : ctx->buf) b.free_();
-  if (ctx->staging) (void)hipHostFree(ctx->staging);
+  for (int k = 0; k < sr_context::kRing; ++k) {
+    if (ctx->staging[k]) (void)hipHostFree(ctx->staging[k]);
+    if (ctx->staging_ev[k]) (void)hipEventDestroy(ctx->staging_ev[k]);
+  }
   for (auto e : ctx->timer.events) (void)hipEventDestroy(e);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -336,6 +354,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   if (fd->frustum_count == 0 || !fd->frusta || !fd->views)
     return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no frusta");
   if (fd->instance_count && !fd->instances) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
+  if (fd->instance_count >= (1u << 24)) return fail(ctx, SR_ERR_UNSUPPORTED, "sr_frame_submit: at most 2^24 instances");
   if (fd->light_count && !fd->lights) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no lights");
   const sr_frame_settings& st = fd->settings;
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
@@ -347,7 +366,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   const uint32_t F = fd->frustum_count, NI = fd->instance_count, NL = fd->light_count;
   // ---- frustum records
   std::vector<FrustumRec> fr(F);
-  uint32_t lights_seen = 0, cams = 0, coarse_total = 0;
+  uint32_t lights_seen = 0, cams = 0, coarse_total = 0, max_cam_tile_px = 1, max_light_tile_px = 1;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
   for (uint32_t i = 0; i < F; ++i) {
     const sr_frustum_desc& d = fd->frusta[i];
@@ -385,6 +404,8 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
       img_px += (uint64_t)d.screen_width * d.screen_height;
     }
     r.enabled = (d.kind == 0u) || (d.kind == 1u && r.light_layer != kInvalid);
+    if (d.kind == 0u) max_cam_tile_px = std::max(max_cam_tile_px, d.froxel_size_x * d.froxel_size_y);
+    if (d.kind == 1u) max_light_tile_px = std::max(max_light_tile_px, d.froxel_size_x * d.froxel_size_y);
     coarse_total = std::max(coarse_total, d.coarse_depth_tile_base + d.coarse_depth_tile_count);
   }
   // ---- instance records
@@ -465,14 +486,20 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
       {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
-  if (ctx->submitted) (void)hipStreamSynchronize(s);  // buffers may be reallocated below
+  bool grow = false;
+  for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > ctx->buf[n.id].bytes;
+  if (grow && ctx->submitted) {  // in-flight frames may still use the buffers
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
   for (const Need& n : needs) {
     if (!ctx->buf[n.id].ensure(n.bytes)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
   }
   // ---- upload per-frame tables through pinned staging
   const uint64_t tbytes = sizeof(FrustumRec) * F + sizeof(InstRec) * NI + sizeof(sr_directional_light) * NL;
-  if (!ensure_staging(ctx, tbytes + 256)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
-  char* stg = (char*)ctx->staging;
+  int slot = 0;
+  char* stg = next_staging(ctx, tbytes + 256, &slot);
+  if (!stg) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
   std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
   if (NI) std::memcpy(stg + sizeof(FrustumRec) * F, ir.data(), sizeof(InstRec) * NI);
   if (NL) std::memcpy(stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI, fd->lights, sizeof(sr_directional_light) * NL);
@@ -483,6 +510,8 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   if (NL)
     HIP_OK(hipMemcpyAsync(ctx->buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
                           sizeof(sr_directional_light) * NL, hipMemcpyHostToDevice, s), "upload lights");
+  HIP_OK(hipEventRecord(ctx->staging_ev[slot], s), "staging event");
+  ctx->staging_used[slot] = true;
 
   FrameParams fp;
   std::memset(&fp, 0, sizeof(fp));
@@ -544,16 +573,20 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.image), img_px * 4, 0u, s);
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.depth), img_px, 0u, s);
+  // texels no segment writes this frame read as zero (the reference leaves
+  // them stale; dumps must be deterministic)
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4, 0u, s);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2, 0u, s);
   launch_prepass(fp, ps, s, &ctx->timer);
   ctx->timer.mark(s, "shadow");
-  launch_shadow(fp, s);
+  launch_shadow(fp, s, max_light_tile_px);
   ctx->timer.mark(s, "stampback");
   for (uint32_t i = 0; i < F; ++i)
     if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
   ctx->timer.mark(s, "shading");
   launch_shading(fp, s);
   ctx->timer.mark(s, "raster");
-  launch_camera_raster(fp, s);
+  launch_camera_raster(fp, s, max_cam_tile_px);
   ctx->timer.mark(s, "composite");
   launch_composite(fp, img_px, s);
   ctx->timer.mark(s, "end");
