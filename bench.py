@@ -43,9 +43,13 @@ def stage_bytes(ctr, w):
     W = ctr["work_items"]
     R = ctr["camera_runs"] + ctr["shadow_runs"]
     tiles16 = 16 * ctr["coarse_tiles"]
+    cand = ctr.get("page_candidates", 0)
     return {
-        "mark": 28 * T,
+        "project_mark": 28 * T,
         "count": 28 * T + 4 * C,
+        # page CSR: candidate lists read (sorted in place), cells written
+        "csr_count": 8 * cand + 28 * cand + 8 * C,
+        "csr_fill": 4 * cand + 28 * cand + 4 * C + 12 * H,
         "prefix": 8 * C + 32 * P,
         "fill": 28 * T + 4 * C + 12 * H,
         "sort_cells": 24 * H,
@@ -90,6 +94,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="named", choices=["named", "small", "batched"])
+    ap.add_argument("--backend", default="scatter", choices=["scatter", "csr"],
+                    help="fine binning backend (FineBinningBackend::SegmentScatter / PageCsr)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -109,6 +115,7 @@ def main():
         w = scene.config_small()
     else:
         w = scene.config_batched()
+    w.settings["fine_binning_backend"] = 1 if args.backend == "csr" else 0
     views = w.camera_count
     segs = w.input_segments
 
@@ -150,7 +157,8 @@ def main():
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
     roof = roofline_for(dom, stage_acc[dom], ctr, w)
-    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w) for k in ("count", "prefix", "fill")}
+    bstages = ("csr_count", "csr_fill") if args.backend == "csr" else ("count", "prefix", "fill")
+    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w) for k in bstages}
     if roof is None:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4),
@@ -180,7 +188,7 @@ def main():
         "config": {"workload": w.name, "mode": "plain", "strands": int(w.assets[0].strand_count),
                    "points_per_strand": int(w.assets[0].meta[0]["count"]), "segments": segs, "views": views,
                    "lights": int(len(w.lights)), "dom_layers": int(w.settings["dom_layers"]),
-                   "froxel": int(w.frusta[0]["froxel_size_x"])},
+                   "froxel": int(w.frusta[0]["froxel_size_x"]), "fine_binning_backend": args.backend},
         "msegments_per_s": round(segs * views * args.steps / dt / 1e6, 2),
         "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_acc.items()},

@@ -79,6 +79,8 @@ enum Counter : uint32_t {
   C_PAGE_DROPS = 10,  // marked pages dropped by the page capacity
   C_CAMERA_HEAD = 11, // camera raster run queue head
   C_SHADOW_HEAD = 12, // shadow raster run queue head
+  C_CANDS = 13,       // page candidates (CSR backend)
+  C_CAND_OVERFLOW = 14,
   C_COUNT = 16
 };
 
@@ -91,6 +93,7 @@ struct FrameParams {
   uint32_t coarse_total;  // coarse tiles over all frusta
   uint32_t task_cap, ref_cap, work_cap, atlas_texels;
   uint32_t dom_layers, page_cap;
+  uint32_t cand_cap, cand_lds_limit;  // CSR: lists longer than cand_lds_limit sort in global memory
   sr_frame_settings st;
 
   uint32_t* counters;  // Counter words
@@ -123,6 +126,13 @@ struct FrameParams {
   sr_raster_work_item* work;         // [work_cap]
   sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
   sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
+  // page CSR backend
+  uint32_t* cand_count_pt;           // [coarse_total * 16] candidates per page-table entry
+  uint32_t* cand_off;                // [page_cap + 1] candidate list offsets
+  uint32_t* cand_cursor;             // [page_cap]
+  uint32_t* cands;                   // [cand_cap] task ids, sorted per page
+  uint32_t* cands_tmp;               // [cand_cap] radix-sort ping-pong for long lists
+  uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // shadow / shading / raster
   float* dom_depth;
   float* dom_opacity;

@@ -195,70 +195,6 @@ __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, u
   return true;
 }
 
-// Pass 1: per slot validity ballot + packed endpoints in slot layout.
-__global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
-  const uint64_t total = (uint64_t)fp.F * fp.slots;
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
-    uint32_t ii, strand, seg;
-    sr_projected_segment ps;
-    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
-    if (ok) slot_proj[s] = ps;
-    unsigned long long m = __ballot(ok);
-    if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
-    const FrustumRec& fr = fp.fr[fi];
-    if (fr.cam_index != kInvalid) fp.shaded[(size_t)fr.cam_index * fp.slots + pos] = ok ? 1 : 0;
-  }
-}
-
-struct MaskSrc {
-  const unsigned long long* masks;
-  __device__ uint32_t load(uint32_t i) const { return (uint32_t)__popcll(masks[i]); }
-};
-struct WaveOffsetDst {
-  uint32_t* offsets;
-  uint32_t* counters;
-  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { offsets[i] = ex; }
-  __device__ void total(uint32_t t) const { counters[C_TASKS] = t; }
-};
-
-__global__ void k_frustum_bases(FrameParams fp) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t waves_per_frustum = fp.slots / 64u;
-  uint32_t cam = 0;
-  for (uint32_t fi = 0; fi < fp.F; ++fi) {
-    uint32_t base = fp.wave_offsets[(size_t)fi * waves_per_frustum];
-    fp.frustum_task_base[fi] = base;
-  }
-  fp.frustum_task_base[fp.F] = fp.counters[C_TASKS];
-  for (uint32_t fi = 0; fi < fp.F; ++fi) {
-    fp.frustum_cam_base[fi] = cam;
-    if (fp.fr[fi].d.kind == 0u) cam += fp.frustum_task_base[fi + 1] - fp.frustum_task_base[fi];
-  }
-  fp.counters[C_CAM_TASKS] = cam;
-}
-
-// Pass 2: compaction with ballot ranks; the queue order is slot order.
-__global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_projected_segment* slot_proj) {
-  const uint64_t total = (uint64_t)fp.F * fp.slots;
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    unsigned long long m = fp.slot_masks[s >> 6];
-    if (!((m >> lane) & 1ull)) continue;
-    uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    uint32_t t = fp.wave_offsets[s >> 6] + rank;
-    if (t >= fp.task_cap) continue;
-    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
-    uint32_t ii = find_instance(fp, pos);
-    const InstRec& in = fp.inst[ii];
-    uint32_t p = pos - in.pos_base;
-    const FrustumRec& fr = fp.fr[fi];
-    fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
-    fp.proj[t] = slot_proj[s];
-    if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
-  }
-}
-
 // ------------------------------------------------------- K7 + K8 LUT
 // Interval merge of build_depth_warp_lut, fed one interval at a time in
 // linked-list walk order.
@@ -330,6 +266,8 @@ __global__ void k_lut(FrameParams fp) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fp.coarse_total) return;
   for (uint32_t z = 0; z < kSlices; ++z) fp.page_table[t * kSlices + z] = 0u;
+  if (fp.st.fine_binning_backend != 0u)
+    for (uint32_t z = 0; z < kSlices; ++z) fp.cand_count_pt[t * kSlices + z] = 0u;
   for (uint32_t z = 0; z < kExt * kExt; ++z) fp.active_flags[t * kExt * kExt + z] = 0u;
   uint32_t fi = frustum_for_coarse_tile(fp, t);
   LutBuilder b;
@@ -364,17 +302,66 @@ __device__ __forceinline__ uint32_t lut_slice_for_depth(const sr_coarse_depth_lu
   return best;
 }
 
-enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2 };
+enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
 
-template <int MODE>
-__device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
-                                  uint32_t local, const sr_fine_seg_ref& ref) {
+// Fine 3D Amanatides-Woo walk through the 4 x 4 x 16 cells of one count page
+// (strand_prepass.wgsl:1517-1573) for a segment already clipped to the coarse
+// tile and the LUT slice [zmin, zmax]; calls emit(cell) per visited cell.
+template <class Emit>
+__device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, float zmax, const sr_frustum_desc& fd,
+                                         Emit&& emit) {
+  float zspan = fmaxf(zmax - zmin, 1.0f / (float)kQuant);
+  float fxs = (float)fd.froxel_size_x, fys = (float)fd.froxel_size_y;
+  float pa[3] = {(q0.x - origin.x) / fxs, (q0.y - origin.y) / fys,
+                 wclamp((q0.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
+  float pb[3] = {(q1.x - origin.x) / fxs, (q1.y - origin.y) / fys,
+                 wclamp((q1.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
+  const int32_t hi[3] = {(int32_t)kExt - 1, (int32_t)kExt - 1, (int32_t)kSlices - 1};
+  int32_t f[3], f1[3], step[3];
+  float delta[3], tmax[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[k] = iclamp(f2i(floorf(pa[k])), 0, hi[k]);
+    f1[k] = iclamp(f2i(floorf(pb[k])), 0, hi[k]);
+    float d = pb[k] - pa[k];
+    step[k] = sgn_i32(d);
+    bool tiny = fabsf(d) < 1e-6f;
+    float safe = tiny ? 1e-6f : d;
+    delta[k] = fabsf(1.0f / safe);
+    float fl = floorf(pa[k]);
+    tmax[k] = step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
+    if (tiny) tmax[k] = 1e38f;
+  }
+  for (uint32_t safety = 0; safety <= kExt + kExt + kSlices + 3u; ++safety) {
+    emit(((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0]);
+    if (f[0] == f1[0] && f[1] == f1[1] && f[2] == f1[2]) break;
+    float mn = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
+    int k = tmax[0] == mn ? 0 : (tmax[1] == mn ? 1 : 2);  // canonical_min_mask: x, then y, then z
+    f[k] += step[k];
+    tmax[k] += delta[k];
+    if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] > hi[0] || f[1] > hi[1] || f[2] > hi[2]) break;
+  }
+}
+
+// Coarse-tile box of a frustum's coarse tile (strand_prepass.wgsl:1422-1433).
+__device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint32_t local, V2* origin, V3* bmax) {
   uint32_t cx = local % fd.coarse_tiles_x, cy = local / fd.coarse_tiles_x;
   uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
-  V2 origin{(float)(cx * cpx), (float)(cy * cpy)};
-  float tmx = fminf(origin.x + (float)cpx, (float)fd.screen_width) - 1e-3f;
-  float tmy = fminf(origin.y + (float)cpy, (float)fd.screen_height) - 1e-3f;
-  V3 bmax{fmaxf(origin.x, tmx), fmaxf(origin.y, tmy), 1.0f};
+  *origin = V2{(float)(cx * cpx), (float)(cy * cpy)};
+  float tmx = fminf(origin->x + (float)cpx, (float)fd.screen_width) - 1e-3f;
+  float tmy = fminf(origin->y + (float)cpy, (float)fd.screen_height) - 1e-3f;
+  *bmax = V3{fmaxf(origin->x, tmx), fmaxf(origin->y, tmy), 1.0f};
+}
+
+// trace_segment_into_coarse_count_page (strand_prepass.wgsl:1412-1575).
+// MARK also counts page candidates for the CSR backend; SCATTER appends the
+// task to each touched page's candidate list.
+template <int MODE>
+__device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
+                                  uint32_t local, const sr_fine_seg_ref& ref, uint32_t task) {
+  V2 origin;
+  V3 bmax;
+  coarse_tile_box(fd, local, &origin, &bmax);
   Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
   const sr_coarse_depth_lut_entry* lut = fp.lut + (size_t)tile * kSlices;
@@ -390,42 +377,25 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     uint32_t pti = tile * kSlices + cz;
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
+      if (fp.st.fine_binning_backend != 0u) atomicAdd(&fp.cand_count_pt[pti], 1u);
       continue;
     }
     uint32_t h = fp.page_table[pti];
     if (h == 0u || h == kMarked) continue;
     uint32_t page = h - 1u;
-    uint32_t ref_base = 0, ref_count = 0;
+    if (MODE == MODE_SCATTER) {
+      uint32_t pos = atomicAdd(&fp.cand_cursor[page], 1u);
+      uint64_t dst = (uint64_t)fp.cand_off[page] + pos;
+      if (dst < fp.cand_cap) fp.cands[dst] = task;
+      continue;
+    }
+    uint32_t ref_base = 0;
     if (MODE == MODE_FILL) {
       ref_base = fp.page_meta[page].seg_ref_base;
-      ref_count = fp.page_meta[page].seg_ref_count;
-      if (ref_count == 0u) continue;
-    }
-    float zspan = fmaxf(zmax - zmin, 1.0f / (float)kQuant);
-    float fxs = (float)fd.froxel_size_x, fys = (float)fd.froxel_size_y;
-    float pa[3] = {(dc.p0.x - origin.x) / fxs, (dc.p0.y - origin.y) / fys,
-                   wclamp((dc.p0.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
-    float pb[3] = {(dc.p1.x - origin.x) / fxs, (dc.p1.y - origin.y) / fys,
-                   wclamp((dc.p1.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
-    const int32_t hi[3] = {(int32_t)kExt - 1, (int32_t)kExt - 1, (int32_t)kSlices - 1};
-    int32_t f[3], f1[3], step[3];
-    float delta[3], tmax[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      f[k] = iclamp(f2i(floorf(pa[k])), 0, hi[k]);
-      f1[k] = iclamp(f2i(floorf(pb[k])), 0, hi[k]);
-      float d = pb[k] - pa[k];
-      step[k] = sgn_i32(d);
-      bool tiny = fabsf(d) < 1e-6f;
-      float safe = tiny ? 1e-6f : d;
-      delta[k] = fabsf(1.0f / safe);
-      float fl = floorf(pa[k]);
-      tmax[k] = step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
-      if (tiny) tmax[k] = 1e38f;
+      if (fp.page_meta[page].seg_ref_count == 0u) continue;
     }
     const uint32_t page_cells = page * kPage;
-    for (uint32_t safety = 0; safety <= kExt + kExt + kSlices + 3u; ++safety) {
-      uint32_t cell = ((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0];
+    fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, [&](uint32_t cell) {
       if (MODE == MODE_COUNT) {
         atomicAdd(&fp.cell_counts[page_cells + cell], 1u);
       } else {
@@ -433,34 +403,19 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
         uint64_t dst = (uint64_t)ref_base + fp.cell_offsets[page_cells + cell] + w;
         if (dst < fp.ref_cap) fp.refs[dst] = ref;
       }
-      if (f[0] == f1[0] && f[1] == f1[1] && f[2] == f1[2]) break;
-      float mn = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
-      int k = tmax[0] == mn ? 0 : (tmax[1] == mn ? 1 : 2);
-      f[k] += step[k];
-      tmax[k] += delta[k];
-      if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] > hi[0] || f[1] > hi[1] || f[2] > hi[2]) break;
-    }
+    });
   }
 }
 
+// Coarse 2D DDA over the asset range's coarse tiles, then the per-tile
+// trace (visit_segment_asset_coarse_tiles, strand_prepass.wgsl:1591-1696).
 template <int MODE>
-__device__ void visit_task(const FrameParams& fp, uint32_t t) {
-  const sr_binning_task tk = fp.tasks[t];
-  const sr_projected_segment ps = fp.proj[t];
-  uint32_t fi = tk.packed_field >> SR_BINNING_FRUSTUM_SHIFT;
-  if (fi >= fp.F) return;
+__device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V3 p0, V3 p1,
+                              const sr_fine_seg_ref& ref, uint32_t task) {
   const sr_frustum_desc& fd = fp.fr[fi].d;
-  uint32_t ii = tk.id_info;
   const sr_coarse_asset_range r = fp.ranges[(size_t)ii * fp.F + fi];
   if (r.geo_id != ii || r.frustum_id != fi) return;
   float W = (float)fd.screen_width, H = (float)fd.screen_height;
-  V2 a = unpack2x16unorm(ps.p0_xy), b = unpack2x16unorm(ps.p1_xy), dz = unpack2x16unorm(ps.depths);
-  V3 p0{a.x * W, a.y * H, dz.x}, p1{b.x * W, b.y * H, dz.y};
-  sr_fine_seg_ref ref{ii, tk.seg_idx, 0u};
-  if (MODE == MODE_FILL) {
-    const InstRec& in = fp.inst[ii];
-    if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
-  }
   uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
   uint32_t min_cx = max(r.min_x, min(f2u(fmaxf(fminf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
   uint32_t max_cx = min(r.max_x, min(f2u(fmaxf(fmaxf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
@@ -494,17 +449,123 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
     uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
     if (local < fd.coarse_depth_tile_count)
-      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref);
+      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref, task);
     if (f[0] == f1[0] && f[1] == f1[1]) break;
     if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
     else { f[1] += step[1]; tmax[1] += delta[1]; }
   }
 }
 
+// Endpoints as every binning pass sees them: the packed ProjectedSegment
+// unpacked and scaled by the frustum's screen size (process_binning_task,
+// strand_prepass.wgsl:1894-1926).
+__device__ __forceinline__ void task_endpoints(const sr_frustum_desc& fd, const sr_projected_segment& ps, V3* p0,
+                                               V3* p1) {
+  float W = (float)fd.screen_width, H = (float)fd.screen_height;
+  V2 a = unpack2x16unorm(ps.p0_xy), b = unpack2x16unorm(ps.p1_xy), dz = unpack2x16unorm(ps.depths);
+  *p0 = V3{a.x * W, a.y * H, dz.x};
+  *p1 = V3{b.x * W, b.y * H, dz.y};
+}
+
+template <int MODE>
+__device__ void visit_task(const FrameParams& fp, uint32_t t) {
+  const sr_binning_task tk = fp.tasks[t];
+  const sr_projected_segment ps = fp.proj[t];
+  uint32_t fi = tk.packed_field >> SR_BINNING_FRUSTUM_SHIFT;
+  if (fi >= fp.F) return;
+  uint32_t ii = tk.id_info;
+  V3 p0, p1;
+  task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
+  sr_fine_seg_ref ref{ii, tk.seg_idx, 0u};
+  if (MODE == MODE_FILL) {
+    const InstRec& in = fp.inst[ii];
+    if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
+  }
+  visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
   const uint32_t n = min(fp.counters[C_TASKS], fp.task_cap);
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) visit_task<MODE>(fp, t);
+}
+
+// Pass 1: per slot validity ballot + packed endpoints in slot layout; valid
+// slots also mark their pages (the LUT is built before projection).
+__global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
+  const uint64_t total = (uint64_t)fp.F * fp.slots;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+    uint32_t ii, strand, seg;
+    sr_projected_segment ps;
+    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
+    if (ok) {
+      slot_proj[s] = ps;
+      // K10: mark the (coarse tile, LUT slice) pages the segment touches,
+      // from the same packed endpoints the count and fill passes unpack
+      V3 p0, p1;
+      task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
+      visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u);
+    }
+    unsigned long long m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
+    const FrustumRec& fr = fp.fr[fi];
+    if (fr.cam_index != kInvalid) fp.shaded[(size_t)fr.cam_index * fp.slots + pos] = ok ? 1 : 0;
+  }
+}
+
+struct MaskSrc {
+  const unsigned long long* masks;
+  __device__ uint32_t load(uint32_t i) const { return (uint32_t)__popcll(masks[i]); }
+};
+struct WaveOffsetDst {
+  uint32_t* offsets;
+  uint32_t* counters;
+  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { offsets[i] = ex; }
+  __device__ void total(uint32_t t) const { counters[C_TASKS] = t; }
+};
+
+__global__ void k_frustum_bases(FrameParams fp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t waves_per_frustum = fp.slots / 64u;
+  uint32_t cam = 0;
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    uint32_t base = fp.wave_offsets[(size_t)fi * waves_per_frustum];
+    fp.frustum_task_base[fi] = base;
+  }
+  fp.frustum_task_base[fp.F] = fp.counters[C_TASKS];
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    fp.frustum_cam_base[fi] = cam;
+    if (fp.fr[fi].d.kind == 0u) cam += fp.frustum_task_base[fi + 1] - fp.frustum_task_base[fi];
+  }
+  fp.counters[C_CAM_TASKS] = cam;
+}
+
+// Pass 2: compaction with ballot ranks; the queue order is slot order.
+__global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_projected_segment* slot_proj) {
+  const uint64_t total = (uint64_t)fp.F * fp.slots;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long m = fp.slot_masks[s >> 6];
+    if (!((m >> lane) & 1ull)) continue;
+    uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint32_t t = fp.wave_offsets[s >> 6] + rank;
+    if (t >= fp.task_cap) continue;
+    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+    uint32_t ii = find_instance(fp, pos);
+    const InstRec& in = fp.inst[ii];
+    uint32_t p = pos - in.pos_base;
+    const FrustumRec& fr = fp.fr[fi];
+    fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
+    const sr_projected_segment ps = slot_proj[s];
+    fp.proj[t] = ps;
+    if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
+    if (fp.st.fine_binning_backend != 0u) {  // K17: append to the candidate lists of the touched pages
+      V3 p0, p1;
+      task_endpoints(fr.d, ps, &p0, &p1);
+      visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, t);
+    }
+  }
 }
 
 // ------------------------------------------------------ K11 allocate
@@ -517,6 +578,8 @@ struct PageAllocDst {
   sr_fine_page_meta* meta;
   uint32_t* counters;
   uint32_t page_cap;
+  const uint32_t* cand_count_pt;  // non-null for the CSR backend
+  uint32_t* page_cands;
   __device__ void store(uint32_t e, uint32_t ex, uint32_t v) const {
     if (!v) return;
     if (ex >= page_cap) {  // the reference drops pages past its capacity (entry 0)
@@ -526,6 +589,7 @@ struct PageAllocDst {
     }
     table[e] = ex + 1u;
     meta[ex] = sr_fine_page_meta{0u, 0u, 0u, 0u, e / kSlices, e % kSlices, 0u, 0u};
+    if (cand_count_pt) page_cands[ex] = cand_count_pt[e];
   }
   __device__ void total(uint32_t t) const {
     counters[C_MARKED] = t;
@@ -597,6 +661,236 @@ struct PageBaseDst {
   }
   __device__ void total(uint32_t t) const { fp.counters[C_REFS] = t; }
 };
+
+// ------------------------------------------------------ page CSR backend
+// K16-K18 (prefix_page_candidates, scatter_page_candidates,
+// build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507): every
+// page owns the list of tasks that touch it.  One workgroup per page sorts
+// its candidate list (so lists are in task order), counts cells in LDS, and
+// after the page-base scan fills references with stable ranks: a chunk of
+// 256 candidates sets one bit per (cell, lane) in an LDS bit matrix and each
+// hit's rank is the cell cursor plus the popcount of lower lanes, so every
+// cell comes out in task order, i.e. sorted by (inst, seg), with no
+// reference sort.
+struct CandSrc {
+  const uint32_t* counts;
+  __device__ uint32_t load(uint32_t i) const { return counts[i]; }
+};
+struct CandDst {
+  uint32_t* off;
+  uint32_t* cursor;
+  uint32_t* counters;
+  __device__ void store(uint32_t p, uint32_t ex, uint32_t) const {
+    off[p] = ex;
+    cursor[p] = 0u;
+  }
+  __device__ void total(uint32_t t) const {
+    counters[C_CANDS] = t;
+    off[counters[C_PAGES]] = t;
+  }
+};
+
+// trace_candidate_inside_page (strand_prepass.wgsl:2240-2337)
+template <class Emit>
+__device__ __forceinline__ void trace_in_page(const FrameParams& fp, uint32_t task, const sr_fine_page_meta& pm,
+                                              uint32_t fi, Emit&& emit) {
+  const sr_binning_task tk = fp.tasks[task];
+  if ((tk.packed_field >> SR_BINNING_FRUSTUM_SHIFT) != fi) return;
+  const sr_frustum_desc& fd = fp.fr[fi].d;
+  V3 p0, p1;
+  task_endpoints(fd, fp.proj[task], &p0, &p1);
+  const sr_coarse_depth_lut_entry e = fp.lut[pm.coarse_tile_id * kSlices + pm.coarse_z];
+  if (e.virtual_count_q == 0u) return;
+  float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
+  V2 origin;
+  V3 bmax;
+  coarse_tile_box(fd, pm.coarse_tile_id - fd.coarse_depth_tile_base, &origin, &bmax);
+  Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
+  if (!c.ok) return;
+  Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
+  if (!dc.ok) return;
+  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, emit);
+}
+
+constexpr uint32_t kCandLds = 8192;
+
+// Stable LSD radix sort (8-bit digits) of a candidate list too long for LDS,
+// ping-ponging between a and b; the same LDS bit-matrix ranks as the fill
+// keep every pass stable.  Returns the buffer holding the sorted list.
+// lds holds 256 * 8 row words + 256 cursors.
+__device__ uint32_t* radix_sort_global(uint32_t* a, uint32_t* b, uint32_t n, uint32_t* lds, uint32_t* lo_hi,
+                                       uint32_t* lds4) {
+  uint32_t* rows = lds;
+  uint32_t* cursor = lds + kPage * 8;
+  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
+  if (lane == 0) { lo_hi[0] = 0xFFFFFFFFu; lo_hi[1] = 0u; }
+  __syncthreads();
+  uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+  for (uint32_t i = lane; i < n; i += blockDim.x) { mn = min(mn, a[i]); mx = max(mx, a[i]); }
+  atomicMin(&lo_hi[0], mn);
+  atomicMax(&lo_hi[1], mx);
+  __syncthreads();
+  const uint32_t lo = lo_hi[0], span = lo_hi[1] - lo;
+  const uint32_t bits = span ? 32u - __clz(span) : 1u;
+  for (uint32_t sh = 0; sh < bits; sh += 8u) {
+    cursor[lane] = 0u;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += blockDim.x) atomicAdd(&cursor[((a[i] - lo) >> sh) & 255u], 1u);
+    __syncthreads();
+    uint32_t total;
+    const uint32_t ex = block_exclusive<uint32_t>(cursor[lane], &total, lds4);
+    cursor[lane] = ex;
+    __syncthreads();
+    for (uint32_t cb = 0; cb < n; cb += 256u) {
+      const uint32_t i = cb + lane;
+      const bool has = i < n;
+      const uint32_t v = has ? a[i] : 0u;
+      const uint32_t d = ((v - lo) >> sh) & 255u;
+      if (has) atomicOr(&rows[d * 8 + word], bit);
+      __syncthreads();
+      if (has) {
+        const uint32_t* row = rows + d * 8;
+        uint32_t rank = cursor[d];
+        for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
+        rank += __popc(row[word] & (bit - 1u));
+        b[rank] = v;
+      }
+      __syncthreads();
+      uint32_t add = 0;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) {
+        add += __popc(rows[lane * 8 + w]);
+        rows[lane * 8 + w] = 0u;
+      }
+      cursor[lane] += add;
+      __syncthreads();
+    }
+    uint32_t* t = a; a = b; b = t;
+  }
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
+  __shared__ uint32_t sc[kCandLds];
+  __shared__ uint32_t scnt[kPage];
+  __shared__ uint32_t lds4[4];
+  __shared__ uint32_t lo_hi[2];
+  __shared__ uint32_t colmask;
+  const uint32_t P = fp.counters[C_PAGES];
+  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+    const uint32_t off = fp.cand_off[page];
+    const uint32_t n = fp.cand_off[page + 1] - off;
+    const sr_fine_page_meta pm = fp.page_meta[page];
+    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
+    __syncthreads();
+    scnt[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) colmask = 0u;
+    const bool valid = (uint64_t)off + n <= fp.cand_cap && fi != kInvalid;
+    uint32_t* gl = fp.cands + off;
+    const bool in_lds = n <= min(kCandLds, fp.cand_lds_limit);
+    const uint32_t* list = in_lds ? sc : gl;
+    if (valid) {
+      if (in_lds) {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sc[i] = gl[i];
+        __syncthreads();
+        uint32_t m = 1;
+        while (m < n) m <<= 1;
+        for (uint32_t k = 2; k <= m; k <<= 1) {
+          for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+              uint32_t partner = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
+              if (partner <= i || partner >= n) continue;
+              uint32_t a = sc[i], b = sc[partner];
+              if (b < a) { sc[i] = b; sc[partner] = a; }
+            }
+            __syncthreads();
+          }
+        }
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gl[i] = sc[i];
+      } else {
+        const uint32_t* sorted = radix_sort_global(gl, fp.cands_tmp + off, n, sc, lo_hi, lds4);
+        if (sorted != gl)
+          for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gl[i] = sorted[i];
+        list = gl;
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t task = list[i];
+        trace_in_page(fp, task, pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
+      }
+    } else if (threadIdx.x == 0 && (uint64_t)off + n > fp.cand_cap) {
+      atomicAdd(&fp.counters[C_CAND_OVERFLOW], 1u);
+    }
+    __syncthreads();
+    const uint32_t c = threadIdx.x;
+    const uint32_t cnt = scnt[c];
+    uint32_t total;
+    uint32_t ex = block_exclusive<uint32_t>(cnt, &total, lds4);
+    fp.cell_counts[page * kPage + c] = cnt;
+    fp.cell_offsets[page * kPage + c] = ex;
+    if (cnt) atomicOr(&colmask, 1u << (c & 15u));
+    __syncthreads();
+    if (c == 0) {
+      fp.page_totals[page] = total;
+      fp.page_xy_mask[page] = colmask;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
+  __shared__ uint32_t rows[kPage * 8];  // bit (cell, lane) of the current chunk
+  __shared__ uint32_t cursor[kPage];
+  __shared__ uint32_t offs[kPage];
+  const uint32_t P = fp.counters[C_PAGES];
+  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
+  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+    const sr_fine_page_meta pm = fp.page_meta[page];
+    if (pm.seg_ref_count == 0u) continue;  // empty or overflowed (uniform)
+    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
+    const uint32_t off = fp.cand_off[page];
+    const uint32_t n = fp.cand_off[page + 1] - off;
+    __syncthreads();
+    cursor[lane] = 0u;
+    offs[lane] = fp.cell_offsets[page * kPage + lane];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
+    __syncthreads();
+    for (uint32_t cb = 0; cb < n; cb += 256u) {
+      const uint32_t i = cb + lane;
+      const bool has = i < n;
+      const uint32_t task = has ? fp.cands[off + i] : 0u;
+      if (has) trace_in_page(fp, task, pm, fi, [&](uint32_t cell) { atomicOr(&rows[cell * 8 + word], bit); });
+      __syncthreads();
+      if (has) {
+        const sr_binning_task tk = fp.tasks[task];
+        const InstRec& in = fp.inst[tk.id_info];
+        sr_fine_seg_ref ref{tk.id_info, tk.seg_idx, 0u};
+        if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
+        trace_in_page(fp, task, pm, fi, [&](uint32_t cell) {
+          const uint32_t* row = rows + cell * 8;
+          uint32_t rank = cursor[cell];
+          for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
+          rank += __popc(row[word] & (bit - 1u));
+          uint64_t dst = (uint64_t)pm.seg_ref_base + offs[cell] + rank;
+          if (dst < fp.ref_cap) fp.refs[dst] = ref;
+        });
+      }
+      __syncthreads();
+      uint32_t add = 0;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) {
+        add += __popc(rows[lane * 8 + w]);
+        rows[lane * 8 + w] = 0u;
+      }
+      cursor[lane] += add;
+      __syncthreads();
+    }
+  }
+}
 
 // ------------------------------------------------ canonical cell order
 // Sorts every cell's references by (inst, seg).  The mirror-bitonic network
@@ -834,33 +1128,49 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   const uint32_t kMaxGrid = 256u * 16u;
   const uint64_t slots_total = (uint64_t)fp.F * fp.slots;
   const uint32_t ptab = fp.coarse_total * kSlices;
+  const bool csr = fp.st.fine_binning_backend != 0u;
 
   timer->mark(s, "broad");
   hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 64, 1024)), dim3(64), 0, s, fp);
-  timer->mark(s, "project");
+  timer->mark(s, "depth_lut");
+  hipLaunchKernelGGL(k_lut, dim3(grid_for(fp.coarse_total, 64, 1u << 20)), dim3(64), 0, s, fp);
+  timer->mark(s, "project_mark");
   hipLaunchKernelGGL(k_project_mark, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  timer->mark(s, "project_compact");
   launch_scan<uint32_t>(MaskSrc{fp.slot_masks}, WaveOffsetDst{fp.wave_offsets, fp.counters},
                         (uint32_t)(slots_total / 64u), nullptr, ps.partials, s);
   hipLaunchKernelGGL(k_frustum_bases, dim3(1), dim3(64), 0, s, fp);
-  hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
-  timer->mark(s, "depth_lut");
-  hipLaunchKernelGGL(k_lut, dim3(grid_for(fp.coarse_total, 64, 1u << 20)), dim3(64), 0, s, fp);
-  timer->mark(s, "mark");
-  hipLaunchKernelGGL((k_visit<MODE_MARK>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
   timer->mark(s, "allocate");
-  launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap}, ptab,
-                        nullptr, ps.partials, s);
-  hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp);
-  timer->mark(s, "count");
-  hipLaunchKernelGGL((k_visit<MODE_COUNT>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
-  timer->mark(s, "prefix");
-  hipLaunchKernelGGL(k_page_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp, ps.page_xy_mask);
-  launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, ps.page_xy_mask}, fp.page_cap, fp.counters + C_PAGES,
-                        ps.partials, s);
-  timer->mark(s, "fill");
-  hipLaunchKernelGGL((k_visit<MODE_FILL>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
-  timer->mark(s, "sort_cells");
-  hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  launch_scan<uint32_t>(PageMarkSrc{fp.page_table},
+                        PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap,
+                                     csr ? fp.cand_count_pt : nullptr, fp.cand_cursor},
+                        ptab, nullptr, ps.partials, s);
+  if (csr)
+    launch_scan<uint32_t>(CandSrc{fp.cand_cursor}, CandDst{fp.cand_off, fp.cand_cursor, fp.counters}, fp.page_cap,
+                          fp.counters + C_PAGES, ps.partials, s);
+  timer->mark(s, "project_write");
+  hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  if (csr) {
+    timer->mark(s, "csr_count");
+    hipLaunchKernelGGL(k_csr_count, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+    timer->mark(s, "page_bases");
+    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,
+                          fp.counters + C_PAGES, ps.partials, s);
+    timer->mark(s, "csr_fill");
+    hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  } else {
+    hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    timer->mark(s, "count");
+    hipLaunchKernelGGL((k_visit<MODE_COUNT>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    timer->mark(s, "prefix");
+    hipLaunchKernelGGL(k_page_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp, fp.page_xy_mask);
+    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,
+                          fp.counters + C_PAGES, ps.partials, s);
+    timer->mark(s, "fill");
+    hipLaunchKernelGGL((k_visit<MODE_FILL>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    timer->mark(s, "sort_cells");
+    hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  }
   timer->mark(s, "compact");
   launch_scan<uint32_t>(FlagSrc{fp.active_flags}, ActiveDst{fp.active_queue, fp.counters},
                         fp.coarse_total * kExt * kExt, nullptr, ps.partials, s);

@@ -130,6 +130,7 @@ struct ShadowBatch {
   float p0x[TPX], p0y[TPX], p0z[TPX];
   float p1x[TPX], p1y[TPX], p1z[TPX], l2[TPX];
   float rmin[TPX], rmax[TPX], absorb[TPX];
+  float mnx[TPX], mny[TPX], mxx[TPX], mxy[TPX];
   uint32_t item[TPX];
 };
 
@@ -157,6 +158,12 @@ __device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, con
   b.rmax[k] = fmaxf(m.max_radius_pixels, minr);
   b.absorb[k] = m.absorption_color[3];
   b.item[k] = w;
+  // conservative pixel box: outside it the distance to the segment exceeds
+  // the largest radius even with the rounding of mix() at 8K coordinates,
+  // so the coverage test below would return false
+  float pad = b.rmax[k] + 4e-3f;
+  b.mnx[k] = fminf(ra.x, rc.x) - pad; b.mny[k] = fminf(ra.y, rc.y) - pad;
+  b.mxx[k] = fmaxf(ra.x, rc.x) + pad; b.mxy[k] = fmaxf(ra.y, rc.y) + pad;
 }
 
 // fragment_position_line_relative + coverage (strand_rasterizer.wgsl:138-147,
@@ -164,6 +171,7 @@ __device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, con
 template <int TPX>
 __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t k, float px, float py, float* pz,
                                              float* cov) {
+  if (px < b.mnx[k] || py < b.mny[k] || px > b.mxx[k] || py > b.mxy[k]) return false;
   float x0 = b.p0x[k], y0 = b.p0y[k], z0 = b.p0z[k];
   float x1 = b.p1x[k], y1 = b.p1y[k], l2 = b.l2[k];
   float t = 0.0f;

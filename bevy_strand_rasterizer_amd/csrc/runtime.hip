@@ -134,14 +134,15 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_FRUSTA, B_INSTS, B_LIGHTS, B_COUNT
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_COUNT
 };
 
 // Shape of the last submitted frame, for dumps.
 struct FrameShape {
   bool valid = false;
   uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
-  uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, atlas_texels = 0, dom_layers = 0;
+  uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, cand_cap = 0, atlas_texels = 0, dom_layers = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
   int hist_cur = 0;
 };
@@ -167,7 +168,8 @@ struct sr_context {
   StageTimer timer;
   hipEvent_t done = nullptr;
   bool submitted = false;
-  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0;
+  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0, cand_cap_hint = 0;
+  uint32_t flags = 0;  // SR_CONFIG_* bits
   int hist_cur = 0;
   uint64_t hbm_budget = 0;
 };
@@ -240,6 +242,7 @@ int sr_create(const sr_device_config* cfg, sr_context** out) {
   ctx->device = dev;
   if (cfg && cfg->slab_bytes) ctx->slabs.slab_bytes = cfg->slab_bytes;
   if (cfg) ctx->hbm_budget = cfg->hbm_budget_bytes;
+  if (cfg) ctx->flags = cfg->flags;
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming)) != hipSuccess) {
     std::string m = std::string("sr_create: ") + hipGetErrorString(e);
@@ -457,6 +460,12 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   uint64_t ref_cap64 = std::max<uint64_t>({(uint64_t)task_cap * 8ull, 1ull << 16, (uint64_t)ctx->ref_cap_hint});
   const uint32_t ref_cap = (uint32_t)std::min<uint64_t>(ref_cap64, 0x7FFFFFFFull);
   const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, ctx->work_cap_hint});
+  // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
+  const bool csr = st.fine_binning_backend != 0u;
+  const uint32_t cand_cap = csr ? (uint32_t)std::min<uint64_t>(
+                                      std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)ctx->cand_cap_hint}),
+                                      0x7FFFFFFFull)
+                                : 0u;
   const uint32_t atlas_w = std::min(std::max(max_idx, 1u), SR_MAX_TEXTURE_EXTENT);
   const uint32_t atlas_h = std::min(std::max((max_idx + atlas_w - 1) / atlas_w, 1u), SR_MAX_TEXTURE_EXTENT);
   const uint32_t atlas_texels = atlas_w * atlas_h;
@@ -484,7 +493,9 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
       {B_DOM_OPACITY, 4ull * dom_px * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
       {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
       {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
-      {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_FRUSTA, sizeof(FrustumRec) * F},
+      {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_CAND_PT, csr ? 4ull * ptab : 0ull},
+      {B_CAND_OFF, csr ? 4ull * (page_cap + 1ull) : 0ull}, {B_CAND_CURSOR, csr ? 4ull * page_cap : 0ull},
+      {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
   bool grow = false;
   for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > ctx->buf[n.id].bytes;
@@ -564,6 +575,14 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   PrepassScratch ps{ctx->buf[B_SLOT_PROJ].as<sr_projected_segment>(), ctx->buf[B_PARTIALS].as<uint32_t>(),
                     ctx->buf[B_PARTIALS4].as<uint4>(), ctx->buf[B_XY_MASK].as<uint32_t>()};
   fp.page_cap = page_cap;
+  fp.page_xy_mask = ps.page_xy_mask;
+  fp.cand_cap = cand_cap;
+  fp.cand_lds_limit = (ctx->flags & SR_CONFIG_CSR_GLOBAL_SORT) ? 0u : 0xFFFFFFFFu;
+  fp.cand_count_pt = ctx->buf[B_CAND_PT].as<uint32_t>();
+  fp.cand_off = ctx->buf[B_CAND_OFF].as<uint32_t>();
+  fp.cand_cursor = ctx->buf[B_CAND_CURSOR].as<uint32_t>();
+  fp.cands = ctx->buf[B_CANDS].as<uint32_t>();
+  fp.cands_tmp = ctx->buf[B_CANDS_TMP].as<uint32_t>();
 
   ctx->timer.begin();
   ctx->timer.mark(s, "clear");
@@ -597,6 +616,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   sh.valid = true;
   sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
+  sh.cand_cap = cand_cap;
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
   sh.hist_cur = hist_next;
@@ -623,7 +643,7 @@ int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
   fc.work_items = c[C_WORK];
   fc.camera_runs = c[C_CAM_RUNS];
   fc.shadow_runs = c[C_SHADOW_RUNS];
-  fc.overflow_pages = c[C_OVERFLOW];
+  fc.overflow_pages = c[C_OVERFLOW] + c[C_CAND_OVERFLOW];
   fc.coarse_tiles = sh.coarse_total;
   fc.task_capacity = sh.task_cap;
   fc.ref_capacity = sh.ref_cap;
@@ -631,7 +651,9 @@ int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
   if (c[C_REFS] > sh.ref_cap) ctx->ref_cap_hint = next_pow2(c[C_REFS]);
   if (c[C_PAGE_DROPS] > 0) ctx->page_cap_hint = next_pow2(c[C_MARKED]);
   if (c[C_WORK] > sh.work_cap) ctx->work_cap_hint = next_pow2(c[C_WORK]);
-  fc.pad[0] = c[C_PAGE_DROPS];
+  if (sh.cand_cap && c[C_CANDS] > sh.cand_cap) ctx->cand_cap_hint = next_pow2(c[C_CANDS]);
+  fc.page_drops = c[C_PAGE_DROPS];
+  fc.page_candidates = sh.cand_cap ? c[C_CANDS] : 0u;
   if (out) *out = fc;
   return SR_OK;
 }

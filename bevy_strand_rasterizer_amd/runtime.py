@@ -15,10 +15,12 @@ from . import abi
 
 
 class StrandRenderer:
-    def __init__(self, device: int = 0, slab_bytes: int = 0, hbm_budget: int = 0, profiling: bool = False):
+    def __init__(self, device: int = 0, slab_bytes: int = 0, hbm_budget: int = 0, profiling: bool = False,
+                 flags: int = 0):
         self.lib = abi.load_library()
         cfg = abi.DeviceConfig()
         cfg.device_index = device
+        cfg.flags = flags
         cfg.slab_bytes = slab_bytes
         cfg.hbm_budget_bytes = hbm_budget
         h = C.c_void_p()

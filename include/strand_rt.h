@@ -399,7 +399,9 @@ typedef struct sr_frame_counters {
   uint32_t coarse_tiles;        /* total coarse tiles over all frusta */
   uint32_t task_capacity;
   uint32_t ref_capacity;
-  uint32_t pad[4];
+  uint32_t page_drops;          /* marked pages past the page capacity (dropped, as the reference) */
+  uint32_t page_candidates;     /* (task, page) pairs of the page CSR backend; 0 for the cell-atomic backend */
+  uint32_t pad[2];
 } sr_frame_counters;
 SR_STATIC_ASSERT(sizeof(sr_frame_counters) == 64, "sr_frame_counters is 64 bytes");
 
@@ -440,10 +442,15 @@ typedef struct sr_context sr_context;
 
 typedef struct sr_device_config {
   int32_t device_index;     /* HIP device ordinal */
-  uint32_t flags;           /* reserved, 0 */
+  uint32_t flags;           /* SR_CONFIG_* bits */
   uint64_t slab_bytes;      /* size of each asset paging slab; 0 = default */
   uint64_t hbm_budget_bytes;/* 0 = no budget */
 } sr_device_config;
+
+/* sr_device_config.flags: the page CSR backend sorts every page candidate
+ * list with its global radix path instead of only lists longer than its LDS
+ * list (exercises that path on small frames; results are identical). */
+#define SR_CONFIG_CSR_GLOBAL_SORT 1u
 
 /* Library version string. */
 const char* sr_version(void);

@@ -41,7 +41,7 @@ STRUCTS = [
 
 
 # numpy field -> C member expression where the C struct uses an array
-C_MEMBER = {("sr_frame_counters", f"pad{i}"): f"pad[{i}]" for i in range(4)}
+C_MEMBER = {("sr_frame_counters", f"pad{i}"): f"pad[{i}]" for i in range(2)}
 
 
 @pytest.fixture(scope="module")
