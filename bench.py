@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="named", choices=["named", "small", "batched"])
-    ap.add_argument("--backend", default="scatter", choices=["scatter", "csr"],
+    ap.add_argument("--backend", default="csr", choices=["scatter", "csr"],
                     help="fine binning backend (FineBinningBackend::SegmentScatter / PageCsr)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)

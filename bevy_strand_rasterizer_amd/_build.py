@@ -17,7 +17,7 @@ LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libstrand_rt.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["runtime.hip", "prepass.hip", "render.hip", "scene.cpp"]
+SOURCES = ["runtime.hip", "prepass.hip", "render.hip", "scene.cpp", "strand_cache.cpp"]
 
 # -ffp-contract=off: the binning arithmetic must round exactly like WGSL.
 # -packed-fp32-ops off: packed FP32 forms are avoided on this hardware pool.

@@ -146,6 +146,10 @@ class ScalpParams(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
+strand_cache_info_dtype = np.dtype([("version", "<u4"), ("scale", "<f4"), ("vertex_count", "<u8"),
+                                    ("strand_count", "<u8"), ("index_count", "<u8"),
+                                    ("max_segments_in_strand", "<u4"), ("format", "<u4"), ("aabb", aabb_dtype)])
+
 _SIGNATURES = {
     "sr_version": (C.c_char_p, []),
     "sr_create": (C.c_int, [C.POINTER(DeviceConfig), C.POINTER(C.c_void_p)]),
@@ -164,6 +168,8 @@ _SIGNATURES = {
     "sr_build_frustum_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "sr_camera_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "sr_light_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]),
+    "sr_strand_cache_parse": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_char_p, C.c_uint64]),
 }
 
 _lib = None

@@ -125,12 +125,13 @@ __device__ __forceinline__ uint32_t item_of(const RunTable<TPX>& t, uint32_t i) 
 }
 
 // --------------------------------------------------------- K25 shadow
+// One reference of a shadow batch, as three 16-byte LDS records so a pixel
+// reads it with three broadcast loads.
 template <int TPX>
 struct ShadowBatch {
-  float p0x[TPX], p0y[TPX], p0z[TPX];
-  float p1x[TPX], p1y[TPX], p1z[TPX], l2[TPX];
-  float rmin[TPX], rmax[TPX], absorb[TPX];
-  float mnx[TPX], mny[TPX], mxx[TPX], mxy[TPX];
+  float4 a[TPX];  // p0.x, p0.y, p0.z (depth01), |p1 - p0|^2 in xy
+  float4 c[TPX];  // p1.x, p1.y, p1.z, absorption alpha
+  float4 r[TPX];  // min radius, max radius, p1.x - p0.x, p1.y - p0.y
   uint32_t item[TPX];
 };
 
@@ -150,20 +151,11 @@ __device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, con
   float minr = fmaxf(m.min_radius_pixels, 1e-4f);
   float z0 = normalize_depth01(ra.z), z1 = normalize_depth01(rc.z);
   float abx = rc.x - ra.x, aby = rc.y - ra.y;
-  float l2 = abx * abx + aby * aby;
-  b.p0x[k] = ra.x; b.p0y[k] = ra.y; b.p0z[k] = z0;
-  b.p1x[k] = rc.x; b.p1y[k] = rc.y; b.p1z[k] = z1;
-  b.l2[k] = l2;  // divided per pixel: t = dot / l2 as the reference
-  b.rmin[k] = minr;
-  b.rmax[k] = fmaxf(m.max_radius_pixels, minr);
-  b.absorb[k] = m.absorption_color[3];
+  float l2 = abx * abx + aby * aby;  // divided per pixel: t = dot / l2 as the reference
+  b.a[k] = make_float4(ra.x, ra.y, z0, l2);
+  b.c[k] = make_float4(rc.x, rc.y, z1, m.absorption_color[3]);
+  b.r[k] = make_float4(minr, fmaxf(m.max_radius_pixels, minr), abx, aby);
   b.item[k] = w;
-  // conservative pixel box: outside it the distance to the segment exceeds
-  // the largest radius even with the rounding of mix() at 8K coordinates,
-  // so the coverage test below would return false
-  float pad = b.rmax[k] + 4e-3f;
-  b.mnx[k] = fminf(ra.x, rc.x) - pad; b.mny[k] = fminf(ra.y, rc.y) - pad;
-  b.mxx[k] = fmaxf(ra.x, rc.x) + pad; b.mxy[k] = fmaxf(ra.y, rc.y) + pad;
 }
 
 // fragment_position_line_relative + coverage (strand_rasterizer.wgsl:138-147,
@@ -171,14 +163,12 @@ __device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, con
 template <int TPX>
 __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t k, float px, float py, float* pz,
                                              float* cov) {
-  if (px < b.mnx[k] || py < b.mny[k] || px > b.mxx[k] || py > b.mxy[k]) return false;
-  float x0 = b.p0x[k], y0 = b.p0y[k], z0 = b.p0z[k];
-  float x1 = b.p1x[k], y1 = b.p1y[k], l2 = b.l2[k];
+  const float4 A = b.a[k], Cc = b.c[k], Rr = b.r[k];
   float t = 0.0f;
-  if (!(l2 == 0.0f)) t = ((px - x0) * (x1 - x0) + (py - y0) * (y1 - y0)) / l2;
+  if (!(A.w == 0.0f)) t = ((px - A.x) * Rr.z + (py - A.y) * Rr.w) / A.w;
   if (t < 0.0f || t > 1.0f) return false;
-  float qx = wmix(x0, x1, t), qy = wmix(y0, y1, t), qz = wmix(z0, b.p1z[k], t);
-  float r = wmix(b.rmin[k], b.rmax[k], wclamp(qz, 0.0f, 1.0f));
+  float qx = wmix(A.x, Cc.x, t), qy = wmix(A.y, Cc.y, t), qz = wmix(A.z, Cc.z, t);
+  float r = wmix(Rr.x, Rr.y, wclamp(qz, 0.0f, 1.0f));
   float dx = px - qx, dy = py - qy;
   float c = wclamp(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f, 1.0f);
   *pz = qz;
@@ -186,6 +176,11 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
   return c > 0.0f;
 }
 
+// K25 (strand_shadow_raster.wgsl): per pixel of a light tile, pass 1 finds
+// the front depth z0 (max depth over the covering references of the nearest
+// work item with a hit), pass 2 accumulates each covering reference's alpha
+// into the two DOM layers around its relative depth (per-pixel partials in
+// LDS, indexed by layer).
 template <int TPX>
 __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
   __shared__ RunTable<TPX> t;
@@ -207,8 +202,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
     const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
     const float pfx = (float)pxu + 0.5f, pfy = (float)pyu + 0.5f;
-    // pass 1: front depth = max depth over the covering references of the
-    // first (nearest) work item with any hit
+    // pass 1
     float z0 = -1.0f;
     bool found = !active, hit = false;
     uint32_t cur = 0;
@@ -250,11 +244,11 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
             float tL = u * (float)L;
             uint32_t si = min(f2u(floorf(tL)), L - 1u);
             float wgt = tL - floorf(tL);
-            float a = cov * b.absorb[k] * 0.5f;
-            float a0 = part_px[si];
+            const float a = cov * b.c[k].w * 0.5f;
+            const float a0 = part_px[si];
             part_px[si] = wclamp(a0 + (1.0f - a0) * (1.0f - wgt) * a, 0.0f, 1.0f);
             if (si + 1u < L) {
-              float a1 = part_px[si + 1u];
+              const float a1 = part_px[si + 1u];
               part_px[si + 1u] = wclamp(a1 + (1.0f - a1) * wgt * a, 0.0f, 1.0f);
             }
           }
@@ -266,16 +260,11 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
       const size_t pix = (size_t)pyu * fr.d.screen_width + pxu;
       float* dd = fp.dom_depth + fr.dom_depth_off;
       float* op = fp.dom_opacity + fr.dom_opacity_off;
-      if (!found) {
-        dd[pix] = 0.0f;
-        for (uint32_t i = 0; i < L; ++i) op[i * plane + pix] = 0.0f;
-      } else {
-        dd[pix] = z0;
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < L; ++i) {
-          acc = acc + (1.0f - acc) * part_px[i];
-          op[i * plane + pix] = acc;
-        }
+      dd[pix] = found ? z0 : 0.0f;
+      float acc = 0.0f;
+      for (uint32_t i = 0; i < L; ++i) {
+        acc = acc + (1.0f - acc) * part_px[i];
+        op[i * plane + pix] = found ? acc : 0.0f;
       }
     }
   }

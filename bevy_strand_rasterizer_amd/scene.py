@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -175,6 +176,65 @@ def asset_from_strands(strands: list, material: dict | None = None, radius_px: f
     for k, val in m.items():
         a.materials[k] = val
     return a
+
+
+class StrandCacheError(ValueError):
+    """A .strands cache that the loader rejects (message as the reference's)."""
+
+
+def parse_strand_cache(data: bytes):
+    """Parses + validates a .strands cache (binary or JSON) through the
+    native loader (StrandCacheAssetLoader, src/strand_cache.rs:223-232).
+    Returns (info record, vertices float4[V] scaled with w = 1,
+    StrandMeta[N], indices u32[I])."""
+    lib = abi.load_library()
+    buf = C.create_string_buffer(bytes(data), len(data))
+    info = np.zeros(1, abi.strand_cache_info_dtype)
+    err = C.create_string_buffer(512)
+    rc = lib.sr_strand_cache_parse(buf, len(data), abi.ptr(info), None, None, None, err, 512)
+    if rc != abi.SR_OK:
+        raise StrandCacheError(err.value.decode())
+    i = info[0]
+    verts = np.zeros((int(i["vertex_count"]), 4), np.float32)
+    meta = np.zeros(int(i["strand_count"]), abi.strand_meta_dtype)
+    idx = np.zeros(int(i["index_count"]), np.uint32)
+    rc = lib.sr_strand_cache_parse(buf, len(data), abi.ptr(info), abi.ptr(verts), abi.ptr(meta),
+                                   abi.ptr(idx) if idx.size else None, err, 512)
+    if rc != abi.SR_OK:
+        raise StrandCacheError(err.value.decode())
+    return info[0], verts, meta, idx
+
+
+def asset_from_cache(data: bytes, material: dict | None = None, radius_px: float = 1.0) -> Asset:
+    """set_cached_strand_geometry (src/plugin.rs:1468-1490): the parsed cache
+    packed into slabs with one material (every StrandMeta.material_idx = 0)."""
+    info, verts, meta, idx = parse_strand_cache(data)
+    a = Asset(vertices=np.ascontiguousarray(verts), indices=idx, meta=meta,
+              geo=np.zeros(1, abi.strand_geo_dtype), materials=np.zeros(1, abi.strand_material_dtype))
+    g = a.geo[0]
+    g["strand_count"] = len(meta)
+    g["max_segments_in_strand"] = int(info["max_segments_in_strand"])
+    g["aabb"] = info["aabb"]
+    a.geo[0] = g
+    m = default_material()
+    if material:
+        m.update(material)
+    m["min_radius_pixels"] = m.get("min_radius_pixels", radius_px)
+    m["max_radius_pixels"] = m.get("max_radius_pixels", radius_px)
+    for k, val in m.items():
+        a.materials[k] = val
+    return a
+
+
+def strand_cache_bytes(vertices, strands_count_offset, indices, scale: float = 1.0) -> bytes:
+    """The binary cache layout written by the reference's converter
+    (tools/alembic_to_strands.py:146-162): magic, '<IfQQQ' header, then the
+    vertex, (count, offset) and index payloads, little-endian."""
+    v = np.ascontiguousarray(vertices, dtype="<f4").reshape(-1, 3)
+    so = np.ascontiguousarray(strands_count_offset, dtype="<u4").reshape(-1, 2)
+    ix = np.ascontiguousarray(indices, dtype="<u4").reshape(-1)
+    head = b"SSRSTRD\0" + struct.pack("<IfQQQ", 1, scale, v.shape[0], so.shape[0], ix.shape[0])
+    return head + v.tobytes() + so.tobytes() + ix.tobytes()
 
 
 def default_material() -> dict:

@@ -529,6 +529,32 @@ int sr_camera_view(const float* eye3, const float* target3, const float* up3, fl
 int sr_light_view(const float* direction_to_light3, const float* center3, float radius,
                   sr_view* out);
 
+/* ------------------------------------------------------------------------ */
+/* .strands cache (src/strand_cache.rs).                                     */
+typedef struct sr_strand_cache_info {
+  uint32_t version;                /* 1 */
+  float scale;                     /* applied to the vertices written */
+  uint64_t vertex_count;
+  uint64_t strand_count;
+  uint64_t index_count;
+  uint32_t max_segments_in_strand; /* max vertex count of a strand (src/plugin.rs:1373) */
+  uint32_t format;                 /* 0 = binary "SSRSTRD\0", 1 = JSON */
+  sr_aabb aabb;                    /* over every scaled vertex */
+} sr_strand_cache_info;
+SR_STATIC_ASSERT(sizeof(sr_strand_cache_info) == 72, "sr_strand_cache_info is 72 bytes");
+
+/* Parse, validate and pack a .strands cache held in memory (replaces
+ * StrandCacheAssetLoader::load, src/strand_cache.rs:223-232, with
+ * parse_cache :165-171 and validate_cache :174-209, plus the packing of
+ * set_cached_strand_geometry / pack_indexed_strands, src/plugin.rs:1468-1490,
+ * 1357-1393).  Call with NULL outputs to read the counts into `info`, then
+ * with host arrays of vertex_count float4 (x, y, z scaled, w = 1),
+ * strand_count sr_strand_meta {count, offset, material 0} and index_count
+ * uint32.  Errors return SR_ERR_INVALID_ARGUMENT with the reference's message
+ * in `error`. */
+int sr_strand_cache_parse(const void* bytes, uint64_t size, sr_strand_cache_info* info, float* vertices,
+                          sr_strand_meta* meta, uint32_t* indices, char* error, uint64_t error_capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,18 @@ def csr(w):
     return w
 
 
+def cache_workload():
+    """A scalp written as a .strands cache and loaded back (one material)."""
+    base = scene.scalp_asset(300, 8, 11, 0.002, 2.0)
+    so = np.stack([base.meta["count"], base.meta["offset"]], axis=1)
+    data = scene.strand_cache_bytes(base.vertices[:, :3], so, base.indices, scale=1.0)
+    a = scene.asset_from_cache(data, radius_px=2.0)
+    eye = (scene.HEAD_CENTER[0], scene.HEAD_CENTER[1] + 0.05, scene.HEAD_CENTER[2] + 0.6)
+    return scene.build_workload("cache-300x8-96", a, [(eye, 96, 96)], 1, 96, 4, 8, 8)
+
+
+CASES["cache-asset"] = cache_workload
+
 CSR_CASES = {
     "tiny-f8-csr": lambda: csr(scene.config_tiny()),
     "tiny-2lights-8layers-csr": lambda: csr(scene.config_tiny(strands=40, lights=2, dom_layers=8, res=72)),
