@@ -23,6 +23,7 @@ INVALID_PTR = 0xFFFFFFFF
 MAX_DOM_LAYERS = 16
 BINNING_FRUSTUM_SHIFT = 3
 CONFIG_CSR_GLOBAL_SORT = 1  # sr_device_config.flags
+CONFIG_CSR_FEW_HITS = 2
 
 # sr_status
 SR_OK = 0

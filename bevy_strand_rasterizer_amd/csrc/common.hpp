@@ -81,7 +81,9 @@ enum Counter : uint32_t {
   C_SHADOW_HEAD = 12, // shadow raster run queue head
   C_CANDS = 13,       // page candidates (CSR backend)
   C_CAND_OVERFLOW = 14,
-  C_COUNT = 16
+  C_CSR_COUNT_HEAD = 15,  // page queue heads of the CSR count and fill kernels
+  C_CSR_FILL_HEAD = 16,
+  C_COUNT = 20
 };
 
 struct FrameParams {
@@ -133,6 +135,14 @@ struct FrameParams {
   uint32_t* cands;                   // [cand_cap] task ids, sorted per page
   uint32_t* cands_tmp;               // [cand_cap] radix-sort ping-pong for long lists
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
+  // fine cells each slot's segment hits, recorded while marking (CSR backend):
+  // slot_hits[s * hit_k + j] = page-table entry << 8 | cell, in traversal
+  // order; slot_hit_count[s] = count, or kHitOverflow past hit_k (those
+  // slots are traced again).  hit_k = 0 disables recording.
+  const sr_projected_segment* slot_proj;  // [F * slots]
+  uint32_t* slot_hits;
+  uint8_t* slot_hit_count;
+  uint32_t hit_k, pad2;
   // shadow / shading / raster
   float* dom_depth;
   float* dom_opacity;

@@ -353,12 +353,22 @@ __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint3
   *bmax = V3{fmaxf(origin->x, tmx), fmaxf(origin->y, tmy), 1.0f};
 }
 
+constexpr uint32_t kHitOverflow = 0xFFu;
+
+// Cells recorded for one slot while marking (CSR backend).
+struct HitRec {
+  uint32_t* dst;  // null: not recording
+  uint32_t n;
+  uint32_t cap;
+};
+
 // trace_segment_into_coarse_count_page (strand_prepass.wgsl:1412-1575).
-// MARK also counts page candidates for the CSR backend; SCATTER appends the
-// task to each touched page's candidate list.
+// MARK also counts page candidates for the CSR backend and records the fine
+// cells of every touched page; SCATTER appends the slot to each touched
+// page's candidate list.
 template <int MODE>
 __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
-                                  uint32_t local, const sr_fine_seg_ref& ref, uint32_t task) {
+                                  uint32_t local, const sr_fine_seg_ref& ref, uint32_t task, HitRec& rec) {
   V2 origin;
   V3 bmax;
   coarse_tile_box(fd, local, &origin, &bmax);
@@ -377,7 +387,19 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     uint32_t pti = tile * kSlices + cz;
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
-      if (fp.st.fine_binning_backend != 0u) atomicAdd(&fp.cand_count_pt[pti], 1u);
+      if (fp.st.fine_binning_backend != 0u) {
+        if (rec.dst) {
+          // entries with a recorded first cell are counted after the walk
+          // from the record; the rest (past the record) here
+          if (rec.n >= rec.cap) atomicAdd(&fp.cand_count_pt[pti], 1u);
+          fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, [&](uint32_t cell) {
+            if (rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
+            ++rec.n;
+          });
+        } else {
+          atomicAdd(&fp.cand_count_pt[pti], 1u);
+        }
+      }
       continue;
     }
     uint32_t h = fp.page_table[pti];
@@ -411,7 +433,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
 // trace (visit_segment_asset_coarse_tiles, strand_prepass.wgsl:1591-1696).
 template <int MODE>
 __device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V3 p0, V3 p1,
-                              const sr_fine_seg_ref& ref, uint32_t task) {
+                              const sr_fine_seg_ref& ref, uint32_t task, HitRec& rec) {
   const sr_frustum_desc& fd = fp.fr[fi].d;
   const sr_coarse_asset_range r = fp.ranges[(size_t)ii * fp.F + fi];
   if (r.geo_id != ii || r.frustum_id != fi) return;
@@ -449,7 +471,7 @@ __device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V
     if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
     uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
     if (local < fd.coarse_depth_tile_count)
-      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref, task);
+      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref, task, rec);
     if (f[0] == f1[0] && f[1] == f1[1]) break;
     if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
     else { f[1] += step[1]; tmax[1] += delta[1]; }
@@ -481,7 +503,8 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     const InstRec& in = fp.inst[ii];
     if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
   }
-  visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t);
+  HitRec none{nullptr, 0u, 0u};
+  visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t, none);
 }
 
 template <int MODE>
@@ -490,22 +513,51 @@ __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) visit_task<MODE>(fp, t);
 }
 
+// The distinct page-table entries of a slot's recorded cells, in order (the
+// cells of one entry are consecutive).
+struct EntryCursor {
+  const uint32_t* h;
+  uint32_t j, n, prev;
+  __device__ uint32_t next() {
+    while (j < n) {
+      const uint32_t v = h[j++] >> 8;
+      if (v != prev) {
+        prev = v;
+        return v;
+      }
+    }
+    return kInvalid;
+  }
+};
+
 // Pass 1: per slot validity ballot + packed endpoints in slot layout; valid
-// slots also mark their pages (the LUT is built before projection).
+// slots also mark their pages (the LUT is built before projection) and, for
+// the CSR backend, record their fine cells and count page candidates.
 __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
   const uint64_t total = (uint64_t)fp.F * fp.slots;
+  const bool record = fp.st.fine_binning_backend != 0u && fp.hit_k != 0u;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
     bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
+    uint32_t recorded = 0;
     if (ok) {
       slot_proj[s] = ps;
       // K10: mark the (coarse tile, LUT slice) pages the segment touches,
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u);
+      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k};
+      visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
+      if (record) {
+        fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
+        recorded = min(rec.n, rec.cap);
+      }
+    }
+    if (record) {  // K10 candidate counts of the recorded entries
+      EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, recorded, kInvalid};
+      for (uint32_t k = cur.next(); k != kInvalid; k = cur.next()) atomicAdd(&fp.cand_count_pt[k], 1u);
     }
     unsigned long long m = __ballot(ok);
     if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
@@ -542,28 +594,48 @@ __global__ void k_frustum_bases(FrameParams fp) {
 }
 
 // Pass 2: compaction with ballot ranks; the queue order is slot order.
+// CSR backend: each slot joins the candidate lists of its pages (K17).
 __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_projected_segment* slot_proj) {
   const uint64_t total = (uint64_t)fp.F * fp.slots;
   const uint32_t lane = threadIdx.x & 63;
+  const unsigned long long lower = (1ull << lane) - 1ull;
+  const bool csr = fp.st.fine_binning_backend != 0u;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    unsigned long long m = fp.slot_masks[s >> 6];
-    if (!((m >> lane) & 1ull)) continue;
-    uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    uint32_t t = fp.wave_offsets[s >> 6] + rank;
-    if (t >= fp.task_cap) continue;
-    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
-    uint32_t ii = find_instance(fp, pos);
-    const InstRec& in = fp.inst[ii];
-    uint32_t p = pos - in.pos_base;
-    const FrustumRec& fr = fp.fr[fi];
-    fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
-    const sr_projected_segment ps = slot_proj[s];
-    fp.proj[t] = ps;
-    if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
-    if (fp.st.fine_binning_backend != 0u) {  // K17: append to the candidate lists of the touched pages
-      V3 p0, p1;
-      task_endpoints(fr.d, ps, &p0, &p1);
-      visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, t);
+    const unsigned long long m = fp.slot_masks[s >> 6];
+    const uint32_t t = fp.wave_offsets[s >> 6] + (uint32_t)__popcll(m & lower);
+    const bool valid = ((m >> lane) & 1ull) && t < fp.task_cap;
+    uint32_t recorded = 0;
+    if (valid) {
+      uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+      uint32_t ii = find_instance(fp, pos);
+      const InstRec& in = fp.inst[ii];
+      uint32_t p = pos - in.pos_base;
+      const FrustumRec& fr = fp.fr[fi];
+      fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
+      const sr_projected_segment ps = slot_proj[s];
+      fp.proj[t] = ps;
+      if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
+      if (csr) {
+        const uint32_t nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
+        if (nh != kHitOverflow) {
+          recorded = nh;
+        } else {  // too many cells to record: trace again
+          V3 p0, p1;
+          task_endpoints(fr.d, ps, &p0, &p1);
+          HitRec none{nullptr, 0u, 0u};
+          visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, none);
+        }
+      }
+    }
+    if (csr && fp.hit_k) {
+      EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, recorded, kInvalid};
+      for (uint32_t key = cur.next(); key != kInvalid; key = cur.next()) {
+        const uint32_t e = fp.page_table[key];
+        if (e == 0u || e == kMarked) continue;
+        const uint32_t page = e - 1u;
+        const uint64_t dst = (uint64_t)fp.cand_off[page] + atomicAdd(&fp.cand_cursor[page], 1u);
+        if (dst < fp.cand_cap) fp.cands[dst] = (uint32_t)s;
+      }
     }
   }
 }
@@ -665,8 +737,9 @@ struct PageBaseDst {
 // ------------------------------------------------------ page CSR backend
 // K16-K18 (prefix_page_candidates, scatter_page_candidates,
 // build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507): every
-// page owns the list of tasks that touch it.  One workgroup per page sorts
-// its candidate list (so lists are in task order), counts cells in LDS, and
+// page owns the list of slots whose segments touch it (slot order is task
+// order).  One workgroup per page sorts its candidate list, counts cells in
+// LDS, and
 // after the page-base scan fills references with stable ranks: a chunk of
 // 256 candidates sets one bit per (cell, lane) in an LDS bit matrix and each
 // hit's rank is the cell cursor plus the popcount of lower lanes, so every
@@ -690,16 +763,28 @@ struct CandDst {
   }
 };
 
-// trace_candidate_inside_page (strand_prepass.wgsl:2240-2337)
+// trace_candidate_inside_page (strand_prepass.wgsl:2240-2337): the cells of
+// page `pm` that candidate slot `slot` hits.  Recorded cells are filtered by
+// the page's table entry; slots whose recording overflowed are traced again
+// from their packed endpoints.
 template <class Emit>
-__device__ __forceinline__ void trace_in_page(const FrameParams& fp, uint32_t task, const sr_fine_page_meta& pm,
+__device__ __forceinline__ void cells_in_page(const FrameParams& fp, uint32_t slot, const sr_fine_page_meta& pm,
                                               uint32_t fi, Emit&& emit) {
-  const sr_binning_task tk = fp.tasks[task];
-  if ((tk.packed_field >> SR_BINNING_FRUSTUM_SHIFT) != fi) return;
+  const uint32_t pti = pm.coarse_tile_id * kSlices + pm.coarse_z;
+  const uint32_t nh = fp.hit_k ? fp.slot_hit_count[slot] : kHitOverflow;
+  if (nh != kHitOverflow) {
+    const uint32_t* h = fp.slot_hits + (size_t)slot * fp.hit_k;
+    for (uint32_t j = 0; j < nh; ++j) {
+      const uint32_t v = h[j];
+      if ((v >> 8) == pti) emit(v & 255u);
+    }
+    return;
+  }
+  if (slot / fp.slots != fi) return;
   const sr_frustum_desc& fd = fp.fr[fi].d;
   V3 p0, p1;
-  task_endpoints(fd, fp.proj[task], &p0, &p1);
-  const sr_coarse_depth_lut_entry e = fp.lut[pm.coarse_tile_id * kSlices + pm.coarse_z];
+  task_endpoints(fd, fp.slot_proj[slot], &p0, &p1);
+  const sr_coarse_depth_lut_entry e = fp.lut[pti];
   if (e.virtual_count_q == 0u) return;
   float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
   V2 origin;
@@ -710,6 +795,19 @@ __device__ __forceinline__ void trace_in_page(const FrameParams& fp, uint32_t ta
   Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
   if (!dc.ok) return;
   fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, emit);
+}
+
+// The reference a candidate slot contributes (make_fine_seg_ref,
+// strand_prepass.wgsl:1313-1327): instance, index position, material.
+__device__ __forceinline__ sr_fine_seg_ref slot_ref(const FrameParams& fp, uint32_t slot) {
+  const uint32_t pos = slot % fp.slots;
+  const uint32_t ii = find_instance(fp, pos);
+  const InstRec& in = fp.inst[ii];
+  const uint32_t p = pos - in.pos_base;
+  sr_fine_seg_ref ref{ii, p, 0u};
+  const uint32_t strand = in.pos_strand[p];
+  if (strand < in.meta_count) ref.material_idx = in.meta[strand].material_idx;
+  return ref;
 }
 
 constexpr uint32_t kCandLds = 8192;
@@ -772,14 +870,24 @@ __device__ uint32_t* radix_sort_global(uint32_t* a, uint32_t* b, uint32_t n, uin
   return a;
 }
 
+// Next page of a dynamic page queue (heavy pages do not pile up on one
+// workgroup as with a static stride).
+__device__ __forceinline__ uint32_t next_page(uint32_t* head, uint32_t* slot) {
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = atomicAdd(head, 1u);
+  __syncthreads();
+  return *slot;
+}
+
 __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
   __shared__ uint32_t sc[kCandLds];
   __shared__ uint32_t scnt[kPage];
   __shared__ uint32_t lds4[4];
   __shared__ uint32_t lo_hi[2];
   __shared__ uint32_t colmask;
+  __shared__ uint32_t page_slot;
   const uint32_t P = fp.counters[C_PAGES];
-  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+  for (uint32_t page; (page = next_page(&fp.counters[C_CSR_COUNT_HEAD], &page_slot)) < P;) {
     const uint32_t off = fp.cand_off[page];
     const uint32_t n = fp.cand_off[page + 1] - off;
     const sr_fine_page_meta pm = fp.page_meta[page];
@@ -819,8 +927,8 @@ __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
     __syncthreads();
     if (valid) {
       for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        uint32_t task = list[i];
-        trace_in_page(fp, task, pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
+        const uint32_t slot = list[i];
+        cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
       }
     } else if (threadIdx.x == 0 && (uint64_t)off + n > fp.cand_cap) {
       atomicAdd(&fp.counters[C_CAND_OVERFLOW], 1u);
@@ -845,9 +953,10 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
   __shared__ uint32_t rows[kPage * 8];  // bit (cell, lane) of the current chunk
   __shared__ uint32_t cursor[kPage];
   __shared__ uint32_t offs[kPage];
+  __shared__ uint32_t page_slot;
   const uint32_t P = fp.counters[C_PAGES];
   const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
-  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+  for (uint32_t page; (page = next_page(&fp.counters[C_CSR_FILL_HEAD], &page_slot)) < P;) {
     const sr_fine_page_meta pm = fp.page_meta[page];
     if (pm.seg_ref_count == 0u) continue;  // empty or overflowed (uniform)
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
@@ -862,15 +971,12 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
     for (uint32_t cb = 0; cb < n; cb += 256u) {
       const uint32_t i = cb + lane;
       const bool has = i < n;
-      const uint32_t task = has ? fp.cands[off + i] : 0u;
-      if (has) trace_in_page(fp, task, pm, fi, [&](uint32_t cell) { atomicOr(&rows[cell * 8 + word], bit); });
+      const uint32_t slot = has ? fp.cands[off + i] : 0u;
+      if (has) cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) { atomicOr(&rows[cell * 8 + word], bit); });
       __syncthreads();
       if (has) {
-        const sr_binning_task tk = fp.tasks[task];
-        const InstRec& in = fp.inst[tk.id_info];
-        sr_fine_seg_ref ref{tk.id_info, tk.seg_idx, 0u};
-        if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
-        trace_in_page(fp, task, pm, fi, [&](uint32_t cell) {
+        const sr_fine_seg_ref ref = slot_ref(fp, slot);
+        cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) {
           const uint32_t* row = rows + cell * 8;
           uint32_t rank = cursor[cell];
           for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);

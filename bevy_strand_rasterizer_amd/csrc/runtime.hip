@@ -134,7 +134,7 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -462,6 +462,11 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, ctx->work_cap_hint});
   // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
   const bool csr = st.fine_binning_backend != 0u;
+  // cells recorded per slot while marking: kHitsPerSlot words per slot, when
+  // the page-table entry fits 24 bits and the table stays within budget
+  const uint32_t kHitsPerSlot = (ctx->flags & SR_CONFIG_CSR_FEW_HITS) ? 2u : 8u;
+  const uint64_t hit_bytes = 4ull * kHitsPerSlot * task_cap;
+  const uint32_t hit_k = (csr && ptab < (1u << 24) && hit_bytes <= (8ull << 30)) ? kHitsPerSlot : 0u;
   const uint32_t cand_cap = csr ? (uint32_t)std::min<uint64_t>(
                                       std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)ctx->cand_cap_hint}),
                                       0x7FFFFFFFull)
@@ -495,7 +500,8 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
       {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_CAND_PT, csr ? 4ull * ptab : 0ull},
       {B_CAND_OFF, csr ? 4ull * (page_cap + 1ull) : 0ull}, {B_CAND_CURSOR, csr ? 4ull * page_cap : 0ull},
-      {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_FRUSTA, sizeof(FrustumRec) * F},
+      {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
+      {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
   bool grow = false;
   for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > ctx->buf[n.id].bytes;
@@ -583,6 +589,10 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   fp.cand_cursor = ctx->buf[B_CAND_CURSOR].as<uint32_t>();
   fp.cands = ctx->buf[B_CANDS].as<uint32_t>();
   fp.cands_tmp = ctx->buf[B_CANDS_TMP].as<uint32_t>();
+  fp.slot_proj = ps.slot_proj;
+  fp.hit_k = hit_k;
+  fp.slot_hits = ctx->buf[B_SLOT_HITS].as<uint32_t>();
+  fp.slot_hit_count = ctx->buf[B_SLOT_HIT_COUNT].as<uint8_t>();
 
   ctx->timer.begin();
   ctx->timer.mark(s, "clear");

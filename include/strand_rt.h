@@ -451,6 +451,10 @@ typedef struct sr_device_config {
  * list with its global radix path instead of only lists longer than its LDS
  * list (exercises that path on small frames; results are identical). */
 #define SR_CONFIG_CSR_GLOBAL_SORT 1u
+/* sr_device_config.flags: the page CSR backend records only 2 fine cells per
+ * segment while marking (instead of 8), so most segments take the re-trace
+ * path of the later passes (exercises it; results are identical). */
+#define SR_CONFIG_CSR_FEW_HITS 2u
 
 /* Library version string. */
 const char* sr_version(void);

@@ -103,13 +103,19 @@ CSR_CASES = {
 }
 
 
+CSR_FLAGS = {"global-sort": abi.CONFIG_CSR_GLOBAL_SORT, "few-hits": abi.CONFIG_CSR_FEW_HITS,
+             "global-sort+few-hits": abi.CONFIG_CSR_GLOBAL_SORT | abi.CONFIG_CSR_FEW_HITS}
+
+
+@pytest.mark.parametrize("flags", list(CSR_FLAGS))
 @pytest.mark.parametrize("case", ["tiny-f8-csr", "small-f8-csr", "dense-one-tile-csr"])
-def test_parity_csr_global_sort(case):
-    """Every page candidate list through the global radix sort."""
+def test_parity_csr_paths(case, flags):
+    """Every page candidate list through the global radix sort, and/or most
+    segments through the re-trace path (2 recorded cells per segment)."""
     w = CSR_CASES[case]()
-    print(f"[parity] {case} (global sort): {w.name}", flush=True)
+    print(f"[parity] {case} ({flags}): {w.name}", flush=True)
     ref = oracle.render(w)
-    ctr, gpu = gpu_frame(w, flags=abi.CONFIG_CSR_GLOBAL_SORT)
+    ctr, gpu = gpu_frame(w, flags=CSR_FLAGS[flags])
     compare(w, ctr, gpu, ref)
 
 
