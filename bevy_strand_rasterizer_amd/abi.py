@@ -31,6 +31,7 @@ STATUS_NAMES = {
     0: "SR_OK", -1: "SR_ERR_INVALID_ARGUMENT", -2: "SR_ERR_OUT_OF_MEMORY", -3: "SR_ERR_DEVICE",
     -4: "SR_ERR_NOT_READY", -5: "SR_ERR_CAPACITY", -6: "SR_ERR_INVALID_ASSET", -7: "SR_ERR_UNSUPPORTED",
 }
+STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
 
 # sr_dump_kind
 DUMP = {
@@ -147,6 +148,27 @@ class ScalpParams(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
+SERVER_MAX_LIGHTS = 4
+request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel_config_dtype),
+                          ("camera_view", view_dtype), ("light_count", "<u4"), ("pad", "<u4"),
+                          ("light_frusta", froxel_config_dtype, SERVER_MAX_LIGHTS),
+                          ("light_views", view_dtype, SERVER_MAX_LIGHTS), ("lights", light_dtype, SERVER_MAX_LIGHTS),
+                          ("settings", np.dtype([("stochastic_cull_enabled", "<u4"), ("target_strands_per_pixel", "<f4"),
+                                                 ("min_keep_probability", "<f4"), ("shadow_keep_probability", "<f4"),
+                                                 ("telemetry_enabled", "<u4"), ("fine_binning_backend", "<u4"),
+                                                 ("dom_layers", "<u4"), ("shadow_history_valid", "<u4"),
+                                                 ("ambient_color", "<f4", 4), ("scene_clear_color", "<f4", 4),
+                                                 ("scene_clear_depth", "<f4"), ("stamp_target_width", "<u4"),
+                                                 ("stamp_target_height", "<u4"), ("pad", "<u4")])),
+                          ("out_rgba", "<u8"), ("out_depth", "<u8"), ("tag", "<u8")], align=True)
+request_result_dtype = np.dtype([("ticket", "<u8"), ("tag", "<u8"), ("status", "<i4"), ("frame_views", "<u4"),
+                                 ("submit_ms", "<f8"), ("launch_ms", "<f8"), ("done_ms", "<f8"),
+                                 ("counters", frame_counters_dtype)], align=True)
+server_config_dtype = np.dtype([("slots", "<u4"), ("max_views_per_frame", "<u4"), ("hbm_budget_bytes", "<u8")])
+server_stats_dtype = np.dtype([("submitted", "<u8"), ("completed", "<u8"), ("failed", "<u8"), ("frames", "<u8"),
+                               ("peak_held_bytes", "<u8"), ("budget_waits", "<u8"), ("max_frame_views", "<u4"),
+                               ("max_concurrent_frames", "<u4")])
+
 strand_cache_info_dtype = np.dtype([("version", "<u4"), ("scale", "<f4"), ("vertex_count", "<u8"),
                                     ("strand_count", "<u8"), ("index_count", "<u8"),
                                     ("max_segments_in_strand", "<u4"), ("format", "<u4"), ("aabb", aabb_dtype)])
@@ -171,6 +193,11 @@ _SIGNATURES = {
     "sr_light_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]),
     "sr_strand_cache_parse": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_char_p, C.c_uint64]),
+    "sr_server_start": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sr_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "sr_server_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int32]),
+    "sr_server_stats_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sr_server_stop": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

@@ -147,14 +147,12 @@ struct FrameShape {
   int hist_cur = 0;
 };
 
-}  // namespace
-
-struct sr_context {
-  int device = 0;
+// Everything one in-flight frame owns: its stream, grow-only buffers, the
+// staging ring for the per-frame tables, the shape of its last frame and the
+// grow-only capacity hints.  The context's slot 0 serves sr_frame_submit; a
+// server owns further slots so that frames overlap on the device.
+struct FrameSlot {
   hipStream_t stream = nullptr;
-  std::string err;
-  SlabAllocator slabs;
-  std::vector<AssetRec> assets;
   DBuf buf[B_COUNT];
   // pinned host staging for per-frame tables: a ring, so the host can run
   // several frames ahead of the device without overwriting pending copies
@@ -169,9 +167,58 @@ struct sr_context {
   hipEvent_t done = nullptr;
   bool submitted = false;
   uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0, cand_cap_hint = 0;
-  uint32_t flags = 0;  // SR_CONFIG_* bits
   int hist_cur = 0;
+
+  uint64_t held_bytes() const {
+    uint64_t b = 0;
+    for (const DBuf& d : buf) b += d.bytes;
+    return b;
+  }
+  void release_buffers() {
+    for (DBuf& d : buf) d.free_();
+    shape.valid = false;
+  }
+};
+
+FrameSlot* new_slot(std::string* err) {
+  FrameSlot* sl = new FrameSlot();
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming)) != hipSuccess) {
+    if (err) *err = std::string("stream creation: ") + hipGetErrorString(e);
+    if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    delete sl;
+    return nullptr;
+  }
+  return sl;
+}
+
+void free_slot(FrameSlot* sl) {
+  if (!sl) return;
+  if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+  sl->release_buffers();
+  for (int k = 0; k < FrameSlot::kRing; ++k) {
+    if (sl->staging[k]) (void)hipHostFree(sl->staging[k]);
+    if (sl->staging_ev[k]) (void)hipEventDestroy(sl->staging_ev[k]);
+  }
+  for (auto e : sl->timer.events) (void)hipEventDestroy(e);
+  if (sl->done) (void)hipEventDestroy(sl->done);
+  if (sl->stream) (void)hipStreamDestroy(sl->stream);
+  delete sl;
+}
+
+}  // namespace
+
+struct sr_context {
+  int device = 0;
+  std::string err;
+  std::mutex mu;  // assets and slabs (a server thread reads them while the caller uploads)
+  SlabAllocator slabs;
+  std::vector<AssetRec> assets;
+  std::vector<FrameSlot*> slots;  // slots[0]: sr_frame_submit
+  uint32_t flags = 0;             // SR_CONFIG_* bits
   uint64_t hbm_budget = 0;
+  FrameSlot& main() { return *slots[0]; }
 };
 
 namespace {
@@ -186,30 +233,44 @@ int hip_fail(sr_context* ctx, hipError_t e, const char* what) {
   return fail(ctx, SR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+#define HIP_OKS(expr, what)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      err = std::string(what) + ": " + hipGetErrorString(_e);                 \
+      return SR_ERR_DEVICE;                                                   \
+    }                                                                         \
+  } while (0)
+
+inline int efail(std::string& err, int code, const std::string& msg) {
+  err = msg;
+  return code;
+}
+
 #define HIP_OK(expr, what)                        \
   do {                                            \
     hipError_t _e = (expr);                       \
     if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
   } while (0)
 
-// Returns the next ring slot with at least `bytes`, after the device has
+// Returns the next ring entry with at least `bytes`, after the device has
 // consumed the copy that last used it.
-char* next_staging(sr_context* ctx, uint64_t bytes, int* slot_out) {
-  int k = ctx->ring_pos;
-  ctx->ring_pos = (k + 1) % sr_context::kRing;
-  if (ctx->staging_used[k]) (void)hipEventSynchronize(ctx->staging_ev[k]);
-  if (!ctx->staging_ev[k] && hipEventCreateWithFlags(&ctx->staging_ev[k], hipEventDisableTiming) != hipSuccess)
+char* next_staging(FrameSlot& sl, uint64_t bytes, int* slot_out) {
+  int k = sl.ring_pos;
+  sl.ring_pos = (k + 1) % FrameSlot::kRing;
+  if (sl.staging_used[k]) (void)hipEventSynchronize(sl.staging_ev[k]);
+  if (!sl.staging_ev[k] && hipEventCreateWithFlags(&sl.staging_ev[k], hipEventDisableTiming) != hipSuccess)
     return nullptr;
-  if (bytes > ctx->staging_bytes[k]) {
-    if (ctx->staging[k]) (void)hipHostFree(ctx->staging[k]);
-    ctx->staging[k] = nullptr;
-    ctx->staging_bytes[k] = 0;
+  if (bytes > sl.staging_bytes[k]) {
+    if (sl.staging[k]) (void)hipHostFree(sl.staging[k]);
+    sl.staging[k] = nullptr;
+    sl.staging_bytes[k] = 0;
     uint64_t b = align_up(bytes * 2, 4096);
-    if (hipHostMalloc(&ctx->staging[k], b, hipHostMallocDefault) != hipSuccess) return nullptr;
-    ctx->staging_bytes[k] = b;
+    if (hipHostMalloc(&sl.staging[k], b, hipHostMallocDefault) != hipSuccess) return nullptr;
+    sl.staging_bytes[k] = b;
   }
   *slot_out = k;
-  return (char*)ctx->staging[k];
+  return (char*)sl.staging[k];
 }
 
 inline uint32_t next_pow2(uint32_t x) {
@@ -243,12 +304,13 @@ int sr_create(const sr_device_config* cfg, sr_context** out) {
   if (cfg && cfg->slab_bytes) ctx->slabs.slab_bytes = cfg->slab_bytes;
   if (cfg) ctx->hbm_budget = cfg->hbm_budget_bytes;
   if (cfg) ctx->flags = cfg->flags;
-  if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming)) != hipSuccess) {
-    std::string m = std::string("sr_create: ") + hipGetErrorString(e);
+  std::string err;
+  FrameSlot* sl = new_slot(&err);
+  if (!sl) {
     delete ctx;
-    return fail(nullptr, SR_ERR_DEVICE, m);
+    return fail(nullptr, SR_ERR_DEVICE, "sr_create: " + err);
   }
+  ctx->slots.push_back(sl);
   *out = ctx;
   return SR_OK;
 }
@@ -256,15 +318,8 @@ int sr_create(const sr_device_config* cfg, sr_context** out) {
 int sr_destroy(sr_context* ctx) {
   if (!ctx) return SR_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  for (auto& b : ctx->buf) b.free_();
-  for (int k = 0; k < sr_context::kRing; ++k) {
-    if (ctx->staging[k]) (void)hipHostFree(ctx->staging[k]);
-    if (ctx->staging_ev[k]) (void)hipEventDestroy(ctx->staging_ev[k]);
-  }
-  for (auto e : ctx->timer.events) (void)hipEventDestroy(e);
-  if (ctx->done) (void)hipEventDestroy(ctx->done);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (FrameSlot* sl : ctx->slots) free_slot(sl);
+  ctx->slots.clear();
   delete ctx;
   return SR_OK;
 }
@@ -300,6 +355,7 @@ int sr_asset_upload(sr_context* ctx, const sr_asset_slabs* a, sr_asset_info* out
     for (uint32_t p = m.offset; p + 1 < m.offset + m.count; ++p) pos_strand[p] = s;
   }
   (void)hipSetDevice(ctx->device);
+  std::lock_guard<std::mutex> lk(ctx->mu);
   AssetRec rec;
   rec.live = true;
   rec.geo = *a->geo;
@@ -330,10 +386,12 @@ int sr_asset_upload(sr_context* ctx, const sr_asset_slabs* a, sr_asset_info* out
 }
 
 int sr_asset_release(sr_context* ctx, uint32_t id) {
-  if (!ctx || id >= ctx->assets.size() || !ctx->assets[id].live)
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (id >= ctx->assets.size() || !ctx->assets[id].live)
     return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_asset_release: unknown asset");
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  for (FrameSlot* sl : ctx->slots) (void)hipStreamSynchronize(sl->stream);  // frames in flight may read it
   AssetRec& r = ctx->assets[id];
   const sr_device_ptr* owned[6] = {&r.info.vertices, &r.info.indices, &r.info.meta,
                                    &r.info.geo,      &r.info.materials, &r.pos_strand};
@@ -348,23 +406,31 @@ int sr_asset_release(sr_context* ctx, uint32_t id) {
 
 int sr_set_profiling(sr_context* ctx, int32_t enabled) {
   if (!ctx) return SR_ERR_INVALID_ARGUMENT;
-  ctx->timer.enabled = enabled != 0;
+  ctx->main().timer.enabled = enabled != 0;
   return SR_OK;
 }
 
-int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) {
-  if (!ctx || !fd) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: NULL argument");
+}  // extern "C"
+
+namespace {
+
+// Plans and launches one frame on `sl`.  need_bytes (optional) receives the
+// device bytes the slot holds once its buffers fit this frame; dry_run stops
+// after planning.  Asset records are read under ctx->mu.
+int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, hipStream_t stream_in,
+                      std::string& err, uint64_t* need_bytes, bool dry_run) {
+  if (!fd) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: NULL argument");
   if (fd->frustum_count == 0 || !fd->frusta || !fd->views)
-    return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no frusta");
-  if (fd->instance_count && !fd->instances) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
-  if (fd->instance_count >= (1u << 24)) return fail(ctx, SR_ERR_UNSUPPORTED, "sr_frame_submit: at most 2^24 instances");
-  if (fd->light_count && !fd->lights) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no lights");
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no frusta");
+  if (fd->instance_count && !fd->instances) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
+  if (fd->instance_count >= (1u << 24)) return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit: at most 2^24 instances");
+  if (fd->light_count && !fd->lights) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no lights");
   const sr_frame_settings& st = fd->settings;
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
-    return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
-  if (st.fine_binning_backend > 1) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
+  if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
   (void)hipSetDevice(ctx->device);
-  hipStream_t s = stream_ptr ? (hipStream_t)stream_ptr : ctx->stream;
+  hipStream_t s = stream_in ? stream_in : sl.stream;
 
   const uint32_t F = fd->frustum_count, NI = fd->instance_count, NL = fd->light_count;
   // ---- frustum records
@@ -374,14 +440,14 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   for (uint32_t i = 0; i < F; ++i) {
     const sr_frustum_desc& d = fd->frusta[i];
     if (d.screen_width == 0 || d.screen_height == 0 || d.froxel_size_x == 0 || d.froxel_size_y == 0)
-      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "frustum with zero extent");
+      return efail(err, SR_ERR_INVALID_ARGUMENT, "frustum with zero extent");
     if (d.froxel_size_x * d.froxel_size_y > 256)
-      return fail(ctx, SR_ERR_UNSUPPORTED, "froxel tiles above 256 pixels are not supported");
+      return efail(err, SR_ERR_UNSUPPORTED, "froxel tiles above 256 pixels are not supported");
     uint32_t ftx = (d.screen_width + d.froxel_size_x - 1) / d.froxel_size_x;
     uint32_t fty = (d.screen_height + d.froxel_size_y - 1) / d.froxel_size_y;
     if (d.coarse_tiles_x * kExt < ftx || d.coarse_tiles_y * kExt < fty ||
         d.coarse_depth_tile_count < d.coarse_tiles_x * d.coarse_tiles_y)
-      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "frustum coarse tiling does not cover its fine tiles");
+      return efail(err, SR_ERR_INVALID_ARGUMENT, "frustum coarse tiling does not cover its fine tiles");
     FrustumRec& r = fr[i];
     std::memset(&r, 0, sizeof(r));
     r.d = d;
@@ -412,6 +478,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
     coarse_total = std::max(coarse_total, d.coarse_depth_tile_base + d.coarse_depth_tile_count);
   }
   // ---- instance records
+  std::unique_lock<std::mutex> asset_lock(ctx->mu);
   std::vector<InstRec> ir(NI);
   uint64_t pos = 0;
   uint32_t max_idx = 1;
@@ -423,7 +490,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
     r.pos_base = (uint32_t)pos;
     auto live = [&](uint32_t id) { return id < ctx->assets.size() && ctx->assets[id].live; };
     if (!live(in.vertex_id) || !live(in.index_id) || !live(in.meta_id) || !live(in.geo_id) || !live(in.material_id))
-      return fail(ctx, SR_ERR_INVALID_ARGUMENT, "instance references an unknown asset id");
+      return efail(err, SR_ERR_INVALID_ARGUMENT, "instance references an unknown asset id");
     const AssetRec& va = ctx->assets[in.vertex_id];
     const AssetRec& ia = ctx->assets[in.index_id];
     const AssetRec& ma = ctx->assets[in.meta_id];
@@ -443,23 +510,23 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
     // kernels index vertices/materials through the meta/index slabs of the
     // same instance; mixing assets requires matching slab shapes
     if (in.index_id != in.meta_id || in.vertex_id != in.index_id || in.material_id != in.meta_id)
-      return fail(ctx, SR_ERR_UNSUPPORTED, "an instance must take all slabs from one asset");
+      return efail(err, SR_ERR_UNSUPPORTED, "an instance must take all slabs from one asset");
     r.valid = 1u;
     pos += r.index_count;
     max_idx = std::max(max_idx, r.index_count);
   }
   const uint32_t slots = (uint32_t)align_up(std::max<uint64_t>(pos, 64), 64);
-  if ((uint64_t)F * slots > 0x7FFFFFFFull) return fail(ctx, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
+  if ((uint64_t)F * slots > 0x7FFFFFFFull) return efail(err, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
   const uint32_t task_cap = F * slots;
   const uint32_t ptab = coarse_total * kSlices;
-  uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, std::max(4096u, ptab / 2)), ctx->page_cap_hint);
+  uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, std::max(4096u, ptab / 2)), sl.page_cap_hint);
   page_cap = std::min(page_cap, ptab);
   // reference slab: 8 references per task slot (the reference sizes it at 4x
   // its binning queue, src/plugin.rs:540-546); work items: half of that.
   // Both grow when a frame reports an overflow.
-  uint64_t ref_cap64 = std::max<uint64_t>({(uint64_t)task_cap * 8ull, 1ull << 16, (uint64_t)ctx->ref_cap_hint});
+  uint64_t ref_cap64 = std::max<uint64_t>({(uint64_t)task_cap * 8ull, 1ull << 16, (uint64_t)sl.ref_cap_hint});
   const uint32_t ref_cap = (uint32_t)std::min<uint64_t>(ref_cap64, 0x7FFFFFFFull);
-  const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, ctx->work_cap_hint});
+  const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, sl.work_cap_hint});
   // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
   const bool csr = st.fine_binning_backend != 0u;
   // cells recorded per slot while marking: kHitsPerSlot words per slot, when
@@ -468,7 +535,7 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   const uint64_t hit_bytes = 4ull * kHitsPerSlot * task_cap;
   const uint32_t hit_k = (csr && ptab < (1u << 24) && hit_bytes <= (8ull << 30)) ? kHitsPerSlot : 0u;
   const uint32_t cand_cap = csr ? (uint32_t)std::min<uint64_t>(
-                                      std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)ctx->cand_cap_hint}),
+                                      std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)sl.cand_cap_hint}),
                                       0x7FFFFFFFull)
                                 : 0u;
   const uint32_t atlas_w = std::min(std::max(max_idx, 1u), SR_MAX_TEXTURE_EXTENT);
@@ -479,9 +546,9 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   const uint32_t scan_tiles = (std::max({task_cap / 64u, ptab, tiles16, 1u}) + kScanTile - 1) / kScanTile + 1;
 
   // history: keep the previous buffer when the atlas layout is unchanged
-  const bool hist_valid = st.shadow_history_valid && ctx->shape.valid && ctx->shape.cams == cams &&
-                          ctx->shape.NI == NI && ctx->shape.atlas_texels == atlas_texels;
-  const int hist_next = hist_valid ? (ctx->hist_cur ^ 1) : 0;
+  const bool hist_valid = st.shadow_history_valid && sl.shape.valid && sl.shape.cams == cams &&
+                          sl.shape.NI == NI && sl.shape.atlas_texels == atlas_texels;
+  const int hist_next = hist_valid ? (sl.hist_cur ^ 1) : 0;
 
   struct Need { BufId id; uint64_t bytes; };
   const Need needs[] = {
@@ -503,99 +570,108 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
       {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
+  if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
+    uint64_t held = sl.held_bytes();
+    for (const Need& n : needs) {
+      const uint64_t want = std::max<uint64_t>(n.bytes, 256);
+      if (want > sl.buf[n.id].bytes) held += align_up(want + want / 4, 4096) - sl.buf[n.id].bytes;
+    }
+    *need_bytes = held;
+  }
+  if (dry_run) return SR_OK;
   bool grow = false;
-  for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > ctx->buf[n.id].bytes;
-  if (grow && ctx->submitted) {  // in-flight frames may still use the buffers
+  for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > sl.buf[n.id].bytes;
+  if (grow && sl.submitted) {  // in-flight frames may still use the buffers
     (void)hipStreamSynchronize(s);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(sl.stream);
   }
   for (const Need& n : needs) {
-    if (!ctx->buf[n.id].ensure(n.bytes)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
+    if (!sl.buf[n.id].ensure(n.bytes)) return efail(err, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
   }
   // ---- upload per-frame tables through pinned staging
   const uint64_t tbytes = sizeof(FrustumRec) * F + sizeof(InstRec) * NI + sizeof(sr_directional_light) * NL;
   int slot = 0;
-  char* stg = next_staging(ctx, tbytes + 256, &slot);
-  if (!stg) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
+  char* stg = next_staging(sl, tbytes + 256, &slot);
+  if (!stg) return efail(err, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
   std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
   if (NI) std::memcpy(stg + sizeof(FrustumRec) * F, ir.data(), sizeof(InstRec) * NI);
   if (NL) std::memcpy(stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI, fd->lights, sizeof(sr_directional_light) * NL);
-  HIP_OK(hipMemcpyAsync(ctx->buf[B_FRUSTA].p, stg, sizeof(FrustumRec) * F, hipMemcpyHostToDevice, s), "upload frusta");
+  HIP_OKS(hipMemcpyAsync(sl.buf[B_FRUSTA].p, stg, sizeof(FrustumRec) * F, hipMemcpyHostToDevice, s), "upload frusta");
   if (NI)
-    HIP_OK(hipMemcpyAsync(ctx->buf[B_INSTS].p, stg + sizeof(FrustumRec) * F, sizeof(InstRec) * NI,
+    HIP_OKS(hipMemcpyAsync(sl.buf[B_INSTS].p, stg + sizeof(FrustumRec) * F, sizeof(InstRec) * NI,
                           hipMemcpyHostToDevice, s), "upload instances");
   if (NL)
-    HIP_OK(hipMemcpyAsync(ctx->buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
+    HIP_OKS(hipMemcpyAsync(sl.buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
                           sizeof(sr_directional_light) * NL, hipMemcpyHostToDevice, s), "upload lights");
-  HIP_OK(hipEventRecord(ctx->staging_ev[slot], s), "staging event");
-  ctx->staging_used[slot] = true;
+  HIP_OKS(hipEventRecord(sl.staging_ev[slot], s), "staging event");
+  sl.staging_used[slot] = true;
 
   FrameParams fp;
   std::memset(&fp, 0, sizeof(fp));
-  fp.fr = ctx->buf[B_FRUSTA].as<FrustumRec>();
-  fp.inst = ctx->buf[B_INSTS].as<InstRec>();
-  fp.lights = ctx->buf[B_LIGHTS].as<sr_directional_light>();
+  fp.fr = sl.buf[B_FRUSTA].as<FrustumRec>();
+  fp.inst = sl.buf[B_INSTS].as<InstRec>();
+  fp.lights = sl.buf[B_LIGHTS].as<sr_directional_light>();
   fp.F = F; fp.NI = NI; fp.NL = NL; fp.cams = cams;
   fp.slots = slots; fp.coarse_total = coarse_total;
   fp.task_cap = task_cap; fp.ref_cap = ref_cap; fp.work_cap = work_cap; fp.atlas_texels = atlas_texels;
   fp.dom_layers = st.dom_layers;
   fp.st = st;
   fp.st.shadow_history_valid = hist_valid ? 1u : 0u;
-  fp.counters = ctx->buf[B_COUNTERS].as<uint32_t>();
-  fp.broad = ctx->buf[B_BROAD].as<sr_broad_instance_meta>();
-  fp.keep = ctx->buf[B_KEEP].as<float>();
-  fp.ranges = ctx->buf[B_RANGES].as<sr_coarse_asset_range>();
-  fp.slot_masks = ctx->buf[B_SLOT_MASKS].as<unsigned long long>();
-  fp.wave_offsets = ctx->buf[B_WAVE_OFF].as<uint32_t>();
-  fp.frustum_task_base = ctx->buf[B_FR_TASK_BASE].as<uint32_t>();
-  fp.frustum_cam_base = ctx->buf[B_FR_CAM_BASE].as<uint32_t>();
-  fp.shaded = ctx->buf[B_SHADED].as<uint8_t>();
-  fp.tasks = ctx->buf[B_TASKS].as<sr_binning_task>();
-  fp.proj = ctx->buf[B_PROJ].as<sr_projected_segment>();
-  fp.shading_queue = ctx->buf[B_SHADING_Q].as<uint32_t>();
-  fp.lut = ctx->buf[B_LUT].as<sr_coarse_depth_lut_entry>();
-  fp.page_table = ctx->buf[B_PAGE_TABLE].as<uint32_t>();
-  fp.page_meta = ctx->buf[B_PAGE_META].as<sr_fine_page_meta>();
-  fp.page_totals = ctx->buf[B_PAGE_TOTALS].as<uint32_t>();
-  fp.cell_counts = ctx->buf[B_CELL_COUNTS].as<uint32_t>();
-  fp.cell_offsets = ctx->buf[B_CELL_OFFSETS].as<uint32_t>();
-  fp.cell_cursors = ctx->buf[B_CELL_CURSORS].as<uint32_t>();
-  fp.refs = ctx->buf[B_REFS].as<sr_fine_seg_ref>();
-  fp.active_flags = ctx->buf[B_ACTIVE_FLAGS].as<uint32_t>();
-  fp.active_queue = ctx->buf[B_ACTIVE_Q].as<uint32_t>();
-  fp.tile_work = ctx->buf[B_TILE_WORK].as<uint4>();
-  fp.tile_bases = ctx->buf[B_TILE_BASES].as<uint4>();
-  fp.work = ctx->buf[B_WORK].as<sr_raster_work_item>();
-  fp.cam_runs = ctx->buf[B_CAM_RUNS].as<sr_raster_tile_run>();
-  fp.shadow_runs = ctx->buf[B_SHADOW_RUNS].as<sr_raster_tile_run>();
-  fp.dom_depth = ctx->buf[B_DOM_DEPTH].as<float>();
-  fp.dom_opacity = ctx->buf[B_DOM_OPACITY].as<float>();
-  fp.stamp = ctx->buf[B_STAMP].as<float>();
-  fp.atlas = ctx->buf[B_ATLAS].as<float4>();
-  fp.history = ctx->buf[hist_next ? B_HIST1 : B_HIST0].as<float2>();
-  fp.history_prev = hist_valid ? ctx->buf[hist_next ? B_HIST0 : B_HIST1].as<float2>() : nullptr;
-  fp.image = ctx->buf[B_IMAGE].as<float4>();
-  fp.depth = ctx->buf[B_DEPTH].as<float>();
-  fp.composite = ctx->buf[B_COMPOSITE].as<float4>();
+  fp.counters = sl.buf[B_COUNTERS].as<uint32_t>();
+  fp.broad = sl.buf[B_BROAD].as<sr_broad_instance_meta>();
+  fp.keep = sl.buf[B_KEEP].as<float>();
+  fp.ranges = sl.buf[B_RANGES].as<sr_coarse_asset_range>();
+  fp.slot_masks = sl.buf[B_SLOT_MASKS].as<unsigned long long>();
+  fp.wave_offsets = sl.buf[B_WAVE_OFF].as<uint32_t>();
+  fp.frustum_task_base = sl.buf[B_FR_TASK_BASE].as<uint32_t>();
+  fp.frustum_cam_base = sl.buf[B_FR_CAM_BASE].as<uint32_t>();
+  fp.shaded = sl.buf[B_SHADED].as<uint8_t>();
+  fp.tasks = sl.buf[B_TASKS].as<sr_binning_task>();
+  fp.proj = sl.buf[B_PROJ].as<sr_projected_segment>();
+  fp.shading_queue = sl.buf[B_SHADING_Q].as<uint32_t>();
+  fp.lut = sl.buf[B_LUT].as<sr_coarse_depth_lut_entry>();
+  fp.page_table = sl.buf[B_PAGE_TABLE].as<uint32_t>();
+  fp.page_meta = sl.buf[B_PAGE_META].as<sr_fine_page_meta>();
+  fp.page_totals = sl.buf[B_PAGE_TOTALS].as<uint32_t>();
+  fp.cell_counts = sl.buf[B_CELL_COUNTS].as<uint32_t>();
+  fp.cell_offsets = sl.buf[B_CELL_OFFSETS].as<uint32_t>();
+  fp.cell_cursors = sl.buf[B_CELL_CURSORS].as<uint32_t>();
+  fp.refs = sl.buf[B_REFS].as<sr_fine_seg_ref>();
+  fp.active_flags = sl.buf[B_ACTIVE_FLAGS].as<uint32_t>();
+  fp.active_queue = sl.buf[B_ACTIVE_Q].as<uint32_t>();
+  fp.tile_work = sl.buf[B_TILE_WORK].as<uint4>();
+  fp.tile_bases = sl.buf[B_TILE_BASES].as<uint4>();
+  fp.work = sl.buf[B_WORK].as<sr_raster_work_item>();
+  fp.cam_runs = sl.buf[B_CAM_RUNS].as<sr_raster_tile_run>();
+  fp.shadow_runs = sl.buf[B_SHADOW_RUNS].as<sr_raster_tile_run>();
+  fp.dom_depth = sl.buf[B_DOM_DEPTH].as<float>();
+  fp.dom_opacity = sl.buf[B_DOM_OPACITY].as<float>();
+  fp.stamp = sl.buf[B_STAMP].as<float>();
+  fp.atlas = sl.buf[B_ATLAS].as<float4>();
+  fp.history = sl.buf[hist_next ? B_HIST1 : B_HIST0].as<float2>();
+  fp.history_prev = hist_valid ? sl.buf[hist_next ? B_HIST0 : B_HIST1].as<float2>() : nullptr;
+  fp.image = sl.buf[B_IMAGE].as<float4>();
+  fp.depth = sl.buf[B_DEPTH].as<float>();
+  fp.composite = sl.buf[B_COMPOSITE].as<float4>();
   // page_cap bounds the page arrays; the allocation scan drops pages past it
-  PrepassScratch ps{ctx->buf[B_SLOT_PROJ].as<sr_projected_segment>(), ctx->buf[B_PARTIALS].as<uint32_t>(),
-                    ctx->buf[B_PARTIALS4].as<uint4>(), ctx->buf[B_XY_MASK].as<uint32_t>()};
+  PrepassScratch ps{sl.buf[B_SLOT_PROJ].as<sr_projected_segment>(), sl.buf[B_PARTIALS].as<uint32_t>(),
+                    sl.buf[B_PARTIALS4].as<uint4>(), sl.buf[B_XY_MASK].as<uint32_t>()};
   fp.page_cap = page_cap;
   fp.page_xy_mask = ps.page_xy_mask;
   fp.cand_cap = cand_cap;
   fp.cand_lds_limit = (ctx->flags & SR_CONFIG_CSR_GLOBAL_SORT) ? 0u : 0xFFFFFFFFu;
-  fp.cand_count_pt = ctx->buf[B_CAND_PT].as<uint32_t>();
-  fp.cand_off = ctx->buf[B_CAND_OFF].as<uint32_t>();
-  fp.cand_cursor = ctx->buf[B_CAND_CURSOR].as<uint32_t>();
-  fp.cands = ctx->buf[B_CANDS].as<uint32_t>();
-  fp.cands_tmp = ctx->buf[B_CANDS_TMP].as<uint32_t>();
+  fp.cand_count_pt = sl.buf[B_CAND_PT].as<uint32_t>();
+  fp.cand_off = sl.buf[B_CAND_OFF].as<uint32_t>();
+  fp.cand_cursor = sl.buf[B_CAND_CURSOR].as<uint32_t>();
+  fp.cands = sl.buf[B_CANDS].as<uint32_t>();
+  fp.cands_tmp = sl.buf[B_CANDS_TMP].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
   fp.hit_k = hit_k;
-  fp.slot_hits = ctx->buf[B_SLOT_HITS].as<uint32_t>();
-  fp.slot_hit_count = ctx->buf[B_SLOT_HIT_COUNT].as<uint8_t>();
+  fp.slot_hits = sl.buf[B_SLOT_HITS].as<uint32_t>();
+  fp.slot_hit_count = sl.buf[B_SLOT_HIT_COUNT].as<uint8_t>();
 
-  ctx->timer.begin();
-  ctx->timer.mark(s, "clear");
+  sl.timer.begin();
+  sl.timer.mark(s, "clear");
   launch_fill_u32(fp.counters, C_COUNT, 0u, s);
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
@@ -606,23 +682,23 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   // them stale; dumps must be deterministic)
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4, 0u, s);
   launch_fill_u32(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2, 0u, s);
-  launch_prepass(fp, ps, s, &ctx->timer);
-  ctx->timer.mark(s, "shadow");
+  launch_prepass(fp, ps, s, &sl.timer);
+  sl.timer.mark(s, "shadow");
   launch_shadow(fp, s, max_light_tile_px);
-  ctx->timer.mark(s, "stampback");
+  sl.timer.mark(s, "stampback");
   for (uint32_t i = 0; i < F; ++i)
     if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
-  ctx->timer.mark(s, "shading");
+  sl.timer.mark(s, "shading");
   launch_shading(fp, s);
-  ctx->timer.mark(s, "raster");
+  sl.timer.mark(s, "raster");
   launch_camera_raster(fp, s, max_cam_tile_px);
-  ctx->timer.mark(s, "composite");
+  sl.timer.mark(s, "composite");
   launch_composite(fp, img_px, s);
-  ctx->timer.mark(s, "end");
-  HIP_OK(hipGetLastError(), "sr_frame_submit: launch");
-  HIP_OK(hipEventRecord(ctx->done, s), "sr_frame_submit: event");
+  sl.timer.mark(s, "end");
+  HIP_OKS(hipGetLastError(), "sr_frame_submit: launch");
+  HIP_OKS(hipEventRecord(sl.done, s), "sr_frame_submit: event");
 
-  FrameShape& sh = ctx->shape;
+  FrameShape& sh = sl.shape;
   sh.valid = true;
   sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
@@ -630,19 +706,18 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
   sh.hist_cur = hist_next;
-  ctx->hist_cur = hist_next;
-  ctx->submitted = true;
+  sl.hist_cur = hist_next;
+  sl.submitted = true;
   return SR_OK;
 }
 
-int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
-  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
-  if (!ctx->submitted) return fail(ctx, SR_ERR_NOT_READY, "sr_frame_wait: no frame submitted");
-  (void)hipSetDevice(ctx->device);
-  HIP_OK(hipEventSynchronize(ctx->done), "sr_frame_wait");
+// Waits for the slot's last frame; counters and grow-only hints.
+int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
+  if (!sl.submitted) return efail(err, SR_ERR_NOT_READY, "sr_frame_wait: no frame submitted");
+  HIP_OKS(hipEventSynchronize(sl.done), "sr_frame_wait");
   uint32_t c[C_COUNT];
-  HIP_OK(hipMemcpy(c, ctx->buf[B_COUNTERS].p, sizeof(c), hipMemcpyDeviceToHost), "sr_frame_wait: counters");
-  const FrameShape& sh = ctx->shape;
+  HIP_OKS(hipMemcpy(c, sl.buf[B_COUNTERS].p, sizeof(c), hipMemcpyDeviceToHost), "sr_frame_wait: counters");
+  const FrameShape& sh = sl.shape;
   sr_frame_counters fc;
   std::memset(&fc, 0, sizeof(fc));
   fc.binning_tasks = c[C_TASKS];
@@ -658,14 +733,34 @@ int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
   fc.task_capacity = sh.task_cap;
   fc.ref_capacity = sh.ref_cap;
   // grow-only capacity planning for the next frame
-  if (c[C_REFS] > sh.ref_cap) ctx->ref_cap_hint = next_pow2(c[C_REFS]);
-  if (c[C_PAGE_DROPS] > 0) ctx->page_cap_hint = next_pow2(c[C_MARKED]);
-  if (c[C_WORK] > sh.work_cap) ctx->work_cap_hint = next_pow2(c[C_WORK]);
-  if (sh.cand_cap && c[C_CANDS] > sh.cand_cap) ctx->cand_cap_hint = next_pow2(c[C_CANDS]);
+  if (c[C_REFS] > sh.ref_cap) sl.ref_cap_hint = next_pow2(c[C_REFS]);
+  if (c[C_PAGE_DROPS] > 0) sl.page_cap_hint = next_pow2(c[C_MARKED]);
+  if (c[C_WORK] > sh.work_cap) sl.work_cap_hint = next_pow2(c[C_WORK]);
+  if (sh.cand_cap && c[C_CANDS] > sh.cand_cap) sl.cand_cap_hint = next_pow2(c[C_CANDS]);
   fc.page_drops = c[C_PAGE_DROPS];
   fc.page_candidates = sh.cand_cap ? c[C_CANDS] : 0u;
   if (out) *out = fc;
   return SR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  std::string err;
+  int rc = frame_submit_slot(ctx, ctx->main(), fd, (hipStream_t)stream_ptr, err, nullptr, false);
+  return rc == SR_OK ? rc : fail(ctx, rc, err);
+}
+
+int sr_frame_wait(sr_context* ctx, sr_frame_counters* out) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  std::string err;
+  int rc = frame_wait_slot(ctx->main(), out, err);
+  return rc == SR_OK ? rc : fail(ctx, rc, err);
 }
 
 int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, uint64_t* out_bytes) {
@@ -673,7 +768,8 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
   sr_frame_counters fc;
   int rc = sr_frame_wait(ctx, &fc);
   if (rc != SR_OK) return rc;
-  const FrameShape& sh = ctx->shape;
+  const FrameShape& sh = ctx->main().shape;
+  DBuf* buf = ctx->main().buf;
   const void* src = nullptr;
   uint64_t bytes = 0;
   uint32_t P = std::min(fc.allocated_pages, sh.page_cap);
@@ -685,33 +781,33 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
         std::memcpy(dst, &fc, sizeof(fc));
       }
       return SR_OK;
-    case SR_DUMP_BROAD_META: src = ctx->buf[B_BROAD].p; bytes = 16ull * sh.NI; break;
-    case SR_DUMP_KEEP: src = ctx->buf[B_KEEP].p; bytes = 4ull * sh.NI * sh.F; break;
-    case SR_DUMP_RANGES: src = ctx->buf[B_RANGES].p; bytes = 32ull * sh.NI * sh.F; break;
-    case SR_DUMP_TASKS: src = ctx->buf[B_TASKS].p; bytes = 16ull * std::min(fc.binning_tasks, sh.task_cap); break;
-    case SR_DUMP_PROJECTED: src = ctx->buf[B_PROJ].p; bytes = 12ull * std::min(fc.binning_tasks, sh.task_cap); break;
-    case SR_DUMP_SHADING_QUEUE: src = ctx->buf[B_SHADING_Q].p; bytes = 4ull * fc.camera_tasks; break;
-    case SR_DUMP_LUT: src = ctx->buf[B_LUT].p; bytes = 16ull * sh.coarse_total * kSlices; break;
-    case SR_DUMP_PAGE_TABLE: src = ctx->buf[B_PAGE_TABLE].p; bytes = 4ull * sh.coarse_total * kSlices; break;
-    case SR_DUMP_PAGE_META: src = ctx->buf[B_PAGE_META].p; bytes = 32ull * P; break;
-    case SR_DUMP_CELL_COUNTS: src = ctx->buf[B_CELL_COUNTS].p; bytes = 4ull * P * kPage; break;
-    case SR_DUMP_CELL_OFFSETS: src = ctx->buf[B_CELL_OFFSETS].p; bytes = 4ull * P * kPage; break;
-    case SR_DUMP_REFS: src = ctx->buf[B_REFS].p; bytes = 12ull * std::min(fc.fine_refs, sh.ref_cap); break;
-    case SR_DUMP_ACTIVE_TILES: src = ctx->buf[B_ACTIVE_Q].p; bytes = 4ull * fc.active_tiles; break;
-    case SR_DUMP_WORK_ITEMS: src = ctx->buf[B_WORK].p; bytes = 32ull * std::min(fc.work_items, sh.work_cap); break;
-    case SR_DUMP_CAMERA_RUNS: src = ctx->buf[B_CAM_RUNS].p; bytes = 32ull * fc.camera_runs; break;
-    case SR_DUMP_SHADOW_RUNS: src = ctx->buf[B_SHADOW_RUNS].p; bytes = 32ull * fc.shadow_runs; break;
-    case SR_DUMP_DOM_DEPTH: src = ctx->buf[B_DOM_DEPTH].p; bytes = 4ull * sh.dom_px; break;
-    case SR_DUMP_DOM_OPACITY: src = ctx->buf[B_DOM_OPACITY].p; bytes = 4ull * sh.dom_px * sh.dom_layers; break;
-    case SR_DUMP_ATLAS: src = ctx->buf[B_ATLAS].p; bytes = 16ull * sh.cams * sh.NI * sh.atlas_texels; break;
+    case SR_DUMP_BROAD_META: src = buf[B_BROAD].p; bytes = 16ull * sh.NI; break;
+    case SR_DUMP_KEEP: src = buf[B_KEEP].p; bytes = 4ull * sh.NI * sh.F; break;
+    case SR_DUMP_RANGES: src = buf[B_RANGES].p; bytes = 32ull * sh.NI * sh.F; break;
+    case SR_DUMP_TASKS: src = buf[B_TASKS].p; bytes = 16ull * std::min(fc.binning_tasks, sh.task_cap); break;
+    case SR_DUMP_PROJECTED: src = buf[B_PROJ].p; bytes = 12ull * std::min(fc.binning_tasks, sh.task_cap); break;
+    case SR_DUMP_SHADING_QUEUE: src = buf[B_SHADING_Q].p; bytes = 4ull * fc.camera_tasks; break;
+    case SR_DUMP_LUT: src = buf[B_LUT].p; bytes = 16ull * sh.coarse_total * kSlices; break;
+    case SR_DUMP_PAGE_TABLE: src = buf[B_PAGE_TABLE].p; bytes = 4ull * sh.coarse_total * kSlices; break;
+    case SR_DUMP_PAGE_META: src = buf[B_PAGE_META].p; bytes = 32ull * P; break;
+    case SR_DUMP_CELL_COUNTS: src = buf[B_CELL_COUNTS].p; bytes = 4ull * P * kPage; break;
+    case SR_DUMP_CELL_OFFSETS: src = buf[B_CELL_OFFSETS].p; bytes = 4ull * P * kPage; break;
+    case SR_DUMP_REFS: src = buf[B_REFS].p; bytes = 12ull * std::min(fc.fine_refs, sh.ref_cap); break;
+    case SR_DUMP_ACTIVE_TILES: src = buf[B_ACTIVE_Q].p; bytes = 4ull * fc.active_tiles; break;
+    case SR_DUMP_WORK_ITEMS: src = buf[B_WORK].p; bytes = 32ull * std::min(fc.work_items, sh.work_cap); break;
+    case SR_DUMP_CAMERA_RUNS: src = buf[B_CAM_RUNS].p; bytes = 32ull * fc.camera_runs; break;
+    case SR_DUMP_SHADOW_RUNS: src = buf[B_SHADOW_RUNS].p; bytes = 32ull * fc.shadow_runs; break;
+    case SR_DUMP_DOM_DEPTH: src = buf[B_DOM_DEPTH].p; bytes = 4ull * sh.dom_px; break;
+    case SR_DUMP_DOM_OPACITY: src = buf[B_DOM_OPACITY].p; bytes = 4ull * sh.dom_px * sh.dom_layers; break;
+    case SR_DUMP_ATLAS: src = buf[B_ATLAS].p; bytes = 16ull * sh.cams * sh.NI * sh.atlas_texels; break;
     case SR_DUMP_HISTORY:
-      src = ctx->buf[sh.hist_cur ? B_HIST1 : B_HIST0].p;
+      src = buf[sh.hist_cur ? B_HIST1 : B_HIST0].p;
       bytes = 8ull * sh.cams * sh.NI * sh.atlas_texels;
       break;
-    case SR_DUMP_IMAGE: src = ctx->buf[B_IMAGE].p; bytes = 16ull * sh.img_px; break;
-    case SR_DUMP_DEPTH: src = ctx->buf[B_DEPTH].p; bytes = 4ull * sh.img_px; break;
-    case SR_DUMP_COMPOSITE: src = ctx->buf[B_COMPOSITE].p; bytes = 16ull * sh.img_px; break;
-    case SR_DUMP_STAMP: src = ctx->buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
+    case SR_DUMP_IMAGE: src = buf[B_IMAGE].p; bytes = 16ull * sh.img_px; break;
+    case SR_DUMP_DEPTH: src = buf[B_DEPTH].p; bytes = 4ull * sh.img_px; break;
+    case SR_DUMP_COMPOSITE: src = buf[B_COMPOSITE].p; bytes = 16ull * sh.img_px; break;
+    case SR_DUMP_STAMP: src = buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
     default: return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_dump: unknown kind");
   }
   *out_bytes = bytes;
@@ -724,7 +820,7 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
 int sr_stage_times(sr_context* ctx, float* dst_ms, int32_t capacity, int32_t* out_count, char* names,
                    int32_t names_capacity) {
   if (!ctx || !out_count) return SR_ERR_INVALID_ARGUMENT;
-  StageTimer& t = ctx->timer;
+  StageTimer& t = ctx->main().timer;
   if (!t.enabled || t.used < 2) { *out_count = 0; return SR_OK; }
   HIP_OK(hipEventSynchronize(t.events[t.used - 1]), "sr_stage_times");
   int n = (int)t.used - 1;
@@ -741,6 +837,413 @@ int sr_stage_times(sr_context* ctx, float* dst_ms, int32_t capacity, int32_t* ou
     names[names_capacity - 1] = 0;
   }
   *out_count = n;
+  return SR_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================== serving
+// A worker thread over frame slots: completes finished frames (copies each
+// request's composite and depth out of pinned staging), then admits queued
+// requests into free slots.  Admission takes the oldest request and every
+// later queued request with the same instance, lights and settings (up to
+// max_views_per_frame cameras) as one multi-view frame, when the bytes held
+// by assets and every slot after this frame stay within the budget; idle
+// slots give their buffers back first.  A request that does not fit the
+// budget even alone fails with SR_ERR_CAPACITY.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct PendingRequest {
+  sr_request req;
+  uint64_t ticket;
+  double submit_ms;
+};
+
+struct ServerSlot {
+  FrameSlot* fs = nullptr;
+  bool busy = false;
+  std::vector<PendingRequest> reqs;  // cameras of the frame, in frustum order
+  std::vector<uint64_t> out_off;     // float offset of each request's composite in staging
+  void* out = nullptr;               // pinned composite + depth of every camera
+  uint64_t out_bytes = 0;
+  hipEvent_t out_ev = nullptr;
+  double launch_ms = 0.0;
+  sr_frame_counters counters{};
+};
+
+bool same_group(const sr_request& a, const sr_request& b) {
+  if (std::memcmp(&a.instance, &b.instance, sizeof(a.instance)) != 0) return false;
+  if (a.light_count != b.light_count) return false;
+  if (std::memcmp(&a.settings, &b.settings, sizeof(a.settings)) != 0) return false;
+  for (uint32_t l = 0; l < a.light_count; ++l) {
+    if (std::memcmp(&a.light_frusta[l], &b.light_frusta[l], sizeof(a.light_frusta[l])) != 0) return false;
+    if (std::memcmp(&a.light_views[l], &b.light_views[l], sizeof(a.light_views[l])) != 0) return false;
+    if (std::memcmp(&a.lights[l], &b.lights[l], sizeof(a.lights[l])) != 0) return false;
+  }
+  return true;
+}
+
+// Frame tables of a group: cameras in request order, then the shared lights.
+struct GroupFrame {
+  std::vector<sr_frustum_desc> frusta;
+  std::vector<sr_view> views;
+  std::vector<sr_directional_light> lights;
+  sr_strand_instance instance;
+  sr_frame_desc fd;
+};
+
+int build_group_frame(const std::vector<PendingRequest>& g, GroupFrame& out) {
+  const sr_request& h = g[0].req;
+  const uint32_t nc = (uint32_t)g.size(), nl = h.light_count, F = nc + nl;
+  std::vector<sr_froxel_config> cfg(F);
+  std::vector<uint32_t> is_light(F, 0u);
+  for (uint32_t i = 0; i < nc; ++i) cfg[i] = g[i].req.camera;
+  for (uint32_t l = 0; l < nl; ++l) {
+    cfg[nc + l] = h.light_frusta[l];
+    is_light[nc + l] = 1u;
+  }
+  out.frusta.resize(F);
+  int rc = sr_build_frustum_table(cfg.data(), is_light.data(), nullptr, F, out.frusta.data());
+  if (rc != SR_OK) return rc;
+  out.views.resize(F);
+  for (uint32_t i = 0; i < nc; ++i) out.views[i] = g[i].req.camera_view;
+  for (uint32_t l = 0; l < nl; ++l) out.views[nc + l] = h.light_views[l];
+  out.lights.assign(h.lights, h.lights + nl);
+  out.instance = h.instance;
+  std::memset(&out.fd, 0, sizeof(out.fd));
+  out.fd.instances = &out.instance;
+  out.fd.instance_count = 1;
+  out.fd.frustum_count = F;
+  out.fd.frusta = out.frusta.data();
+  out.fd.views = out.views.data();
+  out.fd.lights = nl ? out.lights.data() : nullptr;
+  out.fd.light_count = nl;
+  out.fd.settings = h.settings;
+  return SR_OK;
+}
+
+}  // namespace
+
+struct sr_server {
+  sr_context* ctx = nullptr;
+  sr_server_config cfg{};
+  uint64_t budget = 0;
+  std::vector<ServerSlot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<PendingRequest> queue;
+  std::deque<sr_request_result> results;
+  sr_server_stats stats{};
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+  Clock::time_point t0;
+  std::thread worker;
+
+  double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+
+  uint64_t held_bytes() {
+    uint64_t b;
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      b = ctx->slabs.capacity();
+    }
+    for (auto& s : slots) b += s.fs->held_bytes() + s.out_bytes;
+    return b;
+  }
+
+  void finish(ServerSlot& s, int status) {
+    std::vector<sr_request_result> done;
+    const double t = now_ms();
+    for (size_t i = 0; i < s.reqs.size(); ++i) {
+      const PendingRequest& p = s.reqs[i];
+      if (status == SR_OK) {
+        const uint64_t px = (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height;
+        const float* base = static_cast<const float*>(s.out) + s.out_off[i];
+        if (p.req.out_rgba) std::memcpy(p.req.out_rgba, base, px * 16u);
+        if (p.req.out_depth) std::memcpy(p.req.out_depth, base + px * 4u, px * 4u);
+      }
+      sr_request_result r{};
+      r.ticket = p.ticket;
+      r.tag = p.req.tag;
+      r.status = status;
+      r.frame_views = (uint32_t)s.reqs.size();
+      r.submit_ms = p.submit_ms;
+      r.launch_ms = s.launch_ms;
+      r.done_ms = t;
+      r.counters = s.counters;
+      done.push_back(r);
+    }
+    s.reqs.clear();
+    s.busy = false;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& r : done) {
+      results.push_back(r);
+      if (r.status == SR_OK) ++stats.completed; else ++stats.failed;
+    }
+    cv_done.notify_all();
+  }
+
+  void fail_requests(std::vector<PendingRequest>& g, int status) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& p : g) {
+      sr_request_result r{};
+      r.ticket = p.ticket;
+      r.tag = p.req.tag;
+      r.status = status;
+      r.submit_ms = p.submit_ms;
+      r.launch_ms = r.done_ms = now_ms();
+      results.push_back(r);
+      ++stats.failed;
+    }
+    cv_done.notify_all();
+  }
+
+  // Completes finished frames; returns true when any slot is busy.
+  bool reap() {
+    bool any = false;
+    for (auto& s : slots) {
+      if (!s.busy) continue;
+      hipError_t q = hipEventQuery(s.out_ev);
+      if (q == hipErrorNotReady) { any = true; continue; }
+      std::string err;
+      int rc = q == hipSuccess ? frame_wait_slot(*s.fs, &s.counters, err) : SR_ERR_DEVICE;
+      finish(s, rc);
+    }
+    return any;
+  }
+
+  // Launches one group on slot s: frame, then the copies of every camera's
+  // composite and depth into the slot's pinned staging.
+  int launch(ServerSlot& s, std::vector<PendingRequest>& g, GroupFrame& gf, std::string& err) {
+    uint64_t floats = 0;
+    s.out_off.clear();
+    for (auto& p : g) {
+      s.out_off.push_back(floats);
+      floats += (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height * 5u;
+    }
+    if (floats * 4u > s.out_bytes) {
+      if (s.out) (void)hipHostFree(s.out);
+      s.out = nullptr;
+      s.out_bytes = 0;
+      if (hipHostMalloc(&s.out, floats * 4u, hipHostMallocDefault) != hipSuccess)
+        return efail(err, SR_ERR_OUT_OF_MEMORY, "server: pinned output staging");
+      s.out_bytes = floats * 4u;
+    }
+    int rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, nullptr, false);
+    if (rc != SR_OK) return rc;
+    const FrameSlot& fs = *s.fs;
+    const float4* comp = fs.buf[B_COMPOSITE].as<float4>();
+    const float* dep = fs.buf[B_DEPTH].as<float>();
+    uint64_t img_off = 0;
+    for (size_t i = 0; i < g.size(); ++i) {
+      const uint64_t px = (uint64_t)g[i].req.camera.screen_width * g[i].req.camera.screen_height;
+      float* dst = static_cast<float*>(s.out) + s.out_off[i];
+      HIP_OKS(hipMemcpyAsync(dst, comp + img_off, px * 16u, hipMemcpyDeviceToHost, fs.stream), "server: copy image");
+      HIP_OKS(hipMemcpyAsync(dst + px * 4u, dep + img_off, px * 4u, hipMemcpyDeviceToHost, fs.stream),
+              "server: copy depth");
+      img_off += px;
+    }
+    HIP_OKS(hipEventRecord(s.out_ev, fs.stream), "server: event");
+    s.reqs = std::move(g);
+    s.busy = true;
+    s.launch_ms = now_ms();
+    return SR_OK;
+  }
+
+  // Admits queued requests into free slots; returns true when it launched.
+  bool admit() {
+    bool launched = false;
+    for (auto& s : slots) {
+      if (s.busy) continue;
+      std::vector<PendingRequest> g;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (queue.empty()) return launched;
+        g.push_back(queue.front());
+        queue.pop_front();
+        for (auto it = queue.begin(); it != queue.end() && g.size() < cfg.max_views_per_frame;) {
+          if (same_group(g[0].req, it->req)) {
+            g.push_back(*it);
+            it = queue.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      }
+      for (;;) {
+        GroupFrame gf;
+        int rc = build_group_frame(g, gf);
+        std::string err;
+        uint64_t need = 0;
+        if (rc == SR_OK) rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, &need, true);
+        if (rc != SR_OK) {
+          fail_requests(g, rc);
+          break;
+        }
+        uint64_t outb = 0;
+        for (auto& p : g) outb += (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height * 20u;
+        auto total = [&]() { return held_bytes() - s.fs->held_bytes() - s.out_bytes + need + std::max(s.out_bytes, outb); };
+        if (budget && total() > budget) {
+          for (auto& o : slots)  // idle slots give their buffers back
+            if (!o.busy && &o != &s) o.fs->release_buffers();
+          if (total() > budget) {
+            bool others_busy = false;
+            for (auto& o : slots) others_busy = others_busy || (o.busy && &o != &s);
+            if (others_busy) {  // wait for memory to free up; requests go back to the front
+              std::lock_guard<std::mutex> lk(mu);
+              for (auto it = g.rbegin(); it != g.rend(); ++it) queue.push_front(*it);
+              ++stats.budget_waits;
+              return launched;
+            }
+            s.fs->release_buffers();
+            rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, &need, true);
+            if (rc == SR_OK && total() > budget) {
+              if (g.size() > 1) {  // split: the tail goes back to the queue
+                std::lock_guard<std::mutex> lk(mu);
+                for (size_t k = g.size(); k-- > 1;) queue.push_front(g[k]);
+                g.resize(1);
+                continue;
+              }
+              std::vector<PendingRequest> one(1, g[0]);
+              fail_requests(one, SR_ERR_CAPACITY);
+              break;
+            }
+            if (rc != SR_OK) {
+              fail_requests(g, rc);
+              break;
+            }
+          }
+        }
+        const uint64_t held = total();
+        rc = launch(s, g, gf, err);
+        if (rc != SR_OK) {
+          fail_requests(g, rc);
+          break;
+        }
+        launched = true;
+        std::lock_guard<std::mutex> lk(mu);
+        ++stats.frames;
+        stats.peak_held_bytes = std::max(stats.peak_held_bytes, held);
+        stats.max_frame_views = std::max(stats.max_frame_views, (uint32_t)s.reqs.size());
+        uint32_t busy = 0;
+        for (auto& o : slots) busy += o.busy ? 1u : 0u;
+        stats.max_concurrent_frames = std::max(stats.max_concurrent_frames, busy);
+        break;
+      }
+    }
+    return launched;
+  }
+
+  void run() {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+      bool busy = reap();
+      bool launched = admit();
+      busy = busy || launched;
+      std::unique_lock<std::mutex> lk(mu);
+      if (stopping && queue.empty() && !busy) {
+        bool any = false;
+        for (auto& s : slots) any = any || s.busy;
+        if (!any) break;
+      }
+      if (busy) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(50));  // poll the slot events
+      } else if (queue.empty() && !stopping) {
+        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      }
+    }
+  }
+};
+
+extern "C" {
+
+int sr_server_start(sr_context* ctx, const sr_server_config* config, sr_server** out) {
+  if (!ctx || !out) return SR_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  sr_server* srv = new sr_server();
+  srv->ctx = ctx;
+  if (config) srv->cfg = *config;
+  if (srv->cfg.slots == 0) srv->cfg.slots = 2;
+  if (srv->cfg.max_views_per_frame == 0) srv->cfg.max_views_per_frame = 16;
+  srv->budget = srv->cfg.hbm_budget_bytes ? srv->cfg.hbm_budget_bytes : ctx->hbm_budget;
+  srv->slots.resize(srv->cfg.slots);
+  for (auto& s : srv->slots) {
+    std::string err;
+    s.fs = new_slot(&err);
+    if (!s.fs || hipEventCreateWithFlags(&s.out_ev, hipEventDisableTiming) != hipSuccess) {
+      for (auto& o : srv->slots) {
+        if (o.fs) free_slot(o.fs);
+        if (o.out_ev) (void)hipEventDestroy(o.out_ev);
+      }
+      delete srv;
+      return fail(ctx, SR_ERR_DEVICE, "sr_server_start: " + err);
+    }
+  }
+  srv->t0 = Clock::now();
+  srv->worker = std::thread([srv] { srv->run(); });
+  *out = srv;
+  return SR_OK;
+}
+
+int sr_server_submit(sr_server* srv, const sr_request* req, uint64_t* ticket) {
+  if (!srv || !req) return SR_ERR_INVALID_ARGUMENT;
+  if (req->light_count > SR_SERVER_MAX_LIGHTS || req->camera.screen_width == 0 || req->camera.screen_height == 0)
+    return fail(srv->ctx, SR_ERR_INVALID_ARGUMENT, "sr_server_submit: bad request");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (srv->stopping) return fail(srv->ctx, SR_ERR_NOT_READY, "sr_server_submit: server is stopping");
+  PendingRequest p{*req, srv->next_ticket++, srv->now_ms()};
+  srv->queue.push_back(p);
+  ++srv->stats.submitted;
+  if (ticket) *ticket = p.ticket;
+  srv->cv_work.notify_one();
+  return SR_OK;
+}
+
+int sr_server_poll(sr_server* srv, sr_request_result* results, uint32_t capacity, uint32_t* count, int32_t wait_ms) {
+  if (!srv || !count) return SR_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(srv->mu);
+  if (srv->results.empty() && wait_ms > 0)
+    srv->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !srv->results.empty(); });
+  uint32_t n = 0;
+  while (n < capacity && !srv->results.empty()) {
+    if (results) results[n] = srv->results.front();
+    srv->results.pop_front();
+    ++n;
+  }
+  *count = n;
+  return SR_OK;
+}
+
+int sr_server_stats_get(sr_server* srv, sr_server_stats* out) {
+  if (!srv || !out) return SR_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  *out = srv->stats;
+  return SR_OK;
+}
+
+int sr_server_stop(sr_server* srv) {
+  if (!srv) return SR_ERR_INVALID_ARGUMENT;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stopping = true;
+    srv->cv_work.notify_all();
+  }
+  if (srv->worker.joinable()) srv->worker.join();
+  (void)hipSetDevice(srv->ctx->device);
+  for (auto& s : srv->slots) {
+    free_slot(s.fs);
+    if (s.out) (void)hipHostFree(s.out);
+    if (s.out_ev) (void)hipEventDestroy(s.out_ev);
+  }
+  delete srv;
   return SR_OK;
 }
 

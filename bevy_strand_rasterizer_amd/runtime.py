@@ -112,3 +112,79 @@ class StrandRenderer:
             self.close()
         except Exception:
             pass
+
+
+class StrandServer:
+    """The native serving loop (sr_server_*) over a renderer's resident assets.
+
+    ``submit`` copies a request record (``scene.request_from_workload``) and
+    returns a ticket; the composite image (float32 H x W x 4) and depth
+    (H x W) land in arrays owned here and come back from ``poll`` with the
+    request's timing and status."""
+
+    def __init__(self, renderer: StrandRenderer, slots: int = 2, max_views: int = 16, hbm_budget: int = 0):
+        self.r = renderer
+        self.lib = renderer.lib
+        cfg = np.zeros(1, abi.server_config_dtype)
+        cfg["slots"], cfg["max_views_per_frame"], cfg["hbm_budget_bytes"] = slots, max_views, hbm_budget
+        h = C.c_void_p()
+        abi.check(self.lib, self.lib.sr_server_start(renderer.ctx, abi.ptr(cfg), C.byref(h)), renderer.ctx,
+                  "sr_server_start")
+        self.srv = h
+        self._pending = {}
+
+    def submit(self, request: np.ndarray, outputs: bool = True) -> int:
+        req = np.array(request, dtype=abi.request_dtype).reshape(1)
+        w, h = int(req["camera"]["screen_width"][0]), int(req["camera"]["screen_height"][0])
+        rgba = np.zeros((h, w, 4), np.float32) if outputs else None
+        depth = np.zeros((h, w), np.float32) if outputs else None
+        req["out_rgba"] = rgba.ctypes.data if outputs else 0
+        req["out_depth"] = depth.ctypes.data if outputs else 0
+        t = C.c_uint64()
+        abi.check(self.lib, self.lib.sr_server_submit(self.srv, abi.ptr(req), C.byref(t)), self.r.ctx,
+                  "sr_server_submit")
+        self._pending[t.value] = (rgba, depth)
+        return t.value
+
+    def poll(self, wait_ms: int = 0, capacity: int = 256) -> list:
+        res = np.zeros(capacity, abi.request_result_dtype)
+        n = C.c_uint32()
+        abi.check(self.lib, self.lib.sr_server_poll(self.srv, abi.ptr(res), capacity, C.byref(n), wait_ms),
+                  self.r.ctx, "sr_server_poll")
+        out = []
+        for rec in res[: n.value]:
+            rgba, depth = self._pending.pop(int(rec["ticket"]), (None, None))
+            out.append({"ticket": int(rec["ticket"]), "tag": int(rec["tag"]), "status": int(rec["status"]),
+                        "frame_views": int(rec["frame_views"]), "submit_ms": float(rec["submit_ms"]),
+                        "launch_ms": float(rec["launch_ms"]), "done_ms": float(rec["done_ms"]),
+                        "latency_ms": float(rec["done_ms"] - rec["submit_ms"]),
+                        "counters": {k: int(rec["counters"][k]) for k in abi.frame_counters_dtype.names},
+                        "rgba": rgba, "depth": depth})
+        return out
+
+    def drain(self, count: int, timeout_s: float = 120.0) -> list:
+        """Polls until `count` results have arrived."""
+        import time
+
+        got, t0 = [], time.monotonic()
+        while len(got) < count:
+            if time.monotonic() - t0 > timeout_s:
+                raise TimeoutError(f"{len(got)} of {count} results after {timeout_s} s")
+            got += self.poll(wait_ms=50)
+        return got
+
+    def stats(self) -> dict:
+        st = np.zeros(1, abi.server_stats_dtype)
+        abi.check(self.lib, self.lib.sr_server_stats_get(self.srv, abi.ptr(st)), self.r.ctx, "sr_server_stats_get")
+        return {k: int(st[0][k]) for k in abi.server_stats_dtype.names}
+
+    def stop(self) -> None:
+        if getattr(self, "srv", None):
+            self.lib.sr_server_stop(self.srv)
+            self.srv = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()

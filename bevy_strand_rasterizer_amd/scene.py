@@ -344,6 +344,43 @@ def build_workload(name: str, asset: Asset, cameras: list, light_count: int, lig
                     settings=st)
 
 
+def with_asset_id(w: Workload, asset_id: int) -> Workload:
+    """The same workload drawing the uploaded asset `asset_id`."""
+    inst = w.instances.copy()
+    for k in ("vertex_id", "index_id", "meta_id", "geo_id", "material_id"):
+        inst[k] = asset_id
+    return Workload(name=w.name, assets=w.assets, instances=inst, frusta=w.frusta, views=w.views, lights=w.lights,
+                    settings=dict(w.settings))
+
+
+def request_from_workload(w: Workload, tag: int = 0) -> np.ndarray:
+    """A serving request (sr_request) equal to a one-camera workload: its
+    camera frustum (resolution, froxel size), view, lights and settings, and
+    its single instance (asset ids as in the workload)."""
+    cams = np.nonzero(w.frusta["kind"] == 0)[0]
+    lights = np.nonzero(w.frusta["kind"] == 1)[0]
+    if len(cams) != 1 or len(w.instances) != 1 or len(lights) > abi.SERVER_MAX_LIGHTS:
+        raise ValueError("a request holds one camera, one instance and at most 4 lights")
+    r = np.zeros(1, abi.request_dtype)
+    r["instance"] = w.instances[0]
+    f = w.frusta[cams[0]]
+    for k in ("screen_width", "screen_height", "froxel_size_x", "froxel_size_y", "depth_slices"):
+        r["camera"][k] = f[k]
+    r["camera_view"] = w.views[cams[0]]
+    r["light_count"] = len(lights)
+    for j, li in enumerate(lights):
+        for k in ("screen_width", "screen_height", "froxel_size_x", "froxel_size_y", "depth_slices"):
+            r["light_frusta"][0, j][k] = w.frusta[li][k]
+        r["light_views"][0, j] = w.views[li]
+        r["lights"][0, j] = w.lights[j]
+    st = w.frame_settings()
+    for name, _ in st._fields_:
+        v = getattr(st, name)
+        r["settings"][name] = list(v) if not isinstance(v, (int, float)) else v
+    r["tag"] = tag
+    return r
+
+
 def config_small(froxel: int = 16, seed: int = 0x5EEDCAFE) -> Workload:
     """2,000 strands x 16 points, width 0.0008 m, 512x512 camera at 0.6 m,
     one directional light with a 512^2 deep shadow map of 4 layers."""

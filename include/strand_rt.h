@@ -495,6 +495,68 @@ int sr_stage_times(sr_context* ctx, float* dst_ms, int32_t capacity, int32_t* ou
 const char* sr_last_error(sr_context* ctx);
 
 /* ------------------------------------------------------------------------ */
+/* Serving: a request loop over device-resident assets (no reference
+ * counterpart; the reference renders one frame per Bevy render-graph run,
+ * src/plugin.rs:240-260).  A worker thread admits queued requests into frame
+ * slots (each slot owns a stream and grow-only buffers, so frames overlap on
+ * the device) while the HBM held by assets and slots stays within the budget;
+ * requests with the same instance, lights and settings that wait together
+ * are merged into one multi-view frame (cameras of different sizes).  Each
+ * request's image equals the image of the same request rendered alone. */
+#define SR_SERVER_MAX_LIGHTS 4u
+
+typedef struct sr_server sr_server;
+
+typedef struct sr_server_config {
+  uint32_t slots;               /* frames in flight; 0 = 2 */
+  uint32_t max_views_per_frame; /* requests merged into one frame; 0 = 16 */
+  uint64_t hbm_budget_bytes;    /* 0 = the context's budget (0 there = unlimited) */
+} sr_server_config;
+
+typedef struct sr_request {
+  sr_strand_instance instance;  /* asset ids + transform (one instance) */
+  sr_froxel_config camera;      /* resolution and froxel size of the view */
+  sr_view camera_view;
+  uint32_t light_count;         /* 0..SR_SERVER_MAX_LIGHTS */
+  uint32_t pad;
+  sr_froxel_config light_frusta[SR_SERVER_MAX_LIGHTS];
+  sr_view light_views[SR_SERVER_MAX_LIGHTS];
+  sr_directional_light lights[SR_SERVER_MAX_LIGHTS];
+  sr_frame_settings settings;
+  float* out_rgba;              /* host, width * height * 4 (composite, straight alpha) or NULL */
+  float* out_depth;             /* host, width * height or NULL */
+  uint64_t tag;                 /* returned with the result */
+} sr_request;
+
+typedef struct sr_request_result {
+  uint64_t ticket;
+  uint64_t tag;
+  int32_t status;               /* SR_OK, or why the request failed (SR_ERR_CAPACITY: above the budget alone) */
+  uint32_t frame_views;         /* cameras in the frame that served it */
+  double submit_ms;             /* server clock (ms since start) at sr_server_submit */
+  double launch_ms;             /* frame launched */
+  double done_ms;               /* outputs written to the caller's buffers */
+  sr_frame_counters counters;   /* of the frame that served it */
+} sr_request_result;
+
+typedef struct sr_server_stats {
+  uint64_t submitted, completed, failed, frames;
+  uint64_t peak_held_bytes;     /* assets + slot buffers, the most seen at an admission */
+  uint64_t budget_waits;        /* admissions deferred by the budget */
+  uint32_t max_frame_views, max_concurrent_frames;
+} sr_server_stats;
+
+int sr_server_start(sr_context* ctx, const sr_server_config* config, sr_server** out_server);
+/* Copies the request; out buffers must stay valid until its result is polled. */
+int sr_server_submit(sr_server* server, const sr_request* request, uint64_t* out_ticket);
+/* Up to `capacity` finished results; waits up to wait_ms (0 = no wait) for the first. */
+int sr_server_poll(sr_server* server, sr_request_result* results, uint32_t capacity, uint32_t* out_count,
+                   int32_t wait_ms);
+int sr_server_stats_get(sr_server* server, sr_server_stats* out);
+/* Finishes every queued and in-flight request, joins the worker, frees the slots. */
+int sr_server_stop(sr_server* server);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic scene generator (configs of the benchmark)                      */
 /* ------------------------------------------------------------------------ */
 typedef struct sr_scalp_params {

@@ -37,6 +37,11 @@ STRUCTS = [
     ("sr_view", abi.view_dtype, 80, "per-frustum view (new)"),
     ("sr_directional_light", abi.light_dtype, 32, "directional light (new)"),
     ("sr_frame_counters", abi.frame_counters_dtype, 64, "frame counters (new)"),
+    ("sr_strand_cache_info", abi.strand_cache_info_dtype, 72, "strand cache parse result (new)"),
+    ("sr_request", abi.request_dtype, 904, "serving request (new)"),
+    ("sr_request_result", abi.request_result_dtype, 112, "serving result (new)"),
+    ("sr_server_config", abi.server_config_dtype, 16, "serving config (new)"),
+    ("sr_server_stats", abi.server_stats_dtype, 56, "serving stats (new)"),
 ]
 
 
