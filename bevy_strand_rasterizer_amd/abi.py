@@ -24,6 +24,7 @@ MAX_DOM_LAYERS = 16
 BINNING_FRUSTUM_SHIFT = 3
 CONFIG_CSR_GLOBAL_SORT = 1  # sr_device_config.flags
 CONFIG_CSR_FEW_HITS = 2
+CONFIG_HIP_GRAPHS = 4
 
 # sr_status
 SR_OK = 0
@@ -148,6 +149,12 @@ class ScalpParams(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
+tune_trial_dtype = np.dtype([("fine_binning_backend", "<u4"), ("camera_froxel", "<u4"), ("light_froxel", "<u4"),
+                             ("hip_graphs", "<u4"), ("ms_per_frame", "<f4"), ("psnr_db", "<f4"), ("accepted", "<u4"),
+                             ("pad", "<u4")])
+tune_result_dtype = np.dtype([("best", tune_trial_dtype), ("trial_count", "<u4"), ("pad", "<u4"),
+                              ("trials", tune_trial_dtype, 16)])
+
 SERVER_MAX_LIGHTS = 4
 request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel_config_dtype),
                           ("camera_view", view_dtype), ("light_count", "<u4"), ("pad", "<u4"),
@@ -193,6 +200,7 @@ _SIGNATURES = {
     "sr_light_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]),
     "sr_strand_cache_parse": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_char_p, C.c_uint64]),
+    "sr_autotune": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.c_float, C.c_void_p]),
     "sr_server_start": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "sr_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "sr_server_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_int32]),

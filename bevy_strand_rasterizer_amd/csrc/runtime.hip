@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -168,13 +169,27 @@ struct FrameSlot {
   bool submitted = false;
   uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0, cand_cap_hint = 0;
   int hist_cur = 0;
+  // captured launch sequences, keyed by everything the launches bake in
+  struct Graph {
+    std::vector<uint8_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+  };
+  std::vector<Graph> graphs;
+  uint64_t graph_clock = 0, graph_captures = 0, graph_replays = 0;
 
+  void drop_graphs() {
+    for (Graph& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   uint64_t held_bytes() const {
     uint64_t b = 0;
     for (const DBuf& d : buf) b += d.bytes;
     return b;
   }
   void release_buffers() {
+    drop_graphs();  // they point into the buffers
     for (DBuf& d : buf) d.free_();
     shape.valid = false;
   }
@@ -196,7 +211,7 @@ FrameSlot* new_slot(std::string* err) {
 void free_slot(FrameSlot* sl) {
   if (!sl) return;
   if (sl->stream) (void)hipStreamSynchronize(sl->stream);
-  sl->release_buffers();
+  sl->release_buffers();  // and the graphs
   for (int k = 0; k < FrameSlot::kRing; ++k) {
     if (sl->staging[k]) (void)hipHostFree(sl->staging[k]);
     if (sl->staging_ev[k]) (void)hipEventDestroy(sl->staging_ev[k]);
@@ -585,6 +600,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     (void)hipStreamSynchronize(s);
     (void)hipStreamSynchronize(sl.stream);
   }
+  if (grow) sl.drop_graphs();  // captured launches hold the old buffer addresses
   for (const Need& n : needs) {
     if (!sl.buf[n.id].ensure(n.bytes)) return efail(err, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
   }
@@ -670,31 +686,85 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.slot_hits = sl.buf[B_SLOT_HITS].as<uint32_t>();
   fp.slot_hit_count = sl.buf[B_SLOT_HIT_COUNT].as<uint8_t>();
 
+  auto launches = [&]() {
+    sl.timer.mark(s, "clear");
+    launch_fill_u32(fp.counters, C_COUNT, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.image), img_px * 4, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.depth), img_px, 0u, s);
+    // texels no segment writes this frame read as zero (the reference leaves
+    // them stale; dumps must be deterministic)
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4, 0u, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2, 0u, s);
+    launch_prepass(fp, ps, s, &sl.timer);
+    sl.timer.mark(s, "shadow");
+    launch_shadow(fp, s, max_light_tile_px);
+    sl.timer.mark(s, "stampback");
+    for (uint32_t i = 0; i < F; ++i)
+      if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
+    sl.timer.mark(s, "shading");
+    launch_shading(fp, s);
+    sl.timer.mark(s, "raster");
+    launch_camera_raster(fp, s, max_cam_tile_px);
+    sl.timer.mark(s, "composite");
+    launch_composite(fp, img_px, s);
+    sl.timer.mark(s, "end");
+  };
   sl.timer.begin();
-  sl.timer.mark(s, "clear");
-  launch_fill_u32(fp.counters, C_COUNT, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.image), img_px * 4, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.depth), img_px, 0u, s);
-  // texels no segment writes this frame read as zero (the reference leaves
-  // them stale; dumps must be deterministic)
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4, 0u, s);
-  launch_fill_u32(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2, 0u, s);
-  launch_prepass(fp, ps, s, &sl.timer);
-  sl.timer.mark(s, "shadow");
-  launch_shadow(fp, s, max_light_tile_px);
-  sl.timer.mark(s, "stampback");
-  for (uint32_t i = 0; i < F; ++i)
-    if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
-  sl.timer.mark(s, "shading");
-  launch_shading(fp, s);
-  sl.timer.mark(s, "raster");
-  launch_camera_raster(fp, s, max_cam_tile_px);
-  sl.timer.mark(s, "composite");
-  launch_composite(fp, img_px, s);
-  sl.timer.mark(s, "end");
+  if ((ctx->flags & SR_CONFIG_HIP_GRAPHS) && !sl.timer.enabled) {
+    // The sequence reads every per-frame table from device memory, so one
+    // capture serves every frame whose launches bake in the same values:
+    // FrameParams (buffers, capacities, settings), the launch extents and
+    // the stampback targets.
+    std::vector<uint8_t> key(sizeof(fp) + 8 * sizeof(uint64_t) + 2 * sizeof(uint32_t) * F);
+    uint8_t* k = key.data();
+    std::memcpy(k, &fp, sizeof(fp));
+    k += sizeof(fp);
+    const uint64_t ext[8] = {dom_px, stamp_px, img_px, atlas_n, max_light_tile_px, max_cam_tile_px, F, 0};
+    std::memcpy(k, ext, sizeof(ext));
+    k += sizeof(ext);
+    for (uint32_t i = 0; i < F; ++i) {
+      const uint32_t st2[2] = {fr[i].d.kind == 1u ? fr[i].stamp_w : 0u, fr[i].d.kind == 1u ? fr[i].stamp_h : 0u};
+      std::memcpy(k, st2, sizeof(st2));
+      k += sizeof(st2);
+    }
+    FrameSlot::Graph* g = nullptr;
+    for (auto& c : sl.graphs)
+      if (c.key == key) g = &c;
+    if (!g) {
+      if (sl.graphs.size() >= 4) {  // least recently used goes
+        auto lru = std::min_element(sl.graphs.begin(), sl.graphs.end(),
+                                    [](const FrameSlot::Graph& a, const FrameSlot::Graph& b) { return a.last_use < b.last_use; });
+        (void)hipStreamSynchronize(s);  // it may still be running
+        if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+        sl.graphs.erase(lru);
+      }
+      hipGraph_t graph = nullptr;
+      HIP_OKS(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "sr_frame_submit: begin capture");
+      launches();
+      hipError_t ce = hipStreamEndCapture(s, &graph);
+      if (ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return efail(err, SR_ERR_DEVICE, std::string("sr_frame_submit: capture: ") + hipGetErrorString(ce));
+      }
+      FrameSlot::Graph ng;
+      ng.key = std::move(key);
+      hipError_t ie = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return efail(err, SR_ERR_DEVICE, std::string("sr_frame_submit: instantiate: ") + hipGetErrorString(ie));
+      sl.graphs.push_back(std::move(ng));
+      g = &sl.graphs.back();
+      ++sl.graph_captures;
+    } else {
+      ++sl.graph_replays;
+    }
+    g->last_use = ++sl.graph_clock;
+    HIP_OKS(hipGraphLaunch(g->exec, s), "sr_frame_submit: graph launch");
+  } else {
+    launches();
+  }
   HIP_OKS(hipGetLastError(), "sr_frame_submit: launch");
   HIP_OKS(hipEventRecord(sl.done, s), "sr_frame_submit: event");
 
@@ -1248,3 +1318,126 @@ int sr_server_stop(sr_server* srv) {
 }
 
 }  // extern "C"
+
+// ==================================================================== autotune
+// Times candidate configurations of one frame on the context's main slot:
+// fine-binning backend x camera froxel size x light froxel size, with the
+// launch sequence in a HIP graph, plus the frame as given without graphs
+// (the quality reference).  A candidate is accepted when its composite is
+// within min_psnr_db of the reference (tile size changes the order in which
+// overlapping fragments blend); the fastest accepted one is `best`.
+namespace {
+
+struct TuneFrame {
+  std::vector<sr_frustum_desc> frusta;
+  sr_frame_desc fd;
+};
+
+int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint32_t light_fx, TuneFrame& out) {
+  const uint32_t F = src->frustum_count;
+  std::vector<sr_froxel_config> cfg(F);
+  std::vector<uint32_t> is_light(F), cascade(F);
+  for (uint32_t i = 0; i < F; ++i) {
+    const sr_frustum_desc& d = src->frusta[i];
+    const uint32_t fx = d.kind == 1u ? light_fx : cam_fx;
+    cfg[i] = sr_froxel_config{d.screen_width, d.screen_height, fx ? fx : d.froxel_size_x, fx ? fx : d.froxel_size_y,
+                              d.depth_slices};
+    is_light[i] = d.kind == 1u ? 1u : 0u;
+    cascade[i] = d.cascade_index;
+  }
+  out.frusta.resize(F);
+  int rc = sr_build_frustum_table(cfg.data(), is_light.data(), cascade.data(), F, out.frusta.data());
+  if (rc != SR_OK) return rc;
+  out.fd = *src;
+  out.fd.frusta = out.frusta.data();
+  out.fd.settings.fine_binning_backend = backend;
+  return SR_OK;
+}
+
+}  // namespace
+
+extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t frames_per_trial, float min_psnr_db,
+                           sr_tune_result* out) {
+  if (!ctx || !fd || !out || fd->frustum_count == 0) return SR_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  FrameSlot& sl = ctx->main();
+  const uint32_t saved_flags = ctx->flags;
+  const bool saved_timer = sl.timer.enabled;
+  sl.timer.enabled = false;
+  std::memset(out, 0, sizeof(*out));
+  uint32_t cam0 = 0, light0 = 0;
+  for (uint32_t i = 0; i < fd->frustum_count; ++i) {
+    if (fd->frusta[i].kind == 0u && !cam0) cam0 = fd->frusta[i].froxel_size_x;
+    if (fd->frusta[i].kind == 1u && !light0) light0 = fd->frusta[i].froxel_size_x;
+  }
+  struct Cand { uint32_t backend, cam, light, graphs; };
+  std::vector<Cand> cands;
+  cands.push_back({fd->settings.fine_binning_backend, cam0, light0, 0u});  // the reference
+  for (uint32_t b : {0u, 1u})
+    for (uint32_t cf : {8u, 16u})
+      for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, 1u});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<float4> ref_img, img;
+  int rc = SR_OK;
+  std::string err;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SR_ERR_DEVICE;
+  const uint32_t frames = frames_per_trial ? frames_per_trial : 3;
+  float best_ms = 0.0f;
+  for (size_t c = 0; c < cands.size() && rc == SR_OK && out->trial_count < 16; ++c) {
+    const Cand& k = cands[c];
+    TuneFrame tf;
+    rc = tune_frame(fd, k.backend, k.cam, k.light, tf);
+    if (rc != SR_OK) break;
+    ctx->flags = k.graphs ? (saved_flags | SR_CONFIG_HIP_GRAPHS) : (saved_flags & ~SR_CONFIG_HIP_GRAPHS);
+    sr_frame_counters fc;
+    // warm: grows the buffers and captures the graph
+    for (int w = 0; w < 2 && rc == SR_OK; ++w) {
+      rc = frame_submit_slot(ctx, sl, &tf.fd, nullptr, err, nullptr, false);
+      if (rc == SR_OK) rc = frame_wait_slot(sl, &fc, err);
+    }
+    if (rc != SR_OK) break;
+    (void)hipEventRecord(e0, sl.stream);
+    for (uint32_t f = 0; f < frames && rc == SR_OK; ++f) rc = frame_submit_slot(ctx, sl, &tf.fd, nullptr, err, nullptr, false);
+    (void)hipEventRecord(e1, sl.stream);
+    if (rc == SR_OK) rc = frame_wait_slot(sl, &fc, err);
+    if (rc != SR_OK) break;
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    img.resize(sl.shape.img_px);
+    if (!img.empty() &&
+        hipMemcpy(img.data(), sl.buf[B_COMPOSITE].p, img.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = SR_ERR_DEVICE;
+      break;
+    }
+    double mse = 0.0;
+    if (c == 0) {
+      ref_img = img;
+    } else if (ref_img.size() == img.size()) {
+      for (size_t i = 0; i < img.size(); ++i) {
+        const float* a = &img[i].x;
+        const float* b = &ref_img[i].x;
+        for (int q = 0; q < 4; ++q) mse += (double)(a[q] - b[q]) * (a[q] - b[q]);
+      }
+      mse /= 4.0 * (double)std::max<size_t>(img.size(), 1);
+    }
+    sr_tune_trial& t = out->trials[out->trial_count++];
+    t.fine_binning_backend = k.backend;
+    t.camera_froxel = k.cam;
+    t.light_froxel = k.light;
+    t.hip_graphs = k.graphs;
+    t.ms_per_frame = ms / (float)frames;
+    t.psnr_db = mse > 0.0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.0f;
+    t.accepted = t.psnr_db >= min_psnr_db ? 1u : 0u;
+    if (t.accepted && (best_ms == 0.0f || t.ms_per_frame < best_ms)) {
+      best_ms = t.ms_per_frame;
+      out->best = t;
+    }
+  }
+  ctx->flags = saved_flags;
+  sl.timer.enabled = saved_timer;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc != SR_OK) return fail(ctx, rc, err.empty() ? "sr_autotune: device error" : err);
+  return SR_OK;
+}

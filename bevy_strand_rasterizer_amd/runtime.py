@@ -81,6 +81,19 @@ class StrandRenderer:
                       self.ctx, "sr_frame_dump")
         return out
 
+    def autotune(self, workload, frames_per_trial: int = 3, min_psnr_db: float = 50.0) -> dict:
+        """sr_autotune on this workload's frame: every trial and the best."""
+        fd = workload.frame_desc()
+        res = np.zeros(1, abi.tune_result_dtype)
+        abi.check(self.lib, self.lib.sr_autotune(self.ctx, C.byref(fd), frames_per_trial, min_psnr_db,
+                                                 abi.ptr(res)), self.ctx, "sr_autotune")
+        names = abi.tune_trial_dtype.names
+        trials = [{k: (float(t[k]) if k in ("ms_per_frame", "psnr_db") else int(t[k])) for k in names if k != "pad"}
+                  for t in res[0]["trials"][: int(res[0]["trial_count"])]]
+        best = {k: (float(res[0]["best"][k]) if k in ("ms_per_frame", "psnr_db") else int(res[0]["best"][k]))
+                for k in names if k != "pad"}
+        return {"best": best, "trials": trials}
+
     def set_profiling(self, enabled: bool) -> None:
         abi.check(self.lib, self.lib.sr_set_profiling(self.ctx, int(enabled)), self.ctx, "sr_set_profiling")
 

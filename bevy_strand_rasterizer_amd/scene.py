@@ -344,6 +344,23 @@ def build_workload(name: str, asset: Asset, cameras: list, light_count: int, lig
                     settings=st)
 
 
+def retile(w: Workload, camera_froxel: int = 0, light_froxel: int = 0, backend: int | None = None) -> Workload:
+    """The workload with its frustum table rebuilt for other froxel sizes (0
+    keeps a frustum's size) and, optionally, another binning backend: how an
+    autotuner result is applied."""
+    cfgs, kinds = [], []
+    for f in w.frusta:
+        size = int(light_froxel if f["kind"] == 1 else camera_froxel)
+        fx, fy = (size, size) if size else (int(f["froxel_size_x"]), int(f["froxel_size_y"]))
+        cfgs.append((int(f["screen_width"]), int(f["screen_height"]), fx, fy, int(f["depth_slices"])))
+        kinds.append(int(f["kind"]))
+    st = dict(w.settings)
+    if backend is not None:
+        st["fine_binning_backend"] = int(backend)
+    return Workload(name=w.name, assets=w.assets, instances=w.instances, frusta=frustum_table(cfgs, kinds),
+                    views=w.views, lights=w.lights, settings=st)
+
+
 def with_asset_id(w: Workload, asset_id: int) -> Workload:
     """The same workload drawing the uploaded asset `asset_id`."""
     inst = w.instances.copy()

@@ -455,6 +455,11 @@ typedef struct sr_device_config {
  * segment while marking (instead of 8), so most segments take the re-trace
  * path of the later passes (exercises it; results are identical). */
 #define SR_CONFIG_CSR_FEW_HITS 2u
+/* sr_device_config.flags: capture each frame's launch sequence in a HIP graph
+ * and replay it for later frames of the same shape (buffers, capacities,
+ * settings and launch extents); per-frame tables are uploaded before the
+ * replay.  Ignored while profiling is on. */
+#define SR_CONFIG_HIP_GRAPHS 4u
 
 /* Library version string. */
 const char* sr_version(void);
@@ -493,6 +498,37 @@ int sr_stage_times(sr_context* ctx, float* dst_ms, int32_t capacity, int32_t* ou
 
 /* Last error message of the context (or the global one when ctx is NULL). */
 const char* sr_last_error(sr_context* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner (no reference counterpart: the reference fixes its tile sizes and
+ * binning backend per run, src/resources.rs).  Times one frame under the
+ * candidate configurations -- fine-binning backend (segment scatter / page
+ * CSR) x camera froxel size {8, 16} x light froxel size {8, 16}, replayed from
+ * a HIP graph -- against the frame as given without graphs (the quality
+ * reference), on the context's main slot.  A candidate whose composite stays
+ * within min_psnr_db of the reference is accepted; `best` is the fastest
+ * accepted one.  The caller rebuilds its frustum table with best's froxel
+ * sizes (sr_build_frustum_table) and sets the backend and SR_CONFIG_HIP_GRAPHS. */
+typedef struct sr_tune_trial {
+  uint32_t fine_binning_backend;
+  uint32_t camera_froxel;       /* 0 when the frame has no camera */
+  uint32_t light_froxel;        /* 0 when the frame has no light */
+  uint32_t hip_graphs;
+  float ms_per_frame;
+  float psnr_db;                /* composite against the reference configuration */
+  uint32_t accepted;
+  uint32_t pad;
+} sr_tune_trial;
+
+typedef struct sr_tune_result {
+  sr_tune_trial best;
+  uint32_t trial_count;
+  uint32_t pad;
+  sr_tune_trial trials[16];
+} sr_tune_result;
+
+int sr_autotune(sr_context* ctx, const sr_frame_desc* frame, uint32_t frames_per_trial, float min_psnr_db,
+                sr_tune_result* out);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: a request loop over device-resident assets (no reference

@@ -42,6 +42,8 @@ STRUCTS = [
     ("sr_request_result", abi.request_result_dtype, 112, "serving result (new)"),
     ("sr_server_config", abi.server_config_dtype, 16, "serving config (new)"),
     ("sr_server_stats", abi.server_stats_dtype, 56, "serving stats (new)"),
+    ("sr_tune_trial", abi.tune_trial_dtype, 32, "autotuner trial (new)"),
+    ("sr_tune_result", abi.tune_result_dtype, 552, "autotuner result (new)"),
 ]
 
 

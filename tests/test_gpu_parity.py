@@ -164,3 +164,47 @@ def test_deterministic_repeat():
         b = {k: r.dump(k) for k in EXACT + FLOAT}
     for k in EXACT + FLOAT:
         assert a[k].tobytes() == b[k].tobytes(), k
+
+
+def test_graph_replays_match_eager():
+    """Frames replayed from a captured HIP graph equal eager frames, for the
+    frame that was captured and for another frame of the same shape (new
+    per-frame tables), replayed several times with eager work in between."""
+    w1 = scene.config_small(froxel=8)
+    w2 = scene.build_workload("small-orbit", w1.assets[0], [(scene.orbit_eye(3, 8, 0.6), 512, 512)], 1, 512, 4, 8, 8)
+    with StrandRenderer() as eager, StrandRenderer(flags=abi.CONFIG_HIP_GRAPHS) as graph:
+        eager.upload_workload(w1)
+        graph.upload_workload(w1)
+        want = {}
+        for name, w in (("w1", w1), ("w2", w2)):
+            eager.render(w)
+            want[name] = {k: eager.dump(k) for k in EXACT + FLOAT}
+        for rep in range(3):
+            for name, w in (("w1", w1), ("w2", w2)):
+                graph.render(w)
+                got = {k: graph.dump(k) for k in EXACT + FLOAT}
+                eager.render(w2 if name == "w1" else w1)  # eager work between replays
+                for k in EXACT + FLOAT:
+                    assert got[k].tobytes() == want[name][k].tobytes(), f"replay {rep} {name}: {k} differs"
+
+
+def test_autotune_picks_an_accepted_configuration():
+    w = scene.config_small(froxel=8)
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        res = r.autotune(w, frames_per_trial=2, min_psnr_db=40.0)
+    print(f"[autotune] best {res['best']}", flush=True)
+    trials = res["trials"]
+    assert len(trials) == 9
+    assert trials[0]["hip_graphs"] == 0 and trials[0]["psnr_db"] > 900  # the reference itself
+    assert res["best"]["accepted"] == 1 and res["best"]["ms_per_frame"] > 0
+    assert res["best"]["ms_per_frame"] == min(t["ms_per_frame"] for t in trials if t["accepted"])
+    # same froxel sizes and backend as the reference: the graph replay is exact
+    same = [t for t in trials[1:] if t["camera_froxel"] == 8 and t["light_froxel"] == 8
+            and t["fine_binning_backend"] == trials[0]["fine_binning_backend"]]
+    assert same and same[0]["psnr_db"] > 900
+    tuned = scene.retile(w, res["best"]["camera_froxel"], res["best"]["light_froxel"],
+                         res["best"]["fine_binning_backend"])
+    ref = oracle.render(tuned)
+    ctr, gpu = gpu_frame(tuned)
+    compare(tuned, ctr, gpu, ref)
