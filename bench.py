@@ -9,7 +9,14 @@ deep shadow map, 8x8 froxels.  Inputs are resident in HBM before timing.
 
 Prints one JSON line.  value = camera frames per second over K pipelined
 frames (whole job, one GPU); Msegments/s, p50/p99 frame latency, per-stage
-times, the roofline of the dominant kernel and the CPU baseline ride along.
+times, the roofline of the dominant kernel and the CPU baseline ride along,
+and so do the two system patterns:
+  batched -- many views and lights in one frame replayed from a HIP graph
+             with the autotuned configuration: the 2,000 x 16 scalp from 64
+             cameras over {720p, 1080p, 1440p} with 4 lights of 2048^2, and
+             the named asset from 8 1080p cameras sharing its light;
+  serving -- the request loop over 8 resident assets (10k..500k strands x
+             32 points) under a Poisson stream, p50/p99 latency and frames/s.
 """
 from __future__ import annotations
 
@@ -88,33 +95,139 @@ def cpu_baseline(threads: int, fraction: float = 0.25):
             "sample_seconds": round(dt, 2), "msegments_per_s": round(w.input_segments / dt / 1e6, 4)}
 
 
+def percentile_ms(xs):
+    return {"p50": round(percentile(xs, 0.5), 4), "p99": round(percentile(xs, 0.99), 4)}
+
+
+def time_frames(r, w, steps: int, warmup: int):
+    """Latency (one frame at a time) and throughput (`steps` pipelined
+    frames) of workload `w` on renderer `r`."""
+    import torch
+
+    for _ in range(max(1, warmup)):
+        r.submit(w)
+    r.wait()
+    lat = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r.submit(w)
+        r.wait()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.submit(w)
+    ctr = r.wait()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return ctr, lat, dt
+
+
+def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool = True):
+    """One multi-view frame per step, replayed from a HIP graph with the
+    autotuned backend and froxel sizes."""
+    from bevy_strand_rasterizer_amd import abi, scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    r = StrandRenderer(device=0, flags=abi.CONFIG_HIP_GRAPHS)
+    r.upload_workload(w)
+    best, trials = None, []
+    if tune:
+        res = r.autotune(w, frames_per_trial=2, min_psnr_db=50.0)
+        best, trials = res["best"], res["trials"]
+        w = scene.retile(w, best["camera_froxel"], best["light_froxel"], best["fine_binning_backend"])
+    ctr, lat, dt = time_frames(r, w, steps, warmup)
+    r.close()
+    views = w.camera_count
+    fps = views * steps / dt
+    out = {"workload": w.name, "views": views, "lights": int(len(w.lights)),
+           "resolutions": sorted({f"{int(f['screen_width'])}x{int(f['screen_height'])}"
+                                  for f in w.frusta if f["kind"] == 0}),
+           "frames_per_s": round(fps, 3), "msegments_per_s": round(w.input_segments * views * steps / dt / 1e6, 2),
+           "ms_per_step": round(dt * 1e3 / steps, 4), "latency_ms": percentile_ms(lat),
+           "tuned": best, "trials": trials,
+           "counters": {k: v for k, v in ctr.items() if not k.startswith("pad")}}
+    if plain_fps:
+        out["vs_plain"] = round(fps / plain_fps, 3)
+    return out
+
+
+def run_serving(seconds: float, rate: float, slots: int, max_views: int, budget_gib: float, seed: int = 7):
+    """Poisson arrivals at `rate` req/s for `seconds` over the 8 resident
+    serving assets; each request's frame stays on the device (no host
+    copy), its latency is submit -> frame complete on the server clock."""
+    import numpy as np
+
+    from bevy_strand_rasterizer_amd import scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer, StrandServer
+
+    assets = scene.serving_assets()
+    r = StrandRenderer(device=0)
+    ids = [r.upload(a).geo_id for a in assets]
+    rng = np.random.default_rng(seed)
+    n = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = [scene.serving_request(assets, ids, rng, tag=i) for i in range(n)]
+    segs = [assets[int(np.nonzero(np.array(ids) == int(q[0]["instance"]["geo_id"][0]))[0][0])].segment_count
+            for q in reqs]
+    res = []
+    with StrandServer(r, slots=slots, max_views=max_views, hbm_budget=int(budget_gib * (1 << 30))) as srv:
+        for k in range(min(16, n)):  # warm: every slot grows its buffers once
+            srv.submit(reqs[k][0], outputs=False)
+        srv.drain(min(16, n), timeout_s=300)
+        t0 = time.perf_counter()
+        for i in range(n):
+            while time.perf_counter() - t0 < arrivals[i]:
+                res += srv.poll(wait_ms=0)
+                time.sleep(0.0002)
+            srv.submit(reqs[i][0], outputs=False)
+        res += srv.drain(n - len(res), timeout_s=600)
+        st = srv.stats()
+    r.close()
+    ok = [x for x in res if x["status"] == 0]
+    lat = [x["latency_ms"] for x in ok]
+    span = (max(x["done_ms"] for x in ok) - min(x["submit_ms"] for x in ok)) / 1e3 if ok else 0.0
+    return {"workload": "8 resident scalps {10k..500k} x 32, Poisson arrivals",
+            "offered_req_per_s": rate, "seconds": seconds, "requests": n, "completed": len(ok),
+            "failed": len(res) - len(ok), "frames_per_s": round(len(ok) / span, 3) if span else None,
+            "msegments_per_s": round(sum(segs[x["tag"]] for x in ok) / span / 1e6, 2) if span else None,
+            "latency_ms": percentile_ms(lat) if lat else None,
+            "mean_views_per_frame": round(float(np.mean([x["frame_views"] for x in ok])), 3) if ok else None,
+            "slots": slots, "max_views_per_frame": max_views, "hbm_budget_gib": budget_gib,
+            "server": st, "resolutions": [f"{w}x{h}" for w, h in scene.SERVING_RESOLUTIONS]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="named", choices=["named", "small", "batched"])
+    ap.add_argument("--config", default="named", choices=["named", "small"])
     ap.add_argument("--backend", default="csr", choices=["scatter", "csr"],
                     help="fine binning backend (FineBinningBackend::SegmentScatter / PageCsr)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-max-views", type=int, default=16)
+    ap.add_argument("--serve-budget-gib", type=float, default=96.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent frames and views shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    import torch  # initialise the device through torch before the library does
+
+    torch.cuda.init()
 
     from bevy_strand_rasterizer_amd import scene
     from bevy_strand_rasterizer_amd.runtime import StrandRenderer
 
-    if args.config == "named":
-        w = scene.config_named()
-    elif args.config == "small":
-        w = scene.config_small()
-    else:
-        w = scene.config_batched()
+    w = scene.config_named() if args.config == "named" else scene.config_small()
     w.settings["fine_binning_backend"] = 1 if args.backend == "csr" else 0
     views = w.camera_count
     segs = w.input_segments
@@ -124,6 +237,7 @@ def main():
     for _ in range(max(1, args.warmup)):
         r.submit(w)
     ctr = r.wait()
+    print(f"bench: plain warm ({w.name})", file=sys.stderr, flush=True)
 
     # stage timing (HIP events on the library stream), a few frames
     r.set_profiling(True)
@@ -136,24 +250,11 @@ def main():
             stage_acc[k] = stage_acc.get(k, 0.0) + v / nprof
     r.set_profiling(False)
 
-    # latency: one frame at a time
-    lat = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        r.submit(w)
-        r.wait()
-        lat.append((time.perf_counter() - t0) * 1e3)
-
-    # throughput: K pipelined frames
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.submit(w)
-    ctr = r.wait()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    ctr, lat, dt = time_frames(r, w, args.steps, 0)
     ms_per_step = dt * 1e3 / args.steps
     fps = views * args.steps / dt
+    r.close()
+    print(f"bench: plain {fps:.2f} frames/s", file=sys.stderr, flush=True)
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
     roof = roofline_for(dom, stage_acc[dom], ctr, w)
@@ -163,6 +264,20 @@ def main():
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4),
                 "note": "dominant stage is LDS/VALU bound; binning rooflines in binning_roofline"}
+
+    batched = None
+    if not args.no_batched:
+        batched = {}
+        for name, wb in (("64views_4lights_2kx16", scene.config_batched()),
+                         ("named_8views_1light", scene.config_batched_named(8))):
+            batched[name] = run_batched(wb, steps=max(3, args.steps // 4), warmup=2, plain_fps=fps)
+            print(f"bench: batched {name} {batched[name]['frames_per_s']} frames/s", file=sys.stderr, flush=True)
+
+    serving = None
+    if not args.no_serving:
+        serving = run_serving(args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_max_views,
+                              args.serve_budget_gib)
+        print(f"bench: serving {serving['frames_per_s']} frames/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -190,15 +305,16 @@ def main():
                    "lights": int(len(w.lights)), "dom_layers": int(w.settings["dom_layers"]),
                    "froxel": int(w.frusta[0]["froxel_size_x"]), "fine_binning_backend": args.backend},
         "msegments_per_s": round(segs * views * args.steps / dt / 1e6, 2),
-        "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4)},
+        "latency_ms": percentile_ms(lat),
         "stage_ms": {k: round(v, 4) for k, v in stage_acc.items()},
         "counters": {k: v for k, v in ctr.items() if not k.startswith("pad")},
         "roofline": roof,
         "binning_roofline": binning,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
-    r.close()
 
 
 if __name__ == "__main__":

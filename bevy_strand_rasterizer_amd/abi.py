@@ -80,7 +80,7 @@ broad_instance_meta_dtype = np.dtype([("strand_count", "<u4"), ("visible", "<u4"
                                       ("camera_keep_probability", "<f4"), ("shadow_keep_probability", "<f4")])
 frame_counters_dtype = _u32("binning_tasks", "camera_tasks", "allocated_pages", "fine_refs", "active_tiles",
                             "work_items", "camera_runs", "shadow_runs", "overflow_pages", "coarse_tiles",
-                            "task_capacity", "ref_capacity", "page_drops", "page_candidates", "pad0", "pad1")
+                            "task_capacity", "ref_capacity", "page_drops", "page_candidates", "capacity_exceeded", "pad")
 push_constants_dtype = np.dtype([("workgroup_offset", "<u4"), ("num_elements", "<u4"), ("frustum_count", "<u4"),
                                  ("scan_load_base", "<u4"), ("scan_save_base", "<u4"),
                                  ("stochastic_cull_enabled", "<u4"), ("target_strands_per_pixel", "<f4"),

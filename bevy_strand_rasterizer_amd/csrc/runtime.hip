@@ -539,9 +539,15 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   // reference slab: 8 references per task slot (the reference sizes it at 4x
   // its binning queue, src/plugin.rs:540-546); work items: half of that.
   // Both grow when a frame reports an overflow.
-  uint64_t ref_cap64 = std::max<uint64_t>({(uint64_t)task_cap * 8ull, 1ull << 16, (uint64_t)sl.ref_cap_hint});
+  // The first frame of a shape starts at 8 per task slot, at most 2^28
+  // references (3 GiB); a frame that overflows reports capacity_exceeded and
+  // the next one gets the size it needed.
+  uint64_t ref_cap64 = std::max<uint64_t>({std::min<uint64_t>((uint64_t)task_cap * 8ull, 1ull << 28), 1ull << 16,
+                                           (uint64_t)sl.ref_cap_hint});
   const uint32_t ref_cap = (uint32_t)std::min<uint64_t>(ref_cap64, 0x7FFFFFFFull);
-  const uint32_t work_cap = std::max<uint32_t>({ref_cap / 2u, 1u << 15, sl.work_cap_hint});
+  // work items are non-empty cells: never more than the page capacity's cells
+  const uint32_t work_cap = (uint32_t)std::max<uint64_t>(
+      {std::min<uint64_t>(ref_cap / 2u, (uint64_t)page_cap * kPage), 1u << 15, (uint64_t)sl.work_cap_hint});
   // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
   const bool csr = st.fine_binning_backend != 0u;
   // cells recorded per slot while marking: kHitsPerSlot words per slot, when
@@ -809,6 +815,10 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   if (sh.cand_cap && c[C_CANDS] > sh.cand_cap) sl.cand_cap_hint = next_pow2(c[C_CANDS]);
   fc.page_drops = c[C_PAGE_DROPS];
   fc.page_candidates = sh.cand_cap ? c[C_CANDS] : 0u;
+  fc.capacity_exceeded = (c[C_REFS] > sh.ref_cap || c[C_PAGE_DROPS] > 0 || c[C_WORK] > sh.work_cap ||
+                          c[C_OVERFLOW] > 0 || c[C_CAND_OVERFLOW] > 0 || (sh.cand_cap && c[C_CANDS] > sh.cand_cap))
+                             ? 1u
+                             : 0u;
   if (out) *out = fc;
   return SR_OK;
 }
@@ -945,6 +955,7 @@ struct ServerSlot {
   uint64_t out_bytes = 0;
   hipEvent_t out_ev = nullptr;
   double launch_ms = 0.0;
+  uint32_t retries = 0;
   sr_frame_counters counters{};
 };
 
@@ -1084,6 +1095,20 @@ struct sr_server {
       if (q == hipErrorNotReady) { any = true; continue; }
       std::string err;
       int rc = q == hipSuccess ? frame_wait_slot(*s.fs, &s.counters, err) : SR_ERR_DEVICE;
+      if (rc == SR_OK && s.counters.capacity_exceeded && s.retries < 3) {
+        // the slot's hints grew: draw the same frame again
+        ++s.retries;
+        std::vector<PendingRequest> g = std::move(s.reqs);
+        GroupFrame gf;
+        rc = build_group_frame(g, gf);
+        if (rc == SR_OK) rc = launch(s, g, gf, err);
+        if (rc == SR_OK) {
+          any = true;
+          continue;
+        }
+        s.reqs = std::move(g);
+      }
+      if (rc == SR_OK && s.counters.capacity_exceeded) rc = SR_ERR_CAPACITY;
       finish(s, rc);
     }
     return any;
@@ -1198,6 +1223,7 @@ struct sr_server {
           break;
         }
         launched = true;
+        s.retries = 0;
         std::lock_guard<std::mutex> lk(mu);
         ++stats.frames;
         stats.peak_held_bytes = std::max(stats.peak_held_bytes, held);

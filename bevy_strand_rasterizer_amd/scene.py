@@ -416,19 +416,53 @@ def config_named(seed: int = 0x5EEDCAFE, strands: int = 100_000, points: int = 3
     return build_workload(f"scalp-{strands // 1000}k-x{points}-1080p", asset, [(eye, 1920, 1080)], 1, 2048, 4)
 
 
-def config_batched(views: int = 64, seed: int = 0x5EEDCAFE, strands: int = 100_000, points: int = 32,
-                   lights: int = 4) -> Workload:
-    """The named asset from `views` orbit cameras with resolutions drawn from
-    {1280x720, 1920x1080, 2560x1440} and `lights` 2048^2 x 4-layer lights."""
+def config_batched(views: int = 64, seed: int = 0x5EEDCAFE, strands: int = 2000, points: int = 16,
+                   lights: int = 4, width_m: float = 0.0008, resolutions=((1280, 720), (1920, 1080), (2560, 1440)),
+                   light_res: int = 2048, froxel: int = 8) -> Workload:
+    """Many views in one frame: a scalp asset seen from `views` orbit cameras
+    with resolutions drawn (seeded) from `resolutions`, and `lights`
+    light_res^2 x 4-layer lights shared by every view.  The defaults are the
+    batching configuration: the 2,000 x 16 scalp, 64 cameras over {720p,
+    1080p, 1440p}, 4 lights of 2048^2."""
     dist = 0.6
-    asset = scalp_asset(strands, points, seed, 0.0006, radius_px_for(0.0006, 1080, dist))
+    asset = scalp_asset(strands, points, seed, width_m, radius_px_for(width_m, 1080, dist))
     rng = np.random.default_rng(seed)
-    res = [(1280, 720), (1920, 1080), (2560, 1440)]
     cams = []
     for i in range(views):
-        w, h = res[int(rng.integers(0, 3))]
+        w, h = resolutions[int(rng.integers(0, len(resolutions)))]
         cams.append((orbit_eye(i, views, dist), w, h))
-    return build_workload(f"scalp-{strands // 1000}k-x{points}-{views}views", asset, cams, lights, 2048, 4)
+    return build_workload(f"scalp-{strands}x{points}-{views}views-{lights}lights", asset, cams, lights, light_res, 4,
+                          froxel, froxel)
+
+
+def config_batched_named(views: int = 8, seed: int = 0x5EEDCAFE) -> Workload:
+    """The named asset (100k x 32, width 0.0006 m) from `views` 1920x1080
+    orbit cameras sharing one 2048^2 x 4-layer light."""
+    return config_batched(views=views, seed=seed, strands=100_000, points=32, lights=1, width_m=0.0006,
+                          resolutions=((1920, 1080),))
+
+
+SERVING_SIZES = (10_000, 25_000, 50_000, 100_000, 150_000, 200_000, 300_000, 500_000)
+SERVING_RESOLUTIONS = ((256, 256), (512, 512), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
+
+
+def serving_assets(sizes=SERVING_SIZES, points: int = 32, seed: int = 0x5EEDCAFE) -> list:
+    """The serving configuration's device-resident assets: scalps of `sizes`
+    strands x 32 points (width 0.0006 m), one seed each."""
+    return [scalp_asset(n, points, seed + i, 0.0006, radius_px_for(0.0006, 1080, 0.6)) for i, n in enumerate(sizes)]
+
+
+def serving_request(assets: list, asset_ids: list, rng: np.random.Generator, tag: int = 0):
+    """One request of the serving stream: a uniform-random asset, a random
+    orbit camera and a resolution drawn from SERVING_RESOLUTIONS, lit by one
+    2048^2 x 4-layer light (shared by requests of the same asset).  Returns
+    (request record, workload of the same frame drawn alone)."""
+    k = int(rng.integers(0, len(assets)))
+    w, h = SERVING_RESOLUTIONS[int(rng.integers(0, len(SERVING_RESOLUTIONS)))]
+    eye = orbit_eye(int(rng.integers(0, 360)), 360, 0.6)
+    wl = with_asset_id(build_workload(f"serve-{k}-{w}x{h}", assets[k], [(eye, w, h)], 1, 2048, 4, 8, 8),
+                       asset_ids[k])
+    return request_from_workload(wl, tag=tag), wl
 
 
 def config_tiny(strands: int = 24, points: int = 6, res: int = 64, froxel: int = 8, seed: int = 7,

@@ -401,7 +401,8 @@ typedef struct sr_frame_counters {
   uint32_t ref_capacity;
   uint32_t page_drops;          /* marked pages past the page capacity (dropped, as the reference) */
   uint32_t page_candidates;     /* (task, page) pairs of the page CSR backend; 0 for the cell-atomic backend */
-  uint32_t pad[2];
+  uint32_t capacity_exceeded;   /* 1: a capacity overflowed; the context grew it, submit the frame again */
+  uint32_t pad;
 } sr_frame_counters;
 SR_STATIC_ASSERT(sizeof(sr_frame_counters) == 64, "sr_frame_counters is 64 bytes");
 

@@ -48,7 +48,7 @@ STRUCTS = [
 
 
 # numpy field -> C member expression where the C struct uses an array
-C_MEMBER = {("sr_frame_counters", f"pad{i}"): f"pad[{i}]" for i in range(2)}
+C_MEMBER = {}
 
 
 @pytest.fixture(scope="module")
