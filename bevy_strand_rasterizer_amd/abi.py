@@ -40,6 +40,7 @@ DUMP = {
     "lut": 7, "page_table": 8, "page_meta": 9, "cell_counts": 10, "cell_offsets": 11, "refs": 12,
     "active_tiles": 13, "work_items": 14, "camera_runs": 15, "shadow_runs": 16, "dom_depth": 17,
     "dom_opacity": 18, "atlas": 19, "history": 20, "image": 21, "depth": 22, "composite": 23, "stamp": 24,
+    "telemetry": 25,
 }
 
 
@@ -98,6 +99,7 @@ DUMP_DTYPES = {
     "camera_runs": raster_tile_run_dtype, "shadow_runs": raster_tile_run_dtype, "dom_depth": np.dtype("<f4"),
     "dom_opacity": np.dtype("<f4"), "atlas": np.dtype("<f4"), "history": np.dtype("<f4"),
     "image": np.dtype("<f4"), "depth": np.dtype("<f4"), "composite": np.dtype("<f4"), "stamp": np.dtype("<f4"),
+    "telemetry": np.dtype("<u4"),
 }
 
 

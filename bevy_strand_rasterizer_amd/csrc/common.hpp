@@ -64,6 +64,13 @@ struct InstRec {
   float world_from_local[16];
 };
 
+// Telemetry words (strand_prepass.wgsl:72-75, prepass.rs:1345-1396).
+constexpr uint32_t kTelHistBins = 32, kTelHistOff = 8, kTelPageOff = kTelHistOff + kTelHistBins, kTelFrustumStride = 11;
+enum TelFrustumField : uint32_t {
+  TF_VISIBLE_INSTANCES, TF_VISIBLE_STRANDS, TF_RETAINED_STRANDS, TF_EMITTED_SEGMENTS, TF_ALLOCATED_PAGES,
+  TF_PAGE_CANDIDATES, TF_FINE_REFS, TF_RASTER_RUNS, TF_RASTER_LOAD, TF_MAX_PAGE_CANDIDATES, TF_MAX_RASTER_LOAD
+};
+
 // Device counters of one frame (uint32 words).
 enum Counter : uint32_t {
   C_TASKS = 0,
@@ -139,6 +146,9 @@ struct FrameParams {
   // slot_hits[s * hit_k + j] = page-table entry << 8 | cell, in traversal
   // order; slot_hit_count[s] = count, or kHitOverflow past hit_k (those
   // slots are traced again).  hit_k = 0 disables recording.
+  // telemetry (finalize_telemetry layout): 8 global words, 32 histogram
+  // bins, one candidate count per page, 11 words per frustum; null when off
+  uint32_t* tel;
   const sr_projected_segment* slot_proj;  // [F * slots]
   uint32_t* slot_hits;
   uint8_t* slot_hit_count;

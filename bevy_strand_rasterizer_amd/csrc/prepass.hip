@@ -416,6 +416,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
       ref_base = fp.page_meta[page].seg_ref_base;
       if (fp.page_meta[page].seg_ref_count == 0u) continue;
     }
+    if (MODE == MODE_COUNT && fp.tel) atomicAdd(&fp.tel[kTelPageOff + page], 1u);  // (task, page) candidates
     const uint32_t page_cells = page * kPage;
     fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, [&](uint32_t cell) {
       if (MODE == MODE_COUNT) {
@@ -1230,6 +1231,78 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t max_blocks)
   return (uint32_t)g;
 }
 
+// ------------------------------------------------------ K19 telemetry
+// finalize_telemetry (strand_prepass.wgsl:2145-2188) plus the per-frustum
+// words the reference adds along the way (:736-737, 1085, 1136, 2830-2832):
+// pages, tasks, candidates, references, the log2 histogram of candidates
+// per page, and per frustum visible instances / strands, retained strands,
+// emitted segments, pages, candidates, references, raster runs and loads.
+__device__ __forceinline__ void tel_add(const FrameParams& fp, uint32_t fi, uint32_t field, uint32_t v) {
+  atomicAdd(&fp.tel[kTelPageOff + fp.page_cap + fi * kTelFrustumStride + field], v);
+}
+__device__ __forceinline__ void tel_max(const FrameParams& fp, uint32_t fi, uint32_t field, uint32_t v) {
+  atomicMax(&fp.tel[kTelPageOff + fp.page_cap + fi * kTelFrustumStride + field], v);
+}
+
+__global__ __launch_bounds__(256) void k_telemetry(FrameParams fp) {
+  const uint32_t P = min(fp.counters[C_PAGES], fp.page_cap);
+  const uint32_t stride = gridDim.x * blockDim.x, gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) {
+    fp.tel[0] = P;
+    fp.tel[1] = fp.counters[C_TASKS];
+    fp.tel[3] = fp.counters[C_REFS];
+    for (uint32_t fi = 0; fi < fp.F; ++fi)
+      tel_add(fp, fi, TF_EMITTED_SEGMENTS, fp.frustum_task_base[fi + 1] - fp.frustum_task_base[fi]);
+  }
+  const bool csr = fp.st.fine_binning_backend != 0u;
+  for (uint32_t p = gid; p < P; p += stride) {
+    const sr_fine_page_meta pm = fp.page_meta[p];
+    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
+    uint32_t cand = csr ? fp.cand_off[p + 1] - fp.cand_off[p] : fp.tel[kTelPageOff + p];
+    if (csr) fp.tel[kTelPageOff + p] = cand;
+    if (fi != kInvalid) tel_add(fp, fi, TF_ALLOCATED_PAGES, 1u);
+    if (cand == 0u) continue;
+    atomicAdd(&fp.tel[2], cand);
+    atomicAdd(&fp.tel[4], 1u);
+    atomicMax(&fp.tel[5], cand);
+    if (fi != kInvalid) {
+      tel_add(fp, fi, TF_PAGE_CANDIDATES, cand);
+      tel_add(fp, fi, TF_FINE_REFS, pm.seg_ref_count);
+      tel_max(fp, fi, TF_MAX_PAGE_CANDIDATES, cand);
+    }
+    uint32_t bin = 0, rem = cand;
+    while (rem > 1u && bin + 1u < kTelHistBins) { rem >>= 1; ++bin; }
+    atomicAdd(&fp.tel[kTelHistOff + bin], 1u);
+  }
+  const uint32_t Rc = fp.counters[C_CAM_RUNS], Rs = fp.counters[C_SHADOW_RUNS];
+  for (uint32_t r = gid; r < Rc + Rs; r += stride) {
+    const sr_raster_tile_run run = r < Rc ? fp.cam_runs[r] : fp.shadow_runs[r - Rc];
+    if (run.frustum_id >= fp.F) continue;
+    tel_add(fp, run.frustum_id, TF_RASTER_RUNS, 1u);
+    tel_add(fp, run.frustum_id, TF_RASTER_LOAD, run.load_score);
+    tel_max(fp, run.frustum_id, TF_MAX_RASTER_LOAD, run.load_score);
+  }
+  // visible instances / strands and retained strands per (instance, frustum)
+  for (uint32_t ii = 0; ii < fp.NI; ++ii) {
+    const sr_broad_instance_meta b = fp.broad[ii];
+    if (b.visible == 0u) continue;
+    const uint64_t n = (uint64_t)fp.F * b.strand_count;
+    for (uint64_t k = gid; k < n; k += stride) {
+      const uint32_t fi = (uint32_t)(k / b.strand_count), strand = (uint32_t)(k % b.strand_count);
+      const FrustumRec& fr = fp.fr[fi];
+      if (strand == 0u && fp.ranges[(size_t)ii * fp.F + fi].geo_id == ii) {
+        tel_add(fp, fi, TF_VISIBLE_INSTANCES, 1u);
+        tel_add(fp, fi, TF_VISIBLE_STRANDS, b.strand_count);
+      }
+      if (fr.d.kind > 1u || (fr.d.kind == 1u && fr.light_layer == kInvalid)) continue;
+      const float rnd = hash_to_unit_float(wang_hash((ii * 16777619u) ^ strand));
+      if (rnd > fmaxf(b.camera_keep_probability, b.shadow_keep_probability)) continue;
+      if (rnd > fp.keep[(size_t)ii * fp.F + fi]) continue;
+      tel_add(fp, fi, TF_RETAINED_STRANDS, 1u);
+    }
+  }
+}
+
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer) {
   const uint32_t kMaxGrid = 256u * 16u;
   const uint64_t slots_total = (uint64_t)fp.F * fp.slots;
@@ -1285,6 +1358,10 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
                   fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
   hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  if (fp.tel) {
+    timer->mark(s, "telemetry");
+    hipLaunchKernelGGL(k_telemetry, dim3(1024), dim3(256), 0, s, fp);
+  }
 }
 
 }  // namespace srk

@@ -135,7 +135,7 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -144,7 +144,7 @@ struct FrameShape {
   bool valid = false;
   uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
   uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, cand_cap = 0, atlas_texels = 0, dom_layers = 0;
-  uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
+  uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0;
   int hist_cur = 0;
 };
 
@@ -589,7 +589,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_CAND_PT, csr ? 4ull * ptab : 0ull},
       {B_CAND_OFF, csr ? 4ull * (page_cap + 1ull) : 0ull}, {B_CAND_CURSOR, csr ? 4ull * page_cap : 0ull},
       {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
-      {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
+      {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
+      {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
   if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
     uint64_t held = sl.held_bytes();
@@ -688,6 +689,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.cands = sl.buf[B_CANDS].as<uint32_t>();
   fp.cands_tmp = sl.buf[B_CANDS_TMP].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
+  const uint64_t tel_words = st.telemetry_enabled ? kTelPageOff + (uint64_t)page_cap + kTelFrustumStride * F : 0ull;
+  fp.tel = tel_words ? sl.buf[B_TELEMETRY].as<uint32_t>() : nullptr;
   fp.hit_k = hit_k;
   fp.slot_hits = sl.buf[B_SLOT_HITS].as<uint32_t>();
   fp.slot_hit_count = sl.buf[B_SLOT_HIT_COUNT].as<uint8_t>();
@@ -695,6 +698,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   auto launches = [&]() {
     sl.timer.mark(s, "clear");
     launch_fill_u32(fp.counters, C_COUNT, 0u, s);
+    if (fp.tel) launch_fill_u32(fp.tel, tel_words, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
@@ -779,6 +783,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
   sh.cand_cap = cand_cap;
+  sh.tel_words = tel_words;
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
   sh.hist_cur = hist_next;
@@ -888,6 +893,7 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
     case SR_DUMP_DEPTH: src = buf[B_DEPTH].p; bytes = 4ull * sh.img_px; break;
     case SR_DUMP_COMPOSITE: src = buf[B_COMPOSITE].p; bytes = 16ull * sh.img_px; break;
     case SR_DUMP_STAMP: src = buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
+    case SR_DUMP_TELEMETRY: src = buf[B_TELEMETRY].p; bytes = 4ull * sh.tel_words; break;
     default: return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_dump: unknown kind");
   }
   *out_bytes = bytes;

@@ -208,3 +208,93 @@ def test_autotune_picks_an_accepted_configuration():
     ref = oracle.render(tuned)
     ctr, gpu = gpu_frame(tuned)
     compare(tuned, ctr, gpu, ref)
+
+
+def telemetry_from_oracle(w, ref, page_cap):
+    """finalize_telemetry's words (strand_prepass.wgsl:2145-2188) and the
+    per-frustum words (:736-737, 1085, 1136, 2830-2832), restated from the
+    oracle's canonical dumps: a page's candidates are the distinct segments
+    among its references (each (task, page) visit writes at least one)."""
+    lib = oracle.load()
+    F = len(w.frusta)
+    fr = w.frusta
+    ctr = ref.counters()
+    pm, refs, tasks = ref.get("page_meta"), ref.get("refs"), ref.get("tasks")
+    t = np.zeros(40 + page_cap + 11 * F, np.uint64)
+
+    def frustum_of_tile(tile):
+        for fi in range(F):
+            b = int(fr[fi]["coarse_depth_tile_base"])
+            if b <= tile < b + int(fr[fi]["coarse_depth_tile_count"]):
+                return fi
+        return None
+
+    def fw(fi, field):
+        return 40 + page_cap + 11 * fi + field
+
+    P = len(pm)
+    t[0], t[1], t[3] = P, ctr["binning_tasks"], ctr["fine_refs"]
+    for p in range(P):
+        b, n = int(pm["seg_ref_base"][p]), int(pm["seg_ref_count"][p])
+        cand = len({(int(i), int(s)) for i, s in zip(refs["inst_id"][b:b + n], refs["seg_id"][b:b + n])})
+        t[40 + p] = cand
+        fi = frustum_of_tile(int(pm["coarse_tile_id"][p]))
+        if fi is not None:
+            t[fw(fi, 4)] += 1
+        if cand == 0:
+            continue
+        t[2] += cand
+        t[4] += 1
+        t[5] = max(t[5], cand)
+        if fi is not None:
+            t[fw(fi, 5)] += cand
+            t[fw(fi, 6)] += n
+            t[fw(fi, 9)] = max(t[fw(fi, 9)], cand)
+        t[8 + min(cand.bit_length() - 1, 31)] += 1
+    for run in np.concatenate([ref.get("camera_runs"), ref.get("shadow_runs")]):
+        fi = int(run["frustum_id"])
+        t[fw(fi, 7)] += 1
+        t[fw(fi, 8)] += int(run["load_score"])
+        t[fw(fi, 10)] = max(t[fw(fi, 10)], int(run["load_score"]))
+    task_f = tasks["packed_field"] >> abi.BINNING_FRUSTUM_SHIFT
+    for fi in range(F):
+        t[fw(fi, 3)] = int((task_f == fi).sum())
+    broad, keep, ranges = ref.get("broad_meta"), ref.get("keep"), ref.get("ranges")
+    light_layer, nl = {}, len(w.lights)
+    for fi in range(F):
+        if fr[fi]["kind"] == 1:
+            light_layer[fi] = len(light_layer)
+    for ii in range(len(broad)):
+        b = broad[ii]
+        if not b["visible"]:
+            continue
+        sc = int(b["strand_count"])
+        rnd = np.array([lib.oracle_hash_to_unit_float(lib.oracle_wang_hash(((ii * 16777619) & 0xFFFFFFFF) ^ s))
+                        for s in range(sc)], np.float32)
+        mx = max(float(b["camera_keep_probability"]), float(b["shadow_keep_probability"]))
+        for fi in range(F):
+            if int(ranges[ii * F + fi]["geo_id"]) == ii:
+                t[fw(fi, 0)] += 1
+                t[fw(fi, 1)] += sc
+            if fr[fi]["kind"] > 1 or (fr[fi]["kind"] == 1 and light_layer[fi] >= nl):
+                continue
+            t[fw(fi, 2)] += int(((rnd <= np.float32(mx)) & (rnd <= keep[ii * F + fi])).sum())
+    return t
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_telemetry_matches_oracle(backend):
+    w = scene.config_small(froxel=8)
+    w.settings["telemetry_enabled"] = 1
+    w.settings["fine_binning_backend"] = backend
+    ref = oracle.render(w)
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        r.render(w)
+        tel = r.dump("telemetry")
+    page_cap = len(tel) - 40 - 11 * len(w.frusta)
+    want = telemetry_from_oracle(w, ref, page_cap)
+    assert tel.shape == want.shape
+    bad = np.nonzero(tel.astype(np.uint64) != want)[0]
+    assert bad.size == 0, f"telemetry words differ at {bad[:10]}: gpu {tel[bad[:10]]} oracle {want[bad[:10]]}"
+    assert tel[2] > 0 and tel[5] >= tel[2] / max(tel[4], 1)
