@@ -40,7 +40,7 @@ DUMP = {
     "lut": 7, "page_table": 8, "page_meta": 9, "cell_counts": 10, "cell_offsets": 11, "refs": 12,
     "active_tiles": 13, "work_items": 14, "camera_runs": 15, "shadow_runs": 16, "dom_depth": 17,
     "dom_opacity": 18, "atlas": 19, "history": 20, "image": 21, "depth": 22, "composite": 23, "stamp": 24,
-    "telemetry": 25,
+    "telemetry": 25, "opaque_tiles": 26,
 }
 
 
@@ -99,7 +99,7 @@ DUMP_DTYPES = {
     "camera_runs": raster_tile_run_dtype, "shadow_runs": raster_tile_run_dtype, "dom_depth": np.dtype("<f4"),
     "dom_opacity": np.dtype("<f4"), "atlas": np.dtype("<f4"), "history": np.dtype("<f4"),
     "image": np.dtype("<f4"), "depth": np.dtype("<f4"), "composite": np.dtype("<f4"), "stamp": np.dtype("<f4"),
-    "telemetry": np.dtype("<u4"),
+    "telemetry": np.dtype("<u4"), "opaque_tiles": np.dtype([("covered", "<u4"), ("farthest_q", "<u4")]),
 }
 
 
@@ -117,7 +117,8 @@ class FrameSettings(C.Structure):
 class FrameDesc(C.Structure):
     _fields_ = [("instances", C.c_void_p), ("instance_count", C.c_uint32), ("frustum_count", C.c_uint32),
                 ("frusta", C.c_void_p), ("views", C.c_void_p), ("lights", C.c_void_p),
-                ("light_count", C.c_uint32), ("pad", C.c_uint32), ("settings", FrameSettings)]
+                ("light_count", C.c_uint32), ("pad", C.c_uint32), ("settings", FrameSettings),
+                ("opaque_depth", C.c_void_p), ("opaque_samples", C.c_uint32), ("pad2", C.c_uint32)]
 
 
 class AssetSlabs(C.Structure):

@@ -149,6 +149,11 @@ struct FrameParams {
   // telemetry (finalize_telemetry layout): 8 global words, 32 histogram
   // bins, one candidate count per page, 11 words per frustum; null when off
   uint32_t* tel;
+  // opaque scene depth (per camera, opaque_samples per pixel) and its
+  // fine-tile reduction (covered, quantised farthest); null when absent
+  const float* opaque_depth;
+  uint2* opaque_tiles;
+  uint32_t opaque_samples, fine_total;
   const sr_projected_segment* slot_proj;  // [F * slots]
   uint32_t* slot_hits;
   uint8_t* slot_hit_count;

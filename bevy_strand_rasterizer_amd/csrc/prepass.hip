@@ -304,12 +304,41 @@ __device__ __forceinline__ uint32_t lut_slice_for_depth(const sr_coarse_depth_lu
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
 
+// The fine tiles of one coarse tile as fine_tile_occluded_by_opaque
+// (strand_prepass.wgsl:551-573) sees them: a camera fine tile fully covered
+// by opaque depth hides a slice whose nearest depth (z_max, reverse-Z) lies
+// behind the tile's farthest opaque sample.
+struct Occluder {
+  const uint2* tiles;  // null: nothing is occluded
+  uint32_t base, ftx, fty, fx0, fy0;
+  __device__ __forceinline__ bool hides(int32_t fx, int32_t fy, float nearest) const {
+    if (!tiles) return false;
+    const uint32_t x = fx0 + (uint32_t)fx, y = fy0 + (uint32_t)fy;
+    if (x >= ftx || y >= fty) return false;
+    const uint2 t = tiles[base + y * ftx + x];
+    return t.x != 0u && nearest <= dequantize_depth01(t.y) - 1e-4f;
+  }
+};
+
+__device__ __forceinline__ Occluder occluder_for(const FrameParams& fp, const sr_frustum_desc& fd, uint32_t local) {
+  Occluder o{nullptr, 0u, 0u, 0u, 0u, 0u};
+  if (!fp.opaque_tiles || fd.kind != 0u) return o;
+  o.tiles = fp.opaque_tiles;
+  o.base = fd.fine_depth_tile_base;
+  o.ftx = (fd.screen_width + fd.froxel_size_x - 1u) / fd.froxel_size_x;
+  o.fty = (fd.screen_height + fd.froxel_size_y - 1u) / fd.froxel_size_y;
+  o.fx0 = (local % fd.coarse_tiles_x) * kExt;
+  o.fy0 = (local / fd.coarse_tiles_x) * kExt;
+  return o;
+}
+
 // Fine 3D Amanatides-Woo walk through the 4 x 4 x 16 cells of one count page
 // (strand_prepass.wgsl:1517-1573) for a segment already clipped to the coarse
-// tile and the LUT slice [zmin, zmax]; calls emit(cell) per visited cell.
+// tile and the LUT slice [zmin, zmax]; calls emit(cell) per visited cell that
+// opaque depth does not hide (a hidden cell is stepped over, :1541-1555).
 template <class Emit>
 __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, float zmax, const sr_frustum_desc& fd,
-                                         Emit&& emit) {
+                                         const Occluder& occ, Emit&& emit) {
   float zspan = fmaxf(zmax - zmin, 1.0f / (float)kQuant);
   float fxs = (float)fd.froxel_size_x, fys = (float)fd.froxel_size_y;
   float pa[3] = {(q0.x - origin.x) / fxs, (q0.y - origin.y) / fys,
@@ -333,7 +362,7 @@ __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, fl
     if (tiny) tmax[k] = 1e38f;
   }
   for (uint32_t safety = 0; safety <= kExt + kExt + kSlices + 3u; ++safety) {
-    emit(((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0]);
+    if (!occ.hides(f[0], f[1], zmax)) emit(((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0]);
     if (f[0] == f1[0] && f[1] == f1[1] && f[2] == f1[2]) break;
     float mn = fminf(fminf(tmax[0], tmax[1]), tmax[2]);
     int k = tmax[0] == mn ? 0 : (tmax[1] == mn ? 1 : 2);  // canonical_min_mask: x, then y, then z
@@ -341,6 +370,16 @@ __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, fl
     tmax[k] += delta[k];
     if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] > hi[0] || f[1] > hi[1] || f[2] > hi[2]) break;
   }
+}
+
+// Whether the fine walk of a clipped segment reaches a cell opaque depth
+// does not hide (always, without opaque depth).
+__device__ __forceinline__ bool any_visible_cell(const Clipped& dc, V2 origin, float zmin, float zmax,
+                                                 const sr_frustum_desc& fd, const Occluder& occ) {
+  if (!occ.tiles) return true;
+  bool any = false;
+  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occ, [&](uint32_t) { any = true; });
+  return any;
 }
 
 // Coarse-tile box of a frustum's coarse tile (strand_prepass.wgsl:1422-1433).
@@ -388,15 +427,17 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
       if (fp.st.fine_binning_backend != 0u) {
+        // a (segment, page) candidate has at least one cell opaque depth
+        // does not hide; entries with a recorded first cell are counted after
+        // the walk from the record, the rest (past the record) here
+        const uint32_t before = rec.n;
         if (rec.dst) {
-          // entries with a recorded first cell are counted after the walk
-          // from the record; the rest (past the record) here
-          if (rec.n >= rec.cap) atomicAdd(&fp.cand_count_pt[pti], 1u);
-          fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, [&](uint32_t cell) {
+          fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
             if (rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
             ++rec.n;
           });
-        } else {
+          if (before >= rec.cap && rec.n > before) atomicAdd(&fp.cand_count_pt[pti], 1u);
+        } else if (any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) {
           atomicAdd(&fp.cand_count_pt[pti], 1u);
         }
       }
@@ -406,6 +447,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     if (h == 0u || h == kMarked) continue;
     uint32_t page = h - 1u;
     if (MODE == MODE_SCATTER) {
+      if (!any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) continue;
       uint32_t pos = atomicAdd(&fp.cand_cursor[page], 1u);
       uint64_t dst = (uint64_t)fp.cand_off[page] + pos;
       if (dst < fp.cand_cap) fp.cands[dst] = task;
@@ -416,9 +458,10 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
       ref_base = fp.page_meta[page].seg_ref_base;
       if (fp.page_meta[page].seg_ref_count == 0u) continue;
     }
-    if (MODE == MODE_COUNT && fp.tel) atomicAdd(&fp.tel[kTelPageOff + page], 1u);  // (task, page) candidates
     const uint32_t page_cells = page * kPage;
-    fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, [&](uint32_t cell) {
+    uint32_t emitted = 0;
+    fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
+      ++emitted;
       if (MODE == MODE_COUNT) {
         atomicAdd(&fp.cell_counts[page_cells + cell], 1u);
       } else {
@@ -427,6 +470,8 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
         if (dst < fp.ref_cap) fp.refs[dst] = ref;
       }
     });
+    // (segment, page) candidates for telemetry: pairs with a visible cell
+    if (MODE == MODE_COUNT && fp.tel && emitted) atomicAdd(&fp.tel[kTelPageOff + page], 1u);
   }
 }
 
@@ -790,12 +835,13 @@ __device__ __forceinline__ void cells_in_page(const FrameParams& fp, uint32_t sl
   float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
   V2 origin;
   V3 bmax;
-  coarse_tile_box(fd, pm.coarse_tile_id - fd.coarse_depth_tile_base, &origin, &bmax);
+  const uint32_t local = pm.coarse_tile_id - fd.coarse_depth_tile_base;
+  coarse_tile_box(fd, local, &origin, &bmax);
   Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
   Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
   if (!dc.ok) return;
-  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, emit);
+  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), emit);
 }
 
 // The reference a candidate slot contributes (make_fine_seg_ref,
@@ -1231,6 +1277,46 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t max_blocks)
   return (uint32_t)g;
 }
 
+// ------------------------------------------------------ K1 opaque depth
+// reduce_opaque_depth (opaque_depth_reduce.wgsl:35-86): one thread per camera
+// fine tile, over its pixels: covered when every pixel's nearest sample
+// (min over samples, reverse-Z) is > 0, and the quantised farthest of them.
+__global__ __launch_bounds__(256) void k_opaque_reduce(FrameParams fp) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < fp.fine_total; g += gridDim.x * blockDim.x) {
+    uint32_t fi = kInvalid;
+    for (uint32_t i = 0; i < fp.F; ++i) {
+      const sr_frustum_desc& d = fp.fr[i].d;
+      const uint32_t n = fp.fr[i].fine_tiles_x * fp.fr[i].fine_tiles_y;
+      if (g >= d.fine_depth_tile_base && g < d.fine_depth_tile_base + n) { fi = i; break; }
+    }
+    if (fi == kInvalid || fp.fr[fi].d.kind != 0u) {
+      fp.opaque_tiles[g] = make_uint2(0u, 0u);
+      continue;
+    }
+    const FrustumRec& fr = fp.fr[fi];
+    const uint32_t t = g - fr.d.fine_depth_tile_base;
+    const uint32_t x0 = (t % fr.fine_tiles_x) * fr.d.froxel_size_x, y0 = (t / fr.fine_tiles_x) * fr.d.froxel_size_y;
+    const uint32_t x1 = min(x0 + fr.d.froxel_size_x, fr.d.screen_width);
+    const uint32_t y1 = min(y0 + fr.d.froxel_size_y, fr.d.screen_height);
+    const uint32_t S = fp.opaque_samples;
+    const float* img = fp.opaque_depth + fr.image_off * S;
+    uint32_t covered = 0u, total = 0u;
+    float farthest = 1.0f;
+    for (uint32_t y = y0; y < y1; ++y)
+      for (uint32_t x = x0; x < x1; ++x) {
+        ++total;
+        const float* d = img + ((size_t)y * fr.d.screen_width + x) * S;
+        float depth = d[0];
+        for (uint32_t k = 1; k < S; ++k) depth = fminf(depth, d[k]);
+        if (depth > 0.0f) {
+          ++covered;
+          farthest = fminf(farthest, depth);
+        }
+      }
+    fp.opaque_tiles[g] = make_uint2((covered == total && total > 0u) ? 1u : 0u, quantize_depth01(farthest));
+  }
+}
+
 // ------------------------------------------------------ K19 telemetry
 // finalize_telemetry (strand_prepass.wgsl:2145-2188) plus the per-frustum
 // words the reference adds along the way (:736-737, 1085, 1136, 2830-2832):
@@ -1309,6 +1395,10 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   const uint32_t ptab = fp.coarse_total * kSlices;
   const bool csr = fp.st.fine_binning_backend != 0u;
 
+  if (fp.opaque_tiles) {
+    timer->mark(s, "opaque_reduce");
+    hipLaunchKernelGGL(k_opaque_reduce, dim3(grid_for(fp.fine_total, 256, kMaxGrid)), dim3(256), 0, s, fp);
+  }
   timer->mark(s, "broad");
   hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 64, 1024)), dim3(64), 0, s, fp);
   timer->mark(s, "depth_lut");

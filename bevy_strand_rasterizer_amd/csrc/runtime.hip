@@ -135,7 +135,7 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -144,7 +144,7 @@ struct FrameShape {
   bool valid = false;
   uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
   uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, cand_cap = 0, atlas_texels = 0, dom_layers = 0;
-  uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0;
+  uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0, opaque_tiles = 0;
   int hist_cur = 0;
 };
 
@@ -451,6 +451,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   // ---- frustum records
   std::vector<FrustumRec> fr(F);
   uint32_t lights_seen = 0, cams = 0, coarse_total = 0, max_cam_tile_px = 1, max_light_tile_px = 1;
+  uint64_t fine_total = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
   for (uint32_t i = 0; i < F; ++i) {
     const sr_frustum_desc& d = fd->frusta[i];
@@ -491,7 +492,12 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     if (d.kind == 0u) max_cam_tile_px = std::max(max_cam_tile_px, d.froxel_size_x * d.froxel_size_y);
     if (d.kind == 1u) max_light_tile_px = std::max(max_light_tile_px, d.froxel_size_x * d.froxel_size_y);
     coarse_total = std::max(coarse_total, d.coarse_depth_tile_base + d.coarse_depth_tile_count);
+    fine_total = std::max<uint64_t>(fine_total, (uint64_t)d.fine_depth_tile_base + (uint64_t)ftx * fty);
   }
+  if (fd->opaque_depth && (fd->opaque_samples == 0u || fd->opaque_samples > 4u))
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: opaque_samples must be in 1..4");
+  if (fine_total > 0xFFFFFFFFull) return efail(err, SR_ERR_CAPACITY, "sr_frame_submit: too many fine tiles");
+  const uint64_t opaque_floats = fd->opaque_depth ? img_px * fd->opaque_samples : 0ull;
   // ---- instance records
   std::unique_lock<std::mutex> asset_lock(ctx->mu);
   std::vector<InstRec> ir(NI);
@@ -590,7 +596,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_CAND_OFF, csr ? 4ull * (page_cap + 1ull) : 0ull}, {B_CAND_CURSOR, csr ? 4ull * page_cap : 0ull},
       {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
-      {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
+      {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
+      {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
       {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
   if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
     uint64_t held = sl.held_bytes();
@@ -626,6 +633,16 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (NL)
     HIP_OKS(hipMemcpyAsync(sl.buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
                           sizeof(sr_directional_light) * NL, hipMemcpyHostToDevice, s), "upload lights");
+  if (opaque_floats) {  // the opaque depth travels through its own staging entry
+    int oslot = 0;
+    char* ostg = next_staging(sl, 4ull * opaque_floats, &oslot);
+    if (!ostg) return efail(err, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
+    std::memcpy(ostg, fd->opaque_depth, 4ull * opaque_floats);
+    HIP_OKS(hipMemcpyAsync(sl.buf[B_OPAQUE_DEPTH].p, ostg, 4ull * opaque_floats, hipMemcpyHostToDevice, s),
+            "upload opaque depth");
+    HIP_OKS(hipEventRecord(sl.staging_ev[oslot], s), "staging event");
+    sl.staging_used[oslot] = true;
+  }
   HIP_OKS(hipEventRecord(sl.staging_ev[slot], s), "staging event");
   sl.staging_used[slot] = true;
 
@@ -689,6 +706,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.cands = sl.buf[B_CANDS].as<uint32_t>();
   fp.cands_tmp = sl.buf[B_CANDS_TMP].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
+  fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
+  fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;
+  fp.opaque_samples = opaque_floats ? fd->opaque_samples : 0u;
+  fp.fine_total = (uint32_t)fine_total;
   const uint64_t tel_words = st.telemetry_enabled ? kTelPageOff + (uint64_t)page_cap + kTelFrustumStride * F : 0ull;
   fp.tel = tel_words ? sl.buf[B_TELEMETRY].as<uint32_t>() : nullptr;
   fp.hit_k = hit_k;
@@ -784,6 +805,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
   sh.cand_cap = cand_cap;
   sh.tel_words = tel_words;
+  sh.opaque_tiles = opaque_floats ? fine_total : 0ull;
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
   sh.hist_cur = hist_next;
@@ -894,6 +916,7 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
     case SR_DUMP_COMPOSITE: src = buf[B_COMPOSITE].p; bytes = 16ull * sh.img_px; break;
     case SR_DUMP_STAMP: src = buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
     case SR_DUMP_TELEMETRY: src = buf[B_TELEMETRY].p; bytes = 4ull * sh.tel_words; break;
+    case SR_DUMP_OPAQUE_TILES: src = buf[B_OPAQUE_TILES].p; bytes = 8ull * sh.opaque_tiles; break;
     default: return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_dump: unknown kind");
   }
   *out_bytes = bytes;

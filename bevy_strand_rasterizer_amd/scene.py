@@ -57,6 +57,9 @@ class Workload:
     views: np.ndarray
     lights: np.ndarray
     settings: dict = field(default_factory=dict)
+    # opaque scene depth: float32 (camera pixels in frustum order, samples),
+    # reverse-Z, 0 = nothing drawn; None = no opaque geometry
+    opaque: np.ndarray | None = None
 
     @property
     def camera_count(self) -> int:
@@ -99,6 +102,10 @@ class Workload:
         fd.lights = abi.ptr(self.lights) if len(self.lights) else None
         fd.light_count = len(self.lights)
         fd.settings = self.frame_settings()
+        if self.opaque is not None:
+            assert self.opaque.dtype == np.float32 and self.opaque.flags["C_CONTIGUOUS"] and self.opaque.ndim == 2
+            fd.opaque_depth = abi.ptr(self.opaque)
+            fd.opaque_samples = self.opaque.shape[1]
         return fd
 
 
@@ -358,7 +365,28 @@ def retile(w: Workload, camera_froxel: int = 0, light_froxel: int = 0, backend: 
     if backend is not None:
         st["fine_binning_backend"] = int(backend)
     return Workload(name=w.name, assets=w.assets, instances=w.instances, frusta=frustum_table(cfgs, kinds),
-                    views=w.views, lights=w.lights, settings=st)
+                    views=w.views, lights=w.lights, settings=st, opaque=w.opaque)
+
+
+def opaque_disc(w: Workload, depth: float, radius_frac: float = 0.3, samples: int = 4, seed: int = 1) -> np.ndarray:
+    """A synthetic opaque scene depth for every camera of `w`: a disc (an
+    occluder in front of or behind the hair) of constant reverse-Z `depth`
+    around the image centre, jittered per MSAA sample near its rim, zero
+    elsewhere.  Shape (camera pixels, samples), float32."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for f in w.frusta:
+        if f["kind"] != 0:
+            continue
+        W, H = int(f["screen_width"]), int(f["screen_height"])
+        yy, xx = np.mgrid[0:H, 0:W]
+        r = np.hypot(xx + 0.5 - W / 2, yy + 0.5 - H / 2) / (radius_frac * min(W, H))
+        d = np.zeros((H * W, samples), np.float32)
+        for k in range(samples):
+            inside = (r + rng.uniform(-0.02, 0.02, size=r.shape)).reshape(-1) < 1.0
+            d[inside, k] = depth * (1.0 + 0.01 * k)
+        out.append(d)
+    return np.ascontiguousarray(np.concatenate(out))
 
 
 def with_asset_id(w: Workload, asset_id: int) -> Workload:
@@ -367,7 +395,7 @@ def with_asset_id(w: Workload, asset_id: int) -> Workload:
     for k in ("vertex_id", "index_id", "meta_id", "geo_id", "material_id"):
         inst[k] = asset_id
     return Workload(name=w.name, assets=w.assets, instances=inst, frusta=w.frusta, views=w.views, lights=w.lights,
-                    settings=dict(w.settings))
+                    settings=dict(w.settings), opaque=w.opaque)
 
 
 def request_from_workload(w: Workload, tag: int = 0) -> np.ndarray:

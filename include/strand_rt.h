@@ -350,6 +350,14 @@ typedef struct sr_frame_desc {
   uint32_t light_count;
   uint32_t pad;
   sr_frame_settings settings;
+  /* Opaque scene depth per camera (the reference's prepass depth texture,
+   * src/pipelines/prepass.rs:780-816), host memory: for every camera frustum
+   * in frustum order, width * height pixels x opaque_samples floats (the
+   * samples of a pixel adjacent), reverse-Z with 0 = nothing drawn.  NULL: no
+   * opaque geometry, nothing is occluded. */
+  const float* opaque_depth;
+  uint32_t opaque_samples; /* 1..4 (MSAA samples) */
+  uint32_t pad2;
 } sr_frame_desc;
 
 /* Host-side view of one asset's slabs in the reference layout
@@ -433,6 +441,7 @@ typedef enum sr_dump_kind {
   SR_DUMP_DEPTH = 22,        /* float per camera frustum W*H */
   SR_DUMP_COMPOSITE = 23,    /* float4 per camera frustum W*H */
   SR_DUMP_STAMP = 24,        /* float per light frustum stamp target */
+  SR_DUMP_OPAQUE_TILES = 26, /* uint32 (covered, quantised farthest depth) per fine tile when opaque depth was given */
   SR_DUMP_TELEMETRY = 25,    /* uint32 telemetry when settings.telemetry_enabled (finalize_telemetry layout):
                                 [0] pages [1] tasks [2] page candidates [3] fine refs [4] pages with candidates
                                 [5] max candidates per page, [8..39] log2 histogram of candidates per page,

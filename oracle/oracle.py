@@ -31,6 +31,9 @@ def load() -> C.CDLL:
     lib.oracle_render.restype = C.c_int
     lib.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.FrameDesc), C.c_int, C.c_uint32, C.c_int,
                                   C.POINTER(C.c_void_p)]
+    lib.oracle_render_history.restype = C.c_int
+    lib.oracle_render_history.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.FrameDesc), C.c_int, C.c_uint32,
+                                          C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]
     lib.oracle_get.restype = C.c_int
     lib.oracle_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
     lib.oracle_stage_times.restype = C.c_int
@@ -108,14 +111,22 @@ class OracleFrame:
 
 
 def render(workload, threads: int | None = None, ref_capacity: int | None = None,
-           binning_only: bool = False) -> OracleFrame:
+           binning_only: bool = False, prev_history: np.ndarray | None = None,
+           history_valid: bool | None = None) -> OracleFrame:
+    """One frame; `prev_history` (the previous frame's "history" dump, float2
+    per atlas texel) is read as shadow_history_prev when history_valid."""
     lib = load()
     slabs = (abi.AssetSlabs * len(workload.assets))(*[a.slabs() for a in workload.assets])
     fd = workload.frame_desc()
+    if history_valid is not None:
+        fd.settings.shadow_history_valid = int(history_valid)
     out = C.c_void_p()
     cap = ref_capacity if ref_capacity is not None else gpu_ref_capacity(workload)
     nthreads = threads if threads is not None else min(16, os.cpu_count() or 1)
-    rc = lib.oracle_render(slabs, len(workload.assets), C.byref(fd), nthreads, cap, int(binning_only), C.byref(out))
+    hist = None if prev_history is None else np.ascontiguousarray(prev_history, np.float32)
+    rc = lib.oracle_render_history(slabs, len(workload.assets), C.byref(fd), nthreads, cap, int(binning_only),
+                                   None if hist is None else hist.ctypes.data,
+                                   0 if hist is None else hist.size // 2, C.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed ({rc})")
-    return OracleFrame(lib, out.value, (slabs, fd, workload))
+    return OracleFrame(lib, out.value, (slabs, fd, workload, hist))

@@ -200,6 +200,15 @@ struct Frame {
   uint32_t F = 0, NI = 0;
   uint32_t coarse_total = 0;
   uint32_t ref_capacity = 0;
+  // the previous frame's shadow history (float2 per atlas texel, .x = the
+  // filtered visibility read as shadow_history_prev), or none
+  const float* prev_history = nullptr;
+  uint64_t prev_history_texels = 0;
+  // opaque scene depth per camera (host, pixel-major, opaque_samples each)
+  // and its fine-tile reduction (covered, quantised farthest) per fine tile
+  const float* opaque_depth = nullptr;
+  uint32_t opaque_samples = 0;
+  std::vector<uint32_t> opaque_tiles;
 
   // outputs
   sr_frame_counters ctr{};
@@ -608,6 +617,65 @@ struct FillCtx {
   sr_fine_seg_ref ref;
 };
 
+// fine_tile_occluded_by_opaque (strand_prepass.wgsl:551-573): a camera fine
+// tile fully covered by opaque depth hides a slice whose nearest depth
+// (z_max, reverse-Z) lies behind the tile's farthest opaque sample.
+bool opaque_occluded(const Frame& F, const sr_frustum_desc& fd, uint32_t fine_x, uint32_t fine_y, float nearest) {
+  if (F.opaque_tiles.empty() || fd.kind != 0u) return false;
+  uint32_t ftx = (fd.screen_width + fd.froxel_size_x - 1u) / fd.froxel_size_x;
+  uint32_t fty = (fd.screen_height + fd.froxel_size_y - 1u) / fd.froxel_size_y;
+  if (fine_x >= ftx || fine_y >= fty) return false;
+  size_t t = (size_t)fd.fine_depth_tile_base + fine_y * ftx + fine_x;
+  if (2 * t + 1 >= F.opaque_tiles.size()) return false;
+  if (F.opaque_tiles[2 * t] == 0u) return false;
+  return nearest <= dequantize_depth01(F.opaque_tiles[2 * t + 1]) - 1e-4f;
+}
+
+// reduce_opaque_depth (opaque_depth_reduce.wgsl:35-86): per camera fine
+// tile, whether every pixel has opaque depth (min over its samples > 0) and
+// the quantised farthest (smallest) such depth.
+void reduce_opaque_depth(Frame& F) {
+  if (!F.opaque_depth) return;
+  size_t tiles = 0;
+  for (const auto& fd : F.fr) {
+    uint32_t ftx = (fd.screen_width + fd.froxel_size_x - 1u) / fd.froxel_size_x;
+    uint32_t fty = (fd.screen_height + fd.froxel_size_y - 1u) / fd.froxel_size_y;
+    tiles = std::max<size_t>(tiles, (size_t)fd.fine_depth_tile_base + (size_t)ftx * fty);
+  }
+  F.opaque_tiles.assign(2 * tiles, 0u);
+  uint64_t cam_px = 0;
+  const uint32_t S = F.opaque_samples;
+  for (const auto& fd : F.fr) {
+    if (fd.kind != 0u) continue;
+    uint32_t ftx = (fd.screen_width + fd.froxel_size_x - 1u) / fd.froxel_size_x;
+    uint32_t fty = (fd.screen_height + fd.froxel_size_y - 1u) / fd.froxel_size_y;
+    const float* img = F.opaque_depth + cam_px * S;
+    for (uint32_t ty = 0; ty < fty; ++ty)
+      for (uint32_t tx = 0; tx < ftx; ++tx) {
+        uint32_t x0 = tx * fd.froxel_size_x, y0 = ty * fd.froxel_size_y;
+        uint32_t x1 = std::min(x0 + fd.froxel_size_x, fd.screen_width);
+        uint32_t y1 = std::min(y0 + fd.froxel_size_y, fd.screen_height);
+        uint32_t covered = 0, total = 0;
+        float farthest = 1.0f;
+        for (uint32_t y = y0; y < y1; ++y)
+          for (uint32_t x = x0; x < x1; ++x) {
+            ++total;
+            const float* d = img + ((size_t)y * fd.screen_width + x) * S;
+            float depth = d[0];
+            for (uint32_t k = 1; k < S; ++k) depth = std::fmin(depth, d[k]);
+            if (depth > 0.0f) {
+              ++covered;
+              farthest = std::fmin(farthest, depth);
+            }
+          }
+        size_t t = (size_t)fd.fine_depth_tile_base + ty * ftx + tx;
+        F.opaque_tiles[2 * t] = (covered == total && total > 0u) ? 1u : 0u;
+        F.opaque_tiles[2 * t + 1] = quantize_depth01(farthest);
+      }
+    cam_px += (uint64_t)fd.screen_width * fd.screen_height;
+  }
+}
+
 // strand_prepass.wgsl:1412-1575 trace_segment_into_coarse_count_page
 void trace_coarse_tile(Frame& F, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile, uint32_t local,
                        Mode mode, FillCtx* fc, std::atomic<uint32_t>* counts) {
@@ -664,10 +732,10 @@ void trace_coarse_tile(Frame& F, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32
     const sr_fine_page_meta* pm = (mode == Mode::Fill) ? &F.page_meta[page] : nullptr;
     for (;;) {
       if (safety > kExt + kExt + kSlices + 3u) break;
-      // fine_tile_occluded_by_opaque (strand_prepass.wgsl:551-573): no opaque
-      // depth tiles are bound in the headless path, so nothing is occluded.
       uint32_t cell = ((uint32_t)f[2] * kExt + (uint32_t)f[1]) * kExt + (uint32_t)f[0];
-      if (mode == Mode::Count) {
+      if (opaque_occluded(F, fd, cx * kExt + (uint32_t)f[0], cy * kExt + (uint32_t)f[1], zmax)) {
+        // skipped cell, the walk goes on (:1541-1555)
+      } else if (mode == Mode::Count) {
         counts[(size_t)page * kPage + cell].fetch_add(1u, std::memory_order_relaxed);
       } else if (pm->seg_ref_count != 0u) {  // write_fine_seg_ref, :1329-1354
         size_t gc = (size_t)page * kPage + cell;
@@ -1305,7 +1373,11 @@ void shading(Frame& F) {
   size_t layers = (size_t)cams * F.NI;
   F.atlas.assign(layers * F.atlas_texels * 4, 0.0f);
   F.history.assign(layers * F.atlas_texels * 2, 0.0f);
-  std::vector<float> prev_hist(layers * F.atlas_texels, 0.0f);  // cleared history (frame 0)
+  // shadow_history_prev: the caller's previous history when its layout
+  // matches, else cleared (frame 0)
+  std::vector<float> prev_hist(layers * F.atlas_texels, 0.0f);
+  if (F.prev_history && F.prev_history_texels == prev_hist.size())
+    for (size_t i = 0; i < prev_hist.size(); ++i) prev_hist[i] = F.prev_history[2 * i];
   // segment starts that are shaded this frame, per layer
   std::vector<uint8_t> shaded(layers * F.atlas_texels, 0);
   for (uint32_t t : F.shading_queue) {
@@ -1482,11 +1554,31 @@ extern "C" {
 
 struct or_frame { Frame f; };
 
+int oracle_render_history(const sr_asset_slabs* assets, uint32_t asset_count, const sr_frame_desc* fd, int threads,
+                          uint32_t ref_capacity, int stop_after_binning, const float* prev_history,
+                          uint64_t prev_history_texels, or_frame** out);
+
 int oracle_render(const sr_asset_slabs* assets, uint32_t asset_count, const sr_frame_desc* fd, int threads,
                   uint32_t ref_capacity, int stop_after_binning, or_frame** out) {
+  return oracle_render_history(assets, asset_count, fd, threads, ref_capacity, stop_after_binning, nullptr, 0, out);
+}
+
+// One frame whose shading reads `prev_history` (float2 per atlas texel, the
+// previous frame's history dump) as shadow_history_prev
+// (strand_shading.wgsl:660-679).
+int oracle_render_history(const sr_asset_slabs* assets, uint32_t asset_count, const sr_frame_desc* fd, int threads,
+                          uint32_t ref_capacity, int stop_after_binning, const float* prev_history,
+                          uint64_t prev_history_texels, or_frame** out) {
   if (!assets || !fd || !out) return -1;
   or_frame* of = new or_frame();
   Frame& F = of->f;
+  F.prev_history = prev_history;
+  F.prev_history_texels = prev_history_texels;
+  if (fd->opaque_depth) {
+    if (fd->opaque_samples == 0u || fd->opaque_samples > 4u) { delete of; return -1; }
+    F.opaque_depth = fd->opaque_depth;
+    F.opaque_samples = fd->opaque_samples;
+  }
   F.assets = assets; F.asset_count = asset_count;
   F.inst.assign(fd->instances, fd->instances + fd->instance_count);
   F.fr.assign(fd->frusta, fd->frusta + fd->frustum_count);
@@ -1499,6 +1591,7 @@ int oracle_render(const sr_asset_slabs* assets, uint32_t asset_count, const sr_f
   F.coarse_total = 0;
   for (auto& f : F.fr) F.coarse_total = std::max(F.coarse_total, f.coarse_depth_tile_base + f.coarse_depth_tile_count);
   F.ref_capacity = ref_capacity;
+  timed(F, "opaque_reduce", [&] { reduce_opaque_depth(F); });
   timed(F, "broad", [&] { broad_prepass(F); });
   timed(F, "project", [&] { fine_prepass(F); });
   timed(F, "depth_lut", [&] { build_luts(F); });
@@ -1548,6 +1641,7 @@ int oracle_get(or_frame* of, int kind, const void** data, uint64_t* bytes) {
     case SR_DUMP_DOM_OPACITY: RET(F.dom_opacity);
     case SR_DUMP_ATLAS: RET(F.atlas);
     case SR_DUMP_HISTORY: RET(F.history);
+    case SR_DUMP_OPAQUE_TILES: RET(F.opaque_tiles);
     case SR_DUMP_IMAGE: RET(F.image);
     case SR_DUMP_DEPTH: RET(F.depth);
     case SR_DUMP_COMPOSITE: RET(F.composite);

@@ -60,6 +60,9 @@ def c_layout(tmp_path_factory):
         for f in dt.names:
             member = C_MEMBER.get((cname, f), f)
             lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {member}));')
+    for f, *_ in abi.FrameDesc._fields_:
+        lines.append(f'  printf("sr_frame_desc {f} %zu\\n", offsetof(sr_frame_desc, {f}));')
+    lines.append('  printf("sr_frame_desc __sizeof__ %zu\\n", sizeof(sr_frame_desc));')
     lines.append("  return 0; }")
     d = tmp_path_factory.mktemp("abi")
     src, exe = d / "layout.c", d / "layout"
@@ -86,6 +89,9 @@ def test_ctypes_mirrors_match_header(c_layout):
 
     assert C.sizeof(abi.FrameSettings) == 80
     assert C.sizeof(abi.DevicePtr) == 12
+    assert C.sizeof(abi.FrameDesc) == c_layout[("sr_frame_desc", "__sizeof__")]
+    for name, *_ in abi.FrameDesc._fields_:
+        assert getattr(abi.FrameDesc, name).offset == c_layout[("sr_frame_desc", name)], name
 
 
 def declared_functions():

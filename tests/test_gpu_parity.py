@@ -298,3 +298,56 @@ def test_telemetry_matches_oracle(backend):
     bad = np.nonzero(tel.astype(np.uint64) != want)[0]
     assert bad.size == 0, f"telemetry words differ at {bad[:10]}: gpu {tel[bad[:10]]} oracle {want[bad[:10]]}"
     assert tel[2] > 0 and tel[5] >= tel[2] / max(tel[4], 1)
+
+
+def test_shadow_history_across_frames():
+    """Three frames under a moving light with shadow history on: every frame
+    matches the oracle fed the oracle's previous history
+    (strand_shading.wgsl:660-679; history ping-pong, plugin.rs:1120-1140)."""
+    base = scene.config_small(froxel=8)
+    asset = base.assets[0]
+    center, radius = scene.bounding_sphere(asset)
+    dirs = [(0.35, 0.85, 0.4), (-0.5, 0.75, 0.45), (0.1, 0.6, -0.8)]
+    frames = []
+    for k, d in enumerate(dirs):
+        w = scene.config_small(froxel=8)
+        d = np.asarray(d) / np.linalg.norm(d)
+        w.lights["direction_to_light"][0, :3] = d
+        w.views[1] = scene.light_view(d, center, radius)
+        w.settings["shadow_history_valid"] = 1
+        w.name += f"-light{k}"
+        frames.append(w)
+    prev = None
+    with StrandRenderer() as r:
+        r.upload_workload(frames[0])
+        for k, w in enumerate(frames):
+            print(f"[parity] history frame {k}", flush=True)
+            ctr = r.render(w)
+            gpu = {n: r.dump(n) for n in EXACT + FLOAT}
+            ref = oracle.render(w, history_valid=True, prev_history=prev)
+            compare(w, ctr, gpu, ref)
+            prev = ref.get("history")
+            if k > 0:
+                h = gpu["history"].reshape(-1, 2)
+                assert (np.abs(h[:, 0] - h[:, 1]) > 1e-4).any(), "the filter blends in the previous frame"
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("depth", [0.9, 0.2])
+def test_parity_opaque_occlusion(depth, backend):
+    """Opaque depth reduce (K1) and the fine-tile occlusion of every binning
+    walk, against the oracle: an occluder in front of all strands and one
+    cutting through them."""
+    w = scene.config_small(froxel=8)
+    w.opaque = scene.opaque_disc(w, depth)
+    w.settings["fine_binning_backend"] = backend
+    w.name += f"-opaque{depth}-b{backend}"
+    print(f"[parity] {w.name}", flush=True)
+    ref = oracle.render(w)
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        ctr = r.render(w)
+        gpu = {k: r.dump(k) for k in EXACT + FLOAT + ["opaque_tiles"]}
+    compare(w, ctr, gpu, ref)
+    assert_exact("opaque_tiles", gpu["opaque_tiles"], ref.get("opaque_tiles"))
+    assert ctr["fine_refs"] < oracle.render(scene.config_small(froxel=8)).counters()["fine_refs"]

@@ -387,3 +387,107 @@ def test_single_segment_coverage_is_analytic():
             assert img[y, x, 3] == pytest.approx(alpha_w * cov, abs=2e-5)
             checked += cov > 0
     assert checked > 20
+
+
+def test_shadow_history_filter_properties():
+    """strand_shading.wgsl:660-679: with no previous history the filtered
+    visibility is the current one; with a previous frame it lies between the
+    previous filtered value and the current one (an exponential filter);
+    a texel whose previous value is 0 starts fresh."""
+    base = scene.config_tiny(strands=40, res=72, light_res=72)
+    f0 = oracle.render(base, history_valid=True)  # no previous frame: cleared history
+    h0 = f0.get("history").reshape(-1, 2)
+    shaded = h0[:, 1] > 0
+    assert shaded.any()
+    np.testing.assert_array_equal(h0[shaded, 0], h0[shaded, 1])
+    # move the light: visibilities change; the filter pulls towards the past
+    moved = scene.build_workload("tiny-moved-light", base.assets[0],
+                                 [((scene.HEAD_CENTER[0], scene.HEAD_CENTER[1] + 0.05, scene.HEAD_CENTER[2] + 0.6),
+                                   72, 72)], 1, 72, 4, 8, 8)
+    moved.lights["direction_to_light"][0, :3] = np.array([-0.7, 0.5, 0.5]) / np.linalg.norm([-0.7, 0.5, 0.5])
+    moved.views[1] = scene.light_view(moved.lights["direction_to_light"][0, :3], *scene.bounding_sphere(base.assets[0]))
+    f1 = oracle.render(moved, history_valid=True, prev_history=h0)
+    h1 = f1.get("history").reshape(-1, 2)
+    both = shaded & (h1[:, 1] > 0) & (h0[:, 0] > 0)
+    assert both.sum() > 10
+    cur, prev, filt = h1[both, 1], h0[both, 0], h1[both, 0]
+    lo, hi = np.minimum(cur, prev), np.maximum(cur, prev)
+    assert ((filt >= lo - 1e-6) & (filt <= hi + 1e-6)).all()
+    changed = np.abs(cur - prev) > 1e-3
+    assert changed.any(), "the moved light changes some visibilities"
+    # weight 0.88 on the past: the filtered value sits nearer the previous one
+    assert (np.abs(filt - prev)[changed] < np.abs(filt - cur)[changed]).all()
+    # without history_valid the previous frame is ignored
+    f2 = oracle.render(moved, history_valid=False, prev_history=h0)
+    h2 = f2.get("history").reshape(-1, 2)
+    np.testing.assert_array_equal(h2[:, 0], h2[:, 1])
+
+
+def brute_opaque_tiles(w):
+    """reduce_opaque_depth (opaque_depth_reduce.wgsl:35-86) by numpy over the
+    same input: per camera fine tile, all pixels covered (min over samples
+    > 0) and the quantised smallest such depth."""
+    out = {}
+    px = 0
+    for f in w.frusta:
+        if f["kind"] != 0:
+            continue
+        W, H, fx, fy = (int(f[k]) for k in ("screen_width", "screen_height", "froxel_size_x", "froxel_size_y"))
+        d = w.opaque[px:px + W * H].min(axis=1).reshape(H, W)
+        px += W * H
+        ftx, fty = -(-W // fx), -(-H // fy)
+        for ty in range(fty):
+            for tx in range(ftx):
+                blk = d[ty * fy:(ty + 1) * fy, tx * fx:(tx + 1) * fx]
+                cov = blk > 0
+                far = np.float32(blk[cov].min()) if cov.any() else np.float32(1.0)
+                q = int(np.rint(np.float32(min(max(far, 0.0), 1.0)) * np.float32(16777215.0)))
+                out[int(f["fine_depth_tile_base"]) + ty * ftx + tx] = (int(cov.all()), q)
+    return out
+
+
+def test_opaque_reduce_brute_force():
+    w = scene.config_small(froxel=8)
+    w.opaque = scene.opaque_disc(w, 0.5, samples=4)
+    tiles = oracle.render(w, binning_only=True).get("opaque_tiles")
+    want = brute_opaque_tiles(w)
+    assert len(want) > 100 and sum(c for c, _ in want.values()) > 10
+    for t, (c, q) in want.items():
+        assert (int(tiles[t]["covered"]), int(tiles[t]["farthest_q"])) == (c, q), t
+
+
+def test_opaque_occlusion_culls_hidden_cells():
+    base = scene.config_small(froxel=8)
+    ref0 = oracle.render(base)
+    # an occluder behind every strand hides nothing
+    far = scene.config_small(froxel=8)
+    far.opaque = scene.opaque_disc(far, 1e-6)
+    r_far = oracle.render(far)
+    for k in ("refs", "cell_counts", "page_meta", "work_items"):
+        assert r_far.get(k).tobytes() == ref0.get(k).tobytes(), k
+    # an occluder in front of every strand empties its fully covered tiles
+    near = scene.config_small(froxel=8)
+    near.opaque = scene.opaque_disc(near, 0.9)
+    r_near = oracle.render(near)
+    tiles = r_near.get("opaque_tiles")
+    cam = near.frusta[0]
+    ftx = -(-int(cam["screen_width"]) // int(cam["froxel_size_x"]))
+    covered = {int(cam["fine_depth_tile_base"]) + i for i in range(len(tiles)) if tiles[i]["covered"]}
+    assert covered
+    pm, counts = r_near.get("page_meta"), r_near.get("cell_counts").reshape(-1, 256)
+    hidden_refs = 0
+    for p in range(len(pm)):
+        tile = int(pm["coarse_tile_id"][p])
+        if tile >= int(cam["coarse_depth_tile_count"]):
+            continue  # a light page
+        cx, cy = tile % int(cam["coarse_tiles_x"]), tile // int(cam["coarse_tiles_x"])
+        for cell in np.nonzero(counts[p])[0]:
+            fx, fy = cx * 4 + cell % 4, cy * 4 + (cell // 4) % 4
+            hidden_refs += (fy * ftx + fx) in covered
+    assert hidden_refs == 0
+    assert r_near.counters()["fine_refs"] < ref0.counters()["fine_refs"]
+    # strands vanish from the image where the occluder covers whole tiles
+    img = r_near.get("image").reshape(int(cam["screen_height"]), int(cam["screen_width"]), 4)
+    for t in list(covered)[:50]:
+        tx, ty = t % ftx, t // ftx
+        assert (img[ty * 8:(ty + 1) * 8, tx * 8:(tx + 1) * 8, 3] == 0).all()
