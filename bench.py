@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -34,12 +35,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def percentile(samples, f):
-    """Nearest rank on sorted samples, index round((n-1) f), as the reference
-    bench's summarize (examples/strand_bench.rs:931-947)."""
+    """Nearest rank on sorted samples, index round((n-1) f) rounded half away
+    from zero, as the reference bench's summarize
+    (examples/strand_bench.rs:931-947)."""
     s = sorted(samples)
     if not s:
         return None
-    return s[int(round((len(s) - 1) * f))]
+    return s[int(math.floor((len(s) - 1) * f + 0.5))]
 
 
 def stage_bytes(ctr, w):
@@ -121,6 +123,41 @@ def time_frames(r, w, steps: int, warmup: int):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return ctr, lat, dt
+
+
+def write_report(args, r, w):
+    """The reference bench's report for this workload: per-stage GPU times and
+    telemetry over sampled frames (examples/strand_bench.rs:859-909)."""
+    import copy
+
+    import torch
+
+    from bevy_strand_rasterizer_amd import report
+
+    wt = copy.copy(w)
+    wt.settings = dict(w.settings, telemetry_enabled=1)
+    r.set_profiling(True)
+    for _ in range(2):
+        r.render(wt)
+    stages, tele = {}, []
+    for _ in range(args.report_frames):
+        r.render(wt)
+        for k, v in r.stage_times().items():
+            stages.setdefault(k, []).append(v)
+        tele.append(report.telemetry_samples(r.dump("telemetry"), len(wt.frusta)))
+    r.set_profiling(False)
+    cam = wt.frusta[0]
+    a = wt.assets[0]
+    rep = report.make_report(
+        wt.name, int(cam["screen_width"]), int(cam["screen_height"]), 2, args.report_frames, True, args.backend,
+        0x5EEDCAFE,
+        {"geometry_count": len(wt.assets), "strand_count": int(a.strand_count), "vertex_count": int(len(a.vertices)),
+         "clusters": None, "strands_per_cluster": None, "points_per_strand": int(a.meta[0]["count"]),
+         "asset": "synthetic scalp"},
+        {"name": torch.cuda.get_device_name(0), "backend": "HIP", "driver": torch.version.hip, "driver_info": ""},
+        stages, tele)
+    with open(args.report, "w") as f:
+        json.dump(rep, f, indent=1)
 
 
 def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool = True):
@@ -214,6 +251,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-max-views", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=96.0)
+    ap.add_argument("--report", default="", help="also write the reference bench's report (schema_version 1) here, "
+                                                  "from --report-frames frames with telemetry on")
+    ap.add_argument("--report-frames", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -253,6 +293,8 @@ def main():
     ctr, lat, dt = time_frames(r, w, args.steps, 0)
     ms_per_step = dt * 1e3 / args.steps
     fps = views * args.steps / dt
+    if args.report:
+        write_report(args, r, w)
     r.close()
     print(f"bench: plain {fps:.2f} frames/s", file=sys.stderr, flush=True)
 
