@@ -254,6 +254,9 @@ def main():
     ap.add_argument("--report", default="", help="also write the reference bench's report (schema_version 1) here, "
                                                   "from --report-frames frames with telemetry on")
     ap.add_argument("--report-frames", type=int, default=10)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a plain copy kernel of 1 GiB read + 1 GiB written (torch.neg), the reference "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -357,6 +360,12 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out))
+    if args.calibrate_copy:
+        x = torch.ones(1 << 28, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.neg(x, out=y)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
