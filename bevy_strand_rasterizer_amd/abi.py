@@ -22,8 +22,7 @@ DEPTH_QUANT_MAX = 16777215
 INVALID_PTR = 0xFFFFFFFF
 MAX_DOM_LAYERS = 16
 BINNING_FRUSTUM_SHIFT = 3
-CONFIG_CSR_GLOBAL_SORT = 1  # sr_device_config.flags
-CONFIG_CSR_FEW_HITS = 2
+CONFIG_CSR_FEW_HITS = 2  # sr_device_config.flags
 CONFIG_HIP_GRAPHS = 4
 
 # sr_status

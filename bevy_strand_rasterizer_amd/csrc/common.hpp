@@ -87,7 +87,7 @@ enum Counter : uint32_t {
   C_CAMERA_HEAD = 11, // camera raster run queue head
   C_SHADOW_HEAD = 12, // shadow raster run queue head
   C_CANDS = 13,       // page candidates (CSR backend)
-  C_CAND_OVERFLOW = 14,
+  C_CAND_OVERFLOW = 14,  // unused
   C_CSR_COUNT_HEAD = 15,  // page queue heads of the CSR count and fill kernels
   C_CSR_FILL_HEAD = 16,
   C_COUNT = 20
@@ -102,7 +102,7 @@ struct FrameParams {
   uint32_t coarse_total;  // coarse tiles over all frusta
   uint32_t task_cap, ref_cap, work_cap, atlas_texels;
   uint32_t dom_layers, page_cap;
-  uint32_t cand_cap, cand_lds_limit;  // CSR: lists longer than cand_lds_limit sort in global memory
+  uint32_t cand_cap, pad0;  // CSR: entry capacity
   sr_frame_settings st;
 
   uint32_t* counters;  // Counter words
@@ -136,11 +136,16 @@ struct FrameParams {
   sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
   sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
   // page CSR backend
-  uint32_t* cand_count_pt;           // [coarse_total * 16] candidates per page-table entry
-  uint32_t* cand_off;                // [page_cap + 1] candidate list offsets
-  uint32_t* cand_cursor;             // [page_cap]
-  uint32_t* cands;                   // [cand_cap] task ids, sorted per page
-  uint32_t* cands_tmp;               // [cand_cap] radix-sort ping-pong for long lists
+  // (page, slot) entries: each slot writes one per page it touches, in slot
+  // order at scanned offsets; a stable radix sort by page then leaves every
+  // page's candidate slots contiguous and ascending
+  uint32_t* slot_entries;            // [F * slots] entries of each slot (0 when invalid)
+  uint32_t* entry_base;              // [F * slots] exclusive scan of slot_entries
+  uint32_t* ekeys_w;                 // [cand_cap] page of each entry as written (page_cap: dropped page)
+  uint32_t* cands_w;                 // [cand_cap] slot of each entry as written
+  const uint32_t* ekeys;             // [cand_cap] the same, sorted by page
+  const uint32_t* cands;             // [cand_cap]
+  uint2* cand_range;                 // [page_cap] [begin, end) of each page's candidates
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // fine cells each slot's segment hits, recorded while marking (CSR backend):
   // slot_hits[s * hit_k + j] = page-table entry << 8 | cell, in traversal

@@ -30,12 +30,21 @@ struct StageTimer {
   }
 };
 
+// Items per block of the page-CSR entry radix sort (256 threads x 16).
+constexpr uint32_t kRadixBlock = 256u * 16u;
+
 // Scratch owned by the runtime for one frame's prepass.
 struct PrepassScratch {
   sr_projected_segment* slot_proj;  // [F * slots]
   uint32_t* partials;               // scan tile partials
   uint4* partials4;                 // scan tile partials (4-wide)
   uint32_t* page_xy_mask;           // [coarse_total * 16]
+  // page-CSR entry sort: the two (key, value) buffer pairs (the sorted result
+  // lands in pair 0, FrameParams::ekeys / cands), digit histograms, passes
+  uint32_t* ekeys[2];
+  uint32_t* cands[2];
+  uint32_t* radix_hist;             // [256 * radix_blocks]
+  uint32_t radix_blocks, radix_passes;
 };
 
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer);

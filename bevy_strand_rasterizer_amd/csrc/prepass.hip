@@ -266,8 +266,6 @@ __global__ void k_lut(FrameParams fp) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fp.coarse_total) return;
   for (uint32_t z = 0; z < kSlices; ++z) fp.page_table[t * kSlices + z] = 0u;
-  if (fp.st.fine_binning_backend != 0u)
-    for (uint32_t z = 0; z < kSlices; ++z) fp.cand_count_pt[t * kSlices + z] = 0u;
   for (uint32_t z = 0; z < kExt * kExt; ++z) fp.active_flags[t * kExt * kExt + z] = 0u;
   uint32_t fi = frustum_for_coarse_tile(fp, t);
   LutBuilder b;
@@ -394,11 +392,15 @@ __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint3
 
 constexpr uint32_t kHitOverflow = 0xFFu;
 
-// Cells recorded for one slot while marking (CSR backend).
+// Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst,
+// n counts every visible cell even past cap) and the pages with a visible
+// cell (entries).  SCATTER: the slot's next entry is base + n.
 struct HitRec {
   uint32_t* dst;  // null: not recording
   uint32_t n;
   uint32_t cap;
+  uint32_t entries;
+  uint32_t base, slot;
 };
 
 // trace_segment_into_coarse_count_page (strand_prepass.wgsl:1412-1575).
@@ -428,31 +430,32 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
       fp.page_table[pti] = kMarked;
       if (fp.st.fine_binning_backend != 0u) {
         // a (segment, page) candidate has at least one cell opaque depth
-        // does not hide; entries with a recorded first cell are counted after
-        // the walk from the record, the rest (past the record) here
+        // does not hide
         const uint32_t before = rec.n;
         if (rec.dst) {
           fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
             if (rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
             ++rec.n;
           });
-          if (before >= rec.cap && rec.n > before) atomicAdd(&fp.cand_count_pt[pti], 1u);
+          if (rec.n > before) ++rec.entries;
         } else if (any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) {
-          atomicAdd(&fp.cand_count_pt[pti], 1u);
+          ++rec.entries;
         }
       }
       continue;
     }
     uint32_t h = fp.page_table[pti];
-    if (h == 0u || h == kMarked) continue;
-    uint32_t page = h - 1u;
-    if (MODE == MODE_SCATTER) {
+    if (MODE == MODE_SCATTER) {  // the slot's next (page, slot) entry; dropped pages sort last
       if (!any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) continue;
-      uint32_t pos = atomicAdd(&fp.cand_cursor[page], 1u);
-      uint64_t dst = (uint64_t)fp.cand_off[page] + pos;
-      if (dst < fp.cand_cap) fp.cands[dst] = task;
+      const uint64_t idx = (uint64_t)rec.base + rec.n++;
+      if (idx < fp.cand_cap) {
+        fp.ekeys_w[idx] = (h == 0u || h == kMarked) ? fp.page_cap : h - 1u;
+        fp.cands_w[idx] = rec.slot;
+      }
       continue;
     }
+    if (h == 0u || h == kMarked) continue;
+    uint32_t page = h - 1u;
     uint32_t ref_base = 0;
     if (MODE == MODE_FILL) {
       ref_base = fp.page_meta[page].seg_ref_base;
@@ -549,7 +552,7 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     const InstRec& in = fp.inst[ii];
     if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
   }
-  HitRec none{nullptr, 0u, 0u};
+  HitRec none{nullptr, 0u, 0u, 0u, 0u, 0u};
   visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t, none);
 }
 
@@ -581,30 +584,26 @@ struct EntryCursor {
 // the CSR backend, record their fine cells and count page candidates.
 __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
   const uint64_t total = (uint64_t)fp.F * fp.slots;
-  const bool record = fp.st.fine_binning_backend != 0u && fp.hit_k != 0u;
+  const bool csr = fp.st.fine_binning_backend != 0u;
+  const bool record = csr && fp.hit_k != 0u;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
     bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
-    uint32_t recorded = 0;
+    uint32_t entries = 0;
     if (ok) {
       slot_proj[s] = ps;
       // K10: mark the (coarse tile, LUT slice) pages the segment touches,
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k};
+      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u, 0u};
       visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
-      if (record) {
-        fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
-        recorded = min(rec.n, rec.cap);
-      }
+      if (record) fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
+      entries = rec.entries;
     }
-    if (record) {  // K10 candidate counts of the recorded entries
-      EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, recorded, kInvalid};
-      for (uint32_t k = cur.next(); k != kInvalid; k = cur.next()) atomicAdd(&fp.cand_count_pt[k], 1u);
-    }
+    if (csr) fp.slot_entries[s] = entries;  // K16: candidates without a device atomic
     unsigned long long m = __ballot(ok);
     if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
     const FrustumRec& fr = fp.fr[fi];
@@ -650,7 +649,6 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
     const unsigned long long m = fp.slot_masks[s >> 6];
     const uint32_t t = fp.wave_offsets[s >> 6] + (uint32_t)__popcll(m & lower);
     const bool valid = ((m >> lane) & 1ull) && t < fp.task_cap;
-    uint32_t recorded = 0;
     if (valid) {
       uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
       uint32_t ii = find_instance(fp, pos);
@@ -661,26 +659,25 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       const sr_projected_segment ps = slot_proj[s];
       fp.proj[t] = ps;
       if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
-      if (csr) {
+      if (csr) {  // K17: the slot's (page, slot) entries at its scanned offset
         const uint32_t nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
+        const uint32_t base = fp.entry_base[s];
         if (nh != kHitOverflow) {
-          recorded = nh;
+          EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, nh, kInvalid};
+          uint32_t j = 0;
+          for (uint32_t pti = cur.next(); pti != kInvalid; pti = cur.next(), ++j) {
+            const uint64_t idx = (uint64_t)base + j;
+            if (idx >= fp.cand_cap) break;
+            const uint32_t e = fp.page_table[pti];
+            fp.ekeys_w[idx] = (e == 0u || e == kMarked) ? fp.page_cap : e - 1u;
+            fp.cands_w[idx] = (uint32_t)s;
+          }
         } else {  // too many cells to record: trace again
           V3 p0, p1;
           task_endpoints(fr.d, ps, &p0, &p1);
-          HitRec none{nullptr, 0u, 0u};
-          visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, none);
+          HitRec rec{nullptr, 0u, 0u, 0u, base, (uint32_t)s};
+          visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, rec);
         }
-      }
-    }
-    if (csr && fp.hit_k) {
-      EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, recorded, kInvalid};
-      for (uint32_t key = cur.next(); key != kInvalid; key = cur.next()) {
-        const uint32_t e = fp.page_table[key];
-        if (e == 0u || e == kMarked) continue;
-        const uint32_t page = e - 1u;
-        const uint64_t dst = (uint64_t)fp.cand_off[page] + atomicAdd(&fp.cand_cursor[page], 1u);
-        if (dst < fp.cand_cap) fp.cands[dst] = (uint32_t)s;
       }
     }
   }
@@ -696,8 +693,6 @@ struct PageAllocDst {
   sr_fine_page_meta* meta;
   uint32_t* counters;
   uint32_t page_cap;
-  const uint32_t* cand_count_pt;  // non-null for the CSR backend
-  uint32_t* page_cands;
   __device__ void store(uint32_t e, uint32_t ex, uint32_t v) const {
     if (!v) return;
     if (ex >= page_cap) {  // the reference drops pages past its capacity (entry 0)
@@ -707,7 +702,6 @@ struct PageAllocDst {
     }
     table[e] = ex + 1u;
     meta[ex] = sr_fine_page_meta{0u, 0u, 0u, 0u, e / kSlices, e % kSlices, 0u, 0u};
-    if (cand_count_pt) page_cands[ex] = cand_count_pt[e];
   }
   __device__ void total(uint32_t t) const {
     counters[C_MARKED] = t;
@@ -783,31 +777,112 @@ struct PageBaseDst {
 // ------------------------------------------------------ page CSR backend
 // K16-K18 (prefix_page_candidates, scatter_page_candidates,
 // build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507): every
-// page owns the list of slots whose segments touch it (slot order is task
-// order).  One workgroup per page sorts its candidate list, counts cells in
-// LDS, and
-// after the page-base scan fills references with stable ranks: a chunk of
-// 256 candidates sets one bit per (cell, lane) in an LDS bit matrix and each
-// hit's rank is the cell cursor plus the popcount of lower lanes, so every
-// cell comes out in task order, i.e. sorted by (inst, seg), with no
-// reference sort.
-struct CandSrc {
-  const uint32_t* counts;
-  __device__ uint32_t load(uint32_t i) const { return counts[i]; }
+// page owns the list of slots whose segments touch it, ascending (slot order
+// is task order).  Device-scope atomics go to memory on this part and
+// serialise on hot pages, so the lists are built without any: marking counts
+// each slot's (page, slot) entries, a scan gives every slot its offset, the
+// write pass stores the entries in slot order, and a stable LSD radix sort by
+// page leaves each page's slots contiguous and ascending.  One workgroup per
+// page then counts cells in LDS and, after the page-base scan, fills
+// references with stable ranks: a chunk of 256 candidates sets one bit per
+// (cell, lane) in an LDS bit matrix and each hit's rank is the cell cursor
+// plus the popcount of lower lanes, so every cell comes out in task order,
+// i.e. sorted by (inst, seg), with no reference sort.
+struct EntrySrc {
+  const uint32_t* entries;
+  __device__ uint32_t load(uint32_t i) const { return entries[i]; }
 };
-struct CandDst {
-  uint32_t* off;
-  uint32_t* cursor;
+struct EntryDst {
+  uint32_t* base;
   uint32_t* counters;
-  __device__ void store(uint32_t p, uint32_t ex, uint32_t) const {
-    off[p] = ex;
-    cursor[p] = 0u;
-  }
-  __device__ void total(uint32_t t) const {
-    counters[C_CANDS] = t;
-    off[counters[C_PAGES]] = t;
-  }
+  __device__ void store(uint32_t s, uint32_t ex, uint32_t) const { base[s] = ex; }
+  __device__ void total(uint32_t t) const { counters[C_CANDS] = t; }
 };
+
+// ---- stable LSD radix sort of (key, value) pairs, 8-bit digits
+constexpr uint32_t kRadixItems = kRadixBlock / 256u;
+
+__device__ __forceinline__ uint32_t radix_count(const uint32_t* n_dev, uint32_t cap) { return min(*n_dev, cap); }
+
+// Per block of kRadixBlock items: digit histogram, stored digit-major
+// (hist[d * nblocks + b]) so one exclusive scan yields every (digit, block)
+// output base.
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
+                                                    uint32_t shift, uint32_t* hist, uint32_t nblocks) {
+  __shared__ uint32_t cnt[256];
+  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x, base = b * kRadixBlock;
+  cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  for (uint32_t k = 0; k < kRadixItems; ++k) {
+    const uint32_t i = base + k * 256u + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + b] = cnt[threadIdx.x];
+}
+
+struct HistSrc {
+  const uint32_t* hist;
+  __device__ uint32_t load(uint32_t i) const { return hist[i]; }
+};
+struct HistDst {
+  uint32_t* hist;
+  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { hist[i] = ex; }
+  __device__ void total(uint32_t) const {}
+};
+
+// Scatter: items of a block in index order, 256 at a time; each item's rank
+// among equal digits is the digit cursor plus the popcount of lower lanes in
+// the LDS bit matrix (stable).
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals, uint32_t* okeys,
+                                                       uint32_t* ovals, const uint32_t* n_dev, uint32_t cap,
+                                                       uint32_t shift, const uint32_t* hist, uint32_t nblocks) {
+  __shared__ uint32_t rows[256 * 8];
+  __shared__ uint32_t cursor[256];
+  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x, base = b * kRadixBlock;
+  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
+  if (base >= n) return;
+  cursor[lane] = hist[lane * nblocks + b];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
+  __syncthreads();
+  for (uint32_t k = 0; k < kRadixItems; ++k) {
+    const uint32_t i = base + k * 256u + lane;
+    const bool has = i < n;
+    const uint32_t key = has ? keys[i] : 0u, d = (key >> shift) & 255u;
+    if (has) atomicOr(&rows[d * 8 + word], bit);
+    __syncthreads();
+    if (has) {
+      const uint32_t* row = rows + d * 8;
+      uint32_t rank = cursor[d];
+      for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
+      rank += __popc(row[word] & (bit - 1u));
+      okeys[rank] = key;
+      ovals[rank] = vals[i];
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      add += __popc(rows[lane * 8 + w]);
+      rows[lane * 8 + w] = 0u;
+    }
+    cursor[lane] += add;
+    __syncthreads();
+  }
+}
+
+// Each page's [begin, end) in the sorted entries (pages without entries keep
+// the cleared [0, 0)).
+__global__ __launch_bounds__(256) void k_cand_ranges(FrameParams fp) {
+  const uint32_t n = min(fp.counters[C_CANDS], fp.cand_cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t k = fp.ekeys[i];
+    if (k >= fp.page_cap) continue;
+    if (i == 0u || fp.ekeys[i - 1u] != k) fp.cand_range[k].x = i;
+    if (i + 1u == n || fp.ekeys[i + 1u] != k) fp.cand_range[k].y = i + 1u;
+  }
+}
 
 // trace_candidate_inside_page (strand_prepass.wgsl:2240-2337): the cells of
 // page `pm` that candidate slot `slot` hits.  Recorded cells are filtered by
@@ -857,66 +932,6 @@ __device__ __forceinline__ sr_fine_seg_ref slot_ref(const FrameParams& fp, uint3
   return ref;
 }
 
-constexpr uint32_t kCandLds = 8192;
-
-// Stable LSD radix sort (8-bit digits) of a candidate list too long for LDS,
-// ping-ponging between a and b; the same LDS bit-matrix ranks as the fill
-// keep every pass stable.  Returns the buffer holding the sorted list.
-// lds holds 256 * 8 row words + 256 cursors.
-__device__ uint32_t* radix_sort_global(uint32_t* a, uint32_t* b, uint32_t n, uint32_t* lds, uint32_t* lo_hi,
-                                       uint32_t* lds4) {
-  uint32_t* rows = lds;
-  uint32_t* cursor = lds + kPage * 8;
-  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
-  if (lane == 0) { lo_hi[0] = 0xFFFFFFFFu; lo_hi[1] = 0u; }
-  __syncthreads();
-  uint32_t mn = 0xFFFFFFFFu, mx = 0u;
-  for (uint32_t i = lane; i < n; i += blockDim.x) { mn = min(mn, a[i]); mx = max(mx, a[i]); }
-  atomicMin(&lo_hi[0], mn);
-  atomicMax(&lo_hi[1], mx);
-  __syncthreads();
-  const uint32_t lo = lo_hi[0], span = lo_hi[1] - lo;
-  const uint32_t bits = span ? 32u - __clz(span) : 1u;
-  for (uint32_t sh = 0; sh < bits; sh += 8u) {
-    cursor[lane] = 0u;
-#pragma unroll
-    for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
-    __syncthreads();
-    for (uint32_t i = lane; i < n; i += blockDim.x) atomicAdd(&cursor[((a[i] - lo) >> sh) & 255u], 1u);
-    __syncthreads();
-    uint32_t total;
-    const uint32_t ex = block_exclusive<uint32_t>(cursor[lane], &total, lds4);
-    cursor[lane] = ex;
-    __syncthreads();
-    for (uint32_t cb = 0; cb < n; cb += 256u) {
-      const uint32_t i = cb + lane;
-      const bool has = i < n;
-      const uint32_t v = has ? a[i] : 0u;
-      const uint32_t d = ((v - lo) >> sh) & 255u;
-      if (has) atomicOr(&rows[d * 8 + word], bit);
-      __syncthreads();
-      if (has) {
-        const uint32_t* row = rows + d * 8;
-        uint32_t rank = cursor[d];
-        for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
-        rank += __popc(row[word] & (bit - 1u));
-        b[rank] = v;
-      }
-      __syncthreads();
-      uint32_t add = 0;
-#pragma unroll
-      for (int w = 0; w < 8; ++w) {
-        add += __popc(rows[lane * 8 + w]);
-        rows[lane * 8 + w] = 0u;
-      }
-      cursor[lane] += add;
-      __syncthreads();
-    }
-    uint32_t* t = a; a = b; b = t;
-  }
-  return a;
-}
-
 // Next page of a dynamic page queue (heavy pages do not pile up on one
 // workgroup as with a static stride).
 __device__ __forceinline__ uint32_t next_page(uint32_t* head, uint32_t* slot) {
@@ -927,59 +942,21 @@ __device__ __forceinline__ uint32_t next_page(uint32_t* head, uint32_t* slot) {
 }
 
 __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
-  __shared__ uint32_t sc[kCandLds];
   __shared__ uint32_t scnt[kPage];
   __shared__ uint32_t lds4[4];
-  __shared__ uint32_t lo_hi[2];
   __shared__ uint32_t colmask;
   __shared__ uint32_t page_slot;
   const uint32_t P = fp.counters[C_PAGES];
   for (uint32_t page; (page = next_page(&fp.counters[C_CSR_COUNT_HEAD], &page_slot)) < P;) {
-    const uint32_t off = fp.cand_off[page];
-    const uint32_t n = fp.cand_off[page + 1] - off;
+    const uint2 range = fp.cand_range[page];
     const sr_fine_page_meta pm = fp.page_meta[page];
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    __syncthreads();
     scnt[threadIdx.x] = 0u;
     if (threadIdx.x == 0) colmask = 0u;
-    const bool valid = (uint64_t)off + n <= fp.cand_cap && fi != kInvalid;
-    uint32_t* gl = fp.cands + off;
-    const bool in_lds = n <= min(kCandLds, fp.cand_lds_limit);
-    const uint32_t* list = in_lds ? sc : gl;
-    if (valid) {
-      if (in_lds) {
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sc[i] = gl[i];
-        __syncthreads();
-        uint32_t m = 1;
-        while (m < n) m <<= 1;
-        for (uint32_t k = 2; k <= m; k <<= 1) {
-          for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-              uint32_t partner = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
-              if (partner <= i || partner >= n) continue;
-              uint32_t a = sc[i], b = sc[partner];
-              if (b < a) { sc[i] = b; sc[partner] = a; }
-            }
-            __syncthreads();
-          }
-        }
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gl[i] = sc[i];
-      } else {
-        const uint32_t* sorted = radix_sort_global(gl, fp.cands_tmp + off, n, sc, lo_hi, lds4);
-        if (sorted != gl)
-          for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gl[i] = sorted[i];
-        list = gl;
-      }
-    }
     __syncthreads();
-    if (valid) {
-      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t slot = list[i];
-        cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
-      }
-    } else if (threadIdx.x == 0 && (uint64_t)off + n > fp.cand_cap) {
-      atomicAdd(&fp.counters[C_CAND_OVERFLOW], 1u);
-    }
+    if (fi != kInvalid)
+      for (uint32_t i = range.x + threadIdx.x; i < range.y; i += blockDim.x)
+        cells_in_page(fp, fp.cands[i], pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
     __syncthreads();
     const uint32_t c = threadIdx.x;
     const uint32_t cnt = scnt[c];
@@ -1007,18 +984,16 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
     const sr_fine_page_meta pm = fp.page_meta[page];
     if (pm.seg_ref_count == 0u) continue;  // empty or overflowed (uniform)
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    const uint32_t off = fp.cand_off[page];
-    const uint32_t n = fp.cand_off[page + 1] - off;
-    __syncthreads();
+    const uint2 range = fp.cand_range[page];
     cursor[lane] = 0u;
     offs[lane] = fp.cell_offsets[page * kPage + lane];
 #pragma unroll
     for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
     __syncthreads();
-    for (uint32_t cb = 0; cb < n; cb += 256u) {
+    for (uint32_t cb = range.x; cb < range.y; cb += 256u) {
       const uint32_t i = cb + lane;
-      const bool has = i < n;
-      const uint32_t slot = has ? fp.cands[off + i] : 0u;
+      const bool has = i < range.y;
+      const uint32_t slot = has ? fp.cands[i] : 0u;
       if (has) cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) { atomicOr(&rows[cell * 8 + word], bit); });
       __syncthreads();
       if (has) {
@@ -1344,7 +1319,7 @@ __global__ __launch_bounds__(256) void k_telemetry(FrameParams fp) {
   for (uint32_t p = gid; p < P; p += stride) {
     const sr_fine_page_meta pm = fp.page_meta[p];
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    uint32_t cand = csr ? fp.cand_off[p + 1] - fp.cand_off[p] : fp.tel[kTelPageOff + p];
+    uint32_t cand = csr ? fp.cand_range[p].y - fp.cand_range[p].x : fp.tel[kTelPageOff + p];
     if (csr) fp.tel[kTelPageOff + p] = cand;
     if (fi != kInvalid) tel_add(fp, fi, TF_ALLOCATED_PAGES, 1u);
     if (cand == 0u) continue;
@@ -1410,16 +1385,32 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                         (uint32_t)(slots_total / 64u), nullptr, ps.partials, s);
   hipLaunchKernelGGL(k_frustum_bases, dim3(1), dim3(64), 0, s, fp);
   timer->mark(s, "allocate");
-  launch_scan<uint32_t>(PageMarkSrc{fp.page_table},
-                        PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap,
-                                     csr ? fp.cand_count_pt : nullptr, fp.cand_cursor},
+  launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap},
                         ptab, nullptr, ps.partials, s);
-  if (csr)
-    launch_scan<uint32_t>(CandSrc{fp.cand_cursor}, CandDst{fp.cand_off, fp.cand_cursor, fp.counters}, fp.page_cap,
-                          fp.counters + C_PAGES, ps.partials, s);
+  if (csr) {
+    timer->mark(s, "entry_offsets");
+    launch_scan<uint32_t>(EntrySrc{fp.slot_entries}, EntryDst{fp.entry_base, fp.counters}, (uint32_t)slots_total,
+                          nullptr, ps.partials, s);
+  }
   timer->mark(s, "project_write");
   hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
+    timer->mark(s, "sort_entries");
+    // stable LSD passes; the write target was chosen so the last pass lands
+    // in pair 0
+    int src = (ps.radix_passes & 1u) ? 1 : 0;
+    for (uint32_t k = 0; k < ps.radix_passes; ++k, src ^= 1) {
+      const uint32_t shift = 8u * k;
+      hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.ekeys[src], fp.counters + C_CANDS,
+                         fp.cand_cap, shift, ps.radix_hist, ps.radix_blocks);
+      launch_scan<uint32_t>(HistSrc{ps.radix_hist}, HistDst{ps.radix_hist}, 256u * ps.radix_blocks, nullptr,
+                            ps.partials, s);
+      hipLaunchKernelGGL(k_radix_scatter, dim3(ps.radix_blocks), dim3(256), 0, s, ps.ekeys[src], ps.cands[src],
+                         ps.ekeys[src ^ 1], ps.cands[src ^ 1], fp.counters + C_CANDS, fp.cand_cap, shift,
+                         ps.radix_hist, ps.radix_blocks);
+    }
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.cand_range), 2ull * fp.page_cap, 0u, s);
+    hipLaunchKernelGGL(k_cand_ranges, dim3(grid_for(fp.cand_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "csr_count");
     hipLaunchKernelGGL(k_csr_count, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "page_bases");

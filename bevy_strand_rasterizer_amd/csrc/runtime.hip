@@ -135,7 +135,8 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_CAND_PT, B_CAND_OFF, B_CAND_CURSOR, B_CANDS, B_CANDS_TMP, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_ENTRIES, B_ENTRY_BASE, B_CAND_RANGE, B_EKEYS0, B_EKEYS1, B_CANDS0,
+  B_CANDS1, B_RADIX_HIST, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -570,7 +571,14 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const uint32_t atlas_texels = atlas_w * atlas_h;
   const uint64_t atlas_n = (uint64_t)cams * NI * atlas_texels;
   const uint32_t tiles16 = coarse_total * kExt * kExt;
-  const uint32_t scan_tiles = (std::max({task_cap / 64u, ptab, tiles16, 1u}) + kScanTile - 1) / kScanTile + 1;
+  // page-CSR entry sort: 8-bit digits over the page key (page_cap = dropped)
+  uint32_t key_bits = 1;
+  while (key_bits < 32 && (1ull << key_bits) <= page_cap) ++key_bits;
+  const uint32_t radix_passes = csr ? (key_bits + 7u) / 8u : 0u;
+  const uint32_t radix_blocks = csr ? (cand_cap + kRadixBlock - 1u) / kRadixBlock : 0u;
+  const uint64_t scan_len = std::max<uint64_t>({(uint64_t)task_cap, (uint64_t)ptab, (uint64_t)tiles16,
+                                                256ull * radix_blocks, 1ull});
+  const uint32_t scan_tiles = (uint32_t)((scan_len + kScanTile - 1) / kScanTile + 1);
 
   // history: keep the previous buffer when the atlas layout is unchanged
   const bool hist_valid = st.shadow_history_valid && sl.shape.valid && sl.shape.cams == cams &&
@@ -592,9 +600,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_DOM_OPACITY, 4ull * dom_px * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
       {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
       {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
-      {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap}, {B_CAND_PT, csr ? 4ull * ptab : 0ull},
-      {B_CAND_OFF, csr ? 4ull * (page_cap + 1ull) : 0ull}, {B_CAND_CURSOR, csr ? 4ull * page_cap : 0ull},
-      {B_CANDS, 4ull * cand_cap}, {B_CANDS_TMP, 4ull * cand_cap}, {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
+      {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap},
+      {B_SLOT_ENTRIES, csr ? 4ull * task_cap : 0ull}, {B_ENTRY_BASE, csr ? 4ull * task_cap : 0ull},
+      {B_CAND_RANGE, csr ? 8ull * page_cap : 0ull}, {B_EKEYS0, 4ull * cand_cap}, {B_EKEYS1, 4ull * cand_cap},
+      {B_CANDS0, 4ull * cand_cap}, {B_CANDS1, 4ull * cand_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks},
+      {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
       {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
@@ -694,17 +704,29 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.depth = sl.buf[B_DEPTH].as<float>();
   fp.composite = sl.buf[B_COMPOSITE].as<float4>();
   // page_cap bounds the page arrays; the allocation scan drops pages past it
-  PrepassScratch ps{sl.buf[B_SLOT_PROJ].as<sr_projected_segment>(), sl.buf[B_PARTIALS].as<uint32_t>(),
-                    sl.buf[B_PARTIALS4].as<uint4>(), sl.buf[B_XY_MASK].as<uint32_t>()};
+  PrepassScratch ps{};
+  ps.slot_proj = sl.buf[B_SLOT_PROJ].as<sr_projected_segment>();
+  ps.partials = sl.buf[B_PARTIALS].as<uint32_t>();
+  ps.partials4 = sl.buf[B_PARTIALS4].as<uint4>();
+  ps.page_xy_mask = sl.buf[B_XY_MASK].as<uint32_t>();
+  ps.ekeys[0] = sl.buf[B_EKEYS0].as<uint32_t>();
+  ps.ekeys[1] = sl.buf[B_EKEYS1].as<uint32_t>();
+  ps.cands[0] = sl.buf[B_CANDS0].as<uint32_t>();
+  ps.cands[1] = sl.buf[B_CANDS1].as<uint32_t>();
+  ps.radix_hist = sl.buf[B_RADIX_HIST].as<uint32_t>();
+  ps.radix_blocks = radix_blocks;
+  ps.radix_passes = radix_passes;
   fp.page_cap = page_cap;
   fp.page_xy_mask = ps.page_xy_mask;
   fp.cand_cap = cand_cap;
-  fp.cand_lds_limit = (ctx->flags & SR_CONFIG_CSR_GLOBAL_SORT) ? 0u : 0xFFFFFFFFu;
-  fp.cand_count_pt = sl.buf[B_CAND_PT].as<uint32_t>();
-  fp.cand_off = sl.buf[B_CAND_OFF].as<uint32_t>();
-  fp.cand_cursor = sl.buf[B_CAND_CURSOR].as<uint32_t>();
-  fp.cands = sl.buf[B_CANDS].as<uint32_t>();
-  fp.cands_tmp = sl.buf[B_CANDS_TMP].as<uint32_t>();
+  fp.slot_entries = sl.buf[B_SLOT_ENTRIES].as<uint32_t>();
+  fp.entry_base = sl.buf[B_ENTRY_BASE].as<uint32_t>();
+  fp.cand_range = sl.buf[B_CAND_RANGE].as<uint2>();
+  // entries are written to the pair the last sort pass does not write
+  fp.ekeys_w = ps.ekeys[(radix_passes & 1u) ? 1 : 0];
+  fp.cands_w = ps.cands[(radix_passes & 1u) ? 1 : 0];
+  fp.ekeys = ps.ekeys[0];
+  fp.cands = ps.cands[0];
   fp.slot_proj = ps.slot_proj;
   fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
   fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;

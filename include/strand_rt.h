@@ -463,10 +463,6 @@ typedef struct sr_device_config {
   uint64_t hbm_budget_bytes;/* 0 = no budget */
 } sr_device_config;
 
-/* sr_device_config.flags: the page CSR backend sorts every page candidate
- * list with its global radix path instead of only lists longer than its LDS
- * list (exercises that path on small frames; results are identical). */
-#define SR_CONFIG_CSR_GLOBAL_SORT 1u
 /* sr_device_config.flags: the page CSR backend records only 2 fine cells per
  * segment while marking (instead of 8), so most segments take the re-trace
  * path of the later passes (exercises it; results are identical). */

@@ -103,15 +103,14 @@ CSR_CASES = {
 }
 
 
-CSR_FLAGS = {"global-sort": abi.CONFIG_CSR_GLOBAL_SORT, "few-hits": abi.CONFIG_CSR_FEW_HITS,
-             "global-sort+few-hits": abi.CONFIG_CSR_GLOBAL_SORT | abi.CONFIG_CSR_FEW_HITS}
+CSR_FLAGS = {"few-hits": abi.CONFIG_CSR_FEW_HITS, "few-hits+graphs": abi.CONFIG_CSR_FEW_HITS | abi.CONFIG_HIP_GRAPHS}
 
 
 @pytest.mark.parametrize("flags", list(CSR_FLAGS))
 @pytest.mark.parametrize("case", ["tiny-f8-csr", "small-f8-csr", "dense-one-tile-csr"])
 def test_parity_csr_paths(case, flags):
-    """Every page candidate list through the global radix sort, and/or most
-    segments through the re-trace path (2 recorded cells per segment)."""
+    """Most segments through the re-trace path (2 recorded cells per
+    segment), eager and replayed from a HIP graph."""
     w = CSR_CASES[case]()
     print(f"[parity] {case} ({flags}): {w.name}", flush=True)
     ref = oracle.render(w)
