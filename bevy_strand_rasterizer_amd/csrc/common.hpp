@@ -142,9 +142,16 @@ struct FrameParams {
   uint32_t* slot_entries;            // [F * slots] entries of each slot (0 when invalid)
   uint32_t* entry_base;              // [F * slots] exclusive scan of slot_entries
   uint32_t* ekeys_w;                 // [cand_cap] page of each entry as written (page_cap: dropped page)
-  uint32_t* cands_w;                 // [cand_cap] slot of each entry as written
-  const uint32_t* ekeys;             // [cand_cap] the same, sorted by page
-  const uint32_t* cands;             // [cand_cap]
+  uint32_t* cands_w;                 // [cand_cap] entry index as written (the identity)
+  const uint32_t* ekeys;             // [cand_cap] pages, sorted
+  const uint32_t* cands;             // [cand_cap] entry indices in page order
+  // per entry (indexed by entry index): its slot, material, and its fine
+  // cells in the page: cells 0-3 in epay0, cells 4-6 and the count (255:
+  // more than 7, look them up by slot) in epay1
+  uint32_t* eslot;
+  uint32_t* emat;
+  uint32_t* epay0;
+  uint32_t* epay1;
   uint2* cand_range;                 // [page_cap] [begin, end) of each page's candidates
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // fine cells each slot's segment hits, recorded while marking (CSR backend):

@@ -392,6 +392,30 @@ __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint3
 
 constexpr uint32_t kHitOverflow = 0xFFu;
 
+// A (page, slot) entry's cells: up to 7 packed as bytes with the count in
+// the top byte of the second word (255: too many, the cells are found by
+// slot instead).
+struct CellPack {
+  uint32_t p0 = 0u, p1 = 0u, n = 0u;
+  __device__ __forceinline__ void add(uint32_t cell) {
+    if (n < 4u) p0 |= cell << (8u * n);
+    else if (n < 7u) p1 |= cell << (8u * (n - 4u));
+    ++n;
+  }
+  __device__ __forceinline__ uint32_t word1() const { return (p1 & 0xFFFFFFu) | ((n > 7u ? 255u : n) << 24); }
+};
+
+__device__ __forceinline__ void write_entry(const FrameParams& fp, uint64_t idx, uint32_t page_entry, uint32_t slot,
+                                            uint32_t material, const CellPack& c) {
+  if (idx >= fp.cand_cap) return;
+  fp.ekeys_w[idx] = (page_entry == 0u || page_entry == kMarked) ? fp.page_cap : page_entry - 1u;
+  fp.cands_w[idx] = (uint32_t)idx;
+  fp.eslot[idx] = slot;
+  fp.emat[idx] = material;
+  fp.epay0[idx] = c.p0;
+  fp.epay1[idx] = c.word1();
+}
+
 // Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst,
 // n counts every visible cell even past cap) and the pages with a visible
 // cell (entries).  SCATTER: the slot's next entry is base + n.
@@ -400,7 +424,7 @@ struct HitRec {
   uint32_t n;
   uint32_t cap;
   uint32_t entries;
-  uint32_t base, slot;
+  uint32_t base, slot, material;
 };
 
 // trace_segment_into_coarse_count_page (strand_prepass.wgsl:1412-1575).
@@ -446,12 +470,10 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     }
     uint32_t h = fp.page_table[pti];
     if (MODE == MODE_SCATTER) {  // the slot's next (page, slot) entry; dropped pages sort last
-      if (!any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) continue;
-      const uint64_t idx = (uint64_t)rec.base + rec.n++;
-      if (idx < fp.cand_cap) {
-        fp.ekeys_w[idx] = (h == 0u || h == kMarked) ? fp.page_cap : h - 1u;
-        fp.cands_w[idx] = rec.slot;
-      }
+      CellPack cp;
+      fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) { cp.add(cell); });
+      if (cp.n == 0u) continue;
+      write_entry(fp, (uint64_t)rec.base + rec.n++, h, rec.slot, rec.material, cp);
       continue;
     }
     if (h == 0u || h == kMarked) continue;
@@ -552,7 +574,7 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     const InstRec& in = fp.inst[ii];
     if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
   }
-  HitRec none{nullptr, 0u, 0u, 0u, 0u, 0u};
+  HitRec none{nullptr, 0u, 0u, 0u, 0u, 0u, 0u};
   visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t, none);
 }
 
@@ -598,7 +620,7 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u, 0u};
+      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u, 0u, 0u};
       visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
       if (record) fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
       entries = rec.entries;
@@ -662,20 +684,26 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       if (csr) {  // K17: the slot's (page, slot) entries at its scanned offset
         const uint32_t nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
         const uint32_t base = fp.entry_base[s];
-        if (nh != kHitOverflow) {
-          EntryCursor cur{fp.slot_hits + s * fp.hit_k, 0u, nh, kInvalid};
-          uint32_t j = 0;
-          for (uint32_t pti = cur.next(); pti != kInvalid; pti = cur.next(), ++j) {
-            const uint64_t idx = (uint64_t)base + j;
-            if (idx >= fp.cand_cap) break;
-            const uint32_t e = fp.page_table[pti];
-            fp.ekeys_w[idx] = (e == 0u || e == kMarked) ? fp.page_cap : e - 1u;
-            fp.cands_w[idx] = (uint32_t)s;
+        const uint32_t strand = in.pos_strand[p];
+        const uint32_t material = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
+        if (nh != kHitOverflow) {  // the recorded cells, grouped by page (consecutive)
+          const uint32_t* h = fp.slot_hits + s * fp.hit_k;
+          uint32_t j = 0, prev = kInvalid;
+          CellPack cp;
+          for (uint32_t k = 0; k < nh; ++k) {
+            const uint32_t v = h[k], pti = v >> 8;
+            if (pti != prev) {
+              if (prev != kInvalid) write_entry(fp, (uint64_t)base + j++, fp.page_table[prev], (uint32_t)s, material, cp);
+              prev = pti;
+              cp = CellPack{};
+            }
+            cp.add(v & 255u);
           }
+          if (prev != kInvalid) write_entry(fp, (uint64_t)base + j, fp.page_table[prev], (uint32_t)s, material, cp);
         } else {  // too many cells to record: trace again
           V3 p0, p1;
           task_endpoints(fr.d, ps, &p0, &p1);
-          HitRec rec{nullptr, 0u, 0u, 0u, base, (uint32_t)s};
+          HitRec rec{nullptr, 0u, 0u, 0u, base, (uint32_t)s, material};
           visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, rec);
         }
       }
@@ -921,15 +949,24 @@ __device__ __forceinline__ void cells_in_page(const FrameParams& fp, uint32_t sl
 
 // The reference a candidate slot contributes (make_fine_seg_ref,
 // strand_prepass.wgsl:1313-1327): instance, index position, material.
-__device__ __forceinline__ sr_fine_seg_ref slot_ref(const FrameParams& fp, uint32_t slot) {
+__device__ __forceinline__ sr_fine_seg_ref slot_ref(const FrameParams& fp, uint32_t slot, uint32_t material) {
   const uint32_t pos = slot % fp.slots;
   const uint32_t ii = find_instance(fp, pos);
-  const InstRec& in = fp.inst[ii];
-  const uint32_t p = pos - in.pos_base;
-  sr_fine_seg_ref ref{ii, p, 0u};
-  const uint32_t strand = in.pos_strand[p];
-  if (strand < in.meta_count) ref.material_idx = in.meta[strand].material_idx;
-  return ref;
+  return sr_fine_seg_ref{ii, pos - fp.inst[ii].pos_base, material};
+}
+
+// The cells of entry e in page pm: from the entry's packed cells, or by its
+// slot when it has more than 7.
+template <class Emit>
+__device__ __forceinline__ void entry_cells(const FrameParams& fp, uint32_t e, const sr_fine_page_meta& pm, uint32_t fi,
+                                            Emit&& emit) {
+  const uint32_t w1 = fp.epay1[e], n = w1 >> 24;
+  if (n == 255u) {
+    cells_in_page(fp, fp.eslot[e], pm, fi, emit);
+    return;
+  }
+  const uint32_t w0 = fp.epay0[e];
+  for (uint32_t k = 0; k < n; ++k) emit(((k < 4u) ? (w0 >> (8u * k)) : (w1 >> (8u * (k - 4u)))) & 255u);
 }
 
 // Next page of a dynamic page queue (heavy pages do not pile up on one
@@ -956,7 +993,7 @@ __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
     __syncthreads();
     if (fi != kInvalid)
       for (uint32_t i = range.x + threadIdx.x; i < range.y; i += blockDim.x)
-        cells_in_page(fp, fp.cands[i], pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
+        entry_cells(fp, fp.cands[i], pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
     __syncthreads();
     const uint32_t c = threadIdx.x;
     const uint32_t cnt = scnt[c];
@@ -973,10 +1010,16 @@ __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
   }
 }
 
+// References of a page are staged in LDS (structure of arrays) when the page
+// holds at most kFillStage of them and leave as one coalesced copy; larger
+// pages write each reference in place.
+constexpr uint32_t kFillStage = 3072;
+
 __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
   __shared__ uint32_t rows[kPage * 8];  // bit (cell, lane) of the current chunk
   __shared__ uint32_t cursor[kPage];
   __shared__ uint32_t offs[kPage];
+  __shared__ uint32_t stage[3][kFillStage];
   __shared__ uint32_t page_slot;
   const uint32_t P = fp.counters[C_PAGES];
   const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
@@ -985,6 +1028,7 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
     if (pm.seg_ref_count == 0u) continue;  // empty or overflowed (uniform)
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
     const uint2 range = fp.cand_range[page];
+    const bool staged = pm.seg_ref_count <= kFillStage;
     cursor[lane] = 0u;
     offs[lane] = fp.cell_offsets[page * kPage + lane];
 #pragma unroll
@@ -993,19 +1037,39 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
     for (uint32_t cb = range.x; cb < range.y; cb += 256u) {
       const uint32_t i = cb + lane;
       const bool has = i < range.y;
-      const uint32_t slot = has ? fp.cands[i] : 0u;
-      if (has) cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) { atomicOr(&rows[cell * 8 + word], bit); });
+      const uint32_t e = has ? fp.cands[i] : 0u;
+      // this candidate's cells in the page, up to 8 kept as bytes
+      uint32_t c0 = 0u, c1 = 0u, nc = 0u;
+      if (has) entry_cells(fp, e, pm, fi, [&](uint32_t cell) {
+        atomicOr(&rows[cell * 8 + word], bit);
+        if (nc < 4u) c0 |= cell << (8u * nc);
+        else if (nc < 8u) c1 |= cell << (8u * (nc - 4u));
+        ++nc;
+      });
       __syncthreads();
       if (has) {
-        const sr_fine_seg_ref ref = slot_ref(fp, slot);
-        cells_in_page(fp, slot, pm, fi, [&](uint32_t cell) {
+        const uint32_t slot = fp.eslot[e];
+        sr_fine_seg_ref ref = slot_ref(fp, slot, fp.emat[e]);
+        auto put = [&](uint32_t cell) {
           const uint32_t* row = rows + cell * 8;
           uint32_t rank = cursor[cell];
           for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
           rank += __popc(row[word] & (bit - 1u));
-          uint64_t dst = (uint64_t)pm.seg_ref_base + offs[cell] + rank;
-          if (dst < fp.ref_cap) fp.refs[dst] = ref;
-        });
+          const uint32_t local = offs[cell] + rank;
+          if (staged) {
+            stage[0][local] = ref.inst_id;
+            stage[1][local] = ref.seg_id;
+            stage[2][local] = ref.material_idx;
+          } else {
+            const uint64_t dst = (uint64_t)pm.seg_ref_base + local;
+            if (dst < fp.ref_cap) fp.refs[dst] = ref;
+          }
+        };
+        if (nc <= 8u) {
+          for (uint32_t k = 0; k < nc; ++k) put(((k < 4u ? c0 >> (8u * k) : c1 >> (8u * (k - 4u)))) & 255u);
+        } else {
+          entry_cells(fp, e, pm, fi, put);
+        }
       }
       __syncthreads();
       uint32_t add = 0;
@@ -1016,6 +1080,11 @@ __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
       }
       cursor[lane] += add;
       __syncthreads();
+    }
+    if (staged) {  // the page's references as consecutive 32-bit words
+      uint32_t* dst = reinterpret_cast<uint32_t*>(fp.refs + pm.seg_ref_base);
+      const uint32_t words = 3u * pm.seg_ref_count;
+      for (uint32_t w = lane; w < words; w += 256u) dst[w] = stage[w % 3u][w / 3u];
     }
   }
 }

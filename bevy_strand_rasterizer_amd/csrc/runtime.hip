@@ -136,7 +136,7 @@ enum BufId {
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_ENTRIES, B_ENTRY_BASE, B_CAND_RANGE, B_EKEYS0, B_EKEYS1, B_CANDS0,
-  B_CANDS1, B_RADIX_HIST, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_CANDS1, B_RADIX_HIST, B_ESLOT, B_EMAT, B_EPAY0, B_EPAY1, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -557,11 +557,16 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {std::min<uint64_t>(ref_cap / 2u, (uint64_t)page_cap * kPage), 1u << 15, (uint64_t)sl.work_cap_hint});
   // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
   const bool csr = st.fine_binning_backend != 0u;
-  // cells recorded per slot while marking: kHitsPerSlot words per slot, when
-  // the page-table entry fits 24 bits and the table stays within budget
-  const uint32_t kHitsPerSlot = (ctx->flags & SR_CONFIG_CSR_FEW_HITS) ? 2u : 8u;
-  const uint64_t hit_bytes = 4ull * kHitsPerSlot * task_cap;
-  const uint32_t hit_k = (csr && ptab < (1u << 24) && hit_bytes <= (8ull << 30)) ? kHitsPerSlot : 0u;
+  // cells recorded per slot while marking: 24 words per slot (a segment of
+  // the benchmark scalps hits 5 cells on average and more than 24 about once
+  // in 10^5), fewer when the table would pass 8 GiB, none when page-table
+  // entries do not fit 24 bits; a slot with more cells is traced again
+  uint32_t hit_k = 0;
+  if (csr && ptab < (1u << 24))
+    for (uint32_t k : {24u, 16u, 8u})
+      if (4ull * k * task_cap <= (8ull << 30)) { hit_k = k; break; }
+  if (hit_k && (ctx->flags & SR_CONFIG_CSR_FEW_HITS)) hit_k = 2u;
+  const uint64_t hit_bytes = 4ull * hit_k * task_cap;
   const uint32_t cand_cap = csr ? (uint32_t)std::min<uint64_t>(
                                       std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)sl.cand_cap_hint}),
                                       0x7FFFFFFFull)
@@ -604,6 +609,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_SLOT_ENTRIES, csr ? 4ull * task_cap : 0ull}, {B_ENTRY_BASE, csr ? 4ull * task_cap : 0ull},
       {B_CAND_RANGE, csr ? 8ull * page_cap : 0ull}, {B_EKEYS0, 4ull * cand_cap}, {B_EKEYS1, 4ull * cand_cap},
       {B_CANDS0, 4ull * cand_cap}, {B_CANDS1, 4ull * cand_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks},
+      {B_ESLOT, 4ull * cand_cap}, {B_EMAT, 4ull * cand_cap}, {B_EPAY0, 4ull * cand_cap}, {B_EPAY1, 4ull * cand_cap},
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
@@ -727,6 +733,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.cands_w = ps.cands[(radix_passes & 1u) ? 1 : 0];
   fp.ekeys = ps.ekeys[0];
   fp.cands = ps.cands[0];
+  fp.eslot = sl.buf[B_ESLOT].as<uint32_t>();
+  fp.emat = sl.buf[B_EMAT].as<uint32_t>();
+  fp.epay0 = sl.buf[B_EPAY0].as<uint32_t>();
+  fp.epay1 = sl.buf[B_EPAY1].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
   fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
   fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;
