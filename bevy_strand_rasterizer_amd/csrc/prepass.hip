@@ -1013,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
 // References of a page are staged in LDS (structure of arrays) when the page
 // holds at most kFillStage of them and leave as one coalesced copy; larger
 // pages write each reference in place.
-constexpr uint32_t kFillStage = 3072;
+constexpr uint32_t kFillStage = 1024;
 
 __global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
   __shared__ uint32_t rows[kPage * 8];  // bit (cell, lane) of the current chunk

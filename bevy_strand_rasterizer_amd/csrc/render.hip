@@ -176,6 +176,39 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
   return c > 0.0f;
 }
 
+// Conservative coverage: false only when shadow_cover is false for that
+// pixel, whatever its rounding -- the pixel lies outside the reference's
+// t-range or farther than max radius (+ 0.01 px) from its line.  Costs a few
+// multiplies instead of the exact test's divisions and square root.
+__device__ __forceinline__ bool may_cover(float dxp, float dyp, float abx, float aby, float l2, float rm2) {
+  if (l2 == 0.0f) return dxp * dxp + dyp * dyp <= rm2;
+  const float dot = dxp * abx + dyp * aby;
+  const float cross = dxp * aby - dyp * abx;
+  return dot >= -1e-6f * l2 && dot <= 1.00001f * l2 && cross * cross <= rm2 * l2 * 1.0001f;
+}
+
+// Per pixel of the tile, one bit per reference of the batch that may cover
+// it (cols[p][w]: references w*64 .. w*64+63): wave w tests its 64
+// references against every pixel and a ballot transposes the answer.
+template <int TPX>
+__device__ __forceinline__ void batch_columns(const ShadowBatch<TPX>& b, uint32_t nb, uint32_t tx0, uint32_t ty0,
+                                              uint32_t fx, unsigned long long (*cols)[TPX / 64]) {
+  const uint32_t k = threadIdx.x, w = k >> 6;
+  const bool valid = k < nb;
+  float x0 = 0.0f, y0 = 0.0f, abx = 0.0f, aby = 0.0f, l2 = 0.0f, rm2 = 0.0f;
+  if (valid) {
+    const float4 A = b.a[k], Rr = b.r[k];
+    x0 = A.x; y0 = A.y; l2 = A.w; abx = Rr.z; aby = Rr.w;
+    const float rm = Rr.y + 0.01f;
+    rm2 = rm * rm;
+  }
+  for (uint32_t p = 0; p < (uint32_t)TPX; ++p) {
+    const float px = (float)(tx0 + p % fx) + 0.5f, py = (float)(ty0 + p / fx) + 0.5f;
+    const unsigned long long m = __ballot(valid && may_cover(px - x0, py - y0, abx, aby, l2, rm2));
+    if ((k & 63u) == 0u) cols[p][w] = m;
+  }
+}
+
 // K25 (strand_shadow_raster.wgsl): per pixel of a light tile, pass 1 finds
 // the front depth z0 (max depth over the covering references of the nearest
 // work item with a hit), pass 2 accumulates each covering reference's alpha
@@ -186,6 +219,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
   __shared__ RunTable<TPX> t;
   __shared__ ShadowBatch<TPX> b;
   __shared__ float part[TPX * SR_MAX_DOM_LAYERS];
+  __shared__ unsigned long long cols[TPX][TPX / 64];
   const uint32_t R = fp.counters[C_SHADOW_RUNS];
   const uint32_t L = fp.dom_layers;
   while (next_run<TPX>(&fp.counters[C_SHADOW_HEAD], R, t)) {
@@ -206,13 +240,19 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     float z0 = -1.0f;
     bool found = !active, hit = false;
     uint32_t cur = 0;
+    // (references a pixel's column leaves out do not cover it: skipping them
+    // changes no result, item boundaries included)
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
       if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
       __syncthreads();
-      if (!found) {
-        const uint32_t nb = min((uint32_t)TPX, total - bs);
-        for (uint32_t k = 0; k < nb; ++k) {
+      const uint32_t nb = min((uint32_t)TPX, total - bs);
+      batch_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+      __syncthreads();
+      for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !found; ++cw) {
+        for (unsigned long long bits = cols[pidx][cw]; bits;) {
+          const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
+          bits &= bits - 1ull;
           uint32_t w = b.item[k];
           if (w != cur) {
             if (hit) { found = true; break; }
@@ -234,9 +274,14 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
         __syncthreads();
         if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
         __syncthreads();
+        const uint32_t nb = min((uint32_t)TPX, total - bs);
+        batch_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+        __syncthreads();
         if (found) {
-          const uint32_t nb = min((uint32_t)TPX, total - bs);
-          for (uint32_t k = 0; k < nb; ++k) {
+          for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u; ++cw)
+          for (unsigned long long bits = cols[pidx][cw]; bits;) {
+            const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
+            bits &= bits - 1ull;
             float pz, cov;
             if (!shadow_cover(b, k, pfx, pfy, &pz, &cov)) continue;
             float dzp = fmaxf(0.0f, z0 - pz) * inv_span;
