@@ -191,22 +191,26 @@ __device__ __forceinline__ bool may_cover(float dxp, float dyp, float abx, float
 // it (cols[p][w]: references w*64 .. w*64+63): wave w tests its 64
 // references against every pixel and a ballot transposes the answer.
 template <int TPX>
-__device__ __forceinline__ void batch_columns(const ShadowBatch<TPX>& b, uint32_t nb, uint32_t tx0, uint32_t ty0,
-                                              uint32_t fx, unsigned long long (*cols)[TPX / 64]) {
+__device__ __forceinline__ void batch_columns(bool valid, float x0, float y0, float abx, float aby, float l2,
+                                              float rmax, uint32_t tx0, uint32_t ty0, uint32_t fx,
+                                              unsigned long long (*cols)[TPX / 64]) {
   const uint32_t k = threadIdx.x, w = k >> 6;
-  const bool valid = k < nb;
-  float x0 = 0.0f, y0 = 0.0f, abx = 0.0f, aby = 0.0f, l2 = 0.0f, rm2 = 0.0f;
-  if (valid) {
-    const float4 A = b.a[k], Rr = b.r[k];
-    x0 = A.x; y0 = A.y; l2 = A.w; abx = Rr.z; aby = Rr.w;
-    const float rm = Rr.y + 0.01f;
-    rm2 = rm * rm;
-  }
+  const float rm = rmax + 0.01f, rm2 = rm * rm;
   for (uint32_t p = 0; p < (uint32_t)TPX; ++p) {
     const float px = (float)(tx0 + p % fx) + 0.5f, py = (float)(ty0 + p / fx) + 0.5f;
     const unsigned long long m = __ballot(valid && may_cover(px - x0, py - y0, abx, aby, l2, rm2));
     if ((k & 63u) == 0u) cols[p][w] = m;
   }
+}
+
+template <int TPX>
+__device__ __forceinline__ void shadow_columns(const ShadowBatch<TPX>& b, uint32_t nb, uint32_t tx0, uint32_t ty0,
+                                               uint32_t fx, unsigned long long (*cols)[TPX / 64]) {
+  const uint32_t k = threadIdx.x;
+  const bool valid = k < nb;
+  float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), Rr = A;
+  if (valid) { A = b.a[k]; Rr = b.r[k]; }
+  batch_columns<TPX>(valid, A.x, A.y, Rr.z, Rr.w, A.w, Rr.y, tx0, ty0, fx, cols);
 }
 
 // K25 (strand_shadow_raster.wgsl): per pixel of a light tile, pass 1 finds
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
       if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
       __syncthreads();
       const uint32_t nb = min((uint32_t)TPX, total - bs);
-      batch_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+      shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
       __syncthreads();
       for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !found; ++cw) {
         for (unsigned long long bits = cols[pidx][cw]; bits;) {
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
         if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
         __syncthreads();
         const uint32_t nb = min((uint32_t)TPX, total - bs);
-        batch_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+        shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
         __syncthreads();
         if (found) {
           for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u; ++cw)
@@ -626,6 +630,7 @@ template <int TPX>
 __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
   __shared__ RunTable<TPX> t;
   __shared__ ColorBatch<TPX> b;
+  __shared__ unsigned long long cols[TPX][TPX / 64];
   const uint32_t R = fp.counters[C_CAM_RUNS];
   while (next_run<TPX>(&fp.counters[C_CAMERA_HEAD], R, t)) {
     const sr_raster_tile_run run = fp.cam_runs[t.run];
@@ -648,9 +653,22 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
       __syncthreads();
       if (bs + threadIdx.x < total) load_color_ref(fp, fr, t, bs + threadIdx.x, b);
       __syncthreads();
+      {  // references a pixel's column leaves out do not cover it
+        const uint32_t nb = min((uint32_t)TPX, total - bs), k = threadIdx.x;
+        const bool valid = k < nb && !(b.item[k] & 0x80000000u);
+        float x0 = 0.0f, y0 = 0.0f, ax = 0.0f, ay = 0.0f, rmx = 0.0f;
+        if (valid) {
+          x0 = b.p0x[k]; y0 = b.p0y[k]; ax = b.abx[k]; ay = b.aby[k];
+          rmx = fmaxf(b.rmax[k], fmaxf(b.rmin[k], 1e-4f));
+        }
+        batch_columns<TPX>(valid, x0, y0, ax, ay, ax * ax + ay * ay, rmx, tx * fx, ty * fy, fx, cols);
+      }
+      __syncthreads();
       if (!done) {
-        const uint32_t nb = min((uint32_t)TPX, total - bs);
-        for (uint32_t k = 0; k < nb; ++k) {
+        for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !done; ++cw)
+        for (unsigned long long bits = cols[pidx][cw]; bits;) {
+          const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
+          bits &= bits - 1ull;
           const uint32_t item = b.item[k];
           const uint32_t w = item & 0x7FFFFFFFu;
           if (w != cur) {  // end of a work item: early out above 0.98 alpha
