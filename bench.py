@@ -46,19 +46,26 @@ def percentile(samples, f):
 
 def stage_bytes(ctr, w):
     """Algorithmic bytes per stage: every ABI record read or written once.
-    T tasks, C = 256 P cells, H references, W work items, R runs."""
+    T tasks, C = 256 P cells, H references, N sorted hits, W work items,
+    R runs."""
     T, P, H = ctr["binning_tasks"], ctr["allocated_pages"], ctr["fine_refs"]
     C = 256 * P
     W = ctr["work_items"]
     R = ctr["camera_runs"] + ctr["shadow_runs"]
     tiles16 = 16 * ctr["coarse_tiles"]
-    cand = ctr.get("page_candidates", 0)
+    N = ctr.get("page_candidates", 0)
+    ptab = 16 * ctr["coarse_tiles"]  # coarse tiles x depth slices
+    page_cap = min(ptab, max(4096, ptab // 2))  # the runtime's first-frame page capacity
+    passes = (min(max(page_cap.bit_length(), 1), 24) + 8 + 7) // 8
     return {
         "project_mark": 28 * T,
         "count": 28 * T + 4 * C,
-        # page CSR: candidate lists read (sorted in place), cells written
-        "csr_count": 8 * cand + 28 * cand + 8 * C,
-        "csr_fill": 4 * cand + 28 * cand + 4 * C + 12 * H,
+        # page CSR: (key, slot) pairs; each pass reads keys for the digit
+        # histogram and key + slot for the scatter, writes key + slot (the
+        # last one key + the 12-byte reference and reads the slot's material)
+        "sort_hits": passes * 12 * N + (passes - 1) * 8 * N + 20 * N,
+        # run boundaries of the sorted keys, then counts, offsets, page totals
+        "cell_runs": 4 * C + 4 * N + 8 * C + 8 * C + 8 * P,
         "prefix": 8 * C + 32 * P,
         "fill": 28 * T + 4 * C + 12 * H,
         "sort_cells": 24 * H,
@@ -303,7 +310,7 @@ def main():
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
     roof = roofline_for(dom, stage_acc[dom], ctr, w)
-    bstages = ("csr_count", "csr_fill") if args.backend == "csr" else ("count", "prefix", "fill")
+    bstages = ("sort_hits", "cell_runs") if args.backend == "csr" else ("count", "prefix", "fill")
     binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w) for k in bstages}
     if roof is None:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,

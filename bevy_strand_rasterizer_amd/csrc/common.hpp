@@ -86,10 +86,10 @@ enum Counter : uint32_t {
   C_PAGE_DROPS = 10,  // marked pages dropped by the page capacity
   C_CAMERA_HEAD = 11, // camera raster run queue head
   C_SHADOW_HEAD = 12, // shadow raster run queue head
-  C_CANDS = 13,       // page candidates (CSR backend)
-  C_CAND_OVERFLOW = 14,  // unused
-  C_CSR_COUNT_HEAD = 15,  // page queue heads of the CSR count and fill kernels
-  C_CSR_FILL_HEAD = 16,
+  C_CANDS = 13,       // fine-cell hits of the page CSR backend
+  C_UNUSED_14 = 14,
+  C_UNUSED_15 = 15,
+  C_UNUSED_16 = 16,
   C_COUNT = 20
 };
 
@@ -102,7 +102,7 @@ struct FrameParams {
   uint32_t coarse_total;  // coarse tiles over all frusta
   uint32_t task_cap, ref_cap, work_cap, atlas_texels;
   uint32_t dom_layers, page_cap;
-  uint32_t cand_cap, pad0;  // CSR: entry capacity
+  uint32_t pad0, pad00;
   sr_frame_settings st;
 
   uint32_t* counters;  // Counter words
@@ -136,23 +136,16 @@ struct FrameParams {
   sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
   sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
   // page CSR backend
-  // (page, slot) entries: each slot writes one per page it touches, in slot
-  // order at scanned offsets; a stable radix sort by page then leaves every
-  // page's candidate slots contiguous and ascending
-  uint32_t* slot_entries;            // [F * slots] entries of each slot (0 when invalid)
-  uint32_t* entry_base;              // [F * slots] exclusive scan of slot_entries
-  uint32_t* ekeys_w;                 // [cand_cap] page of each entry as written (page_cap: dropped page)
-  uint32_t* cands_w;                 // [cand_cap] entry index as written (the identity)
-  const uint32_t* ekeys;             // [cand_cap] pages, sorted
-  const uint32_t* cands;             // [cand_cap] entry indices in page order
-  // per entry (indexed by entry index): its slot, material, and its fine
-  // cells in the page: cells 0-3 in epay0, cells 4-6 and the count (255:
-  // more than 7, look them up by slot) in epay1
-  uint32_t* eslot;
-  uint32_t* emat;
-  uint32_t* epay0;
-  uint32_t* epay1;
-  uint2* cand_range;                 // [page_cap] [begin, end) of each page's candidates
+  // page CSR backend: every (page, cell, slot) hit is written in slot order
+  // at scanned offsets, then one stable radix sort by (page, cell) puts the
+  // references in canonical order; cell runs of the sorted keys give the
+  // cell counts and offsets
+  uint32_t* slot_hit_total;          // [F * slots] visible fine cells of each slot (0 when invalid)
+  uint32_t* hit_base;                // [F * slots] exclusive scan of slot_hit_total
+  uint32_t* slot_material;           // [F * slots] material index of each valid slot's segment
+  uint32_t* hkeys_w;                 // [ref_cap] page * 256 + cell of each hit as written
+  uint32_t* hvals_w;                 // [ref_cap] slot of each hit as written
+  const uint32_t* hkeys;             // [ref_cap] keys after the sort
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // fine cells each slot's segment hits, recorded while marking (CSR backend):
   // slot_hits[s * hit_k + j] = page-table entry << 8 | cell, in traversal

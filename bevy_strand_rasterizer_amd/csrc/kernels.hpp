@@ -30,7 +30,7 @@ struct StageTimer {
   }
 };
 
-// Items per block of the page-CSR entry radix sort (256 threads x 16).
+// Items per block of the page-CSR hit radix sort (256 threads x 16).
 constexpr uint32_t kRadixBlock = 256u * 16u;
 
 // Scratch owned by the runtime for one frame's prepass.
@@ -39,10 +39,11 @@ struct PrepassScratch {
   uint32_t* partials;               // scan tile partials
   uint4* partials4;                 // scan tile partials (4-wide)
   uint32_t* page_xy_mask;           // [coarse_total * 16]
-  // page-CSR entry sort: the two (key, value) buffer pairs (the sorted result
-  // lands in pair 0, FrameParams::ekeys / cands), digit histograms, passes
-  uint32_t* ekeys[2];
-  uint32_t* cands[2];
+  // page-CSR hit sort: the two (key, slot) buffer pairs (pair 0 holds the
+  // hits as written and, after the last pass, the sorted keys), digit
+  // histograms, block and pass counts
+  uint32_t* hkeys[2];
+  uint32_t* hvals[2];
   uint32_t* radix_hist;             // [256 * radix_blocks]
   uint32_t radix_blocks, radix_passes;
 };

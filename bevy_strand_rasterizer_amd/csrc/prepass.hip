@@ -370,16 +370,6 @@ __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, fl
   }
 }
 
-// Whether the fine walk of a clipped segment reaches a cell opaque depth
-// does not hide (always, without opaque depth).
-__device__ __forceinline__ bool any_visible_cell(const Clipped& dc, V2 origin, float zmin, float zmax,
-                                                 const sr_frustum_desc& fd, const Occluder& occ) {
-  if (!occ.tiles) return true;
-  bool any = false;
-  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occ, [&](uint32_t) { any = true; });
-  return any;
-}
-
 // Coarse-tile box of a frustum's coarse tile (strand_prepass.wgsl:1422-1433).
 __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint32_t local, V2* origin, V3* bmax) {
   uint32_t cx = local % fd.coarse_tiles_x, cy = local / fd.coarse_tiles_x;
@@ -392,40 +382,28 @@ __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint3
 
 constexpr uint32_t kHitOverflow = 0xFFu;
 
-// A (page, slot) entry's cells: up to 7 packed as bytes with the count in
-// the top byte of the second word (255: too many, the cells are found by
-// slot instead).
-struct CellPack {
-  uint32_t p0 = 0u, p1 = 0u, n = 0u;
-  __device__ __forceinline__ void add(uint32_t cell) {
-    if (n < 4u) p0 |= cell << (8u * n);
-    else if (n < 7u) p1 |= cell << (8u * (n - 4u));
-    ++n;
-  }
-  __device__ __forceinline__ uint32_t word1() const { return (p1 & 0xFFFFFFu) | ((n > 7u ? 255u : n) << 24); }
-};
-
-__device__ __forceinline__ void write_entry(const FrameParams& fp, uint64_t idx, uint32_t page_entry, uint32_t slot,
-                                            uint32_t material, const CellPack& c) {
-  if (idx >= fp.cand_cap) return;
-  fp.ekeys_w[idx] = (page_entry == 0u || page_entry == kMarked) ? fp.page_cap : page_entry - 1u;
-  fp.cands_w[idx] = (uint32_t)idx;
-  fp.eslot[idx] = slot;
-  fp.emat[idx] = material;
-  fp.epay0[idx] = c.p0;
-  fp.epay1[idx] = c.word1();
-}
-
-// Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst,
-// n counts every visible cell even past cap) and the pages with a visible
-// cell (entries).  SCATTER: the slot's next entry is base + n.
+// Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst;
+// n counts every visible cell, also past cap).  SCATTER: the slot's next hit
+// goes to base + n.
 struct HitRec {
   uint32_t* dst;  // null: not recording
   uint32_t n;
   uint32_t cap;
-  uint32_t entries;
-  uint32_t base, slot, material;
+  uint32_t base, slot;
 };
+
+// The sort key of a hit: page * 256 + cell; a dropped page's hits carry the
+// page id page_cap and sort after every kept one.
+__device__ __forceinline__ uint32_t hit_key(const FrameParams& fp, uint32_t page_entry, uint32_t cell) {
+  const uint32_t page = (page_entry == 0u || page_entry == kMarked) ? fp.page_cap : page_entry - 1u;
+  return (page << 8) | cell;
+}
+
+__device__ __forceinline__ void write_hit(const FrameParams& fp, uint64_t idx, uint32_t key, uint32_t slot) {
+  if (idx >= fp.ref_cap) return;
+  fp.hkeys_w[idx] = key;
+  fp.hvals_w[idx] = slot;
+}
 
 // trace_segment_into_coarse_count_page (strand_prepass.wgsl:1412-1575).
 // MARK also counts page candidates for the CSR backend and records the fine
@@ -452,28 +430,21 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
     uint32_t pti = tile * kSlices + cz;
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
-      if (fp.st.fine_binning_backend != 0u) {
-        // a (segment, page) candidate has at least one cell opaque depth
-        // does not hide
-        const uint32_t before = rec.n;
-        if (rec.dst) {
-          fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
-            if (rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
-            ++rec.n;
-          });
-          if (rec.n > before) ++rec.entries;
-        } else if (any_visible_cell(dc, origin, zmin, zmax, fd, occluder_for(fp, fd, local))) {
-          ++rec.entries;
-        }
-      }
+      if (fp.st.fine_binning_backend != 0u)  // the visible cells: recorded (up to cap) and counted
+        fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
+          if (rec.dst && rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
+          ++rec.n;
+        });
       continue;
     }
     uint32_t h = fp.page_table[pti];
-    if (MODE == MODE_SCATTER) {  // the slot's next (page, slot) entry; dropped pages sort last
-      CellPack cp;
-      fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) { cp.add(cell); });
-      if (cp.n == 0u) continue;
-      write_entry(fp, (uint64_t)rec.base + rec.n++, h, rec.slot, rec.material, cp);
+    if (MODE == MODE_SCATTER) {  // the slot's hits in walk order
+      const uint32_t before = rec.n;
+      fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
+        write_hit(fp, (uint64_t)rec.base + rec.n++, hit_key(fp, h, cell), rec.slot);
+      });
+      if (fp.tel && rec.n > before && h != 0u && h != kMarked)  // (segment, page) candidates
+        atomicAdd(&fp.tel[kTelPageOff + h - 1u], 1u);
       continue;
     }
     if (h == 0u || h == kMarked) continue;
@@ -574,7 +545,7 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     const InstRec& in = fp.inst[ii];
     if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
   }
-  HitRec none{nullptr, 0u, 0u, 0u, 0u, 0u, 0u};
+  HitRec none{nullptr, 0u, 0u, 0u, 0u};
   visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t, none);
 }
 
@@ -583,23 +554,6 @@ __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
   const uint32_t n = min(fp.counters[C_TASKS], fp.task_cap);
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) visit_task<MODE>(fp, t);
 }
-
-// The distinct page-table entries of a slot's recorded cells, in order (the
-// cells of one entry are consecutive).
-struct EntryCursor {
-  const uint32_t* h;
-  uint32_t j, n, prev;
-  __device__ uint32_t next() {
-    while (j < n) {
-      const uint32_t v = h[j++] >> 8;
-      if (v != prev) {
-        prev = v;
-        return v;
-      }
-    }
-    return kInvalid;
-  }
-};
 
 // Pass 1: per slot validity ballot + packed endpoints in slot layout; valid
 // slots also mark their pages (the LUT is built before projection) and, for
@@ -613,19 +567,19 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
     bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
-    uint32_t entries = 0;
+    uint32_t hits = 0;
     if (ok) {
       slot_proj[s] = ps;
       // K10: mark the (coarse tile, LUT slice) pages the segment touches,
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u, 0u, 0u};
+      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u};
       visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
       if (record) fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
-      entries = rec.entries;
+      hits = rec.n;
     }
-    if (csr) fp.slot_entries[s] = entries;  // K16: candidates without a device atomic
+    if (csr) fp.slot_hit_total[s] = hits;  // offsets by a scan: no device atomic
     unsigned long long m = __ballot(ok);
     if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
     const FrustumRec& fr = fp.fr[fi];
@@ -681,29 +635,27 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       const sr_projected_segment ps = slot_proj[s];
       fp.proj[t] = ps;
       if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
-      if (csr) {  // K17: the slot's (page, slot) entries at its scanned offset
+      if (csr) {  // K17: the slot's hits at its scanned offset
         const uint32_t nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
-        const uint32_t base = fp.entry_base[s];
+        const uint32_t base = fp.hit_base[s];
         const uint32_t strand = in.pos_strand[p];
-        const uint32_t material = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
-        if (nh != kHitOverflow) {  // the recorded cells, grouped by page (consecutive)
+        fp.slot_material[s] = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
+        if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
           const uint32_t* h = fp.slot_hits + s * fp.hit_k;
-          uint32_t j = 0, prev = kInvalid;
-          CellPack cp;
+          uint32_t prev = kInvalid, entry = 0u;
           for (uint32_t k = 0; k < nh; ++k) {
             const uint32_t v = h[k], pti = v >> 8;
             if (pti != prev) {
-              if (prev != kInvalid) write_entry(fp, (uint64_t)base + j++, fp.page_table[prev], (uint32_t)s, material, cp);
               prev = pti;
-              cp = CellPack{};
+              entry = fp.page_table[pti];
+              if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
             }
-            cp.add(v & 255u);
+            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, v & 255u), (uint32_t)s);
           }
-          if (prev != kInvalid) write_entry(fp, (uint64_t)base + j, fp.page_table[prev], (uint32_t)s, material, cp);
         } else {  // too many cells to record: trace again
           V3 p0, p1;
           task_endpoints(fr.d, ps, &p0, &p1);
-          HitRec rec{nullptr, 0u, 0u, 0u, base, (uint32_t)s, material};
+          HitRec rec{nullptr, 0u, 0u, base, (uint32_t)s};
           visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, rec);
         }
       }
@@ -737,9 +689,10 @@ struct PageAllocDst {
   }
 };
 
-__global__ __launch_bounds__(256) void k_zero_cells(FrameParams fp) {
+// Clears one per-cell array over the allocated pages.
+__global__ __launch_bounds__(256) void k_zero_cells(FrameParams fp, uint32_t* cells) {
   const uint32_t n = fp.counters[C_PAGES] * (kPage / 4u);
-  uint4* c = reinterpret_cast<uint4*>(fp.cell_counts);
+  uint4* c = reinterpret_cast<uint4*>(cells);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     c[i] = make_uint4(0u, 0u, 0u, 0u);
 }
@@ -804,23 +757,22 @@ struct PageBaseDst {
 
 // ------------------------------------------------------ page CSR backend
 // K16-K18 (prefix_page_candidates, scatter_page_candidates,
-// build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507): every
-// page owns the list of slots whose segments touch it, ascending (slot order
-// is task order).  Device-scope atomics go to memory on this part and
-// serialise on hot pages, so the lists are built without any: marking counts
-// each slot's (page, slot) entries, a scan gives every slot its offset, the
-// write pass stores the entries in slot order, and a stable LSD radix sort by
-// page leaves each page's slots contiguous and ascending.  One workgroup per
-// page then counts cells in LDS and, after the page-base scan, fills
-// references with stable ranks: a chunk of 256 candidates sets one bit per
-// (cell, lane) in an LDS bit matrix and each hit's rank is the cell cursor
-// plus the popcount of lower lanes, so every cell comes out in task order,
-// i.e. sorted by (inst, seg), with no reference sort.
-struct EntrySrc {
-  const uint32_t* entries;
-  __device__ uint32_t load(uint32_t i) const { return entries[i]; }
+// build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507) build each
+// page's candidate list with atomics and count / fill per page.  Here the
+// same references come out of one stable sort, without a device-scope
+// atomic (on this part those go to memory and serialise on hot pages) and
+// without per-page workgroups: marking counts each slot's visible fine cells,
+// a scan gives every slot its offset, the write pass stores one
+// (page * 256 + cell, slot) hit per visible cell in slot order, and a stable
+// LSD radix sort by key orders the hits by page, then cell, then slot --
+// exactly the canonical reference order; its last pass writes the
+// references themselves.  Cell runs of the sorted keys give cell counts; the
+// page prefix and page-base scan are the segment-scatter backend's.
+struct HitTotalSrc {
+  const uint32_t* totals;
+  __device__ uint32_t load(uint32_t i) const { return totals[i]; }
 };
-struct EntryDst {
+struct HitBaseDst {
   uint32_t* base;
   uint32_t* counters;
   __device__ void store(uint32_t s, uint32_t ex, uint32_t) const { base[s] = ex; }
@@ -828,22 +780,49 @@ struct EntryDst {
 };
 
 // ---- stable LSD radix sort of (key, value) pairs, 8-bit digits
-constexpr uint32_t kRadixItems = kRadixBlock / 256u;
+// A block sorts kRadixBlock items; wave w of its four owns the w-th quarter,
+// read 64 at a time.  Lanes holding the same digit find each other with eight
+// ballots (one per digit bit), so a wave ranks its items without LDS atomics
+// or barriers: rank = the wave's running count of the digit + the peers in
+// lower lanes.
+constexpr uint32_t kRadixRounds = kRadixBlock / 256u;  // rounds of 64 per wave
+constexpr uint32_t kRadixChunk = kRadixRounds * 64u;   // items per wave
 
 __device__ __forceinline__ uint32_t radix_count(const uint32_t* n_dev, uint32_t cap) { return min(*n_dev, cap); }
 
-// Per block of kRadixBlock items: digit histogram, stored digit-major
-// (hist[d * nblocks + b]) so one exclusive scan yields every (digit, block)
-// output base.
+// Lanes of the wave whose (valid) digit equals this lane's.
+__device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) {
+  unsigned long long m = __ballot(has);
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    const bool set = (d >> bit) & 1u;
+    const unsigned long long bb = __ballot(set);
+    m &= set ? bb : ~bb;
+  }
+  return m;
+}
+
+// Per block: digit histogram, stored digit-major (hist[d * nblocks + b]) so
+// one exclusive scan yields every (digit, block) output base.  The lowest
+// lane of each peer group adds the group's size.
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
                                                     uint32_t shift, uint32_t* hist, uint32_t nblocks) {
   __shared__ uint32_t cnt[256];
-  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x, base = b * kRadixBlock;
+  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const unsigned long long lower = (1ull << lane) - 1ull;
   cnt[threadIdx.x] = 0u;
   __syncthreads();
-  for (uint32_t k = 0; k < kRadixItems; ++k) {
-    const uint32_t i = base + k * 256u + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  const uint32_t chunk = b * kRadixBlock + w * kRadixChunk;
+  if (chunk < n) {
+#pragma unroll 4
+    for (uint32_t k = 0; k < kRadixRounds; ++k) {
+      const uint32_t i = chunk + k * 64u + lane;
+      const bool has = i < n;
+      const uint32_t d = has ? (keys[i] >> shift) & 255u : 0u;
+      const unsigned long long m = digit_peers(d, has);
+      if (has && (m & lower) == 0ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+    }
   }
   __syncthreads();
   hist[threadIdx.x * nblocks + b] = cnt[threadIdx.x];
@@ -859,233 +838,121 @@ struct HistDst {
   __device__ void total(uint32_t) const {}
 };
 
-// Scatter: items of a block in index order, 256 at a time; each item's rank
-// among equal digits is the digit cursor plus the popcount of lower lanes in
-// the LDS bit matrix (stable).
-__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals, uint32_t* okeys,
-                                                       uint32_t* ovals, const uint32_t* n_dev, uint32_t cap,
-                                                       uint32_t shift, const uint32_t* hist, uint32_t nblocks) {
-  __shared__ uint32_t rows[256 * 8];
-  __shared__ uint32_t cursor[256];
+// Scatter: ranks by ballots, then the block's items are reordered by digit in
+// LDS and written out in that order, so consecutive lanes write consecutive
+// addresses of one digit's output range.  REFS: the last pass writes the
+// reference of each hit's slot (make_fine_seg_ref,
+// strand_prepass.wgsl:1313-1327) in place of the value.
+template <bool REFS>
+__global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uint32_t* keys, const uint32_t* vals,
+                                                       uint32_t* okeys, uint32_t* ovals, const uint32_t* n_dev,
+                                                       uint32_t cap, uint32_t shift, const uint32_t* hist,
+                                                       uint32_t nblocks) {
+  __shared__ uint32_t lkeys[kRadixBlock];
+  __shared__ uint32_t lvals[kRadixBlock];
+  __shared__ uint32_t wcnt[4][256];  // per wave: running digit count, then the wave's start of each digit
+  __shared__ uint32_t dstart[256];   // block-local start of each digit
+  __shared__ uint32_t gbase[256];    // output start of each digit for this block
+  __shared__ uint32_t lds4[4];
   const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x, base = b * kRadixBlock;
-  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
   if (base >= n) return;
-  cursor[lane] = hist[lane * nblocks + b];
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const uint32_t nvalid = min(kRadixBlock, n - base);
+  const unsigned long long lower = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
+  for (int q = 0; q < 4; ++q) wcnt[q][t] = 0u;
+  gbase[t] = hist[t * nblocks + b];
   __syncthreads();
-  for (uint32_t k = 0; k < kRadixItems; ++k) {
-    const uint32_t i = base + k * 256u + lane;
-    const bool has = i < n;
-    const uint32_t key = has ? keys[i] : 0u, d = (key >> shift) & 255u;
-    if (has) atomicOr(&rows[d * 8 + word], bit);
-    __syncthreads();
+  const uint32_t chunk = base + w * kRadixChunk;
+  uint32_t key[kRadixRounds], val[kRadixRounds], rank[kRadixRounds];
+#pragma unroll
+  for (uint32_t k = 0; k < kRadixRounds; ++k) {
+    const uint32_t i = chunk + k * 64u + lane;
+    key[k] = i < n ? keys[i] : 0u;
+    val[k] = i < n ? vals[i] : 0u;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kRadixRounds; ++k) {
+    const bool has = chunk + k * 64u + lane < n;
+    const uint32_t d = (key[k] >> shift) & 255u;
+    const unsigned long long m = digit_peers(d, has);
+    uint32_t r = 0u;
     if (has) {
-      const uint32_t* row = rows + d * 8;
-      uint32_t rank = cursor[d];
-      for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
-      rank += __popc(row[word] & (bit - 1u));
-      okeys[rank] = key;
-      ovals[rank] = vals[i];
+      // every lane reads the count before the group's lowest lane updates it
+      const uint32_t before = wcnt[w][d];
+      r = before + (uint32_t)__popcll(m & lower);
+      if ((m & lower) == 0ull) wcnt[w][d] = before + (uint32_t)__popcll(m);
     }
-    __syncthreads();
-    uint32_t add = 0;
+    rank[k] = r;
+  }
+  __syncthreads();
+  const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
+  uint32_t total;
+  const uint32_t ex = block_exclusive<uint32_t>(c0 + c1 + c2 + c3, &total, lds4);
+  wcnt[0][t] = ex;
+  wcnt[1][t] = ex + c0;
+  wcnt[2][t] = ex + c0 + c1;
+  wcnt[3][t] = ex + c0 + c1 + c2;
+  dstart[t] = ex;
+  __syncthreads();
 #pragma unroll
-    for (int w = 0; w < 8; ++w) {
-      add += __popc(rows[lane * 8 + w]);
-      rows[lane * 8 + w] = 0u;
+  for (uint32_t k = 0; k < kRadixRounds; ++k) {
+    if (chunk + k * 64u + lane < n) {
+      const uint32_t pos = wcnt[w][(key[k] >> shift) & 255u] + rank[k];
+      lkeys[pos] = key[k];
+      lvals[pos] = val[k];
     }
-    cursor[lane] += add;
-    __syncthreads();
+  }
+  __syncthreads();
+  for (uint32_t pos = t; pos < nvalid; pos += 256u) {
+    const uint32_t k = lkeys[pos], d = (k >> shift) & 255u;
+    const uint32_t g = gbase[d] + (pos - dstart[d]);
+    const uint32_t v = lvals[pos];
+    okeys[g] = k;
+    if (REFS) {
+      const uint32_t slot = v % fp.slots, ii = find_instance(fp, slot);
+      fp.refs[g] = sr_fine_seg_ref{ii, slot - fp.inst[ii].pos_base, fp.slot_material[v]};
+    } else {
+      ovals[g] = v;
+    }
   }
 }
 
-// Each page's [begin, end) in the sorted entries (pages without entries keep
-// the cleared [0, 0)).
-__global__ __launch_bounds__(256) void k_cand_ranges(FrameParams fp) {
-  const uint32_t n = min(fp.counters[C_CANDS], fp.cand_cap);
+// Runs of equal keys among the sorted hits: a cell's run starts at
+// cell_offsets[cell] and ends before cell_cursors[cell] (cleared to 0: a
+// cell without hits keeps 0); dropped pages' hits sort last and are skipped.
+__global__ __launch_bounds__(256) void k_hit_runs(FrameParams fp) {
+  const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t k = fp.ekeys[i];
-    if (k >= fp.page_cap) continue;
-    if (i == 0u || fp.ekeys[i - 1u] != k) fp.cand_range[k].x = i;
-    if (i + 1u == n || fp.ekeys[i + 1u] != k) fp.cand_range[k].y = i + 1u;
+    const uint32_t k = fp.hkeys[i];
+    if ((k >> 8) >= fp.page_cap) continue;
+    if (i == 0u || fp.hkeys[i - 1u] != k) fp.cell_offsets[k] = i;
+    if (i + 1u == n || fp.hkeys[i + 1u] != k) fp.cell_cursors[k] = i + 1u;
   }
 }
 
-// trace_candidate_inside_page (strand_prepass.wgsl:2240-2337): the cells of
-// page `pm` that candidate slot `slot` hits.  Recorded cells are filtered by
-// the page's table entry; slots whose recording overflowed are traced again
-// from their packed endpoints.
-template <class Emit>
-__device__ __forceinline__ void cells_in_page(const FrameParams& fp, uint32_t slot, const sr_fine_page_meta& pm,
-                                              uint32_t fi, Emit&& emit) {
-  const uint32_t pti = pm.coarse_tile_id * kSlices + pm.coarse_z;
-  const uint32_t nh = fp.hit_k ? fp.slot_hit_count[slot] : kHitOverflow;
-  if (nh != kHitOverflow) {
-    const uint32_t* h = fp.slot_hits + (size_t)slot * fp.hit_k;
-    for (uint32_t j = 0; j < nh; ++j) {
-      const uint32_t v = h[j];
-      if ((v >> 8) == pti) emit(v & 255u);
-    }
-    return;
-  }
-  if (slot / fp.slots != fi) return;
-  const sr_frustum_desc& fd = fp.fr[fi].d;
-  V3 p0, p1;
-  task_endpoints(fd, fp.slot_proj[slot], &p0, &p1);
-  const sr_coarse_depth_lut_entry e = fp.lut[pti];
-  if (e.virtual_count_q == 0u) return;
-  float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
-  V2 origin;
-  V3 bmax;
-  const uint32_t local = pm.coarse_tile_id - fd.coarse_depth_tile_base;
-  coarse_tile_box(fd, local, &origin, &bmax);
-  Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
-  if (!c.ok) return;
-  Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
-  if (!dc.ok) return;
-  fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), emit);
-}
-
-// The reference a candidate slot contributes (make_fine_seg_ref,
-// strand_prepass.wgsl:1313-1327): instance, index position, material.
-__device__ __forceinline__ sr_fine_seg_ref slot_ref(const FrameParams& fp, uint32_t slot, uint32_t material) {
-  const uint32_t pos = slot % fp.slots;
-  const uint32_t ii = find_instance(fp, pos);
-  return sr_fine_seg_ref{ii, pos - fp.inst[ii].pos_base, material};
-}
-
-// The cells of entry e in page pm: from the entry's packed cells, or by its
-// slot when it has more than 7.
-template <class Emit>
-__device__ __forceinline__ void entry_cells(const FrameParams& fp, uint32_t e, const sr_fine_page_meta& pm, uint32_t fi,
-                                            Emit&& emit) {
-  const uint32_t w1 = fp.epay1[e], n = w1 >> 24;
-  if (n == 255u) {
-    cells_in_page(fp, fp.eslot[e], pm, fi, emit);
-    return;
-  }
-  const uint32_t w0 = fp.epay0[e];
-  for (uint32_t k = 0; k < n; ++k) emit(((k < 4u) ? (w0 >> (8u * k)) : (w1 >> (8u * (k - 4u)))) & 255u);
-}
-
-// Next page of a dynamic page queue (heavy pages do not pile up on one
-// workgroup as with a static stride).
-__device__ __forceinline__ uint32_t next_page(uint32_t* head, uint32_t* slot) {
-  __syncthreads();
-  if (threadIdx.x == 0) *slot = atomicAdd(head, 1u);
-  __syncthreads();
-  return *slot;
-}
-
-__global__ __launch_bounds__(256) void k_csr_count(FrameParams fp) {
-  __shared__ uint32_t scnt[kPage];
+// K14 for the sorted hits: per page, cell counts from the runs, the
+// page-local exclusive scan, page total and the mask of fine tiles with work.
+__global__ __launch_bounds__(256) void k_csr_prefix(FrameParams fp) {
   __shared__ uint32_t lds4[4];
   __shared__ uint32_t colmask;
-  __shared__ uint32_t page_slot;
   const uint32_t P = fp.counters[C_PAGES];
-  for (uint32_t page; (page = next_page(&fp.counters[C_CSR_COUNT_HEAD], &page_slot)) < P;) {
-    const uint2 range = fp.cand_range[page];
-    const sr_fine_page_meta pm = fp.page_meta[page];
-    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    scnt[threadIdx.x] = 0u;
-    if (threadIdx.x == 0) colmask = 0u;
-    __syncthreads();
-    if (fi != kInvalid)
-      for (uint32_t i = range.x + threadIdx.x; i < range.y; i += blockDim.x)
-        entry_cells(fp, fp.cands[i], pm, fi, [&](uint32_t cell) { atomicAdd(&scnt[cell], 1u); });
-    __syncthreads();
-    const uint32_t c = threadIdx.x;
-    const uint32_t cnt = scnt[c];
+  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
+    const uint32_t c = threadIdx.x, g = page * kPage + c;
+    const uint32_t hi = fp.cell_cursors[g];
+    const uint32_t cnt = hi ? hi - fp.cell_offsets[g] : 0u;
+    if (c == 0) colmask = 0u;
     uint32_t total;
-    uint32_t ex = block_exclusive<uint32_t>(cnt, &total, lds4);
-    fp.cell_counts[page * kPage + c] = cnt;
-    fp.cell_offsets[page * kPage + c] = ex;
+    const uint32_t ex = block_exclusive<uint32_t>(cnt, &total, lds4);
+    fp.cell_counts[g] = cnt;
+    fp.cell_offsets[g] = ex;
     if (cnt) atomicOr(&colmask, 1u << (c & 15u));
     __syncthreads();
     if (c == 0) {
       fp.page_totals[page] = total;
       fp.page_xy_mask[page] = colmask;
     }
-  }
-}
-
-// References of a page are staged in LDS (structure of arrays) when the page
-// holds at most kFillStage of them and leave as one coalesced copy; larger
-// pages write each reference in place.
-constexpr uint32_t kFillStage = 1024;
-
-__global__ __launch_bounds__(256) void k_csr_fill(FrameParams fp) {
-  __shared__ uint32_t rows[kPage * 8];  // bit (cell, lane) of the current chunk
-  __shared__ uint32_t cursor[kPage];
-  __shared__ uint32_t offs[kPage];
-  __shared__ uint32_t stage[3][kFillStage];
-  __shared__ uint32_t page_slot;
-  const uint32_t P = fp.counters[C_PAGES];
-  const uint32_t lane = threadIdx.x, word = lane >> 5, bit = 1u << (lane & 31u);
-  for (uint32_t page; (page = next_page(&fp.counters[C_CSR_FILL_HEAD], &page_slot)) < P;) {
-    const sr_fine_page_meta pm = fp.page_meta[page];
-    if (pm.seg_ref_count == 0u) continue;  // empty or overflowed (uniform)
-    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    const uint2 range = fp.cand_range[page];
-    const bool staged = pm.seg_ref_count <= kFillStage;
-    cursor[lane] = 0u;
-    offs[lane] = fp.cell_offsets[page * kPage + lane];
-#pragma unroll
-    for (int w = 0; w < 8; ++w) rows[lane * 8 + w] = 0u;
     __syncthreads();
-    for (uint32_t cb = range.x; cb < range.y; cb += 256u) {
-      const uint32_t i = cb + lane;
-      const bool has = i < range.y;
-      const uint32_t e = has ? fp.cands[i] : 0u;
-      // this candidate's cells in the page, up to 8 kept as bytes
-      uint32_t c0 = 0u, c1 = 0u, nc = 0u;
-      if (has) entry_cells(fp, e, pm, fi, [&](uint32_t cell) {
-        atomicOr(&rows[cell * 8 + word], bit);
-        if (nc < 4u) c0 |= cell << (8u * nc);
-        else if (nc < 8u) c1 |= cell << (8u * (nc - 4u));
-        ++nc;
-      });
-      __syncthreads();
-      if (has) {
-        const uint32_t slot = fp.eslot[e];
-        sr_fine_seg_ref ref = slot_ref(fp, slot, fp.emat[e]);
-        auto put = [&](uint32_t cell) {
-          const uint32_t* row = rows + cell * 8;
-          uint32_t rank = cursor[cell];
-          for (uint32_t w = 0; w < word; ++w) rank += __popc(row[w]);
-          rank += __popc(row[word] & (bit - 1u));
-          const uint32_t local = offs[cell] + rank;
-          if (staged) {
-            stage[0][local] = ref.inst_id;
-            stage[1][local] = ref.seg_id;
-            stage[2][local] = ref.material_idx;
-          } else {
-            const uint64_t dst = (uint64_t)pm.seg_ref_base + local;
-            if (dst < fp.ref_cap) fp.refs[dst] = ref;
-          }
-        };
-        if (nc <= 8u) {
-          for (uint32_t k = 0; k < nc; ++k) put(((k < 4u ? c0 >> (8u * k) : c1 >> (8u * (k - 4u)))) & 255u);
-        } else {
-          entry_cells(fp, e, pm, fi, put);
-        }
-      }
-      __syncthreads();
-      uint32_t add = 0;
-#pragma unroll
-      for (int w = 0; w < 8; ++w) {
-        add += __popc(rows[lane * 8 + w]);
-        rows[lane * 8 + w] = 0u;
-      }
-      cursor[lane] += add;
-      __syncthreads();
-    }
-    if (staged) {  // the page's references as consecutive 32-bit words
-      uint32_t* dst = reinterpret_cast<uint32_t*>(fp.refs + pm.seg_ref_base);
-      const uint32_t words = 3u * pm.seg_ref_count;
-      for (uint32_t w = lane; w < words; w += 256u) dst[w] = stage[w % 3u][w / 3u];
-    }
   }
 }
 
@@ -1388,8 +1255,7 @@ __global__ __launch_bounds__(256) void k_telemetry(FrameParams fp) {
   for (uint32_t p = gid; p < P; p += stride) {
     const sr_fine_page_meta pm = fp.page_meta[p];
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    uint32_t cand = csr ? fp.cand_range[p].y - fp.cand_range[p].x : fp.tel[kTelPageOff + p];
-    if (csr) fp.tel[kTelPageOff + p] = cand;
+    const uint32_t cand = fp.tel[kTelPageOff + p];
     if (fi != kInvalid) tel_add(fp, fi, TF_ALLOCATED_PAGES, 1u);
     if (cand == 0u) continue;
     atomicAdd(&fp.tel[2], cand);
@@ -1457,38 +1323,45 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap},
                         ptab, nullptr, ps.partials, s);
   if (csr) {
-    timer->mark(s, "entry_offsets");
-    launch_scan<uint32_t>(EntrySrc{fp.slot_entries}, EntryDst{fp.entry_base, fp.counters}, (uint32_t)slots_total,
+    timer->mark(s, "hit_offsets");
+    launch_scan<uint32_t>(HitTotalSrc{fp.slot_hit_total}, HitBaseDst{fp.hit_base, fp.counters}, (uint32_t)slots_total,
                           nullptr, ps.partials, s);
   }
   timer->mark(s, "project_write");
   hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
-    timer->mark(s, "sort_entries");
-    // stable LSD passes; the write target was chosen so the last pass lands
-    // in pair 0
-    int src = (ps.radix_passes & 1u) ? 1 : 0;
+    timer->mark(s, "sort_hits");
+    // stable LSD passes over (key, slot) pairs; the last one writes the
+    // references and leaves the sorted keys in pair 0
+    int src = (ps.radix_passes & 1u) ? 1 : 0;  // the pair the hits were written to
     for (uint32_t k = 0; k < ps.radix_passes; ++k, src ^= 1) {
       const uint32_t shift = 8u * k;
-      hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.ekeys[src], fp.counters + C_CANDS,
-                         fp.cand_cap, shift, ps.radix_hist, ps.radix_blocks);
+      const bool last = k + 1u == ps.radix_passes;
+      const int dst = last ? 0 : (src ^ 1);
+      hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters + C_CANDS,
+                         fp.ref_cap, shift, ps.radix_hist, ps.radix_blocks);
       launch_scan<uint32_t>(HistSrc{ps.radix_hist}, HistDst{ps.radix_hist}, 256u * ps.radix_blocks, nullptr,
                             ps.partials, s);
-      hipLaunchKernelGGL(k_radix_scatter, dim3(ps.radix_blocks), dim3(256), 0, s, ps.ekeys[src], ps.cands[src],
-                         ps.ekeys[src ^ 1], ps.cands[src ^ 1], fp.counters + C_CANDS, fp.cand_cap, shift,
-                         ps.radix_hist, ps.radix_blocks);
+      if (last)
+        hipLaunchKernelGGL(k_radix_scatter<true>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
+                           ps.hvals[src], ps.hkeys[dst], nullptr, fp.counters + C_CANDS, fp.ref_cap, shift,
+                           ps.radix_hist, ps.radix_blocks);
+      else
+        hipLaunchKernelGGL(k_radix_scatter<false>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
+                           ps.hvals[src], ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift,
+                           ps.radix_hist, ps.radix_blocks);
     }
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.cand_range), 2ull * fp.page_cap, 0u, s);
-    hipLaunchKernelGGL(k_cand_ranges, dim3(grid_for(fp.cand_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
-    timer->mark(s, "csr_count");
-    hipLaunchKernelGGL(k_csr_count, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+    timer->mark(s, "cell_runs");
+    hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,
+                       fp.cell_cursors);
+    hipLaunchKernelGGL(k_hit_runs, dim3(grid_for(fp.ref_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_csr_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "page_bases");
     launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,
                           fp.counters + C_PAGES, ps.partials, s);
-    timer->mark(s, "csr_fill");
-    hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
   } else {
-    hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,
+                       fp.cell_counts);
     timer->mark(s, "count");
     hipLaunchKernelGGL((k_visit<MODE_COUNT>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "prefix");

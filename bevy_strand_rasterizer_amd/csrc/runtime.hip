@@ -135,8 +135,8 @@ enum BufId {
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
-  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_ENTRIES, B_ENTRY_BASE, B_CAND_RANGE, B_EKEYS0, B_EKEYS1, B_CANDS0,
-  B_CANDS1, B_RADIX_HIST, B_ESLOT, B_EMAT, B_EPAY0, B_EPAY1, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
+  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -144,7 +144,7 @@ enum BufId {
 struct FrameShape {
   bool valid = false;
   uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
-  uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, cand_cap = 0, atlas_texels = 0, dom_layers = 0;
+  uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, hit_cap = 0, atlas_texels = 0, dom_layers = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0, opaque_tiles = 0;
   int hist_cur = 0;
 };
@@ -168,7 +168,7 @@ struct FrameSlot {
   StageTimer timer;
   hipEvent_t done = nullptr;
   bool submitted = false;
-  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0, cand_cap_hint = 0;
+  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0;
   int hist_cur = 0;
   // captured launch sequences, keyed by everything the launches bake in
   struct Graph {
@@ -555,7 +555,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   // work items are non-empty cells: never more than the page capacity's cells
   const uint32_t work_cap = (uint32_t)std::max<uint64_t>(
       {std::min<uint64_t>(ref_cap / 2u, (uint64_t)page_cap * kPage), 1u << 15, (uint64_t)sl.work_cap_hint});
-  // page CSR backend: candidate (task, page) pairs, 2 per task slot to start
+  // page CSR backend: one (page * 256 + cell, slot) hit per visible fine
+  // cell, at most ref_cap of them (a hit is a reference of a kept page)
   const bool csr = st.fine_binning_backend != 0u;
   // cells recorded per slot while marking: 24 words per slot (a segment of
   // the benchmark scalps hits 5 cells on average and more than 24 about once
@@ -567,20 +568,18 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       if (4ull * k * task_cap <= (8ull << 30)) { hit_k = k; break; }
   if (hit_k && (ctx->flags & SR_CONFIG_CSR_FEW_HITS)) hit_k = 2u;
   const uint64_t hit_bytes = 4ull * hit_k * task_cap;
-  const uint32_t cand_cap = csr ? (uint32_t)std::min<uint64_t>(
-                                      std::max<uint64_t>({2ull * task_cap, 1ull << 16, (uint64_t)sl.cand_cap_hint}),
-                                      0x7FFFFFFFull)
-                                : 0u;
+  const uint32_t hit_cap = csr ? ref_cap : 0u;
   const uint32_t atlas_w = std::min(std::max(max_idx, 1u), SR_MAX_TEXTURE_EXTENT);
   const uint32_t atlas_h = std::min(std::max((max_idx + atlas_w - 1) / atlas_w, 1u), SR_MAX_TEXTURE_EXTENT);
   const uint32_t atlas_texels = atlas_w * atlas_h;
   const uint64_t atlas_n = (uint64_t)cams * NI * atlas_texels;
   const uint32_t tiles16 = coarse_total * kExt * kExt;
-  // page-CSR entry sort: 8-bit digits over the page key (page_cap = dropped)
+  // page-CSR hit sort: 8-bit digits over page * 256 + cell (page_cap = dropped)
   uint32_t key_bits = 1;
-  while (key_bits < 32 && (1ull << key_bits) <= page_cap) ++key_bits;
+  while (key_bits < 24 && (1ull << key_bits) <= page_cap) ++key_bits;
+  key_bits += 8u;
   const uint32_t radix_passes = csr ? (key_bits + 7u) / 8u : 0u;
-  const uint32_t radix_blocks = csr ? (cand_cap + kRadixBlock - 1u) / kRadixBlock : 0u;
+  const uint32_t radix_blocks = csr ? (hit_cap + kRadixBlock - 1u) / kRadixBlock : 0u;
   const uint64_t scan_len = std::max<uint64_t>({(uint64_t)task_cap, (uint64_t)ptab, (uint64_t)tiles16,
                                                 256ull * radix_blocks, 1ull});
   const uint32_t scan_tiles = (uint32_t)((scan_len + kScanTile - 1) / kScanTile + 1);
@@ -606,10 +605,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
       {B_COMPOSITE, 16ull * img_px}, {B_SLOT_PROJ, 12ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap},
-      {B_SLOT_ENTRIES, csr ? 4ull * task_cap : 0ull}, {B_ENTRY_BASE, csr ? 4ull * task_cap : 0ull},
-      {B_CAND_RANGE, csr ? 8ull * page_cap : 0ull}, {B_EKEYS0, 4ull * cand_cap}, {B_EKEYS1, 4ull * cand_cap},
-      {B_CANDS0, 4ull * cand_cap}, {B_CANDS1, 4ull * cand_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks},
-      {B_ESLOT, 4ull * cand_cap}, {B_EMAT, 4ull * cand_cap}, {B_EPAY0, 4ull * cand_cap}, {B_EPAY1, 4ull * cand_cap},
+      {B_SLOT_HIT_TOTAL, csr ? 4ull * task_cap : 0ull}, {B_HIT_BASE, csr ? 4ull * task_cap : 0ull},
+      {B_SLOT_MATERIAL, csr ? 4ull * task_cap : 0ull}, {B_HKEYS0, 4ull * hit_cap}, {B_HKEYS1, 4ull * hit_cap},
+      {B_HVALS0, 4ull * hit_cap}, {B_HVALS1, 4ull * hit_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks},
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
@@ -715,28 +713,23 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   ps.partials = sl.buf[B_PARTIALS].as<uint32_t>();
   ps.partials4 = sl.buf[B_PARTIALS4].as<uint4>();
   ps.page_xy_mask = sl.buf[B_XY_MASK].as<uint32_t>();
-  ps.ekeys[0] = sl.buf[B_EKEYS0].as<uint32_t>();
-  ps.ekeys[1] = sl.buf[B_EKEYS1].as<uint32_t>();
-  ps.cands[0] = sl.buf[B_CANDS0].as<uint32_t>();
-  ps.cands[1] = sl.buf[B_CANDS1].as<uint32_t>();
+  ps.hkeys[0] = sl.buf[B_HKEYS0].as<uint32_t>();
+  ps.hkeys[1] = sl.buf[B_HKEYS1].as<uint32_t>();
+  ps.hvals[0] = sl.buf[B_HVALS0].as<uint32_t>();
+  ps.hvals[1] = sl.buf[B_HVALS1].as<uint32_t>();
   ps.radix_hist = sl.buf[B_RADIX_HIST].as<uint32_t>();
   ps.radix_blocks = radix_blocks;
   ps.radix_passes = radix_passes;
   fp.page_cap = page_cap;
   fp.page_xy_mask = ps.page_xy_mask;
-  fp.cand_cap = cand_cap;
-  fp.slot_entries = sl.buf[B_SLOT_ENTRIES].as<uint32_t>();
-  fp.entry_base = sl.buf[B_ENTRY_BASE].as<uint32_t>();
-  fp.cand_range = sl.buf[B_CAND_RANGE].as<uint2>();
-  // entries are written to the pair the last sort pass does not write
-  fp.ekeys_w = ps.ekeys[(radix_passes & 1u) ? 1 : 0];
-  fp.cands_w = ps.cands[(radix_passes & 1u) ? 1 : 0];
-  fp.ekeys = ps.ekeys[0];
-  fp.cands = ps.cands[0];
-  fp.eslot = sl.buf[B_ESLOT].as<uint32_t>();
-  fp.emat = sl.buf[B_EMAT].as<uint32_t>();
-  fp.epay0 = sl.buf[B_EPAY0].as<uint32_t>();
-  fp.epay1 = sl.buf[B_EPAY1].as<uint32_t>();
+  fp.slot_hit_total = sl.buf[B_SLOT_HIT_TOTAL].as<uint32_t>();
+  fp.hit_base = sl.buf[B_HIT_BASE].as<uint32_t>();
+  fp.slot_material = sl.buf[B_SLOT_MATERIAL].as<uint32_t>();
+  // hits are written to the pair the last sort pass does not write; the
+  // sorted keys land in pair 0
+  fp.hkeys_w = ps.hkeys[(radix_passes & 1u) ? 1 : 0];
+  fp.hvals_w = ps.hvals[(radix_passes & 1u) ? 1 : 0];
+  fp.hkeys = ps.hkeys[0];
   fp.slot_proj = ps.slot_proj;
   fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
   fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;
@@ -835,7 +828,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.valid = true;
   sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
-  sh.cand_cap = cand_cap;
+  sh.hit_cap = hit_cap;
   sh.tel_words = tel_words;
   sh.opaque_tiles = opaque_floats ? fine_total : 0ull;
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
@@ -863,7 +856,7 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   fc.work_items = c[C_WORK];
   fc.camera_runs = c[C_CAM_RUNS];
   fc.shadow_runs = c[C_SHADOW_RUNS];
-  fc.overflow_pages = c[C_OVERFLOW] + c[C_CAND_OVERFLOW];
+  fc.overflow_pages = c[C_OVERFLOW];
   fc.coarse_tiles = sh.coarse_total;
   fc.task_capacity = sh.task_cap;
   fc.ref_capacity = sh.ref_cap;
@@ -871,11 +864,11 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   if (c[C_REFS] > sh.ref_cap) sl.ref_cap_hint = next_pow2(c[C_REFS]);
   if (c[C_PAGE_DROPS] > 0) sl.page_cap_hint = next_pow2(c[C_MARKED]);
   if (c[C_WORK] > sh.work_cap) sl.work_cap_hint = next_pow2(c[C_WORK]);
-  if (sh.cand_cap && c[C_CANDS] > sh.cand_cap) sl.cand_cap_hint = next_pow2(c[C_CANDS]);
+  if (sh.hit_cap && c[C_CANDS] > sh.hit_cap) sl.ref_cap_hint = std::max(sl.ref_cap_hint, next_pow2(c[C_CANDS]));
   fc.page_drops = c[C_PAGE_DROPS];
-  fc.page_candidates = sh.cand_cap ? c[C_CANDS] : 0u;
+  fc.page_candidates = sh.hit_cap ? c[C_CANDS] : 0u;
   fc.capacity_exceeded = (c[C_REFS] > sh.ref_cap || c[C_PAGE_DROPS] > 0 || c[C_WORK] > sh.work_cap ||
-                          c[C_OVERFLOW] > 0 || c[C_CAND_OVERFLOW] > 0 || (sh.cand_cap && c[C_CANDS] > sh.cand_cap))
+                          c[C_OVERFLOW] > 0 || (sh.hit_cap && c[C_CANDS] > sh.hit_cap))
                              ? 1u
                              : 0u;
   if (out) *out = fc;

@@ -408,7 +408,7 @@ typedef struct sr_frame_counters {
   uint32_t task_capacity;
   uint32_t ref_capacity;
   uint32_t page_drops;          /* marked pages past the page capacity (dropped, as the reference) */
-  uint32_t page_candidates;     /* (task, page) pairs of the page CSR backend; 0 for the cell-atomic backend */
+  uint32_t page_candidates;     /* fine-cell hits sorted by the page CSR backend (references, dropped pages included); 0 for the cell-atomic backend */
   uint32_t capacity_exceeded;   /* 1: a capacity overflowed; the context grew it, submit the frame again */
   uint32_t pad;
 } sr_frame_counters;
