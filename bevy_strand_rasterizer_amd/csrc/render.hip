@@ -19,6 +19,7 @@
 // LDS (pixel x layer), so no dynamic register indexing spills.  Shadow maps
 // are dense per light (no virtual paging), outputs stay FP32.
 #include "common.hpp"
+#include "coverage.hpp"
 #include "kernels.hpp"
 
 namespace srk {
@@ -176,41 +177,51 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
   return c > 0.0f;
 }
 
-// Conservative coverage: false only when shadow_cover is false for that
-// pixel, whatever its rounding -- the pixel lies outside the reference's
-// t-range or farther than max radius (+ 0.01 px) from its line.  Costs a few
-// multiplies instead of the exact test's divisions and square root.
-__device__ __forceinline__ bool may_cover(float dxp, float dyp, float abx, float aby, float l2, float rm2) {
-  if (l2 == 0.0f) return dxp * dxp + dyp * dyp <= rm2;
-  const float dot = dxp * abx + dyp * aby;
-  const float cross = dxp * aby - dyp * abx;
-  return dot >= -1e-6f * l2 && dot <= 1.00001f * l2 && cross * cross <= rm2 * l2 * 1.0001f;
+// 64 x 64 bit-matrix transpose across the wave: lane l holds row l on entry
+// and column l on return (bit j = bit l of lane j's row).  Six butterfly
+// stages swap the off-diagonal blocks of halving size.
+__device__ __forceinline__ unsigned long long wave_transpose64(unsigned long long x) {
+  const uint32_t l = threadIdx.x & 63u;
+  const unsigned long long keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                      0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t sh = 32u >> i;
+    const unsigned long long y = __shfl_xor(x, (int)sh);
+    x = (l & sh) ? ((x & ~keep[i]) | ((y & ~keep[i]) >> sh)) : ((x & keep[i]) | ((y & keep[i]) << sh));
+  }
+  return x;
 }
 
 // Per pixel of the tile, one bit per reference of the batch that may cover
-// it (cols[p][w]: references w*64 .. w*64+63): wave w tests its 64
-// references against every pixel and a ballot transposes the answer.
+// it (cols[p][w]: references w*64 .. w*64+63): each lane builds its
+// reference's conservative pixel mask from row spans, and a wave transpose
+// turns the 64 masks into per-pixel columns.
 template <int TPX>
-__device__ __forceinline__ void batch_columns(bool valid, float x0, float y0, float abx, float aby, float l2,
-                                              float rmax, uint32_t tx0, uint32_t ty0, uint32_t fx,
+__device__ __forceinline__ void batch_columns(bool valid, float x0, float y0, float abx, float aby, float rmax,
+                                              uint32_t tx0, uint32_t ty0, uint32_t fx, uint32_t fy,
                                               unsigned long long (*cols)[TPX / 64]) {
-  const uint32_t k = threadIdx.x, w = k >> 6;
-  const float rm = rmax + 0.01f, rm2 = rm * rm;
-  for (uint32_t p = 0; p < (uint32_t)TPX; ++p) {
-    const float px = (float)(tx0 + p % fx) + 0.5f, py = (float)(ty0 + p / fx) + 0.5f;
-    const unsigned long long m = __ballot(valid && may_cover(px - x0, py - y0, abx, aby, l2, rm2));
-    if ((k & 63u) == 0u) cols[p][w] = m;
+  constexpr int W = TPX / 64;
+  const uint32_t k = threadIdx.x, w = k >> 6, l = k & 63u;
+  uint64_t m[W];
+  if (valid) {
+    span_mask<W>(span_setup(x0, y0, abx, aby, rmax), tx0, ty0, fx, fy, m);
+  } else {
+#pragma unroll
+    for (int j = 0; j < W; ++j) m[j] = 0ull;
   }
+#pragma unroll
+  for (int j = 0; j < W; ++j) cols[j * 64 + l][w] = wave_transpose64(m[j]);
 }
 
 template <int TPX>
 __device__ __forceinline__ void shadow_columns(const ShadowBatch<TPX>& b, uint32_t nb, uint32_t tx0, uint32_t ty0,
-                                               uint32_t fx, unsigned long long (*cols)[TPX / 64]) {
+                                               uint32_t fx, uint32_t fy, unsigned long long (*cols)[TPX / 64]) {
   const uint32_t k = threadIdx.x;
   const bool valid = k < nb;
   float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), Rr = A;
   if (valid) { A = b.a[k]; Rr = b.r[k]; }
-  batch_columns<TPX>(valid, A.x, A.y, Rr.z, Rr.w, A.w, Rr.y, tx0, ty0, fx, cols);
+  batch_columns<TPX>(valid, A.x, A.y, Rr.z, Rr.w, Rr.y, tx0, ty0, fx, fy, cols);
 }
 
 // K25 (strand_shadow_raster.wgsl): per pixel of a light tile, pass 1 finds
@@ -251,7 +262,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
       if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
       __syncthreads();
       const uint32_t nb = min((uint32_t)TPX, total - bs);
-      shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+      shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
       __syncthreads();
       for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !found; ++cw) {
         for (unsigned long long bits = cols[pidx][cw]; bits;) {
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
         if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
         __syncthreads();
         const uint32_t nb = min((uint32_t)TPX, total - bs);
-        shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, cols);
+        shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
         __syncthreads();
         if (found) {
           for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u; ++cw)
@@ -661,7 +672,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           x0 = b.p0x[k]; y0 = b.p0y[k]; ax = b.abx[k]; ay = b.aby[k];
           rmx = fmaxf(b.rmax[k], fmaxf(b.rmin[k], 1e-4f));
         }
-        batch_columns<TPX>(valid, x0, y0, ax, ay, ax * ax + ay * ay, rmx, tx * fx, ty * fy, fx, cols);
+        batch_columns<TPX>(valid, x0, y0, ax, ay, rmx, tx * fx, ty * fy, fx, fy, cols);
       }
       __syncthreads();
       if (!done) {
