@@ -121,6 +121,7 @@ struct FrameParams {
   uint32_t* shading_queue;           // [task_cap]
   // binning
   sr_coarse_depth_lut_entry* lut;    // [coarse_total * 16]
+  uint2* lut_z;                      // [coarse_total * 16] (z_min_q, z_max_q) per slice; unused: (~0u, 0)
   uint32_t* page_table;              // [coarse_total * 16]
   sr_fine_page_meta* page_meta;      // [coarse_total * 16]
   uint32_t* page_totals;             // [coarse_total * 16]
@@ -148,9 +149,10 @@ struct FrameParams {
   const uint32_t* hkeys;             // [ref_cap] keys after the sort
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // fine cells each slot's segment hits, recorded while marking (CSR backend):
-  // slot_hits[s * hit_k + j] = page-table entry << 8 | cell, in traversal
-  // order; slot_hit_count[s] = count, or kHitOverflow past hit_k (those
-  // slots are traced again).  hit_k = 0 disables recording.
+  // slot_hits[j * F * slots + s] = page-table entry << 8 | cell, in traversal
+  // order (slot-minor, so a wave's slots touch consecutive words);
+  // slot_hit_count[s] = count, or kHitOverflow past hit_k (those slots are
+  // traced again).  hit_k = 0 disables recording.
   // telemetry (finalize_telemetry layout): 8 global words, 32 histogram
   // bins, one candidate count per page, 11 words per frustum; null when off
   uint32_t* tel;

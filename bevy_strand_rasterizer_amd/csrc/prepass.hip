@@ -283,21 +283,47 @@ __global__ void k_lut(FrameParams fp) {
   }
   sr_coarse_depth_lut_entry out[kSlices];
   b.finish(out);
-  for (uint32_t z = 0; z < kSlices; ++z) fp.lut[(size_t)t * kSlices + z] = out[z];
+  for (uint32_t z = 0; z < kSlices; ++z) {
+    fp.lut[(size_t)t * kSlices + z] = out[z];
+    fp.lut_z[(size_t)t * kSlices + z] =
+        out[z].virtual_count_q ? make_uint2(out[z].z_min_q, out[z].z_max_q) : make_uint2(~0u, 0u);
+  }
 }
 
 // ------------------------------------------------ binning traversal
-__device__ __forceinline__ uint32_t lut_slice_for_depth(const sr_coarse_depth_lut_entry* lut, float z) {
-  uint32_t zq = quantize_depth01(z);
-  uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
-  for (uint32_t i = 0; i < kSlices; ++i) {
-    const sr_coarse_depth_lut_entry e = lut[i];
-    if (e.virtual_count_q == 0u) continue;
-    if (zq >= e.z_min_q && zq <= e.z_max_q) return i;
-    uint32_t d = zq > e.z_max_q ? zq - e.z_max_q : e.z_min_q - zq;
+// coarse_depth_slice_for_depth: the first used slice whose range holds the
+// quantised depth, else the first nearest one.  Both endpoint depths are
+// looked up in one pass over the tile's 16 (z_min_q, z_max_q) pairs, loaded
+// together (a used slice has z_min_q <= z_max_q; unused ones are (~0, 0)).
+struct SliceSearch {
+  uint32_t zq, hit, best, best_d;
+  __device__ __forceinline__ void see(uint32_t i, uint2 e) {
+    if (hit != kInvalid || e.x > e.y) return;
+    if (zq >= e.x && zq <= e.y) { hit = i; return; }
+    const uint32_t d = zq > e.y ? zq - e.y : e.x - zq;
     if (d < best_d) { best_d = d; best = i; }
   }
-  return best;
+  __device__ __forceinline__ uint32_t result() const { return hit != kInvalid ? hit : best; }
+};
+
+__device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, float za, float zb, uint32_t* sa,
+                                                      uint32_t* sb) {
+  SliceSearch a{quantize_depth01(za), kInvalid, kInvalid, 0xFFFFFFFFu};
+  SliceSearch b{quantize_depth01(zb), kInvalid, kInvalid, 0xFFFFFFFFu};
+  const uint4* l4 = reinterpret_cast<const uint4*>(lz);
+  uint4 q[kSlices / 2];
+#pragma unroll
+  for (uint32_t i = 0; i < kSlices / 2; ++i) q[i] = l4[i];
+#pragma unroll
+  for (uint32_t i = 0; i < kSlices / 2; ++i) {
+    const uint2 e0 = make_uint2(q[i].x, q[i].y), e1 = make_uint2(q[i].z, q[i].w);
+    a.see(2 * i, e0);
+    b.see(2 * i, e0);
+    a.see(2 * i + 1, e1);
+    b.see(2 * i + 1, e1);
+  }
+  *sa = a.result();
+  *sb = b.result();
 }
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
@@ -386,10 +412,11 @@ constexpr uint32_t kHitOverflow = 0xFFu;
 // n counts every visible cell, also past cap).  SCATTER: the slot's next hit
 // goes to base + n.
 struct HitRec {
-  uint32_t* dst;  // null: not recording
+  uint32_t* dst;  // null: not recording; hit j at dst[j * stride]
   uint32_t n;
   uint32_t cap;
   uint32_t base, slot;
+  uint32_t stride;
 };
 
 // The sort key of a hit: page * 256 + cell; a dropped page's hits carry the
@@ -417,14 +444,15 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
   coarse_tile_box(fd, local, &origin, &bmax);
   Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
-  const sr_coarse_depth_lut_entry* lut = fp.lut + (size_t)tile * kSlices;
-  uint32_t z0s = lut_slice_for_depth(lut, c.p0.z), z1s = lut_slice_for_depth(lut, c.p1.z);
+  const uint2* lz = fp.lut_z + (size_t)tile * kSlices;
+  uint32_t z0s, z1s;
+  lut_slices_for_depths(lz, c.p0.z, c.p1.z, &z0s, &z1s);
   if (z0s == kInvalid || z1s == kInvalid) return;
   uint32_t zlo = min(z0s, z1s), zhi = max(z0s, z1s);
   for (uint32_t cz = zlo; cz <= zhi; ++cz) {
-    const sr_coarse_depth_lut_entry e = lut[cz];
-    if (e.virtual_count_q == 0u) continue;
-    float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
+    const uint2 e = lz[cz];
+    if (e.x > e.y) continue;  // unused slice
+    float zmin = dequantize_depth01(e.x), zmax = dequantize_depth01(e.y);
     Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
     if (!dc.ok) continue;
     uint32_t pti = tile * kSlices + cz;
@@ -432,7 +460,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_
       fp.page_table[pti] = kMarked;
       if (fp.st.fine_binning_backend != 0u)  // the visible cells: recorded (up to cap) and counted
         fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
-          if (rec.dst && rec.n < rec.cap) rec.dst[rec.n] = (pti << 8) | cell;
+          if (rec.dst && rec.n < rec.cap) rec.dst[(size_t)rec.n * rec.stride] = (pti << 8) | cell;
           ++rec.n;
         });
       continue;
@@ -545,7 +573,7 @@ __device__ void visit_task(const FrameParams& fp, uint32_t t) {
     const InstRec& in = fp.inst[ii];
     if (tk.chunk_id < in.meta_count) ref.material_idx = in.meta[tk.chunk_id].material_idx;
   }
-  HitRec none{nullptr, 0u, 0u, 0u, 0u};
+  HitRec none{nullptr, 0u, 0u, 0u, 0u, 0u};
   visit_segment<MODE>(fp, fi, ii, p0, p1, ref, t, none);
 }
 
@@ -574,7 +602,7 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      HitRec rec{record ? fp.slot_hits + s * fp.hit_k : nullptr, 0u, fp.hit_k, 0u, 0u};
+      HitRec rec{record ? fp.slot_hits + s : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
       visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
       if (record) fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
       hits = rec.n;
@@ -641,10 +669,10 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
         const uint32_t strand = in.pos_strand[p];
         fp.slot_material[s] = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
         if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
-          const uint32_t* h = fp.slot_hits + s * fp.hit_k;
+          const uint32_t* h = fp.slot_hits + s;
           uint32_t prev = kInvalid, entry = 0u;
           for (uint32_t k = 0; k < nh; ++k) {
-            const uint32_t v = h[k], pti = v >> 8;
+            const uint32_t v = h[(size_t)k * total], pti = v >> 8;
             if (pti != prev) {
               prev = pti;
               entry = fp.page_table[pti];
@@ -655,7 +683,7 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
         } else {  // too many cells to record: trace again
           V3 p0, p1;
           task_endpoints(fr.d, ps, &p0, &p1);
-          HitRec rec{nullptr, 0u, 0u, base, (uint32_t)s};
+          HitRec rec{nullptr, 0u, 0u, base, (uint32_t)s, 0u};
           visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, rec);
         }
       }

@@ -132,7 +132,7 @@ struct DBuf {
 
 enum BufId {
   B_COUNTERS, B_BROAD, B_KEEP, B_RANGES, B_SLOT_MASKS, B_WAVE_OFF, B_FR_TASK_BASE, B_FR_CAM_BASE, B_SHADED,
-  B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
+  B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_LUT_Z, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
@@ -595,7 +595,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_RANGES, 32ull * NI * F + 32}, {B_SLOT_MASKS, 8ull * task_cap / 64}, {B_WAVE_OFF, 4ull * task_cap / 64},
       {B_FR_TASK_BASE, 4ull * (F + 1)}, {B_FR_CAM_BASE, 4ull * F}, {B_SHADED, (uint64_t)std::max(cams, 1u) * slots},
       {B_TASKS, 16ull * task_cap}, {B_PROJ, 12ull * task_cap}, {B_SHADING_Q, 4ull * task_cap},
-      {B_LUT, 16ull * ptab}, {B_PAGE_TABLE, 4ull * ptab}, {B_PAGE_META, 32ull * page_cap},
+      {B_LUT, 16ull * ptab}, {B_LUT_Z, 8ull * ptab}, {B_PAGE_TABLE, 4ull * ptab}, {B_PAGE_META, 32ull * page_cap},
       {B_PAGE_TOTALS, 4ull * page_cap}, {B_CELL_COUNTS, 4ull * page_cap * kPage},
       {B_CELL_OFFSETS, 4ull * page_cap * kPage}, {B_CELL_CURSORS, 4ull * page_cap * kPage},
       {B_REFS, 12ull * ref_cap}, {B_ACTIVE_FLAGS, 4ull * tiles16}, {B_ACTIVE_Q, 4ull * tiles16},
@@ -684,6 +684,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.proj = sl.buf[B_PROJ].as<sr_projected_segment>();
   fp.shading_queue = sl.buf[B_SHADING_Q].as<uint32_t>();
   fp.lut = sl.buf[B_LUT].as<sr_coarse_depth_lut_entry>();
+  fp.lut_z = sl.buf[B_LUT_Z].as<uint2>();
   fp.page_table = sl.buf[B_PAGE_TABLE].as<uint32_t>();
   fp.page_meta = sl.buf[B_PAGE_META].as<sr_fine_page_meta>();
   fp.page_totals = sl.buf[B_PAGE_TOTALS].as<uint32_t>();
