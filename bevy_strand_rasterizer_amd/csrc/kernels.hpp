@@ -44,7 +44,8 @@ struct PrepassScratch {
   // histograms, block and pass counts
   uint32_t* hkeys[2];
   uint32_t* hvals[2];
-  uint32_t* radix_hist;             // [256 * radix_blocks]
+  uint32_t* radix_hist;             // [radix_blocks * 256] digit counts, block-major
+  uint32_t* radix_base;             // [256 * radix_blocks] output bases, digit-major
   uint32_t radix_blocks, radix_passes;
 };
 

@@ -803,8 +803,12 @@ struct HitTotalSrc {
 struct HitBaseDst {
   uint32_t* base;
   uint32_t* counters;
+  uint32_t cap;
   __device__ void store(uint32_t s, uint32_t ex, uint32_t) const { base[s] = ex; }
-  __device__ void total(uint32_t t) const { counters[C_CANDS] = t; }
+  __device__ void total(uint32_t t) const {
+    counters[C_CANDS] = t;
+    counters[C_RADIX_LEN] = 256u * ((min(t, cap) + kRadixBlock - 1u) / kRadixBlock);
+  }
 };
 
 // ---- stable LSD radix sort of (key, value) pairs, 8-bit digits
@@ -830,39 +834,53 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) 
   return m;
 }
 
-// Per block: digit histogram, stored digit-major (hist[d * nblocks + b]) so
-// one exclusive scan yields every (digit, block) output base.  The lowest
-// lane of each peer group adds the group's size.
+// Per block holding hits: digit histogram, stored block-major
+// (hist[b * 256 + d], one coalesced row per block).
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
-                                                    uint32_t shift, uint32_t* hist, uint32_t nblocks) {
+                                                    uint32_t shift, uint32_t* hist) {
   __shared__ uint32_t cnt[256];
   const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x;
+  if (b * kRadixBlock >= n) return;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const unsigned long long lower = (1ull << lane) - 1ull;
   cnt[threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t chunk = b * kRadixBlock + w * kRadixChunk;
   if (chunk < n) {
-#pragma unroll 4
+    uint32_t key[kRadixRounds];
+#pragma unroll
     for (uint32_t k = 0; k < kRadixRounds; ++k) {
       const uint32_t i = chunk + k * 64u + lane;
-      const bool has = i < n;
-      const uint32_t d = has ? (keys[i] >> shift) & 255u : 0u;
+      key[k] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRadixRounds; ++k) {
+      const bool has = chunk + k * 64u + lane < n;
+      const uint32_t d = (key[k] >> shift) & 255u;
       const unsigned long long m = digit_peers(d, has);
       if (has && (m & lower) == 0ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
     }
   }
   __syncthreads();
-  hist[threadIdx.x * nblocks + b] = cnt[threadIdx.x];
+  hist[b * 256u + threadIdx.x] = cnt[threadIdx.x];
 }
 
+// The exclusive scan runs over (digit, block) in digit-major order: every
+// block's output base of every digit.  Source index i = d * nb + b reads the
+// block-major counts; nb = blocks holding hits this frame.
 struct HistSrc {
   const uint32_t* hist;
-  __device__ uint32_t load(uint32_t i) const { return hist[i]; }
+  const uint32_t* n_dev;
+  uint32_t cap;
+  __device__ uint32_t load(uint32_t i) const {
+    const uint32_t nb = (radix_count(n_dev, cap) + kRadixBlock - 1u) / kRadixBlock;
+    const uint32_t d = i / nb, b = i - d * nb;
+    return hist[b * 256u + d];
+  }
 };
 struct HistDst {
-  uint32_t* hist;
-  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { hist[i] = ex; }
+  uint32_t* base;
+  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { base[i] = ex; }
   __device__ void total(uint32_t) const {}
 };
 
@@ -874,8 +892,7 @@ struct HistDst {
 template <bool REFS>
 __global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uint32_t* keys, const uint32_t* vals,
                                                        uint32_t* okeys, uint32_t* ovals, const uint32_t* n_dev,
-                                                       uint32_t cap, uint32_t shift, const uint32_t* hist,
-                                                       uint32_t nblocks) {
+                                                       uint32_t cap, uint32_t shift, const uint32_t* bases) {
   __shared__ uint32_t lkeys[kRadixBlock];
   __shared__ uint32_t lvals[kRadixBlock];
   __shared__ uint32_t wcnt[4][256];  // per wave: running digit count, then the wave's start of each digit
@@ -886,10 +903,11 @@ __global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uin
   if (base >= n) return;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t nvalid = min(kRadixBlock, n - base);
+  const uint32_t nb = (n + kRadixBlock - 1u) / kRadixBlock;
   const unsigned long long lower = (1ull << lane) - 1ull;
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[q][t] = 0u;
-  gbase[t] = hist[t * nblocks + b];
+  gbase[t] = bases[t * nb + b];
   __syncthreads();
   const uint32_t chunk = base + w * kRadixChunk;
   uint32_t key[kRadixRounds], val[kRadixRounds], rank[kRadixRounds];
@@ -1352,7 +1370,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                         ptab, nullptr, ps.partials, s);
   if (csr) {
     timer->mark(s, "hit_offsets");
-    launch_scan<uint32_t>(HitTotalSrc{fp.slot_hit_total}, HitBaseDst{fp.hit_base, fp.counters}, (uint32_t)slots_total,
+    launch_scan<uint32_t>(HitTotalSrc{fp.slot_hit_total}, HitBaseDst{fp.hit_base, fp.counters, fp.ref_cap}, (uint32_t)slots_total,
                           nullptr, ps.partials, s);
   }
   timer->mark(s, "project_write");
@@ -1367,17 +1385,17 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
       const bool last = k + 1u == ps.radix_passes;
       const int dst = last ? 0 : (src ^ 1);
       hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters + C_CANDS,
-                         fp.ref_cap, shift, ps.radix_hist, ps.radix_blocks);
-      launch_scan<uint32_t>(HistSrc{ps.radix_hist}, HistDst{ps.radix_hist}, 256u * ps.radix_blocks, nullptr,
-                            ps.partials, s);
+                         fp.ref_cap, shift, ps.radix_hist);
+      launch_scan<uint32_t>(HistSrc{ps.radix_hist, fp.counters + C_CANDS, fp.ref_cap}, HistDst{ps.radix_base},
+                            256u * ps.radix_blocks, fp.counters + C_RADIX_LEN, ps.partials, s);
       if (last)
         hipLaunchKernelGGL(k_radix_scatter<true>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
                            ps.hvals[src], ps.hkeys[dst], nullptr, fp.counters + C_CANDS, fp.ref_cap, shift,
-                           ps.radix_hist, ps.radix_blocks);
+                           ps.radix_base);
       else
         hipLaunchKernelGGL(k_radix_scatter<false>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
                            ps.hvals[src], ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift,
-                           ps.radix_hist, ps.radix_blocks);
+                           ps.radix_base);
     }
     timer->mark(s, "cell_runs");
     hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,

@@ -426,10 +426,24 @@ __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, 
   return total;
 }
 
+// Per Gauss-Legendre point: the clamped offset h, gamma_i = asin(h), its
+// cosine and 2 gamma_i -- the same for every segment, computed once per
+// workgroup with the functions the per-segment code would call.
+struct LegPoint {
+  float h, gamma_i, cos_gamma_i, ai;
+};
+
+__device__ __forceinline__ LegPoint leg_point(int i) {
+  const float h = wclamp(-1.0f + 1e-6f, 1.0f - 1e-6f, kLegRoots[i]);
+  const float gi = asinf(h);
+  return LegPoint{h, gi, cosf(gi), 2.0f * asinf(h)};
+}
+
 // The three Np lobes of weta_strand_bsdf (strand_shading.wgsl:525-577) in
 // one pass over the 10 quadrature points: the per-point terms of A and Phi
 // (gamma_i, gamma_t, Fresnel, absorption) are shared by the three paths.
-__device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 mu, float beta, float scat, V3* out) {
+__device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 mu, float beta, float scat,
+                         const LegPoint* leg, V3* out) {
   float theta_d = (theta_r - theta_i) * 0.5f;
   float sti = sinf(theta_i);
   float ctd = cosf(theta_d);
@@ -443,14 +457,15 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
   int kmax = (int)ceilf(sqrtf(220.0f * var) / TWO_PI_F) + 1;
   V3 acc[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
   for (int i = 0; i < 10; ++i) {
-    float h = wclamp(-1.0f + 1e-6f, 1.0f - 1e-6f, kLegRoots[i]);
-    float gamma_t = asinf(h / eta_prime);
-    float gamma_i = asinf(h);
-    float f = fresnel_reflectance(eta, sacos(ctd * cosf(gamma_i)));
+    const LegPoint lp = leg[i];
+    const float x = lp.h / eta_prime;
+    float gamma_t = asinf(x);
+    float f = fresnel_reflectance(eta, sacos(ctd * lp.cos_gamma_i));
     float e = 1.0f + cosf(2.0f * gamma_t);
     V3 T{__expf(-2.0f * mup.x * e), __expf(-2.0f * mup.y * e), __expf(-2.0f * mup.z * e)};
-    float gt_phi = asinf(wclamp(h / eta_prime, -1.0f, 1.0f));
-    float ai = 2.0f * asinf(h);
+    // asin(clamp(x, -1, 1)) is gamma_t whenever x is already in range
+    float gt_phi = fabsf(x) <= 1.0f ? gamma_t : asinf(wclamp(x, -1.0f, 1.0f));
+    float ai = lp.ai;
     float f_ = 1.0f - f;
     float wgt = kLegWeights[i];
 #pragma unroll
@@ -486,7 +501,8 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
   *out = total;
 }
 
-__device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material& m, float occlusion) {
+__device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material& m, float occlusion,
+                        const LegPoint* leg) {
   float theta_i = sacos(v3dot(dir, light_n));
   float theta_r = sacos(v3dot(dir, view_n));
   V3 lp = v3norm(v3sub(light_n, v3scale(dir, v3dot(light_n, dir))));
@@ -502,7 +518,7 @@ __device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material&
   V3 total{mp * m.specular_color[0] * m.specular_color[3] * direct, mp * m.specular_color[1] * m.specular_color[3] * direct,
            mp * m.specular_color[2] * m.specular_color[3] * direct};
   V3 lobes;
-  np_lobes(phi, theta_i, theta_r, m.eta, sigma, beta_azim, scat, &lobes);
+  np_lobes(phi, theta_i, theta_r, m.eta, sigma, beta_azim, scat, leg, &lobes);
   total.x = total.x + lobes.x;
   total.y = total.y + lobes.y;
   total.z = total.z + lobes.z;
@@ -535,6 +551,9 @@ __device__ float dom_visibility(const FrameParams& fp, V3 pw, uint32_t layer) {
 }
 
 __global__ __launch_bounds__(128) void k_shading(FrameParams fp) {
+  __shared__ LegPoint leg[10];
+  if (threadIdx.x < 10u) leg[threadIdx.x] = leg_point((int)threadIdx.x);
+  __syncthreads();
   const uint32_t n = min(fp.counters[C_CAM_TASKS], fp.task_cap);
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const uint32_t t = fp.shading_queue[q];
@@ -570,7 +589,7 @@ __global__ __launch_bounds__(128) void k_shading(FrameParams fp) {
     for (uint32_t j = 0; j < fp.NL; ++j) {
       const sr_directional_light lt = fp.lights[j];
       V3 L = v3norm(V3{lt.direction_to_light[0], lt.direction_to_light[1], lt.direction_to_light[2]});
-      V3 bc = marschner(L, Vv, U, m, occ);
+      V3 bc = marschner(L, Vv, U, m, occ, leg);
       float amb = m.ambient_factor;
       float cx = bc.x * (lt.color[0] / 255.0f), cy = bc.y * (lt.color[1] / 255.0f), cz = bc.z * (lt.color[2] / 255.0f);
       float ax = m.absorption_color[0] * amb + (fp.st.ambient_color[0] / 255.0f) * amb;
