@@ -967,13 +967,30 @@ __global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uin
 // Runs of equal keys among the sorted hits: a cell's run starts at
 // cell_offsets[cell] and ends before cell_cursors[cell] (cleared to 0: a
 // cell without hits keeps 0); dropped pages' hits sort last and are skipped.
+// Four consecutive keys per thread, read as one vector.
 __global__ __launch_bounds__(256) void k_hit_runs(FrameParams fp) {
   const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t k = fp.hkeys[i];
-    if ((k >> 8) >= fp.page_cap) continue;
-    if (i == 0u || fp.hkeys[i - 1u] != k) fp.cell_offsets[k] = i;
-    if (i + 1u == n || fp.hkeys[i + 1u] != k) fp.cell_cursors[k] = i + 1u;
+  const uint32_t quads = (n + 3u) / 4u;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+    const uint32_t i0 = 4u * q;
+    uint32_t k[4];
+    if (i0 + 3u < n) {
+      const uint4 v = reinterpret_cast<const uint4*>(fp.hkeys)[q];
+      k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) k[j] = i0 + j < n ? fp.hkeys[i0 + j] : ~0u;
+    }
+    const uint32_t prev = i0 ? fp.hkeys[i0 - 1u] : ~0u;
+    const uint32_t next = i0 + 4u < n ? fp.hkeys[i0 + 4u] : ~0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = i0 + j;
+      if (i >= n || (k[j] >> 8) >= fp.page_cap) continue;
+      const uint32_t before = j ? k[j - 1] : prev, after = j < 3 ? k[j + 1] : next;
+      if (i == 0u || before != k[j]) fp.cell_offsets[k[j]] = i;
+      if (i + 1u == n || after != k[j]) fp.cell_cursors[k[j]] = i + 1u;
+    }
   }
 }
 
@@ -1400,7 +1417,8 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
     timer->mark(s, "cell_runs");
     hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,
                        fp.cell_cursors);
-    hipLaunchKernelGGL(k_hit_runs, dim3(grid_for(fp.ref_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_hit_runs, dim3(grid_for(((uint64_t)fp.ref_cap + 3u) / 4u, 256, 1u << 20)), dim3(256), 0, s,
+                       fp);
     hipLaunchKernelGGL(k_csr_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "page_bases");
     launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,

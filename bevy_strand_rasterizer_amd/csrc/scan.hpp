@@ -110,13 +110,25 @@ __global__ __launch_bounds__(kScanWg) void scan_down_k(Src src, Dst dst, uint32_
   const uint32_t tile = blockIdx.x * (uint32_t)kScanTile;
   if (tile >= n) return;
   T carry = partials[blockIdx.x];
+  // all loads first, then the block scans, then every store: no barrier
+  // sits between a store and the next item, so consumers whose store reads
+  // more memory overlap their latencies
+  T v[kScanItems], ex[kScanItems];
+#pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    uint32_t i = tile + k * kScanWg + threadIdx.x;
-    T v = i < n ? src.load(i) : zero_v<T>();
+    const uint32_t i = tile + k * kScanWg + threadIdx.x;
+    v[k] = i < n ? src.load(i) : zero_v<T>();
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
     T total;
-    T ex = block_exclusive(v, &total, lds4);
-    if (i < n) dst.store(i, carry + ex, v);
+    ex[k] = carry + block_exclusive(v[k], &total, lds4);
     carry = carry + total;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = tile + k * kScanWg + threadIdx.x;
+    if (i < n) dst.store(i, ex[k], v[k]);
   }
 }
 
