@@ -757,7 +757,6 @@ struct PageTotalSrc {
 };
 struct PageBaseDst {
   FrameParams fp;
-  const uint32_t* xy_mask;
   __device__ void store(uint32_t p, uint32_t ex, uint32_t v) const {
     sr_fine_page_meta& pm = fp.page_meta[p];
     uint32_t flags = (v > 0u && (uint64_t)ex + v > fp.ref_cap) ? 1u : 0u;
@@ -766,22 +765,32 @@ struct PageBaseDst {
     pm.work_base = 0u; pm.work_count = 0u; pm.first_depth_key = 0u;
     pm.flags = flags;
     if (flags) atomicAdd(&fp.counters[C_OVERFLOW], 1u);
-    uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
-    if (flags || fi == kInvalid) return;
-    const FrustumRec& fr = fp.fr[fi];
-    uint32_t lc = pm.coarse_tile_id - fr.d.coarse_depth_tile_base;
-    uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
+  }
+  __device__ void total(uint32_t t) const { fp.counters[C_REFS] = t; }
+};
+
+// The fine tiles with work of every page that kept its references become
+// active (one thread per page, after the page bases).
+__global__ __launch_bounds__(256) void k_page_active(FrameParams fp, const uint32_t* xy_mask) {
+  const uint32_t P = fp.counters[C_PAGES];
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+    const sr_fine_page_meta pm = fp.page_meta[p];
     uint32_t m = xy_mask[p];
+    if (pm.flags || m == 0u) continue;
+    const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);
+    if (fi == kInvalid) continue;
+    const FrustumRec& fr = fp.fr[fi];
+    const uint32_t lc = pm.coarse_tile_id - fr.d.coarse_depth_tile_base;
+    const uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
     while (m) {
-      uint32_t xy = __ffs(m) - 1u;
+      const uint32_t xy = __ffs(m) - 1u;
       m &= m - 1u;
-      uint32_t fx = cx * kExt + xy % kExt, fy = cy * kExt + xy / kExt;
+      const uint32_t fx = cx * kExt + xy % kExt, fy = cy * kExt + xy / kExt;
       if (fx >= fr.fine_tiles_x || fy >= fr.fine_tiles_y) continue;
       fp.active_flags[pm.coarse_tile_id * kExt * kExt + xy] = 1u;
     }
   }
-  __device__ void total(uint32_t t) const { fp.counters[C_REFS] = t; }
-};
+}
 
 // ------------------------------------------------------ page CSR backend
 // K16-K18 (prefix_page_candidates, scatter_page_candidates,
@@ -1421,7 +1430,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                        fp);
     hipLaunchKernelGGL(k_csr_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "page_bases");
-    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,
+    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp}, fp.page_cap,
                           fp.counters + C_PAGES, ps.partials, s);
   } else {
     hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,
@@ -1430,7 +1439,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
     hipLaunchKernelGGL((k_visit<MODE_COUNT>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
     timer->mark(s, "prefix");
     hipLaunchKernelGGL(k_page_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp, fp.page_xy_mask);
-    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp, fp.page_xy_mask}, fp.page_cap,
+    launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp}, fp.page_cap,
                           fp.counters + C_PAGES, ps.partials, s);
     timer->mark(s, "fill");
     hipLaunchKernelGGL((k_visit<MODE_FILL>), dim3(grid_for(fp.task_cap, 256, kMaxGrid)), dim3(256), 0, s, fp);
@@ -1438,6 +1447,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
     hipLaunchKernelGGL(k_sort_cells, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
   }
   timer->mark(s, "compact");
+  hipLaunchKernelGGL(k_page_active, dim3(grid_for(fp.page_cap, 256, kMaxGrid)), dim3(256), 0, s, fp, fp.page_xy_mask);
   launch_scan<uint32_t>(FlagSrc{fp.active_flags}, ActiveDst{fp.active_queue, fp.counters},
                         fp.coarse_total * kExt * kExt, nullptr, ps.partials, s);
   timer->mark(s, "emit");
