@@ -18,6 +18,23 @@
 
 namespace srk {
 
+// Largest radius the exact tests can use along a segment: they evaluate
+// r = mix(minr, maxr, clamp(z, 0, 1)) at a depth z between the endpoint
+// depths, and r grows with z (minr <= maxr), so the deeper endpoint bounds
+// it (rounding is absorbed by the span pad).
+SR_HD float span_radius(float minr, float maxr, float zdeep01) {
+  return wmix(minr, maxr, wclamp(zdeep01, 0.0f, 1.0f));
+}
+
+// The largest normalize_depth01 over raw depths in [za, zb] (camera
+// segments): it is the identity on [0, 1] and z / 2 + 1/2 clamped outside,
+// so below 0 it approaches 1/2.
+SR_HD float span_depth01_max(float za, float zb) {
+  const float lo = fminf(za, zb), hi = fmaxf(za, zb);
+  const float n = normalize_depth01(hi);
+  return lo < 0.0f ? fmaxf(n, 0.5f) : n;
+}
+
 struct SpanRef {
   float x0, y0, abx, aby;
   float bu0, bu1, bv0, bv1;  // box of p - p0, grown by the padded radius

@@ -220,8 +220,13 @@ __device__ __forceinline__ void shadow_columns(const ShadowBatch<TPX>& b, uint32
   const uint32_t k = threadIdx.x;
   const bool valid = k < nb;
   float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), Rr = A;
-  if (valid) { A = b.a[k]; Rr = b.r[k]; }
-  batch_columns<TPX>(valid, A.x, A.y, Rr.z, Rr.w, Rr.y, tx0, ty0, fx, fy, cols);
+  float rb = 0.0f;
+  if (valid) {
+    A = b.a[k];
+    Rr = b.r[k];
+    rb = span_radius(Rr.x, Rr.y, fmaxf(A.z, b.c[k].z));
+  }
+  batch_columns<TPX>(valid, A.x, A.y, Rr.z, Rr.w, rb, tx0, ty0, fx, fy, cols);
 }
 
 // K25 (strand_shadow_raster.wgsl): per pixel of a light tile, pass 1 finds
@@ -689,7 +694,8 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
         float x0 = 0.0f, y0 = 0.0f, ax = 0.0f, ay = 0.0f, rmx = 0.0f;
         if (valid) {
           x0 = b.p0x[k]; y0 = b.p0y[k]; ax = b.abx[k]; ay = b.aby[k];
-          rmx = fmaxf(b.rmax[k], fmaxf(b.rmin[k], 1e-4f));
+          const float minr = fmaxf(b.rmin[k], 1e-4f);
+          rmx = span_radius(minr, fmaxf(b.rmax[k], minr), span_depth01_max(b.d0[k], b.d1[k]));
         }
         batch_columns<TPX>(valid, x0, y0, ax, ay, rmx, tx * fx, ty * fy, fx, fy, cols);
       }
