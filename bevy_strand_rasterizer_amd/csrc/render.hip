@@ -375,6 +375,19 @@ __device__ __forceinline__ V3 v3scale(V3 a, float s) { return V3{a.x * s, a.y * 
 __device__ __forceinline__ V3 v3cross(V3 a, V3 b) {
   return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// The BSDF runs at the shading tolerance, not bit for bit: it uses the
+// hardware reciprocal, square root and exp (about 1 ulp) and sin / cos
+// (absolute error near 2^-21 for the angles here).  The atlas differs from
+// the correctly rounded evaluation by far less than its 1e-3 parity bound;
+// visibility (which picks DOM texels and slices) stays exact.
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fsin(float x) { return __sinf(x); }
+__device__ __forceinline__ float fcos(float x) { return __cosf(x); }
+__device__ __forceinline__ float fexp(float x) { return __expf(x); }
+
+// (normalisation stays correctly rounded: acos of a dot product near +-1
+// would magnify its error)
 __device__ __forceinline__ V3 v3norm(V3 a) {
   float l = sqrtf(v3dot(a, a));
   return V3{a.x / l, a.y / l, a.z / l};
@@ -385,10 +398,10 @@ __device__ float fresnel_reflectance(float eta_t, float c_in) {  // eta_i = 1
   float c = fabsf(c_in);
   float cc = wclamp(0.0f, 1.0f, c);
   float sin2 = 1.0f - cc * cc;
-  float ratio = 1.0f / eta_t;
-  float cos_t = sqrtf(1.0f - ratio * ratio * sin2);
-  float rpar = ((eta_t * c) - (1.0f * cos_t)) / ((eta_t * c) + (1.0f * cos_t));
-  float rper = ((1.0f * c) - (eta_t * cos_t)) / ((1.0f * c) + (eta_t * cos_t));
+  float ratio = __builtin_amdgcn_rcpf(eta_t);
+  float cos_t = fsqrt(1.0f - ratio * ratio * sin2);
+  float rpar = fdiv((eta_t * c) - (1.0f * cos_t), (eta_t * c) + (1.0f * cos_t));
+  float rper = fdiv((1.0f * c) - (eta_t * cos_t), (1.0f * c) + (eta_t * cos_t));
   return (rpar * rpar + rper * rper) * 0.5f;
 }
 
@@ -397,12 +410,12 @@ __device__ __forceinline__ float bessel5(float x) {
   float x2 = x * x;
   float y = 1.0f;
   float term = 1.0f;
-  const float den[5] = {1.0f, 4.0f, 64.0f, 2304.0f, 147456.0f};  // 4^i (i!)^2
+  const float inv_den[5] = {1.0f, 1.0f / 4.0f, 1.0f / 64.0f, 1.0f / 2304.0f, 1.0f / 147456.0f};  // 1 / (4^i (i!)^2)
   float xp = 1.0f;
 #pragma unroll
   for (int i = 1; i < 5; ++i) {
     xp = xp * x2;
-    term = xp / den[i];
+    term = xp * inv_den[i];
     y = y + term;
   }
   return y;
@@ -412,21 +425,22 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
   float trs = theta_r + shift;
   float theta_i = wclamp(PI_HALF_LO, PI_HALF_HI, theta_i_);
   trs = wclamp(PI_HALF_LO, PI_HALF_HI, trs);
-  float cti = cosf(-theta_i), ctr = cosf(trs);
-  float sti = sinf(-theta_i), str = sinf(trs);
+  float cti = fcos(-theta_i), ctr = fcos(trs);
+  float sti = fsin(-theta_i), str = fsin(trs);
   if (v < 1e-6f) return fabsf(trs + theta_i) < DELTA_TOL_F ? 0.0f : 1e6f / DELTA_TOL_F;
-  return bessel5((cti * ctr) / v) * expf((sti * str) / v);
+  const float iv = __builtin_amdgcn_rcpf(v);
+  return bessel5((cti * ctr) * iv) * fexp((sti * str) * iv);
 }
 
 // Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)); terms whose
 // exponent is below -110 are zero in FP32 and are not evaluated.
 __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, int kmax) {
-  float kc = rintf(pd / TWO_PI_F);
+  float kc = rintf(pd * (float)(1.0 / (2.0 * 3.14159265359)));
   int k0 = max(-10, (int)kc - kmax), k1 = min(10, (int)kc + kmax);
   float total = 0.0f;
   for (int k = k0; k <= k1; ++k) {
     float ang = pd - TWO_PI_F * (float)k;
-    total = total + __expf(ang * ang * neg_inv_2var);
+    total = total + fexp(ang * ang * neg_inv_2var);
   }
   return total;
 }
@@ -450,24 +464,26 @@ __device__ __forceinline__ LegPoint leg_point(int i) {
 __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 mu, float beta, float scat,
                          const LegPoint* leg, V3* out) {
   float theta_d = (theta_r - theta_i) * 0.5f;
-  float sti = sinf(theta_i);
-  float ctd = cosf(theta_d);
-  float eta_prime = sqrtf(fmaxf(0.0f, eta * eta - sti * sti)) / ctd;
-  V3 mup{mu.x / ctd, mu.y / ctd, mu.z / ctd};
+  float sti = fsin(theta_i);
+  float ctd = fcos(theta_d);
+  const float ictd = __builtin_amdgcn_rcpf(ctd);
+  float eta_prime = fsqrt(fmaxf(0.0f, eta * eta - sti * sti)) * ictd;
+  const float ieta = __builtin_amdgcn_rcpf(eta_prime);
+  V3 mup{mu.x * ictd, mu.y * ictd, mu.z * ictd};
   float var = beta * beta;
   bool narrow = var < 1e-6f;
-  float norm = 1.0f / (SQRT_2PI_F * beta);
-  float neg_inv_2var = -1.0f / (2.0f * var);
+  float norm = __builtin_amdgcn_rcpf(SQRT_2PI_F * beta);
+  float neg_inv_2var = -__builtin_amdgcn_rcpf(2.0f * var);
   // |d - 2 pi k| above sqrt(110 * 2 var) contributes exactly zero
   int kmax = (int)ceilf(sqrtf(220.0f * var) / TWO_PI_F) + 1;
   V3 acc[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
   for (int i = 0; i < 10; ++i) {
     const LegPoint lp = leg[i];
-    const float x = lp.h / eta_prime;
+    const float x = lp.h * ieta;
     float gamma_t = asinf(x);
     float f = fresnel_reflectance(eta, sacos(ctd * lp.cos_gamma_i));
-    float e = 1.0f + cosf(2.0f * gamma_t);
-    V3 T{__expf(-2.0f * mup.x * e), __expf(-2.0f * mup.y * e), __expf(-2.0f * mup.z * e)};
+    float e = 1.0f + fcos(2.0f * gamma_t);
+    V3 T{fexp(-2.0f * mup.x * e), fexp(-2.0f * mup.y * e), fexp(-2.0f * mup.z * e)};
     // asin(clamp(x, -1, 1)) is gamma_t whenever x is already in range
     float gt_phi = fabsf(x) <= 1.0f ? gamma_t : asinf(wclamp(x, -1.0f, 1.0f));
     float ai = lp.ai;
@@ -498,7 +514,7 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
   V3 total{0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
-    float s = ((float)(p + 1) * scat) / 3.0f;
+    float s = ((float)(p + 1) * scat) * (1.0f / 3.0f);
     total.x = total.x + (acc[p].x * 0.5f) * s;
     total.y = total.y + (acc[p].y * 0.5f) * s;
     total.z = total.z + (acc[p].z * 0.5f) * s;
@@ -516,7 +532,7 @@ __device__ V3 marschner(V3 dir, V3 view_n, V3 light_n, const sr_strand_material&
   float phi = sacos(v3dot(v3norm(lp), v3norm(vp))) * (sg > 0.0f ? 1.0f : (sg < 0.0f ? -1.0f : 0.0f));
   V3 sigma{1.0f - m.absorption_color[0], 1.0f - m.absorption_color[1], 1.0f - m.absorption_color[2]};
   float v_long = m.alpha * m.alpha, v_azim = m.beta * m.beta;
-  float beta_azim = sqrtf(v_azim);
+  float beta_azim = fsqrt(v_azim);
   float scat = wclamp(1.0f - occlusion, 0.0f, 1.0f);
   float direct = wmix(0.18f, 1.0f, scat);
   float mp = Mp(v_long, theta_i, theta_r, m.shift);
