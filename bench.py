@@ -44,10 +44,26 @@ def percentile(samples, f):
     return s[int(math.floor((len(s) - 1) * f + 0.5))]
 
 
-def stage_bytes(ctr, w):
+def refs_by_kind(r, w):
+    """Fine references of camera and of light frusta: page reference counts
+    summed over each frustum's coarse tiles (page_meta dump)."""
+    import numpy as np
+
+    pm = r.dump("page_meta")[: int(r.dump("counters")[0]["allocated_pages"])]
+    out = {0: 0, 1: 0}
+    for f in w.frusta:
+        lo = int(f["coarse_depth_tile_base"])
+        sel = (pm["coarse_tile_id"] >= lo) & (pm["coarse_tile_id"] < lo + int(f["coarse_depth_tile_count"]))
+        out[int(f["kind"])] += int(pm["seg_ref_count"][sel].astype(np.int64).sum())
+    return out
+
+
+def stage_bytes(ctr, w, refs=None):
     """Algorithmic bytes per stage: every ABI record read or written once.
     T tasks, C = 256 P cells, H references, N sorted hits, W work items,
-    R runs."""
+    R runs; raster stages: per reference its 12-byte record, two vertex
+    indices and two float4 vertices (+ two atlas texels and a history texel
+    for the camera), plus the framebuffers written once."""
     T, P, H = ctr["binning_tasks"], ctr["allocated_pages"], ctr["fine_refs"]
     C = 256 * P
     W = ctr["work_items"]
@@ -57,7 +73,7 @@ def stage_bytes(ctr, w):
     ptab = 16 * ctr["coarse_tiles"]  # coarse tiles x depth slices
     page_cap = min(ptab, max(4096, ptab // 2))  # the runtime's first-frame page capacity
     passes = (min(max(page_cap.bit_length(), 1), 24) + 8 + 7) // 8
-    return {
+    out = {
         "project_mark": 28 * T,
         "count": 28 * T + 4 * C,
         # page CSR: (key, slot) pairs; each pass reads keys for the digit
@@ -71,17 +87,57 @@ def stage_bytes(ctr, w):
         "sort_cells": 24 * H,
         "compact": 8 * tiles16,
         "emit": 4 * C + 32 * W + 32 * R,
+        # per camera task: task record, four vertex words, meta, atlas and
+        # history texels written
+        "shading": ctr["camera_tasks"] * (16 + 2 * 4 + 2 * 16 + 16 + 16 + 8),
     }
+    if refs is not None:
+        L = int(w.settings["dom_layers"])
+        light_px = sum(int(f["screen_width"]) * int(f["screen_height"]) for f in w.frusta if f["kind"] == 1)
+        cam_px = sum(int(f["screen_width"]) * int(f["screen_height"]) for f in w.frusta if f["kind"] == 0)
+        out["shadow"] = refs[1] * (12 + 8 + 32) + light_px * 4 * (1 + L)
+        out["raster"] = refs[0] * (12 + 8 + 32 + 32 + 8) + cam_px * (16 + 4)
+    return out
 
 
-def roofline_for(stage, ms, ctr, w):
-    b = stage_bytes(ctr, w).get(stage)
+def profile_row(kernel: str):
+    """(HBM bytes, VALU busy) per launch of `kernel` from the committed
+    rocprofv3 summary of the named workload, or (None, None)."""
+    f = Path(__file__).resolve().parent / "profiles" / "named_1080p_csr_kernels.md"
+    if not f.exists():
+        return None, None
+    for line in f.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].strip("`").startswith(kernel):
+            try:
+                return int((float(cells[3]) + float(cells[4])) * 1e6), float(cells[6])
+            except ValueError:
+                return None, None
+    return None, None
+
+
+STAGE_KERNEL = {"shadow": "k_shadow_raster", "raster": "k_camera_raster", "shading": "k_shading",
+                "project_mark": "k_project_mark", "project_write": "k_project_write"}
+
+
+def roofline_for(stage, ms, ctr, w, refs=None):
+    b = stage_bytes(ctr, w, refs).get(stage)
     if b is None or ms <= 0:
         return None
     gbs = b / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": stage,
-            "bytes_per_launch": int(b), "avg_ms": round(ms, 4)}
+    out = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": stage,
+           "bytes_per_launch": int(b), "avg_ms": round(ms, 4)}
+    kern = STAGE_KERNEL.get(stage)
+    if kern:
+        traffic, valu = profile_row(kern)
+        out["kernel"] = kern
+        out["traffic"] = traffic
+        if valu is not None:
+            out["valu_busy"] = valu
+            if valu > 0.6:
+                out["note"] = "VALU-bound (exact per-pixel coverage arithmetic); traffic and VALU busy from profiles/"
+    return out
 
 
 def cpu_baseline(threads: int, fraction: float = 0.25):
@@ -303,19 +359,19 @@ def main():
     ctr, lat, dt = time_frames(r, w, args.steps, 0)
     ms_per_step = dt * 1e3 / args.steps
     fps = views * args.steps / dt
+    refs = refs_by_kind(r, w)
     if args.report:
         write_report(args, r, w)
     r.close()
     print(f"bench: plain {fps:.2f} frames/s", file=sys.stderr, flush=True)
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
-    roof = roofline_for(dom, stage_acc[dom], ctr, w)
+    roof = roofline_for(dom, stage_acc[dom], ctr, w, refs)
     bstages = ("sort_hits", "cell_runs") if args.backend == "csr" else ("count", "prefix", "fill")
-    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w) for k in bstages}
+    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w, refs) for k in bstages}
     if roof is None:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4),
-                "note": "dominant stage is LDS/VALU bound; binning rooflines in binning_roofline"}
+                "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4)}
 
     batched = None
     if not args.no_batched:
@@ -360,6 +416,7 @@ def main():
         "latency_ms": percentile_ms(lat),
         "stage_ms": {k: round(v, 4) for k, v in stage_acc.items()},
         "counters": {k: v for k, v in ctr.items() if not k.startswith("pad")},
+        "refs_by_kind": {"camera": refs[0], "light": refs[1]},
         "roofline": roof,
         "binning_roofline": binning,
         "cpu_baseline": cpu,
