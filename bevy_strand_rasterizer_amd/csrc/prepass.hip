@@ -843,15 +843,14 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) 
   return m;
 }
 
-// Per block holding hits: digit histogram, stored block-major
-// (hist[b * 256 + d], one coalesced row per block).
+// Per block holding hits: digit histogram (LDS atomics; all keys loaded
+// first), stored block-major (hist[b * 256 + d], one coalesced row per block).
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
                                                     uint32_t shift, uint32_t* hist) {
   __shared__ uint32_t cnt[256];
   const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x;
   if (b * kRadixBlock >= n) return;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  const unsigned long long lower = (1ull << lane) - 1ull;
   cnt[threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t chunk = b * kRadixBlock + w * kRadixChunk;
@@ -865,9 +864,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
 #pragma unroll
     for (uint32_t k = 0; k < kRadixRounds; ++k) {
       const bool has = chunk + k * 64u + lane < n;
-      const uint32_t d = (key[k] >> shift) & 255u;
-      const unsigned long long m = digit_peers(d, has);
-      if (has && (m & lower) == 0ull) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+      if (has) atomicAdd(&cnt[(key[k] >> shift) & 255u], 1u);
     }
   }
   __syncthreads();
