@@ -171,7 +171,14 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
   float qx = wmix(A.x, Cc.x, t), qy = wmix(A.y, Cc.y, t), qz = wmix(A.z, Cc.z, t);
   float r = wmix(Rr.x, Rr.y, wclamp(qz, 0.0f, 1.0f));
   float dx = px - qx, dy = py - qy;
-  float c = wclamp(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f, 1.0f);
+  // d / r from the hardware square root and reciprocal (a few ulp): outside
+  // [1 - 1e-5, 1 + 1e-5] it decides coverage as the correctly rounded
+  // quotient would; inside, the quotient is computed exactly.  Coverage is a
+  // DOM value (1e-3 tolerance); t, q and the depth stay exact.
+  const float s2 = dx * dx + dy * dy;
+  float q = __builtin_amdgcn_sqrtf(s2) * __builtin_amdgcn_rcpf(r);
+  if (!(q < 0.99999f || q > 1.00001f)) q = sqrtf(s2) / r;
+  float c = wclamp(1.0f - q, 0.0f, 1.0f);
   *pz = qz;
   *cov = c;
   return c > 0.0f;
