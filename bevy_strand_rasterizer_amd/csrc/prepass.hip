@@ -397,8 +397,8 @@ __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, fl
 }
 
 // Coarse-tile box of a frustum's coarse tile (strand_prepass.wgsl:1422-1433).
-__device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint32_t local, V2* origin, V3* bmax) {
-  uint32_t cx = local % fd.coarse_tiles_x, cy = local / fd.coarse_tiles_x;
+__device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint32_t cx, uint32_t cy, V2* origin,
+                                                V3* bmax) {
   uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
   *origin = V2{(float)(cx * cpx), (float)(cy * cpy)};
   float tmx = fminf(origin->x + (float)cpx, (float)fd.screen_width) - 1e-3f;
@@ -438,10 +438,11 @@ __device__ __forceinline__ void write_hit(const FrameParams& fp, uint64_t idx, u
 // page's candidate list.
 template <int MODE>
 __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
-                                  uint32_t local, const sr_fine_seg_ref& ref, uint32_t task, HitRec& rec) {
+                                  uint32_t local, uint32_t cx, uint32_t cy, const sr_fine_seg_ref& ref, uint32_t task,
+                                  HitRec& rec) {
   V2 origin;
   V3 bmax;
-  coarse_tile_box(fd, local, &origin, &bmax);
+  coarse_tile_box(fd, cx, cy, &origin, &bmax);
   Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
   const uint2* lz = fp.lut_z + (size_t)tile * kSlices;
@@ -541,7 +542,8 @@ __device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V
     if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
     uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
     if (local < fd.coarse_depth_tile_count)
-      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, ref, task, rec);
+      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, (uint32_t)f[0], (uint32_t)f[1],
+                              ref, task, rec);
     if (f[0] == f1[0] && f[1] == f1[1]) break;
     if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
     else { f[1] += step[1]; tmax[1] += delta[1]; }
