@@ -72,16 +72,18 @@ def stage_bytes(ctr, w, refs=None):
     N = ctr.get("page_candidates", 0)
     ptab = 16 * ctr["coarse_tiles"]  # coarse tiles x depth slices
     page_cap = min(ptab, max(4096, ptab // 2))  # the runtime's first-frame page capacity
-    passes = (min(max(page_cap.bit_length(), 1), 24) + 8 + 7) // 8
+    passes = (min(max(page_cap.bit_length(), 1), 24) + 7) // 8  # radix passes over the page bits
     out = {
         "project_mark": 28 * T,
         "count": 28 * T + 4 * C,
-        # page CSR: (key, slot) pairs; each pass reads keys for the digit
-        # histogram and key + slot for the scatter, writes key + slot (the
-        # last one key + the 12-byte reference and reads the slot's material)
-        "sort_hits": passes * 12 * N + (passes - 1) * 8 * N + 20 * N,
-        # run boundaries of the sorted keys, then counts, offsets, page totals
-        "cell_runs": 4 * C + 4 * N + 8 * C + 8 * C + 8 * P,
+        # page CSR: (key, slot) pairs; each radix pass over the page bits
+        # reads keys for the digit histogram, key + slot for the scatter and
+        # writes key + slot
+        "sort_hits": passes * 20 * N,
+        # page runs of the sorted keys, then per page: keys for the cell
+        # histogram, key + slot + the slot's material for the stable cell
+        # sort, the 12-byte references, cell counts / offsets, page totals
+        "page_cells": 4 * N + 4 * N + 12 * N + 12 * N + 8 * C + 8 * P,
         "prefix": 8 * C + 32 * P,
         "fill": 28 * T + 4 * C + 12 * H,
         "sort_cells": 24 * H,
@@ -367,7 +369,7 @@ def main():
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
     roof = roofline_for(dom, stage_acc[dom], ctr, w, refs)
-    bstages = ("sort_hits", "cell_runs") if args.backend == "csr" else ("count", "prefix", "fill")
+    bstages = ("sort_hits", "page_cells") if args.backend == "csr" else ("count", "prefix", "fill")
     binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w, refs) for k in bstages}
     if roof is None:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,

@@ -798,15 +798,14 @@ __global__ __launch_bounds__(256) void k_page_active(FrameParams fp, const uint3
 // K16-K18 (prefix_page_candidates, scatter_page_candidates,
 // build_fine_pages_csr; strand_prepass.wgsl:1698-2049, 2240-2507) build each
 // page's candidate list with atomics and count / fill per page.  Here the
-// same references come out of one stable sort, without a device-scope
-// atomic (on this part those go to memory and serialise on hot pages) and
-// without per-page workgroups: marking counts each slot's visible fine cells,
-// a scan gives every slot its offset, the write pass stores one
-// (page * 256 + cell, slot) hit per visible cell in slot order, and a stable
-// LSD radix sort by key orders the hits by page, then cell, then slot --
-// exactly the canonical reference order; its last pass writes the
-// references themselves.  Cell runs of the sorted keys give cell counts; the
-// page prefix and page-base scan are the segment-scatter backend's.
+// same references come out of a stable sort, without a device-scope atomic
+// (on this part those go to memory and serialise on hot pages): marking
+// counts each slot's visible fine cells, a scan gives every slot its offset,
+// the write pass stores one (page * 256 + cell, slot) hit per visible cell in
+// slot order, a stable LSD radix sort on the page bits groups the hits by
+// page (slot order kept), and one workgroup per page sorts its hits by cell
+// (stable) and writes the references -- the canonical order (page, cell,
+// slot).  The page-base scan is the segment-scatter backend's.
 struct HitTotalSrc {
   const uint32_t* totals;
   __device__ uint32_t load(uint32_t i) const { return totals[i]; }
@@ -894,11 +893,8 @@ struct HistDst {
 
 // Scatter: ranks by ballots, then the block's items are reordered by digit in
 // LDS and written out in that order, so consecutive lanes write consecutive
-// addresses of one digit's output range.  REFS: the last pass writes the
-// reference of each hit's slot (make_fine_seg_ref,
-// strand_prepass.wgsl:1313-1327) in place of the value.
-template <bool REFS>
-__global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uint32_t* keys, const uint32_t* vals,
+// addresses of one digit's output range.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals,
                                                        uint32_t* okeys, uint32_t* ovals, const uint32_t* n_dev,
                                                        uint32_t cap, uint32_t shift, const uint32_t* bases) {
   __shared__ uint32_t lkeys[kRadixBlock];
@@ -961,69 +957,138 @@ __global__ __launch_bounds__(256) void k_radix_scatter(FrameParams fp, const uin
   for (uint32_t pos = t; pos < nvalid; pos += 256u) {
     const uint32_t k = lkeys[pos], d = (k >> shift) & 255u;
     const uint32_t g = gbase[d] + (pos - dstart[d]);
-    const uint32_t v = lvals[pos];
     okeys[g] = k;
-    if (REFS) {
-      const uint32_t slot = v % fp.slots, ii = find_instance(fp, slot);
-      fp.refs[g] = sr_fine_seg_ref{ii, slot - fp.inst[ii].pos_base, fp.slot_material[v]};
-    } else {
-      ovals[g] = v;
-    }
+    ovals[g] = lvals[pos];
   }
 }
 
-// Runs of equal keys among the sorted hits: a cell's run starts at
-// cell_offsets[cell] and ends before cell_cursors[cell] (cleared to 0: a
-// cell without hits keeps 0); dropped pages' hits sort last and are skipped.
-// Four consecutive keys per thread, read as one vector.
-__global__ __launch_bounds__(256) void k_hit_runs(FrameParams fp) {
+// Page runs of the page-sorted hits: [start, end) of every kept page (a
+// page without a visible hit keeps (0, 0)); dropped pages' hits sort last
+// and are skipped.  Four consecutive keys per thread, read as one vector.
+__global__ __launch_bounds__(256) void k_page_runs(FrameParams fp) {
   const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
   const uint32_t quads = (n + 3u) / 4u;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
     const uint32_t i0 = 4u * q;
-    uint32_t k[4];
+    uint32_t pg[4];
     if (i0 + 3u < n) {
       const uint4 v = reinterpret_cast<const uint4*>(fp.hkeys)[q];
-      k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+      pg[0] = v.x >> 8; pg[1] = v.y >> 8; pg[2] = v.z >> 8; pg[3] = v.w >> 8;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) k[j] = i0 + j < n ? fp.hkeys[i0 + j] : ~0u;
+      for (int j = 0; j < 4; ++j) pg[j] = i0 + j < n ? fp.hkeys[i0 + j] >> 8 : ~0u;
     }
-    const uint32_t prev = i0 ? fp.hkeys[i0 - 1u] : ~0u;
-    const uint32_t next = i0 + 4u < n ? fp.hkeys[i0 + 4u] : ~0u;
+    const uint32_t prev = i0 ? fp.hkeys[i0 - 1u] >> 8 : ~0u;
+    const uint32_t next = i0 + 4u < n ? fp.hkeys[i0 + 4u] >> 8 : ~0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t i = i0 + j;
-      if (i >= n || (k[j] >> 8) >= fp.page_cap) continue;
-      const uint32_t before = j ? k[j - 1] : prev, after = j < 3 ? k[j + 1] : next;
-      if (i == 0u || before != k[j]) fp.cell_offsets[k[j]] = i;
-      if (i + 1u == n || after != k[j]) fp.cell_cursors[k[j]] = i + 1u;
+      if (i >= n || pg[j] >= fp.page_cap) continue;
+      const uint32_t before = j ? pg[j - 1] : prev, after = j < 3 ? pg[j + 1] : next;
+      if (before != pg[j]) fp.page_run[2u * pg[j]] = i;
+      if (after != pg[j]) fp.page_run[2u * pg[j] + 1u] = i + 1u;
     }
   }
 }
 
-// K14 for the sorted hits: per page, cell counts from the runs, the
-// page-local exclusive scan, page total and the mask of fine tiles with work.
-__global__ __launch_bounds__(256) void k_csr_prefix(FrameParams fp) {
+// K14 + the final order for the page-sorted hits, one workgroup per page:
+// the page's cell histogram, the page-local exclusive scan (cell counts,
+// offsets, page total, mask of fine tiles with work), then a stable counting
+// sort by cell that writes each hit's reference (make_fine_seg_ref,
+// strand_prepass.wgsl:1313-1327) at its final position.  Hits arrive in slot
+// order; kPageStep per step, wave w ranks the w-th quarter by ballots in
+// rounds of 64, so each cell keeps slot order = (inst, seg) order; the step
+// is reordered by cell in LDS before the references are written.
+constexpr uint32_t kPageRounds = 8u;                    // rounds of 64 per wave and step
+constexpr uint32_t kPageStep = 4u * 64u * kPageRounds;  // hits per step
+
+__global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32_t* keys, const uint32_t* vals) {
+  __shared__ uint32_t cur[256];       // output position of each cell's next reference
+  __shared__ uint32_t wcnt[4][256];   // per wave: digit counts, then its start per cell in the step
+  __shared__ uint32_t lstart[256], gbase[256];
+  __shared__ uint32_t lcell[kPageStep], lval[kPageStep];  // the step's hits in cell order
   __shared__ uint32_t lds4[4];
   __shared__ uint32_t colmask;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const unsigned long long lower = (1ull << lane) - 1ull;
   const uint32_t P = fp.counters[C_PAGES];
   for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
-    const uint32_t c = threadIdx.x, g = page * kPage + c;
-    const uint32_t hi = fp.cell_cursors[g];
-    const uint32_t cnt = hi ? hi - fp.cell_offsets[g] : 0u;
-    if (c == 0) colmask = 0u;
+    const uint32_t start = fp.page_run[2u * page], end = fp.page_run[2u * page + 1u];
+    cur[t] = 0u;
+    if (t == 0) colmask = 0u;
+    __syncthreads();
+    for (uint32_t i = start + t; i < end; i += 256u) atomicAdd(&cur[keys[i] & 255u], 1u);
+    __syncthreads();
+    const uint32_t cnt = cur[t];
     uint32_t total;
     const uint32_t ex = block_exclusive<uint32_t>(cnt, &total, lds4);
+    const uint32_t g = page * kPage + t;
     fp.cell_counts[g] = cnt;
     fp.cell_offsets[g] = ex;
-    if (cnt) atomicOr(&colmask, 1u << (c & 15u));
+    if (cnt) atomicOr(&colmask, 1u << (t & 15u));
+    cur[t] = start + ex;  // absolute position of the cell's next reference
     __syncthreads();
-    if (c == 0) {
+    if (t == 0) {
       fp.page_totals[page] = total;
       fp.page_xy_mask[page] = colmask;
     }
-    __syncthreads();
+    for (uint32_t b = start; b < end; b += kPageStep) {
+      const uint32_t nstep = min(kPageStep, end - b);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) wcnt[q][t] = 0u;
+      __syncthreads();
+      const uint32_t chunk = b + w * 64u * kPageRounds;
+      uint32_t key[kPageRounds], val[kPageRounds], rank[kPageRounds];
+#pragma unroll
+      for (uint32_t r = 0; r < kPageRounds; ++r) {
+        const uint32_t i = chunk + r * 64u + lane;
+        key[r] = i < end ? keys[i] : 0u;
+        val[r] = i < end ? vals[i] : 0u;
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < kPageRounds; ++r) {
+        const bool has = chunk + r * 64u + lane < end;
+        const uint32_t d = key[r] & 255u;
+        const unsigned long long m = digit_peers(d, has);
+        uint32_t rk = 0u;
+        if (has) {  // every lane reads the count before the group's lowest lane updates it
+          const uint32_t before = wcnt[w][d];
+          rk = before + (uint32_t)__popcll(m & lower);
+          if ((m & lower) == 0ull) wcnt[w][d] = before + (uint32_t)__popcll(m);
+        }
+        rank[r] = rk;
+      }
+      __syncthreads();
+      // per cell: its start within the step (cell-major), each wave's start
+      // after it, and the output position of the step's first hit of the cell
+      const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
+      uint32_t step_total;
+      const uint32_t ls = block_exclusive<uint32_t>(c0 + c1 + c2 + c3, &step_total, lds4);
+      wcnt[0][t] = ls;
+      wcnt[1][t] = ls + c0;
+      wcnt[2][t] = ls + c0 + c1;
+      wcnt[3][t] = ls + c0 + c1 + c2;
+      lstart[t] = ls;
+      gbase[t] = cur[t];
+      cur[t] += c0 + c1 + c2 + c3;
+      __syncthreads();
+#pragma unroll
+      for (uint32_t r = 0; r < kPageRounds; ++r) {
+        if (chunk + r * 64u + lane >= end) continue;
+        const uint32_t d = key[r] & 255u, l = wcnt[w][d] + rank[r];
+        lcell[l] = d;
+        lval[l] = val[r];
+      }
+      __syncthreads();
+      // consecutive lanes write consecutive references of a cell
+      for (uint32_t l = t; l < nstep; l += 256u) {
+        const uint32_t d = lcell[l], pos = gbase[d] + (l - lstart[d]);
+        if (pos >= fp.ref_cap) continue;
+        const uint32_t v = lval[l], slot = v % fp.slots, ii = find_instance(fp, slot);
+        fp.refs[pos] = sr_fine_seg_ref{ii, slot - fp.inst[ii].pos_base, fp.slot_material[v]};
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1402,32 +1467,25 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
     timer->mark(s, "sort_hits");
-    // stable LSD passes over (key, slot) pairs; the last one writes the
-    // references and leaves the sorted keys in pair 0
+    // stable LSD passes over the page bits of (key, slot) pairs; the last
+    // leaves the page-sorted hits in pair 0
     int src = (ps.radix_passes & 1u) ? 1 : 0;  // the pair the hits were written to
     for (uint32_t k = 0; k < ps.radix_passes; ++k, src ^= 1) {
-      const uint32_t shift = 8u * k;
-      const bool last = k + 1u == ps.radix_passes;
-      const int dst = last ? 0 : (src ^ 1);
+      const uint32_t shift = 8u + 8u * k;
+      const int dst = k + 1u == ps.radix_passes ? 0 : (src ^ 1);
       hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters + C_CANDS,
                          fp.ref_cap, shift, ps.radix_hist);
       launch_scan<uint32_t>(HistSrc{ps.radix_hist, fp.counters + C_CANDS, fp.ref_cap}, HistDst{ps.radix_base},
                             256u * ps.radix_blocks, fp.counters + C_RADIX_LEN, ps.partials, s);
-      if (last)
-        hipLaunchKernelGGL(k_radix_scatter<true>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
-                           ps.hvals[src], ps.hkeys[dst], nullptr, fp.counters + C_CANDS, fp.ref_cap, shift,
-                           ps.radix_base);
-      else
-        hipLaunchKernelGGL(k_radix_scatter<false>, dim3(ps.radix_blocks), dim3(256), 0, s, fp, ps.hkeys[src],
-                           ps.hvals[src], ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift,
-                           ps.radix_base);
+      hipLaunchKernelGGL(k_radix_scatter, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], ps.hvals[src],
+                         ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift, ps.radix_base);
     }
-    timer->mark(s, "cell_runs");
-    hipLaunchKernelGGL(k_zero_cells, dim3(grid_for((uint64_t)fp.page_cap * 64u, 256, kMaxGrid)), dim3(256), 0, s, fp,
-                       fp.cell_cursors);
-    hipLaunchKernelGGL(k_hit_runs, dim3(grid_for(((uint64_t)fp.ref_cap + 3u) / 4u, 256, 1u << 20)), dim3(256), 0, s,
+    timer->mark(s, "page_cells");
+    launch_fill_u32(fp.page_run, 2ull * fp.page_cap, 0u, s);
+    hipLaunchKernelGGL(k_page_runs, dim3(grid_for(((uint64_t)fp.ref_cap + 3u) / 4u, 256, 1u << 20)), dim3(256), 0, s,
                        fp);
-    hipLaunchKernelGGL(k_csr_prefix, dim3(grid_for(fp.page_cap, 1, kMaxGrid)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_page_cells, dim3(grid_for(fp.page_cap, 1, 1u << 20)), dim3(256), 0, s, fp, ps.hkeys[0],
+                       ps.hvals[0]);
     timer->mark(s, "page_bases");
     launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp}, fp.page_cap,
                           fp.counters + C_PAGES, ps.partials, s);

@@ -136,7 +136,7 @@ enum BufId {
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SLOT_PROJ,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
-  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_PAGE_RUN, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -574,11 +574,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const uint32_t atlas_texels = atlas_w * atlas_h;
   const uint64_t atlas_n = (uint64_t)cams * NI * atlas_texels;
   const uint32_t tiles16 = coarse_total * kExt * kExt;
-  // page-CSR hit sort: 8-bit digits over page * 256 + cell (page_cap = dropped)
-  uint32_t key_bits = 1;
-  while (key_bits < 24 && (1ull << key_bits) <= page_cap) ++key_bits;
-  key_bits += 8u;
-  const uint32_t radix_passes = csr ? (key_bits + 7u) / 8u : 0u;
+  // page-CSR hit sort: 8-bit digits over the page of page * 256 + cell
+  // (page_cap = dropped); cells are sorted per page afterwards
+  uint32_t page_bits = 1;
+  while (page_bits < 24 && (1ull << page_bits) <= page_cap) ++page_bits;
+  const uint32_t radix_passes = csr ? (page_bits + 7u) / 8u : 0u;
   const uint32_t radix_blocks = csr ? (hit_cap + kRadixBlock - 1u) / kRadixBlock : 0u;
   const uint64_t scan_len = std::max<uint64_t>({(uint64_t)task_cap, (uint64_t)ptab, (uint64_t)tiles16,
                                                 256ull * radix_blocks, 1ull});
@@ -607,7 +607,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap},
       {B_SLOT_HIT_TOTAL, csr ? 4ull * task_cap : 0ull}, {B_HIT_BASE, csr ? 4ull * task_cap : 0ull},
       {B_SLOT_MATERIAL, csr ? 4ull * task_cap : 0ull}, {B_HKEYS0, 4ull * hit_cap}, {B_HKEYS1, 4ull * hit_cap},
-      {B_HVALS0, 4ull * hit_cap}, {B_HVALS1, 4ull * hit_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks}, {B_RADIX_BASE, 4ull * 256u * radix_blocks},
+      {B_HVALS0, 4ull * hit_cap}, {B_HVALS1, 4ull * hit_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks}, {B_RADIX_BASE, 4ull * 256u * radix_blocks}, {B_PAGE_RUN, csr ? 8ull * page_cap : 0ull},
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
@@ -732,6 +732,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.hkeys_w = ps.hkeys[(radix_passes & 1u) ? 1 : 0];
   fp.hvals_w = ps.hvals[(radix_passes & 1u) ? 1 : 0];
   fp.hkeys = ps.hkeys[0];
+  fp.page_run = sl.buf[B_PAGE_RUN].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
   fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
   fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;
