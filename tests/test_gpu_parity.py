@@ -98,8 +98,11 @@ CSR_CASES = {
     "tiny-2lights-8layers-csr": lambda: csr(scene.config_tiny(strands=40, lights=2, dom_layers=8, res=72)),
     "small-f16-csr": lambda: csr(scene.config_small(froxel=16)),
     "small-f8-csr": lambda: csr(scene.config_small(froxel=8)),
-    # thousands of strands over one coarse tile: long page candidate lists
+    # thousands of strands over one coarse tile: pages of thousands of hits
+    # (several steps of the per-page cell sort)
     "dense-one-tile-csr": lambda: csr(scene.config_tiny(strands=3000, points=8, res=32, light_res=32)),
+    # a 4096^2 light: page capacity above 2^16, three radix passes over the page bits
+    "wide-light-csr": lambda: csr(scene.config_tiny(res=64, light_res=4096)),
 }
 
 
