@@ -245,7 +245,7 @@ template <int TPX>
 __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
   __shared__ RunTable<TPX> t;
   __shared__ ShadowBatch<TPX> b;
-  __shared__ float part[TPX * SR_MAX_DOM_LAYERS];
+  extern __shared__ float part[];  // [dom_layers * TPX], layer-major: layer i of pixel p at part[i * TPX + p]
   __shared__ unsigned long long cols[TPX][TPX / 64];
   const uint32_t R = fp.counters[C_SHADOW_RUNS];
   const uint32_t L = fp.dom_layers;
@@ -293,8 +293,8 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     }
     if (hit) found = true;
     found = found && active;
-    float* part_px = part + threadIdx.x * SR_MAX_DOM_LAYERS;
-    for (uint32_t i = 0; i < L; ++i) part_px[i] = 0.0f;
+    float* part_px = part + threadIdx.x;
+    for (uint32_t i = 0; i < L; ++i) part_px[i * TPX] = 0.0f;
     const float inv_span = 1.0f / fmaxf(1e-6f, found ? z0 : 1.0f);
     if (__syncthreads_count(found) != 0) {
       for (uint32_t bs = 0; bs < total; bs += TPX) {
@@ -317,11 +317,11 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
             uint32_t si = min(f2u(floorf(tL)), L - 1u);
             float wgt = tL - floorf(tL);
             const float a = cov * b.c[k].w * 0.5f;
-            const float a0 = part_px[si];
-            part_px[si] = wclamp(a0 + (1.0f - a0) * (1.0f - wgt) * a, 0.0f, 1.0f);
+            const float a0 = part_px[si * TPX];
+            part_px[si * TPX] = wclamp(a0 + (1.0f - a0) * (1.0f - wgt) * a, 0.0f, 1.0f);
             if (si + 1u < L) {
-              const float a1 = part_px[si + 1u];
-              part_px[si + 1u] = wclamp(a1 + (1.0f - a1) * wgt * a, 0.0f, 1.0f);
+              const float a1 = part_px[(si + 1u) * TPX];
+              part_px[(si + 1u) * TPX] = wclamp(a1 + (1.0f - a1) * wgt * a, 0.0f, 1.0f);
             }
           }
         }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
       dd[pix] = found ? z0 : 0.0f;
       float acc = 0.0f;
       for (uint32_t i = 0; i < L; ++i) {
-        acc = acc + (1.0f - acc) * part_px[i];
+        acc = acc + (1.0f - acc) * part_px[i * TPX];
         op[i * plane + pix] = found ? acc : 0.0f;
       }
     }
@@ -795,9 +795,9 @@ __global__ void k_composite(FrameParams fp, uint64_t pixels) {
 // ------------------------------------------------------------ launchers
 void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
   if (max_tile_px <= 64)
-    hipLaunchKernelGGL(k_shadow_raster<64>, dim3(256u * 16u), dim3(64), 0, s, fp);
+    hipLaunchKernelGGL(k_shadow_raster<64>, dim3(256u * 16u), dim3(64), 64u * 4u * fp.dom_layers, s, fp);
   else
-    hipLaunchKernelGGL(k_shadow_raster<256>, dim3(256u * 4u), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_shadow_raster<256>, dim3(256u * 4u), dim3(256), 256u * 4u * fp.dom_layers, s, fp);
 }
 
 void launch_shading(const FrameParams& fp, hipStream_t s) {
