@@ -35,78 +35,93 @@ SR_HD float span_depth01_max(float za, float zb) {
   return lo < 0.0f ? fmaxf(n, 0.5f) : n;
 }
 
+// The rectangle's interval on pixel row y of a tile is
+//   x in [max(box_lo, al0 + y al1, cl0 + y cl1), min(box_hi, ah0 + y al1, ch0 + y cl1)]
+// (x, y relative to the tile's first pixel centre): the along and across
+// limits are linear in y, so the coefficients are set up once per segment
+// and every row costs a few fused multiply-adds.  A zero component of ab
+// leaves that limit independent of x; it then bounds the rows instead.
 struct SpanRef {
-  float x0, y0, abx, aby;
-  float bu0, bu1, bv0, bv1;  // box of p - p0, grown by the padded radius
-  float dlo, dhi, c;         // padded along / across limits
-  float ia, ib;              // 1 / abx, 1 / aby (unused when zero)
-  float pad;
+  float box_lo, box_hi;  // x range of the padded box
+  float al0, ah0, al1;   // along limits: low / high intercepts, common slope
+  float cl0, ch0, cl1;   // across limits
+  float eps;             // absolute rounding bound of the x limits
+  int32_t y0, y1;        // rows that can meet the rectangle
 };
 
-SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax) {
+SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, uint32_t tx0, uint32_t ty0,
+                         uint32_t fy) {
   SpanRef s;
   const float l2 = abx * abx + aby * aby;
   const float len = sqrtf(l2);
-  s.pad = 0.03f + 4e-6f * (fabsf(x0) + fabsf(y0) + len + rmax);
-  const float rp = rmax + s.pad;
-  s.x0 = x0;
-  s.y0 = y0;
-  s.abx = abx;
-  s.aby = aby;
-  s.bu0 = fminf(0.0f, abx) - rp;
-  s.bu1 = fmaxf(0.0f, abx) + rp;
-  s.bv0 = fminf(0.0f, aby) - rp;
-  s.bv1 = fmaxf(0.0f, aby) + rp;
-  s.dlo = -s.pad * len;
-  s.dhi = l2 + s.pad * len;
-  s.c = rp * len;
-  s.ia = abx != 0.0f ? 1.0f / abx : 0.0f;
-  s.ib = aby != 0.0f ? 1.0f / aby : 0.0f;
+  const float pad = 0.03f + 4e-6f * (fabsf(x0) + fabsf(y0) + len + rmax);
+  const float rp = rmax + pad;
+  const float dlo = -pad * len, dhi = l2 + pad * len, c = rp * len;
+  const float c0 = (float)tx0 + 0.5f - x0;  // u = px - x0 of pixel x is x + c0
+  const float d0 = (float)ty0 + 0.5f - y0;  // v = py - y0 of row y is y + d0
+  s.box_lo = (fminf(0.0f, abx) - rp) - c0;
+  s.box_hi = (fmaxf(0.0f, abx) + rp) - c0;
+  float vlo = fminf(0.0f, aby) - rp, vhi = fmaxf(0.0f, aby) + rp;  // rows, in v
+  const float inf = __builtin_huge_valf();
+  float m = fabsf(c0) + fabsf(s.box_lo) + fabsf(s.box_hi);
+  if (abx != 0.0f) {  // u abx in [dlo - v aby, dhi - v aby]
+    const float ia = 1.0f / abx;
+    const float e0 = (dlo - d0 * aby) * ia - c0, e1 = (dhi - d0 * aby) * ia - c0;
+    s.al0 = fminf(e0, e1);
+    s.ah0 = fmaxf(e0, e1);
+    s.al1 = -aby * ia;
+    m += fabsf(e0) + fabsf(e1) + fabsf((dlo - d0 * aby) * ia) + fabsf((dhi - d0 * aby) * ia) +
+         (float)fy * fabsf(s.al1);
+  } else {  // v aby in [dlo, dhi]: a row range when aby != 0, else every row (l2 = 0)
+    s.al0 = -inf;
+    s.ah0 = inf;
+    s.al1 = 0.0f;
+    if (aby != 0.0f) {
+      const float w0 = dlo / aby, w1 = dhi / aby;
+      vlo = fmaxf(vlo, fminf(w0, w1));
+      vhi = fminf(vhi, fmaxf(w0, w1));
+    }
+  }
+  if (aby != 0.0f) {  // u aby in [v abx - c, v abx + c]
+    const float ib = 1.0f / aby;
+    const float e0 = (d0 * abx - c) * ib - c0, e1 = (d0 * abx + c) * ib - c0;
+    s.cl0 = fminf(e0, e1);
+    s.ch0 = fmaxf(e0, e1);
+    s.cl1 = abx * ib;
+    m += fabsf(e0) + fabsf(e1) + fabsf((d0 * abx - c) * ib) + fabsf((d0 * abx + c) * ib) +
+         (float)fy * fabsf(s.cl1);
+  } else {  // |v abx| <= c
+    s.cl0 = -inf;
+    s.ch0 = inf;
+    s.cl1 = 0.0f;
+    if (abx != 0.0f) {
+      const float w = c / fabsf(abx);
+      vlo = fmaxf(vlo, -w);
+      vhi = fminf(vhi, w);
+    }
+  }
+  s.eps = 0.01f + 1e-6f * m;
+  // rows y with y + d0 in [vlo, vhi], widened like the x limits
+  const float ey = 0.01f + 1e-6f * (fabsf(d0) + fabsf(vlo) + fabsf(vhi));
+  const float fy0 = ceilf(vlo - d0 - ey), fy1 = floorf(vhi - d0 + ey);
+  s.y0 = (int32_t)fmaxf(fy0, 0.0f);
+  s.y1 = (int32_t)fminf(fy1, (float)fy - 1.0f);
+  if (!(fy0 <= fy1)) s.y1 = -1;
   return s;
 }
 
-// Interval [*lo, *hi] of u = px - x0 on the row v = py - y0; false when the
-// row misses the rectangle.
-SR_HD bool span_row(const SpanRef& s, float v, float* lo, float* hi) {
-  if (!(v >= s.bv0 && v <= s.bv1)) return false;
-  float a = s.bu0, b = s.bu1;
-  const float va = v * s.aby;  // along: u * abx in [dlo - v aby, dhi - v aby]
-  if (s.abx != 0.0f) {
-    const float e0 = (s.dlo - va) * s.ia, e1 = (s.dhi - va) * s.ia;
-    a = fmaxf(a, fminf(e0, e1));
-    b = fminf(b, fmaxf(e0, e1));
-  } else if (!(va >= s.dlo && va <= s.dhi)) {
-    return false;
-  }
-  const float vc = v * s.abx;  // across: u * aby in [v abx - c, v abx + c]
-  if (s.aby != 0.0f) {
-    const float e0 = (vc - s.c) * s.ib, e1 = (vc + s.c) * s.ib;
-    a = fmaxf(a, fminf(e0, e1));
-    b = fminf(b, fmaxf(e0, e1));
-  } else if (!(fabsf(vc) <= s.c)) {
-    return false;
-  }
-  *lo = a;
-  *hi = b;
-  return a <= b;
-}
-
-// Pixel mask of a fx x fy tile at (tx0, ty0) (pixel y * fx + x, W words of
-// 64 pixels): the pixels whose centres fall in the segment's row spans.
+// Pixel mask of a fx x fy tile (pixel y * fx + x, W words of 64 pixels):
+// the pixels whose centres fall in the segment's row spans.
 template <int W>
-SR_HD void span_mask(const SpanRef& s, uint32_t tx0, uint32_t ty0, uint32_t fx, uint32_t fy, uint64_t (&m)[W]) {
+SR_HD void span_mask(const SpanRef& s, uint32_t fx, uint64_t (&m)[W]) {
   for (int j = 0; j < W; ++j) m[j] = 0ull;
-  const float c0 = (float)tx0 + 0.5f - s.x0;  // u of pixel x is x + c0
-  for (uint32_t y = 0; y < fy; ++y) {
-    const float v = ((float)(ty0 + y) + 0.5f) - s.y0;
-    float lo, hi;
-    if (!span_row(s, v, &lo, &hi)) continue;
-    const float xl = lo - c0, xh = hi - c0;
-    const float fl = ceilf(xl - (0.01f + 1e-6f * (fabsf(lo) + fabsf(c0))));
-    const float fh = floorf(xh + (0.01f + 1e-6f * (fabsf(hi) + fabsf(c0))));
-    const float cl = fmaxf(fl, 0.0f), ch = fminf(fh, (float)(fx - 1u));
+  for (int32_t y = s.y0; y <= s.y1; ++y) {
+    const float fy = (float)y;
+    const float lo = fmaxf(s.box_lo, fmaxf(s.al0 + fy * s.al1, s.cl0 + fy * s.cl1));
+    const float hi = fminf(s.box_hi, fminf(s.ah0 + fy * s.al1, s.ch0 + fy * s.cl1));
+    const float cl = fmaxf(ceilf(lo - s.eps), 0.0f), ch = fminf(floorf(hi + s.eps), (float)(fx - 1u));
     if (!(cl <= ch)) continue;
-    const uint32_t b0 = y * fx + (uint32_t)cl, b1 = y * fx + (uint32_t)ch + 1u;  // bits [b0, b1)
+    const uint32_t b0 = (uint32_t)y * fx + (uint32_t)cl, b1 = (uint32_t)y * fx + (uint32_t)ch + 1u;  // bits [b0, b1)
     for (int j = 0; j < W; ++j) {
       const uint32_t lo_b = max(b0, 64u * j), hi_b = min(b1, 64u * j + 64u);
       if (lo_b >= hi_b) continue;

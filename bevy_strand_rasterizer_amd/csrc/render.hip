@@ -212,7 +212,7 @@ __device__ __forceinline__ void batch_columns(bool valid, float x0, float y0, fl
   const uint32_t k = threadIdx.x, w = k >> 6, l = k & 63u;
   uint64_t m[W];
   if (valid) {
-    span_mask<W>(span_setup(x0, y0, abx, aby, rmax), tx0, ty0, fx, fy, m);
+    span_mask<W>(span_setup(x0, y0, abx, aby, rmax, tx0, ty0, fy), fx, m);
   } else {
 #pragma unroll
     for (int j = 0; j < W; ++j) m[j] = 0ull;

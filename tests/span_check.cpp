@@ -82,10 +82,10 @@ static void run(std::mt19937& g, long cases, uint32_t fx, uint32_t fy, long* mis
     const float abx = x1 - x0, aby = y1 - y0;
     uint64_t ms[W], mc[W];
     // the kernels' span radii: the exact radius at the deeper endpoint
-    span_mask<W>(span_setup(x0, y0, abx, aby, span_radius(minr_s, maxr_s, fmaxf(z0, z1))), tx0, ty0, fx, fy, ms);
+    span_mask<W>(span_setup(x0, y0, abx, aby, span_radius(minr_s, maxr_s, fmaxf(z0, z1)), tx0, ty0, fy), fx, ms);
     const float minr_c = fmaxf(minr, 1e-4f), maxr_c = fmaxf(maxr, minr_c);
     const float rcam = span_radius(minr_c, maxr_c, span_depth01_max(z0, z1));
-    span_mask<W>(span_setup(x0, y0, abx, aby, rcam), tx0, ty0, fx, fy, mc);
+    span_mask<W>(span_setup(x0, y0, abx, aby, rcam, tx0, ty0, fy), fx, mc);
     for (uint32_t y = 0; y < fy; ++y)
       for (uint32_t x = 0; x < fx; ++x) {
         const uint32_t p = y * fx + x;
