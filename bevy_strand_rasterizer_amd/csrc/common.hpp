@@ -166,6 +166,9 @@ struct FrameParams {
   uint32_t* slot_hits;
   uint8_t* slot_hit_count;
   uint32_t hit_k, pad2;
+  // shadow raster: span columns of each workgroup's first batches, kept
+  // from its first pass for the second (kShadowColBytes per frame)
+  unsigned long long* shadow_cols;
   // shadow / shading / raster
   float* dom_depth;
   float* dom_opacity;

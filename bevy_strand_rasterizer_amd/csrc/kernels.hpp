@@ -49,6 +49,9 @@ struct PrepassScratch {
   uint32_t radix_blocks, radix_passes;
 };
 
+// Bytes of the shadow raster's column cache (FrameParams::shadow_cols).
+constexpr uint64_t kShadowColBytes = 64ull << 20;
+
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer);
 void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, hipStream_t s);

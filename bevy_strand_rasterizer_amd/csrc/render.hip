@@ -249,6 +249,11 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
   __shared__ unsigned long long cols[TPX][TPX / 64];
   const uint32_t R = fp.counters[C_SHADOW_RUNS];
   const uint32_t L = fp.dom_layers;
+  // this workgroup's slice of the column cache: kKeep batches of TPX pixels
+  // x TPX / 64 words; pixel p's words of batch k at [k][p][.]
+  constexpr uint32_t kWords = TPX / 64u;
+  const uint32_t kKeep = (uint32_t)(kShadowColBytes / (8ull * TPX * kWords * gridDim.x));
+  unsigned long long* keep = fp.shadow_cols + (size_t)blockIdx.x * kKeep * TPX * kWords;
   while (next_run<TPX>(&fp.counters[C_SHADOW_HEAD], R, t)) {
     const sr_raster_tile_run run = fp.shadow_runs[t.run];
     const uint32_t fi = run.frustum_id;
@@ -266,16 +271,21 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     // pass 1
     float z0 = -1.0f;
     bool found = !active, hit = false;
-    uint32_t cur = 0;
+    uint32_t cur = 0, kept = 0;
     // (references a pixel's column leaves out do not cover it: skipping them
     // changes no result, item boundaries included)
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
       if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
       __syncthreads();
-      const uint32_t nb = min((uint32_t)TPX, total - bs);
+      const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
       shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
       __syncthreads();
+      if (bi < kKeep) {  // each pixel keeps its own column words for pass 2
+#pragma unroll
+        for (uint32_t cw = 0; cw < kWords; ++cw) keep[((size_t)bi * TPX + pidx) * kWords + cw] = cols[pidx][cw];
+        kept = bi + 1u;
+      }
       for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !found; ++cw) {
         for (unsigned long long bits = cols[pidx][cw]; bits;) {
           const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
@@ -301,12 +311,13 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
         __syncthreads();
         if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
         __syncthreads();
-        const uint32_t nb = min((uint32_t)TPX, total - bs);
-        shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
+        const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
+        if (bi >= kept) shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
         __syncthreads();
         if (found) {
           for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u; ++cw)
-          for (unsigned long long bits = cols[pidx][cw]; bits;) {
+          for (unsigned long long bits = bi < kept ? keep[((size_t)bi * TPX + pidx) * kWords + cw] : cols[pidx][cw];
+               bits;) {
             const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
             bits &= bits - 1ull;
             float pz, cov;
