@@ -672,8 +672,27 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
         fp.slot_material[s] = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
         if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
           const uint32_t* h = fp.slot_hits + s;
+          // the first kPre cells and their page-table entries are loaded
+          // together, so their latencies overlap
+          constexpr uint32_t kPre = 8u;
+          uint32_t hv[kPre], he[kPre];
+#pragma unroll
+          for (uint32_t k = 0; k < kPre; ++k) hv[k] = k < nh ? h[(size_t)k * total] : 0u;
+#pragma unroll
+          for (uint32_t k = 0; k < kPre; ++k) he[k] = k < nh ? fp.page_table[hv[k] >> 8] : 0u;
           uint32_t prev = kInvalid, entry = 0u;
-          for (uint32_t k = 0; k < nh; ++k) {
+#pragma unroll
+          for (uint32_t k = 0; k < kPre; ++k) {
+            if (k >= nh) break;
+            const uint32_t pti = hv[k] >> 8;
+            if (pti != prev) {
+              prev = pti;
+              entry = he[k];
+              if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
+            }
+            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, hv[k] & 255u), (uint32_t)s);
+          }
+          for (uint32_t k = kPre; k < nh; ++k) {
             const uint32_t v = h[(size_t)k * total], pti = v >> 8;
             if (pti != prev) {
               prev = pti;
