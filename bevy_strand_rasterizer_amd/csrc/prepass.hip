@@ -656,30 +656,36 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
     const uint32_t t = fp.wave_offsets[s >> 6] + (uint32_t)__popcll(m & lower);
     const bool valid = ((m >> lane) & 1ull) && t < fp.task_cap;
     if (valid) {
+      // every load first (recorded cells up to kPre and their page-table
+      // entries, index clamped; values past the slot's count are unused),
+      // then the stores, so the latencies overlap
+      constexpr uint32_t kPre = 8u;
       uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
       uint32_t ii = find_instance(fp, pos);
       const InstRec& in = fp.inst[ii];
       uint32_t p = pos - in.pos_base;
       const FrustumRec& fr = fp.fr[fi];
-      fp.tasks[t] = sr_binning_task{ii, in.pos_strand[p], p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
+      const uint32_t strand = in.pos_strand[p];
       const sr_projected_segment ps = slot_proj[s];
+      uint32_t nh = kHitOverflow, base = 0u, mat = 0u;
+      uint32_t hv[kPre], he[kPre];
+      const uint32_t* h = fp.slot_hits + s;
+      if (csr) {
+        nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
+        base = fp.hit_base[s];
+        const uint32_t pt_last = fp.coarse_total * kSlices - 1u;
+#pragma unroll
+        for (uint32_t k = 0; k < kPre; ++k) hv[k] = k < fp.hit_k ? h[(size_t)k * total] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kPre; ++k) he[k] = fp.page_table[min(hv[k] >> 8, pt_last)];
+        mat = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
+      }
+      fp.tasks[t] = sr_binning_task{ii, strand, p, pack_binning_field(fr.d.kind == 1u ? 1u : 0u, fi)};
       fp.proj[t] = ps;
       if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
       if (csr) {  // K17: the slot's hits at its scanned offset
-        const uint32_t nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
-        const uint32_t base = fp.hit_base[s];
-        const uint32_t strand = in.pos_strand[p];
-        fp.slot_material[s] = strand < in.meta_count ? in.meta[strand].material_idx : 0u;
+        fp.slot_material[s] = mat;
         if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
-          const uint32_t* h = fp.slot_hits + s;
-          // the first kPre cells and their page-table entries are loaded
-          // together, so their latencies overlap
-          constexpr uint32_t kPre = 8u;
-          uint32_t hv[kPre], he[kPre];
-#pragma unroll
-          for (uint32_t k = 0; k < kPre; ++k) hv[k] = k < nh ? h[(size_t)k * total] : 0u;
-#pragma unroll
-          for (uint32_t k = 0; k < kPre; ++k) he[k] = k < nh ? fp.page_table[hv[k] >> 8] : 0u;
           uint32_t prev = kInvalid, entry = 0u;
 #pragma unroll
           for (uint32_t k = 0; k < kPre; ++k) {
