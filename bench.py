@@ -254,6 +254,52 @@ def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool 
     return out
 
 
+def run_stress(views: int, views_per_frame: int, backend: int, repeats: int = 2):
+    """The stress configuration in frames of `views_per_frame` 4K views plus
+    the 4 lights: each frame shape first renders until its capacities fit
+    (the runtime grows them after a frame reports capacity_exceeded), then
+    `repeats` passes over all views are timed."""
+    import torch
+
+    from bevy_strand_rasterizer_amd import scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    ws = [scene.retile(w, 0, 0, backend) for w in scene.config_stress_chunks(views=views, views_per_frame=views_per_frame)]
+    r = StrandRenderer(device=0)
+    r.upload(ws[0].assets[0])
+    grow = 0
+    for w in ws[:1]:
+        for _ in range(6):
+            ctr = r.render(w)
+            print(f"bench: stress warm {w.name}: refs {ctr['fine_refs']} / {ctr['ref_capacity']}, "
+                  f"exceeded {ctr['capacity_exceeded']}", file=sys.stderr, flush=True)
+            if not ctr["capacity_exceeded"]:
+                break
+            grow += 1
+    if ctr["capacity_exceeded"]:
+        r.close()
+        return {"error": "capacity not reached", "views_per_frame": views_per_frame, "counters": ctr}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exceeded = 0
+    for _ in range(repeats):
+        for w in ws:
+            r.submit(w)
+            c = r.wait()
+            exceeded += int(c["capacity_exceeded"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r.close()
+    segs = ws[0].input_segments
+    return {"workload": f"stress-1000k-x64-4k-{views}views-4lights-8layers", "views": views,
+            "views_per_frame": views_per_frame, "frames": len(ws) * repeats,
+            "fine_binning_backend": ["scatter", "csr"][backend],
+            "frames_per_s": round(views * repeats / dt, 3), "ms_per_view": round(dt * 1e3 / (views * repeats), 3),
+            "msegments_per_s": round(segs * views * repeats / dt / 1e6, 2), "warm_growths": grow,
+            "frames_over_capacity": exceeded,
+            "counters_last_frame": {k: v for k, v in c.items() if not k.startswith("pad")}}
+
+
 def run_serving(seconds: float, rate: float, slots: int, max_views: int, budget_gib: float, seed: int = 7):
     """Poisson arrivals at `rate` req/s for `seconds` over the 8 resident
     serving assets; each request's frame stays on the device (no host
@@ -319,6 +365,9 @@ def main():
     ap.add_argument("--report", default="", help="also write the reference bench's report (schema_version 1) here, "
                                                   "from --report-frames frames with telemetry on")
     ap.add_argument("--report-frames", type=int, default=10)
+    ap.add_argument("--stress", action="store_true",
+                    help="also run the stress configuration (1M x 64, 16 4K views, 4 lights x 8 layers)")
+    ap.add_argument("--stress-views-per-frame", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a plain copy kernel of 1 GiB read + 1 GiB written (torch.neg), the reference "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -389,6 +438,11 @@ def main():
                               args.serve_budget_gib)
         print(f"bench: serving {serving['frames_per_s']} frames/s", file=sys.stderr, flush=True)
 
+    stress = None
+    if args.stress:
+        stress = run_stress(16, args.stress_views_per_frame, 1 if args.backend == "csr" else 0)
+        print(f"bench: stress {stress.get('frames_per_s')} frames/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -424,6 +478,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "stress": stress,
     }
     print(json.dumps(out))
     if args.calibrate_copy:

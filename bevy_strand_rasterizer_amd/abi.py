@@ -24,6 +24,7 @@ MAX_DOM_LAYERS = 16
 BINNING_FRUSTUM_SHIFT = 3
 CONFIG_CSR_FEW_HITS = 2  # sr_device_config.flags
 CONFIG_HIP_GRAPHS = 4
+CONFIG_TIGHT_CAPACITY = 8
 
 # sr_status
 SR_OK = 0

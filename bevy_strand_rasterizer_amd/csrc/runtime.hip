@@ -541,7 +541,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if ((uint64_t)F * slots > 0x7FFFFFFFull) return efail(err, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
   const uint32_t task_cap = F * slots;
   const uint32_t ptab = coarse_total * kSlices;
-  uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, std::max(4096u, ptab / 2)), sl.page_cap_hint);
+  const bool tight = (ctx->flags & SR_CONFIG_TIGHT_CAPACITY) != 0u;
+  uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, tight ? 16u : std::max(4096u, ptab / 2)),
+                                         sl.page_cap_hint);
   page_cap = std::min(page_cap, ptab);
   // reference slab: 8 references per task slot (the reference sizes it at 4x
   // its binning queue, src/plugin.rs:540-546); work items: half of that.
@@ -549,12 +551,13 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   // The first frame of a shape starts at 8 per task slot, at most 2^28
   // references (3 GiB); a frame that overflows reports capacity_exceeded and
   // the next one gets the size it needed.
-  uint64_t ref_cap64 = std::max<uint64_t>({std::min<uint64_t>((uint64_t)task_cap * 8ull, 1ull << 28), 1ull << 16,
-                                           (uint64_t)sl.ref_cap_hint});
+  uint64_t ref_cap64 = std::max<uint64_t>({tight ? 4096ull : std::min<uint64_t>((uint64_t)task_cap * 8ull, 1ull << 28),
+                                           tight ? 4096ull : 1ull << 16, (uint64_t)sl.ref_cap_hint});
   const uint32_t ref_cap = (uint32_t)std::min<uint64_t>(ref_cap64, 0x7FFFFFFFull);
   // work items are non-empty cells: never more than the page capacity's cells
   const uint32_t work_cap = (uint32_t)std::max<uint64_t>(
-      {std::min<uint64_t>(ref_cap / 2u, (uint64_t)page_cap * kPage), 1u << 15, (uint64_t)sl.work_cap_hint});
+      {std::min<uint64_t>(ref_cap / 2u, (uint64_t)page_cap * kPage), tight ? 1024u : 1u << 15,
+       (uint64_t)sl.work_cap_hint});
   // page CSR backend: one (page * 256 + cell, slot) hit per visible fine
   // cell, at most ref_cap of them (a hit is a reference of a kept page)
   const bool csr = st.fine_binning_backend != 0u;

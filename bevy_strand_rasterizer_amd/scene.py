@@ -470,6 +470,23 @@ def config_batched_named(views: int = 8, seed: int = 0x5EEDCAFE) -> Workload:
                           resolutions=((1920, 1080),))
 
 
+def config_stress_chunks(views: int = 16, views_per_frame: int = 2, seed: int = 0x5EEDCAFE, strands: int = 1_000_000,
+                         points: int = 64, lights: int = 4, light_res: int = 2048, dom_layers: int = 8,
+                         width: int = 3840, height: int = 2160) -> list:
+    """The stress configuration: 1M strands x 64 points (63M segments), curl
+    radius up to 0.02 m, `views` 3840x2160 orbit cameras and 4 lights with
+    2048^2 x 8-layer deep shadow maps.  One frame of all views would need
+    more fine references than 32-bit offsets address, so the views come as
+    frames of `views_per_frame` cameras, each with every light; all frames
+    draw the same asset (id 0, upload it once)."""
+    dist = 0.6
+    asset = scalp_asset(strands, points, seed, 0.0006, radius_px_for(0.0006, height, dist), curl_max=0.02)
+    cams = [(orbit_eye(i, views, dist), width, height) for i in range(views)]
+    return [build_workload(f"stress-{strands // 1000}k-x{points}-views{i}-{i + views_per_frame - 1}", asset,
+                           cams[i:i + views_per_frame], lights, light_res, dom_layers)
+            for i in range(0, views, views_per_frame)]
+
+
 SERVING_SIZES = (10_000, 25_000, 50_000, 100_000, 150_000, 200_000, 300_000, 500_000)
 SERVING_RESOLUTIONS = ((256, 256), (512, 512), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
 

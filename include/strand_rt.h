@@ -472,6 +472,11 @@ typedef struct sr_device_config {
  * settings and launch extents); per-frame tables are uploaded before the
  * replay.  Ignored while profiling is on. */
 #define SR_CONFIG_HIP_GRAPHS 4u
+/* sr_device_config.flags: a frame slot's first frame of a shape starts with
+ * minimal reference, work-item and page capacities (4096, 1024 and 16), so it
+ * overflows and reports capacity_exceeded, and the next frame runs with the
+ * capacities it needed (exercises the growth path). */
+#define SR_CONFIG_TIGHT_CAPACITY 8u
 
 /* Library version string. */
 const char* sr_version(void);

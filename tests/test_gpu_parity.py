@@ -121,6 +121,26 @@ def test_parity_csr_paths(case, flags):
     compare(w, ctr, gpu, ref)
 
 
+@pytest.mark.parametrize("case", ["small-f8", "small-f8-csr"])
+def test_capacity_growth(case):
+    """A slot starting at minimal reference, work and page capacities: the
+    first frame overflows and says so, later frames of the same shape grow
+    to what they needed and then match the oracle exactly."""
+    w = {**CASES, **CSR_CASES}[case]()
+    print(f"[parity] capacity growth {case}: {w.name}", flush=True)
+    ref = oracle.render(w)
+    with StrandRenderer(flags=abi.CONFIG_TIGHT_CAPACITY) as r:
+        r.upload_workload(w)
+        exceeded = [r.render(w)["capacity_exceeded"]]
+        while exceeded[-1] and len(exceeded) < 5:
+            exceeded.append(r.render(w)["capacity_exceeded"])
+        ctr = r.render(w)
+        gpu = {k: r.dump(k) for k in EXACT + FLOAT}
+    assert exceeded[0] == 1 and exceeded[-1] == 0, exceeded
+    assert ctr["capacity_exceeded"] == 0
+    compare(w, ctr, gpu, ref)
+
+
 @pytest.mark.parametrize("case", list(CASES) + list(CSR_CASES))
 def test_parity(case):
     w = {**CASES, **CSR_CASES}[case]()
