@@ -87,7 +87,7 @@ enum Counter : uint32_t {
   C_CAMERA_HEAD = 11, // camera raster run queue head
   C_SHADOW_HEAD = 12, // shadow raster run queue head
   C_CANDS = 13,       // fine-cell hits of the page CSR backend
-  C_RADIX_LEN = 14,   // 256 x radix blocks holding hits (the digit scan length)
+  C_UNUSED_14 = 14,
   C_UNUSED_15 = 15,
   C_UNUSED_16 = 16,
   C_COUNT = 20

@@ -32,6 +32,9 @@ struct StageTimer {
 
 // Items per block of the page-CSR hit radix sort (256 threads x 16).
 constexpr uint32_t kRadixBlock = 256u * 16u;
+// Row chunks of the radix digit-base scan; its chunk sums (kDigitChunks x
+// 256 words) live in PrepassScratch::partials.
+constexpr uint32_t kDigitChunks = 128u;
 
 // Scratch owned by the runtime for one frame's prepass.
 struct PrepassScratch {
@@ -45,7 +48,7 @@ struct PrepassScratch {
   uint32_t* hkeys[2];
   uint32_t* hvals[2];
   uint32_t* radix_hist;             // [radix_blocks * 256] digit counts, block-major
-  uint32_t* radix_base;             // [256 * radix_blocks] output bases, digit-major
+  uint32_t* radix_base;             // [radix_blocks * 256] output bases, block-major
   uint32_t radix_blocks, radix_passes;
 };
 

@@ -838,12 +838,8 @@ struct HitTotalSrc {
 struct HitBaseDst {
   uint32_t* base;
   uint32_t* counters;
-  uint32_t cap;
   __device__ void store(uint32_t s, uint32_t ex, uint32_t) const { base[s] = ex; }
-  __device__ void total(uint32_t t) const {
-    counters[C_CANDS] = t;
-    counters[C_RADIX_LEN] = 256u * ((min(t, cap) + kRadixBlock - 1u) / kRadixBlock);
-  }
+  __device__ void total(uint32_t t) const { counters[C_CANDS] = t; }
 };
 
 // ---- stable LSD radix sort of (key, value) pairs, 8-bit digits
@@ -897,24 +893,69 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
   hist[b * 256u + threadIdx.x] = cnt[threadIdx.x];
 }
 
-// The exclusive scan runs over (digit, block) in digit-major order: every
-// block's output base of every digit.  Source index i = d * nb + b reads the
-// block-major counts; nb = blocks holding hits this frame.
-struct HistSrc {
-  const uint32_t* hist;
-  const uint32_t* n_dev;
-  uint32_t cap;
-  __device__ uint32_t load(uint32_t i) const {
-    const uint32_t nb = (radix_count(n_dev, cap) + kRadixBlock - 1u) / kRadixBlock;
-    const uint32_t d = i / nb, b = i - d * nb;
-    return hist[b * 256u + d];
+// Output bases of every (block, digit): base[b][d] = the items of smaller
+// digits + the items of digit d in earlier blocks.  The block-major counts
+// are scanned down their columns in kDigitChunks chunks of rows: chunk sums,
+// one workgroup that scans the chunks per digit and adds the digit offsets,
+// then each chunk writes its rows' bases.  Every access is a coalesced row.
+
+__device__ __forceinline__ uint32_t radix_nb(const uint32_t* n_dev, uint32_t cap) {
+  return (radix_count(n_dev, cap) + kRadixBlock - 1u) / kRadixBlock;
+}
+
+__global__ __launch_bounds__(256) void k_digit_chunk_sums(const uint32_t* hist, const uint32_t* n_dev, uint32_t cap,
+                                                         uint32_t* chunk) {
+  const uint32_t nb = radix_nb(n_dev, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
+  const uint32_t c = blockIdx.x, d = threadIdx.x, b0 = c * rows, b1 = min(nb, b0 + rows);
+  uint32_t sum = 0u;
+  for (uint32_t bl = b0; bl < b1; bl += 8u) {  // eight rows' loads in flight
+    uint32_t v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; ++j) v[j] = bl + j < b1 ? hist[(bl + j) * 256u + d] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; ++j) sum += v[j];
   }
-};
-struct HistDst {
-  uint32_t* base;
-  __device__ void store(uint32_t i, uint32_t ex, uint32_t) const { base[i] = ex; }
-  __device__ void total(uint32_t) const {}
-};
+  chunk[c * 256u + d] = sum;
+}
+
+__global__ __launch_bounds__(256) void k_digit_chunk_scan(uint32_t* chunk) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t d = threadIdx.x;
+  uint32_t run = 0u;
+  uint32_t ex[kDigitChunks / 16u][16];
+  for (uint32_t g = 0; g < kDigitChunks / 16u; ++g) {  // sixteen loads in flight
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) v[j] = chunk[(g * 16u + j) * 256u + d];
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) {
+      ex[g][j] = run;
+      run += v[j];
+    }
+  }
+  uint32_t total;
+  const uint32_t off = block_exclusive<uint32_t>(run, &total, lds4);
+  for (uint32_t g = 0; g < kDigitChunks / 16u; ++g)
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) chunk[(g * 16u + j) * 256u + d] = ex[g][j] + off;
+}
+
+__global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* hist, const uint32_t* n_dev, uint32_t cap,
+                                                    const uint32_t* chunk, uint32_t* base) {
+  const uint32_t nb = radix_nb(n_dev, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
+  const uint32_t c = blockIdx.x, d = threadIdx.x, b0 = c * rows, b1 = min(nb, b0 + rows);
+  uint32_t run = chunk[c * 256u + d];
+  for (uint32_t bl = b0; bl < b1; bl += 8u) {  // eight rows' loads, then their stores
+    uint32_t v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; ++j) v[j] = bl + j < b1 ? hist[(bl + j) * 256u + d] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; ++j) {
+      if (bl + j < b1) base[(bl + j) * 256u + d] = run;
+      run += v[j];
+    }
+  }
+}
 
 // Scatter: ranks by ballots, then the block's items are reordered by digit in
 // LDS and written out in that order, so consecutive lanes write consecutive
@@ -936,7 +977,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
   const unsigned long long lower = (1ull << lane) - 1ull;
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[q][t] = 0u;
-  gbase[t] = bases[t * nb + b];
+  gbase[t] = bases[b * 256u + t];
   __syncthreads();
   const uint32_t chunk = base + w * kRadixChunk;
   uint32_t key[kRadixRounds], val[kRadixRounds], rank[kRadixRounds];
@@ -1485,7 +1526,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                         ptab, nullptr, ps.partials, s);
   if (csr) {
     timer->mark(s, "hit_offsets");
-    launch_scan<uint32_t>(HitTotalSrc{fp.slot_hit_total}, HitBaseDst{fp.hit_base, fp.counters, fp.ref_cap}, (uint32_t)slots_total,
+    launch_scan<uint32_t>(HitTotalSrc{fp.slot_hit_total}, HitBaseDst{fp.hit_base, fp.counters}, (uint32_t)slots_total,
                           nullptr, ps.partials, s);
   }
   timer->mark(s, "project_write");
@@ -1500,8 +1541,11 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
       const int dst = k + 1u == ps.radix_passes ? 0 : (src ^ 1);
       hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters + C_CANDS,
                          fp.ref_cap, shift, ps.radix_hist);
-      launch_scan<uint32_t>(HistSrc{ps.radix_hist, fp.counters + C_CANDS, fp.ref_cap}, HistDst{ps.radix_base},
-                            256u * ps.radix_blocks, fp.counters + C_RADIX_LEN, ps.partials, s);
+      hipLaunchKernelGGL(k_digit_chunk_sums, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist,
+                         fp.counters + C_CANDS, fp.ref_cap, ps.partials);
+      hipLaunchKernelGGL(k_digit_chunk_scan, dim3(1), dim3(256), 0, s, ps.partials);
+      hipLaunchKernelGGL(k_digit_bases, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist, fp.counters + C_CANDS,
+                         fp.ref_cap, ps.partials, ps.radix_base);
       hipLaunchKernelGGL(k_radix_scatter, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], ps.hvals[src],
                          ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift, ps.radix_base);
     }

@@ -585,7 +585,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const uint32_t radix_blocks = csr ? (hit_cap + kRadixBlock - 1u) / kRadixBlock : 0u;
   const uint64_t scan_len = std::max<uint64_t>({(uint64_t)task_cap, (uint64_t)ptab, (uint64_t)tiles16,
                                                 256ull * radix_blocks, 1ull});
-  const uint32_t scan_tiles = (uint32_t)((scan_len + kScanTile - 1) / kScanTile + 1);
+  // scan tile partials; the radix digit-base scan keeps its chunk sums there too
+  const uint32_t scan_tiles = std::max<uint32_t>((uint32_t)((scan_len + kScanTile - 1) / kScanTile + 1),
+                                                 csr ? kDigitChunks * 256u : 0u);
 
   // history: keep the previous buffer when the atlas layout is unchanged
   const bool hist_valid = st.shadow_history_valid && sl.shape.valid && sl.shape.cams == cams &&
