@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU restatement (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's cpu_baseline leg.  The shipped library never loads it.
+reference result) and bench.py's cpu_baseline leg.  The shipped library never
+loads it.
 """
 from __future__ import annotations
 

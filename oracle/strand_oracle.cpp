@@ -1,6 +1,6 @@
 // strand_oracle.cpp -- CPU restatement of the reference's strand rendering
-// hot path, used ONLY as test infrastructure: the correctness checker of the
-// HIP kernels and the timed CPU baseline.  Nothing in the shipped library
+// hot path, used ONLY as test infrastructure: the correctness reference for
+// the HIP kernels and the timed CPU baseline.  Nothing in the shipped library
 // links, loads or calls this file.
 //
 // Every function below restates one WGSL function of the reference and cites
