@@ -450,8 +450,9 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
   return bessel5((cti * ctr) * iv) * fexp((sti * str) * iv);
 }
 
-// Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)); terms whose
-// exponent is below -110 are zero in FP32 and are not evaluated.
+// Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)) over the
+// window k in kc +- kmax around the nearest k; the caller sizes the window so
+// that every term left out is below e^-16 (1e-7 of the largest term).
 __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, int kmax) {
   float kc = rintf(pd * (float)(1.0 / (2.0 * 3.14159265359)));
   int k0 = max(-10, (int)kc - kmax), k1 = min(10, (int)kc + kmax);
@@ -492,8 +493,9 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
   bool narrow = var < 1e-6f;
   float norm = __builtin_amdgcn_rcpf(SQRT_2PI_F * beta);
   float neg_inv_2var = -__builtin_amdgcn_rcpf(2.0f * var);
-  // |d - 2 pi k| above sqrt(110 * 2 var) contributes exactly zero
-  int kmax = (int)ceilf(sqrtf(220.0f * var) / TWO_PI_F) + 1;
+  // a term with |d - 2 pi k| above r = sqrt(16 * 2 var) is below e^-16; those
+  // within r lie at |k - kc| <= (r + pi) / (2 pi)
+  int kmax = (int)ceilf((sqrtf(32.0f * var) + kPiF) / TWO_PI_F);
   V3 acc[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
   for (int i = 0; i < 10; ++i) {
     const LegPoint lp = leg[i];
