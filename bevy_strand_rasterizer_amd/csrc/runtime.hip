@@ -1003,6 +1003,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Segment slots (frusta x index positions) a served frame may hold.
+constexpr uint64_t kFrameSlots = 1ull << 27;
+
 struct PendingRequest {
   sr_request req;
   uint64_t ticket;
@@ -1226,7 +1229,22 @@ struct sr_server {
         if (queue.empty()) return launched;
         g.push_back(queue.front());
         queue.pop_front();
-        for (auto it = queue.begin(); it != queue.end() && g.size() < cfg.max_views_per_frame;) {
+        // views per frame: at most max_views_per_frame, and at most what
+        // keeps (views + lights) x the asset's index positions within
+        // kFrameSlots, so frames of large assets stay a modest share of HBM
+        uint32_t vmax = cfg.max_views_per_frame;
+        {
+          std::lock_guard<std::mutex> alk(ctx->mu);
+          const uint32_t id = g[0].req.instance.index_id;
+          if (id < ctx->assets.size() && ctx->assets[id].live) {
+            const uint64_t idx = std::max<uint64_t>(ctx->assets[id].info.index_count, 1);
+            const uint64_t frusta = kFrameSlots / idx;
+            vmax = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(vmax, frusta > g[0].req.light_count
+                                                                              ? frusta - g[0].req.light_count
+                                                                              : 1));
+          }
+        }
+        for (auto it = queue.begin(); it != queue.end() && g.size() < vmax;) {
           if (same_group(g[0].req, it->req)) {
             g.push_back(*it);
             it = queue.erase(it);
