@@ -865,8 +865,10 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) 
   return m;
 }
 
-// Per block holding hits: digit histogram (LDS atomics; all keys loaded
-// first), stored block-major (hist[b * 256 + d], one coalesced row per block).
+// Per block holding hits: digit histogram (all keys loaded first; per round
+// the wave's first two digit groups add their sizes with one LDS atomic each,
+// the other lanes one by one), stored block-major (hist[b * 256 + d], one
+// coalesced row per block).
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
                                                     uint32_t shift, uint32_t* hist) {
   __shared__ uint32_t cnt[256];
@@ -885,8 +887,22 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
     }
 #pragma unroll
     for (uint32_t k = 0; k < kRadixRounds; ++k) {
-      const bool has = chunk + k * 64u + lane < n;
-      if (has) atomicAdd(&cnt[(key[k] >> shift) & 255u], 1u);
+      bool pending = chunk + k * 64u + lane < n;
+      const uint32_t d = (key[k] >> shift) & 255u;
+      // the first two digit groups of the wave add their sizes at once
+      // (sorted keys repeat digits); the rest count one by one
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const unsigned long long act = __ballot(pending);
+        if (act == 0ull) break;
+        const int first = __ffsll((long long)act) - 1;
+        const uint32_t dl = (uint32_t)__shfl((int)d, first);
+        const bool same = pending && d == dl;
+        const unsigned long long grp = __ballot(same);
+        if ((int)lane == first) atomicAdd(&cnt[dl], (uint32_t)__popcll(grp));
+        if (same) pending = false;
+      }
+      if (pending) atomicAdd(&cnt[d], 1u);
     }
   }
   __syncthreads();
