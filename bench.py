@@ -359,6 +359,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-light-rate", type=float, default=100.0,
+                    help="a second serving run at this offered load (0: none)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-max-views", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=96.0)
@@ -437,6 +439,11 @@ def main():
         serving = run_serving(args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_max_views,
                               args.serve_budget_gib)
         print(f"bench: serving {serving['frames_per_s']} frames/s", file=sys.stderr, flush=True)
+        if args.serve_light_rate > 0:  # latency below saturation, beside the named stream's
+            serving["below_saturation"] = run_serving(args.serve_seconds, args.serve_light_rate, args.serve_slots,
+                                                      args.serve_max_views, args.serve_budget_gib, seed=11)
+            print(f"bench: serving at {args.serve_light_rate} req/s: p50/p99 "
+                  f"{serving['below_saturation']['latency_ms']}", file=sys.stderr, flush=True)
 
     stress = None
     if args.stress:
