@@ -143,9 +143,9 @@ struct FrameParams {
   // cell counts and offsets
   uint32_t* slot_hit_total;          // [F * slots] visible fine cells of each slot (0 when invalid)
   uint32_t* hit_base;                // [F * slots] exclusive scan of slot_hit_total
-  uint32_t* slot_material;           // [F * slots] material index of each valid slot's segment
+  uint32_t* slot_material;           // [slots] material index of the segment at each index position
   uint32_t* hkeys_w;                 // [ref_cap] page * 256 + cell of each hit as written
-  uint32_t* hvals_w;                 // [ref_cap] slot of each hit as written
+  uint32_t* hvals_w;                 // [ref_cap] index position of each hit's segment, as written
   const uint32_t* hkeys;             // [ref_cap] keys after the page sort
   uint32_t* page_run;                // [page_cap * 2] start, end of each page's sorted hits
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page

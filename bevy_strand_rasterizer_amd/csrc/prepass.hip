@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       fp.proj[t] = ps;
       if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
       if (csr) {  // K17: the slot's hits at its scanned offset
-        fp.slot_material[s] = mat;
+        fp.slot_material[pos] = mat;  // the same value from every frustum
         if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
           uint32_t prev = kInvalid, entry = 0u;
 #pragma unroll
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
               entry = he[k];
               if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
             }
-            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, hv[k] & 255u), (uint32_t)s);
+            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, hv[k] & 255u), pos);
           }
           for (uint32_t k = kPre; k < nh; ++k) {
             const uint32_t v = h[(size_t)k * total], pti = v >> 8;
@@ -705,12 +705,12 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
               entry = fp.page_table[pti];
               if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
             }
-            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, v & 255u), (uint32_t)s);
+            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, v & 255u), pos);
           }
         } else {  // too many cells to record: trace again
           V3 p0, p1;
           task_endpoints(fr.d, ps, &p0, &p1);
-          HitRec rec{nullptr, 0u, 0u, base, (uint32_t)s, 0u};
+          HitRec rec{nullptr, 0u, 0u, base, pos, 0u};
           visit_segment<MODE_SCATTER>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, p, 0u}, (uint32_t)s, rec);
         }
       }
@@ -826,11 +826,12 @@ __global__ __launch_bounds__(256) void k_page_active(FrameParams fp, const uint3
 // same references come out of a stable sort, without a device-scope atomic
 // (on this part those go to memory and serialise on hot pages): marking
 // counts each slot's visible fine cells, a scan gives every slot its offset,
-// the write pass stores one (page * 256 + cell, slot) hit per visible cell in
-// slot order, a stable LSD radix sort on the page bits groups the hits by
-// page (slot order kept), and one workgroup per page sorts its hits by cell
-// (stable) and writes the references -- the canonical order (page, cell,
-// slot).  The page-base scan is the segment-scatter backend's.
+// the write pass stores one (page * 256 + cell, index position) hit per
+// visible cell in slot order, a stable LSD radix sort on the page bits groups
+// the hits by page (slot order kept; a page belongs to one frustum, so the
+// index position names the segment), and one workgroup per page sorts its
+// hits by cell (stable) and writes the references -- the canonical order
+// (page, cell, slot).  The page-base scan is the segment-scatter backend's.
 struct HitTotalSrc {
   const uint32_t* totals;
   __device__ uint32_t load(uint32_t i) const { return totals[i]; }
@@ -1166,8 +1167,8 @@ __global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32
       for (uint32_t l = t; l < nstep; l += 256u) {
         const uint32_t d = lcell[l], pos = gbase[d] + (l - lstart[d]);
         if (pos >= fp.ref_cap) continue;
-        const uint32_t v = lval[l], slot = v % fp.slots, ii = find_instance(fp, slot);
-        fp.refs[pos] = sr_fine_seg_ref{ii, slot - fp.inst[ii].pos_base, fp.slot_material[v]};
+        const uint32_t v = lval[l], ii = find_instance(fp, v);  // v: the segment's index position
+        fp.refs[pos] = sr_fine_seg_ref{ii, v - fp.inst[ii].pos_base, fp.slot_material[v]};
       }
       __syncthreads();
     }
@@ -1549,7 +1550,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
     timer->mark(s, "sort_hits");
-    // stable LSD passes over the page bits of (key, slot) pairs; the last
+    // stable LSD passes over the page bits of (key, position) pairs; the last
     // leaves the page-sorted hits in pair 0
     int src = (ps.radix_passes & 1u) ? 1 : 0;  // the pair the hits were written to
     for (uint32_t k = 0; k < ps.radix_passes; ++k, src ^= 1) {
