@@ -23,6 +23,7 @@ constexpr uint32_t kPage = SR_COARSE_COUNT_PAGE_SIZE;   // 256 cells per count p
 constexpr uint32_t kQuant = SR_DEPTH_QUANT_MAX;
 constexpr uint32_t kInvalid = SR_INVALID_PTR;
 constexpr uint32_t kMarked = 0xFFFFFFFEu;
+constexpr uint32_t kOrderBlocks = 64u;  // blocks per run kind of the raster run-order sort
 
 // ------------------------------------------------------------- records
 // Per-frustum record built on the host for every frame.
@@ -136,6 +137,9 @@ struct FrameParams {
   sr_raster_work_item* work;         // [work_cap]
   sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
   sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
+  uint32_t* cam_order;               // [coarse_total * 16] camera runs, heaviest first
+  uint32_t* shadow_order;            // [coarse_total * 16] shadow runs, heaviest first
+  uint32_t* run_hist;                // [2 * kOrderBlocks * 32] run-order bucket counts
   // page CSR backend
   // page CSR backend: every (page, cell, slot) hit is written in slot order
   // at scanned offsets, then one stable radix sort by (page, cell) puts the

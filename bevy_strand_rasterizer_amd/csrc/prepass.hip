@@ -1399,6 +1399,92 @@ __global__ __launch_bounds__(256) void k_emit_write(FrameParams fp) {
   }
 }
 
+// Raster run order: each kind's runs by descending log2 of their load
+// (references), so a persistent raster starts the longest runs first.  A
+// counting sort over the 32 buckets in two launches of kOrderBlocks blocks
+// per kind: k_run_hist counts each block's chunk of runs per bucket,
+// k_run_scatter places them.  The order inside a bucket does not matter:
+// every run writes only its own tile.
+__device__ __forceinline__ uint32_t run_bucket(uint32_t load) { return 31u - (uint32_t)__clz(load | 1u); }
+
+struct RunChunk {
+  const sr_raster_tile_run* runs;
+  uint32_t* order;
+  uint32_t* hist;  // [kOrderBlocks][32] bucket counts of this kind
+  uint32_t blk, lo, hi;
+};
+
+__device__ __forceinline__ RunChunk run_chunk(const FrameParams& fp) {
+  const bool cam = blockIdx.x < kOrderBlocks;
+  RunChunk c;
+  c.runs = cam ? fp.cam_runs : fp.shadow_runs;
+  c.order = cam ? fp.cam_order : fp.shadow_order;
+  c.hist = fp.run_hist + (cam ? 0u : kOrderBlocks * 32u);
+  c.blk = blockIdx.x % kOrderBlocks;
+  const uint32_t n = fp.counters[cam ? C_CAM_RUNS : C_SHADOW_RUNS];
+  const uint32_t chunk = (n + kOrderBlocks - 1u) / kOrderBlocks;
+  c.lo = min(n, c.blk * chunk);
+  c.hi = min(n, c.lo + chunk);
+  return c;
+}
+
+// Adds this wave's runs [i0, hi) to the bucket counters cnt, one LDS atomic
+// per distinct bucket; with order set, also writes each run at its place
+// (the lanes sharing a bucket take consecutive places).
+__device__ __forceinline__ void run_bucket_add(const RunChunk& c, uint32_t i0, uint32_t* cnt, bool place) {
+  const uint32_t lane = threadIdx.x & 63u, i = i0 + threadIdx.x;
+  const bool valid = i < c.hi;
+  const uint32_t b = valid ? run_bucket(c.runs[i].load_score) : 0u;
+  uint64_t todo = __ballot(valid);
+  while (todo) {
+    const int first = __ffsll((long long)todo) - 1;
+    const uint32_t fb = (uint32_t)__shfl((int)b, first);
+    const uint64_t m = __ballot(valid && b == fb) & todo;
+    uint32_t at = 0u;
+    if ((int)lane == first) at = atomicAdd(&cnt[fb], (uint32_t)__popcll(m));
+    if (place) {
+      at = (uint32_t)__shfl((int)at, first);
+      if ((m >> lane) & 1ull) c.order[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+    todo &= ~m;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_run_hist(FrameParams fp) {
+  __shared__ uint32_t cnt[32];
+  const RunChunk c = run_chunk(fp);
+  if (threadIdx.x < 32u) cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  for (uint32_t i0 = c.lo; i0 < c.hi; i0 += blockDim.x) run_bucket_add(c, i0, cnt, false);
+  __syncthreads();
+  if (threadIdx.x < 32u) c.hist[c.blk * 32u + threadIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_run_scatter(FrameParams fp) {
+  __shared__ uint32_t cur[32], tot[32];
+  const RunChunk c = run_chunk(fp);
+  if (c.lo >= c.hi) return;
+  if (threadIdx.x < 32u) {  // bucket totals and this block's start inside its bucket
+    uint32_t t = 0u, before = 0u;
+#pragma unroll 8
+    for (uint32_t k = 0; k < kOrderBlocks; ++k) {
+      const uint32_t v = c.hist[k * 32u + threadIdx.x];
+      t += v;
+      before += k < c.blk ? v : 0u;
+    }
+    tot[threadIdx.x] = t;
+    cur[threadIdx.x] = before;
+  }
+  __syncthreads();
+  if (threadIdx.x < 32u) {  // heavier buckets come first
+    uint32_t heavier = 0u;
+    for (uint32_t k = threadIdx.x + 1u; k < 32u; ++k) heavier += tot[k];
+    cur[threadIdx.x] += heavier;
+  }
+  __syncthreads();
+  for (uint32_t i0 = c.lo; i0 < c.hi; i0 += blockDim.x) run_bucket_add(c, i0, cur, true);
+}
+
 // ------------------------------------------------------------ sequencer
 static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t max_blocks) {
   uint64_t g = (n + block - 1) / block;
@@ -1598,6 +1684,8 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
                   fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
   hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  hipLaunchKernelGGL(k_run_hist, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
+  hipLaunchKernelGGL(k_run_scatter, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
   if (fp.tel) {
     timer->mark(s, "telemetry");
     hipLaunchKernelGGL(k_telemetry, dim3(1024), dim3(256), 0, s, fp);

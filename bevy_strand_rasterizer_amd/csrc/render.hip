@@ -58,10 +58,16 @@ struct RunTable {
   uint32_t run;
 };
 
+// The workgroup's next tile run: tickets from `head` index `order` (runs
+// sorted heaviest first, so the longest runs start early and none is left
+// for the end).
 template <int TPX>
-__device__ bool next_run(uint32_t* head, uint32_t count, RunTable<TPX>& t) {
+__device__ bool next_run(uint32_t* head, uint32_t count, const uint32_t* order, RunTable<TPX>& t) {
   __syncthreads();
-  if (threadIdx.x == 0) t.run = atomicAdd(head, 1u);
+  if (threadIdx.x == 0) {
+    const uint32_t k = atomicAdd(head, 1u);
+    t.run = k < count ? order[k] : count;
+  }
   __syncthreads();
   return t.run < count;
 }
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
   constexpr uint32_t kWords = TPX / 64u;
   const uint32_t kKeep = (uint32_t)(kShadowColBytes / (8ull * TPX * kWords * gridDim.x));
   unsigned long long* keep = fp.shadow_cols + (size_t)blockIdx.x * kKeep * TPX * kWords;
-  while (next_run<TPX>(&fp.counters[C_SHADOW_HEAD], R, t)) {
+  while (next_run<TPX>(&fp.counters[C_SHADOW_HEAD], R, fp.shadow_order, t)) {
     const sr_raster_tile_run run = fp.shadow_runs[t.run];
     const uint32_t fi = run.frustum_id;
     const FrustumRec& fr = fp.fr[fi];
@@ -703,7 +709,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
   __shared__ ColorBatch<TPX> b;
   __shared__ unsigned long long cols[TPX][TPX / 64];
   const uint32_t R = fp.counters[C_CAM_RUNS];
-  while (next_run<TPX>(&fp.counters[C_CAMERA_HEAD], R, t)) {
+  while (next_run<TPX>(&fp.counters[C_CAMERA_HEAD], R, fp.cam_order, t)) {
     const sr_raster_tile_run run = fp.cam_runs[t.run];
     const uint32_t fi = run.frustum_id;
     const FrustumRec& fr = fp.fr[fi];
