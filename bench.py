@@ -80,10 +80,10 @@ def stage_bytes(ctr, w, refs=None):
         # reads keys for the digit histogram, key + slot for the scatter and
         # writes key + slot
         "sort_hits": passes * 20 * N,
-        # page runs of the sorted keys, then per page: keys for the cell
-        # histogram, key + slot + the slot's material for the stable cell
-        # sort, the 12-byte references, cell counts / offsets, page totals
-        "page_cells": 4 * N + 4 * N + 12 * N + 12 * N + 8 * C + 8 * P,
+        # per page: keys for the cell histogram, key + slot + the slot's
+        # material for the stable cell sort, the 12-byte references, cell
+        # counts / offsets, page totals (the page's bounds are a few probes)
+        "page_cells": 4 * N + 12 * N + 12 * N + 8 * C + 8 * P,
         "prefix": 8 * C + 32 * P,
         "fill": 28 * T + 4 * C + 12 * H,
         "sort_cells": 24 * H,

@@ -151,7 +151,6 @@ struct FrameParams {
   uint32_t* hkeys_w;                 // [ref_cap] page * 256 + cell of each hit as written
   uint32_t* hvals_w;                 // [ref_cap] index position of each hit's segment, as written
   const uint32_t* hkeys;             // [ref_cap] keys after the page sort
-  uint32_t* page_run;                // [page_cap * 2] start, end of each page's sorted hits
   uint32_t* page_xy_mask;            // [page_cap] fine tiles with work per page
   // fine cells each slot's segment hits, recorded while marking (CSR backend):
   // slot_hits[j * F * slots + s] = page-table entry << 8 | cell, in traversal

@@ -1045,35 +1045,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
   }
 }
 
-// Page runs of the page-sorted hits: [start, end) of every kept page (a
-// page without a visible hit keeps (0, 0)); dropped pages' hits sort last
-// and are skipped.  Four consecutive keys per thread, read as one vector.
-__global__ __launch_bounds__(256) void k_page_runs(FrameParams fp) {
-  const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
-  const uint32_t quads = (n + 3u) / 4u;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
-    const uint32_t i0 = 4u * q;
-    uint32_t pg[4];
-    if (i0 + 3u < n) {
-      const uint4 v = reinterpret_cast<const uint4*>(fp.hkeys)[q];
-      pg[0] = v.x >> 8; pg[1] = v.y >> 8; pg[2] = v.z >> 8; pg[3] = v.w >> 8;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) pg[j] = i0 + j < n ? fp.hkeys[i0 + j] >> 8 : ~0u;
-    }
-    const uint32_t prev = i0 ? fp.hkeys[i0 - 1u] >> 8 : ~0u;
-    const uint32_t next = i0 + 4u < n ? fp.hkeys[i0 + 4u] >> 8 : ~0u;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t i = i0 + j;
-      if (i >= n || pg[j] >= fp.page_cap) continue;
-      const uint32_t before = j ? pg[j - 1] : prev, after = j < 3 ? pg[j + 1] : next;
-      if (before != pg[j]) fp.page_run[2u * pg[j]] = i;
-      if (after != pg[j]) fp.page_run[2u * pg[j] + 1u] = i + 1u;
-    }
-  }
-}
-
 // K14 + the final order for the page-sorted hits, one workgroup per page:
 // the page's cell histogram, the page-local exclusive scan (cell counts,
 // offsets, page total, mask of fine tiles with work), then a stable counting
@@ -1081,7 +1052,11 @@ __global__ __launch_bounds__(256) void k_page_runs(FrameParams fp) {
 // strand_prepass.wgsl:1313-1327) at its final position.  Hits arrive in slot
 // order; kPageStep per step, wave w ranks the w-th quarter by ballots in
 // rounds of 64, so each cell keeps slot order = (inst, seg) order; the step
-// is reordered by cell in LDS before the references are written.
+// is reordered by cell in LDS before the references are written.  A page's
+// hits [start, end) are found first: the lower bounds of (page << 8) and
+// ((page + 1) << 8) in the sorted keys, each searched by half the workgroup
+// in rounds of a 128-way split (four rounds cover 2^28 keys).  Dropped
+// pages' hits (page >= page_cap) sort after every kept page.
 constexpr uint32_t kPageRounds = 8u;                    // rounds of 64 per wave and step
 constexpr uint32_t kPageStep = 4u * 64u * kPageRounds;  // hits per step
 
@@ -1091,12 +1066,46 @@ __global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32
   __shared__ uint32_t lstart[256], gbase[256];
   __shared__ uint32_t lcell[kPageStep], lval[kPageStep];  // the step's hits in cell order
   __shared__ uint32_t lds4[4];
+  __shared__ uint32_t bound[4];  // lo of both searches, then hi
   __shared__ uint32_t colmask;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const unsigned long long lower = (1ull << lane) - 1ull;
   const uint32_t P = fp.counters[C_PAGES];
+  const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
+  const uint32_t half = t >> 7, j = t & 127u;  // search half, its sample
   for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
-    const uint32_t start = fp.page_run[2u * page], end = fp.page_run[2u * page + 1u];
+    // the search keeps the lower bound in [lo, hi]; samples j < 127 split
+    // [lo, hi) evenly, below are monotone in j, and the last sample below
+    // and the first not below close the interval in
+    const uint64_t target = (uint64_t)(page + half) << 8;
+    if (t < 2u) {
+      bound[t] = 0u;
+      bound[2u + t] = n;
+    }
+    __syncthreads();
+    for (;;) {
+      const uint32_t lo = bound[half], hi = bound[2u + half];
+      const bool more = bound[0] != bound[2] || bound[1] != bound[3];
+      const bool sample = hi > lo && j < 127u;
+      uint32_t m = lo;
+      bool below = false;
+      if (sample) {
+        m = lo + (uint32_t)(((uint64_t)(hi - lo) * (j + 1u)) >> 7);
+        below = (uint64_t)keys[m] < target;
+      }
+      __syncthreads();
+      if (!more) break;
+      wcnt[0][t] = below ? 1u : 0u;
+      __syncthreads();
+      if (sample) {
+        const bool next_below = j < 126u && wcnt[0][t + 1u] != 0u;
+        const bool prev_below = j == 0u || wcnt[0][t - 1u] != 0u;
+        if (below && !next_below) bound[half] = m + 1u;
+        if (!below && prev_below) bound[2u + half] = m;
+      }
+      __syncthreads();
+    }
+    const uint32_t start = bound[0], end = bound[1];
     cur[t] = 0u;
     if (t == 0) colmask = 0u;
     __syncthreads();
@@ -1653,9 +1662,6 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                          ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift, ps.radix_base);
     }
     timer->mark(s, "page_cells");
-    launch_fill_u32(fp.page_run, 2ull * fp.page_cap, 0u, s);
-    hipLaunchKernelGGL(k_page_runs, dim3(grid_for(((uint64_t)fp.ref_cap + 3u) / 4u, 256, 1u << 20)), dim3(256), 0, s,
-                       fp);
     hipLaunchKernelGGL(k_page_cells, dim3(grid_for(fp.page_cap, 1, 1u << 20)), dim3(256), 0, s, fp, ps.hkeys[0],
                        ps.hvals[0]);
     timer->mark(s, "page_bases");

@@ -136,7 +136,7 @@ enum BufId {
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS, B_CAM_ORDER, B_SHADOW_ORDER, B_RUN_HIST,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SHADOW_COLS, B_SLOT_PROJ,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
-  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_PAGE_RUN, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
   B_COUNT
 };
 
@@ -612,7 +612,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap},
       {B_SLOT_HIT_TOTAL, csr ? 4ull * task_cap : 0ull}, {B_HIT_BASE, csr ? 4ull * task_cap : 0ull},
       {B_SLOT_MATERIAL, csr ? 4ull * task_cap : 0ull}, {B_HKEYS0, 4ull * hit_cap}, {B_HKEYS1, 4ull * hit_cap},
-      {B_HVALS0, 4ull * hit_cap}, {B_HVALS1, 4ull * hit_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks}, {B_RADIX_BASE, 4ull * 256u * radix_blocks}, {B_PAGE_RUN, csr ? 8ull * page_cap : 0ull},
+      {B_HVALS0, 4ull * hit_cap}, {B_HVALS1, 4ull * hit_cap}, {B_RADIX_HIST, 4ull * 256u * radix_blocks}, {B_RADIX_BASE, 4ull * 256u * radix_blocks},
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
@@ -741,7 +741,6 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.hkeys_w = ps.hkeys[(radix_passes & 1u) ? 1 : 0];
   fp.hvals_w = ps.hvals[(radix_passes & 1u) ? 1 : 0];
   fp.hkeys = ps.hkeys[0];
-  fp.page_run = sl.buf[B_PAGE_RUN].as<uint32_t>();
   fp.slot_proj = ps.slot_proj;
   fp.opaque_depth = opaque_floats ? sl.buf[B_OPAQUE_DEPTH].as<float>() : nullptr;
   fp.opaque_tiles = opaque_floats ? sl.buf[B_OPAQUE_TILES].as<uint2>() : nullptr;
