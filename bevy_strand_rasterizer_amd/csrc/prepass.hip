@@ -1310,8 +1310,7 @@ struct ActiveDst {
 };
 
 // ------------------------------------------------------ K23 emission
-__device__ __forceinline__ uint32_t raster_depth_key(const FrameParams& fp, uint32_t pti, uint32_t fz) {
-  const sr_coarse_depth_lut_entry e = fp.lut[pti];
+__device__ __forceinline__ uint32_t raster_depth_key(const sr_coarse_depth_lut_entry& e, uint32_t fz) {
   if (e.virtual_count_q == 0u) return fz;
   return e.virtual_start_q + (e.virtual_count_q * fz) / kSlices;
 }
@@ -1320,39 +1319,57 @@ struct TileLane {
   uint32_t count, base, page, cell, pti, fz;
 };
 
-// Lane l of a 256-lane workgroup serves depth lane 255 - l, so lanes run
-// near (large keys) to far.
-__device__ __forceinline__ TileLane tile_lane(const FrameParams& fp, uint32_t tile, uint32_t lx, uint32_t ly) {
-  TileLane tl{0u, 0u, kInvalid, 0u, 0u, 0u};
-  uint32_t dl = (kPage - 1u) - threadIdx.x;
-  uint32_t cz = dl / kSlices, fz = dl % kSlices;
-  uint32_t pti = tile * kSlices + cz;
-  tl.pti = pti; tl.fz = fz;
-  uint32_t h = fp.page_table[pti];
-  if (h == 0u || h == kMarked) return tl;
-  uint32_t page = h - 1u;
-  const sr_fine_page_meta& pm = fp.page_meta[page];
-  uint32_t cell = (fz * kExt + ly) * kExt + lx;
-  tl.page = page; tl.cell = cell;
-  if (!(pm.seg_ref_count > 0u && pm.flags == 0u)) return tl;
-  uint32_t n = fp.cell_counts[page * kPage + cell];
-  if (n == 0u) return tl;
-  tl.count = n;
-  tl.base = pm.seg_ref_base + fp.cell_offsets[page * kPage + cell];
-  return tl;
+// A fine tile's 256 depth lanes, four per thread of a wave: lane
+// l = 4 * thread + i serves depth lane 255 - l, so lanes run near (large
+// keys) to far.  A thread's four depth lanes share one page (a page holds
+// the 16 fine slices of a LUT slice), so its page-table and page-meta words
+// are read once and the four cells' counts and offsets are read together.
+constexpr uint32_t kEmitPer = kPage / 64u;
+
+__device__ __forceinline__ void tile_lanes(const FrameParams& fp, uint32_t tile, uint32_t lx, uint32_t ly,
+                                           TileLane (&tl)[kEmitPer]) {
+  const uint32_t dl0 = (kPage - 1u) - threadIdx.x * kEmitPer;
+  const uint32_t pti = tile * kSlices + dl0 / kSlices;
+  const uint32_t h = fp.page_table[pti];
+  const bool paged = h != 0u && h != kMarked;
+  const uint32_t page = paged ? h - 1u : 0u;
+  uint32_t ref_base = 0u;
+  bool ok = false;
+  if (paged) {
+    const sr_fine_page_meta& pm = fp.page_meta[page];
+    ref_base = pm.seg_ref_base;
+    ok = pm.seg_ref_count > 0u && pm.flags == 0u;
+  }
+  uint32_t n[kEmitPer], off[kEmitPer];
+#pragma unroll
+  for (uint32_t i = 0; i < kEmitPer; ++i) {
+    const uint32_t fz = (dl0 - i) % kSlices, cell = (fz * kExt + ly) * kExt + lx;
+    tl[i] = TileLane{0u, 0u, paged ? page : kInvalid, paged ? cell : 0u, pti, fz};
+    n[i] = ok ? fp.cell_counts[page * kPage + cell] : 0u;
+    off[i] = ok ? fp.cell_offsets[page * kPage + cell] : 0u;
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kEmitPer; ++i) {
+    tl[i].count = n[i];
+    tl[i].base = ref_base + off[i];
+  }
 }
 
-__global__ __launch_bounds__(256) void k_emit_count(FrameParams fp) {
-  __shared__ uint32_t lds4[4];
-  __shared__ uint32_t lds4b[4];
+__global__ __launch_bounds__(64) void k_emit_count(FrameParams fp) {
   const uint32_t A = fp.counters[C_ACTIVE];
   for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
     uint32_t at = fp.active_queue[a];
     uint32_t tile = at / (kExt * kExt), lf = at % (kExt * kExt);
-    TileLane tl = tile_lane(fp, tile, lf % kExt, lf / kExt);
-    uint32_t works, load;
-    (void)block_exclusive<uint32_t>(tl.count ? 1u : 0u, &works, lds4);
-    (void)block_exclusive<uint32_t>(tl.count, &load, lds4b);
+    TileLane tl[kEmitPer];
+    tile_lanes(fp, tile, lf % kExt, lf / kExt, tl);
+    uint32_t works = 0u, load = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kEmitPer; ++i) {
+      works += tl[i].count ? 1u : 0u;
+      load += tl[i].count;
+    }
+    works = (uint32_t)__shfl((int)wave_inclusive<uint32_t>(works), 63);
+    load = (uint32_t)__shfl((int)wave_inclusive<uint32_t>(load), 63);
     if (threadIdx.x == 0) {
       uint32_t fi = frustum_for_coarse_tile(fp, tile);
       uint32_t kind = fi == kInvalid ? 2u : fp.fr[fi].d.kind;
@@ -1377,8 +1394,7 @@ struct TileBaseDst {
   }
 };
 
-__global__ __launch_bounds__(256) void k_emit_write(FrameParams fp) {
-  __shared__ uint32_t lds4[4];
+__global__ __launch_bounds__(64) void k_emit_write(FrameParams fp) {
   const uint32_t A = fp.counters[C_ACTIVE];
   for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
     uint4 tw = fp.tile_work[a];
@@ -1392,14 +1408,21 @@ __global__ __launch_bounds__(256) void k_emit_write(FrameParams fp) {
     uint32_t lc = tile - fr.d.coarse_depth_tile_base;
     uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
     uint32_t screen_tile = (cy * kExt + ly) * fr.fine_tiles_x + (cx * kExt + lx);
-    TileLane tl = tile_lane(fp, tile, lx, ly);
-    uint32_t total;
-    uint32_t rank = block_exclusive<uint32_t>(tl.count ? 1u : 0u, &total, lds4);
-    if (tl.count) {
-      uint32_t w = tb.x + rank;
+    TileLane tl[kEmitPer];
+    tile_lanes(fp, tile, lx, ly, tl);
+    uint32_t mine = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kEmitPer; ++i) mine += tl[i].count ? 1u : 0u;
+    uint32_t w = tb.x + wave_inclusive<uint32_t>(mine) - mine;  // lane order
+    const sr_coarse_depth_lut_entry e = fp.lut[tl[0].pti];      // the thread's one page slice
+#pragma unroll
+    for (uint32_t i = 0; i < kEmitPer; ++i) {
+      if (!tl[i].count) continue;
       if (w < fp.work_cap)
-        fp.work[w] = sr_raster_work_item{tl.base, tl.count, fi, screen_tile, raster_depth_key(fp, tl.pti, tl.fz),
-                                         tl.count, tl.page, tl.cell};
+        fp.work[w] = sr_raster_work_item{tl[i].base, tl[i].count, fi, screen_tile,
+                                         raster_depth_key(e, tl[i].fz), tl[i].count, tl[i].page,
+                                         tl[i].cell};
+      ++w;
     }
     if (threadIdx.x == 0) {
       sr_raster_tile_run run{tb.x, tw.x, fi, screen_tile, tw.w, 0u, 0u, 0u};
@@ -1686,10 +1709,10 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   launch_scan<uint32_t>(FlagSrc{fp.active_flags}, ActiveDst{fp.active_queue, fp.counters},
                         fp.coarse_total * kExt * kExt, nullptr, ps.partials, s);
   timer->mark(s, "emit");
-  hipLaunchKernelGGL(k_emit_count, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  hipLaunchKernelGGL(k_emit_count, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, 4u * kMaxGrid)), dim3(64), 0, s, fp);
   launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
                   fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
-  hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, kMaxGrid)), dim3(256), 0, s, fp);
+  hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, 4u * kMaxGrid)), dim3(64), 0, s, fp);
   hipLaunchKernelGGL(k_run_hist, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
   hipLaunchKernelGGL(k_run_scatter, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
   if (fp.tel) {
