@@ -137,6 +137,7 @@ struct FrameParams {
   sr_raster_work_item* work;         // [work_cap]
   sr_raster_tile_run* cam_runs;      // [coarse_total * 16]
   sr_raster_tile_run* shadow_runs;   // [coarse_total * 16]
+  float4* slot_screen;               // [F * slots][2] raw screen x, y, z, w of each valid slot's endpoints
   uint32_t* cam_order;               // [coarse_total * 16] camera runs, heaviest first
   uint32_t* shadow_order;            // [coarse_total * 16] shadow runs, heaviest first
   uint32_t* run_hist;                // [2 * kOrderBlocks * 32] run-order bucket counts

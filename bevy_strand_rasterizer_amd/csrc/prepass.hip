@@ -155,8 +155,10 @@ __device__ __forceinline__ uint32_t find_instance(const FrameParams& fp, uint32_
 
 // Evaluates one (frustum, index position) slot.  Returns true and the
 // packed endpoints when the segment survives culling and clipping.
+// scr receives the endpoints' raw screen coordinates as the rasterizers
+// compute them (world point with w = 1 through world_to_screen_raw).
 __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, uint32_t* ii_out, uint32_t* strand_out,
-                             uint32_t* seg_out, sr_projected_segment* out) {
+                             uint32_t* seg_out, sr_projected_segment* out, float4* scr) {
   if (fp.NI == 0u) return false;
   const FrustumRec& fr = fp.fr[fi];
   if (!fr.enabled) return false;
@@ -189,6 +191,10 @@ __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, u
   out->p0_xy = pack2x16unorm(wclamp(c.p0.x / ssx, 0.0f, 1.0f), wclamp(c.p0.y / ssy, 0.0f, 1.0f));
   out->p1_xy = pack2x16unorm(wclamp(c.p1.x / ssx, 0.0f, 1.0f), wclamp(c.p1.y / ssy, 0.0f, 1.0f));
   out->depths = pack2x16unorm(wclamp(c.p0.z, 0.0f, 1.0f), wclamp(c.p1.z, 0.0f, 1.0f));
+  const V4 s0 = p0w.w == 1.0f ? r0 : world_to_screen_raw(V4{p0w.x, p0w.y, p0w.z, 1.0f}, fr.clip, vw, vh);
+  const V4 s1 = p1w.w == 1.0f ? r1 : world_to_screen_raw(V4{p1w.x, p1w.y, p1w.z, 1.0f}, fr.clip, vw, vh);
+  scr[0] = make_float4(s0.x, s0.y, s0.z, s0.w);
+  scr[1] = make_float4(s1.x, s1.y, s1.z, s1.w);
   *ii_out = ii;
   *strand_out = strand;
   *seg_out = p;
@@ -596,10 +602,13 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
     uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
-    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps);
+    float4 scr[2];
+    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps, scr);
     uint32_t hits = 0;
     if (ok) {
       slot_proj[s] = ps;
+      fp.slot_screen[2 * s] = scr[0];
+      fp.slot_screen[2 * s + 1] = scr[1];
       // K10: mark the (coarse tile, LUT slice) pages the segment touches,
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;

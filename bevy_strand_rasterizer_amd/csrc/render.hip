@@ -142,18 +142,25 @@ struct ShadowBatch {
   uint32_t item[TPX];
 };
 
+// The endpoints' raw screen coordinates of a reference's segment in frustum
+// fi (world_v3 then world_to_screen_raw, w = 1), as the projection pass
+// stored them for the segment's slot.
+__device__ __forceinline__ void ref_screen(const FrameParams& fp, uint32_t fi, const InstRec& in, uint32_t seg,
+                                           float4* a, float4* c) {
+  const float4* sc = fp.slot_screen + 2ull * ((size_t)fi * fp.slots + in.pos_base + seg);
+  *a = sc[0];
+  *c = sc[1];
+}
+
 template <int TPX>
-__device__ void load_shadow_ref(const FrameParams& fp, const FrustumRec& fr, const RunTable<TPX>& t, uint32_t i,
+__device__ void load_shadow_ref(const FrameParams& fp, uint32_t fi, const RunTable<TPX>& t, uint32_t i,
                                 ShadowBatch<TPX>& b) {
   const uint32_t k = threadIdx.x;
   const uint32_t w = item_of(t, i);
   const sr_fine_seg_ref sr = fp.refs[t.base[w] + (i - t.pref[w])];
   const InstRec& in = fp.inst[sr.inst_id];
-  V3 a = world_v3(in, in.indices[sr.seg_id]);
-  V3 c = world_v3(in, in.indices[sr.seg_id + 1u]);
-  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
-  V4 ra = world_to_screen_raw(V4{a.x, a.y, a.z, 1.0f}, fr.clip, vw, vh);
-  V4 rc = world_to_screen_raw(V4{c.x, c.y, c.z, 1.0f}, fr.clip, vw, vh);
+  float4 ra, rc;
+  ref_screen(fp, fi, in, sr.seg_id, &ra, &rc);
   const sr_strand_material& m = in.materials[sr.material_idx];
   float minr = fmaxf(m.min_radius_pixels, 1e-4f);
   float z0 = normalize_depth01(ra.z), z1 = normalize_depth01(rc.z);
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     // changes no result, item boundaries included)
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
-      if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
+      if (bs + threadIdx.x < total) load_shadow_ref(fp, fi, t, bs + threadIdx.x, b);
       __syncthreads();
       const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
       shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     if (__syncthreads_count(found) != 0) {
       for (uint32_t bs = 0; bs < total; bs += TPX) {
         __syncthreads();
-        if (bs + threadIdx.x < total) load_shadow_ref(fp, fr, t, bs + threadIdx.x, b);
+        if (bs + threadIdx.x < total) load_shadow_ref(fp, fi, t, bs + threadIdx.x, b);
         __syncthreads();
         const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
         if (bi >= kept) shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
@@ -672,17 +679,14 @@ struct ColorBatch {
 };
 
 template <int TPX>
-__device__ void load_color_ref(const FrameParams& fp, const FrustumRec& fr, const RunTable<TPX>& t, uint32_t i,
-                               ColorBatch<TPX>& b) {
+__device__ void load_color_ref(const FrameParams& fp, const FrustumRec& fr, uint32_t fi, const RunTable<TPX>& t,
+                               uint32_t i, ColorBatch<TPX>& b) {
   const uint32_t k = threadIdx.x;
   const uint32_t w = item_of(t, i);
   const sr_fine_seg_ref sr = fp.refs[t.base[w] + (i - t.pref[w])];
   const InstRec& in = fp.inst[sr.inst_id];
-  const float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
-  V3 a3 = world_v3(in, in.indices[sr.seg_id]);
-  V3 c3 = world_v3(in, in.indices[sr.seg_id + 1u]);
-  V4 a = world_to_screen_raw(V4{a3.x, a3.y, a3.z, 1.0f}, fr.clip, vw, vh);
-  V4 c = world_to_screen_raw(V4{c3.x, c3.y, c3.z, 1.0f}, fr.clip, vw, vh);
+  float4 a, c;
+  ref_screen(fp, fi, in, sr.seg_id, &a, &c);
   b.item[k] = w | ((a.x < 0.0f && c.x < 0.0f) ? 0x80000000u : 0u);
   if (a.x < 0.0f && c.x < 0.0f) return;
   const sr_strand_material& m = in.materials[sr.material_idx];
@@ -728,7 +732,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
     uint32_t cur = 0;
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
-      if (bs + threadIdx.x < total) load_color_ref(fp, fr, t, bs + threadIdx.x, b);
+      if (bs + threadIdx.x < total) load_color_ref(fp, fr, fi, t, bs + threadIdx.x, b);
       __syncthreads();
       {  // references a pixel's column leaves out do not cover it
         const uint32_t nb = min((uint32_t)TPX, total - bs), k = threadIdx.x;
