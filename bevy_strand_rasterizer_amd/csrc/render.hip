@@ -152,24 +152,58 @@ __device__ __forceinline__ void ref_screen(const FrameParams& fp, uint32_t fi, c
   *c = sc[1];
 }
 
+// A reference of a shadow batch is fetched in two stages, a batch apart, so
+// a batch's loads are in flight while the batch before it is rasterised:
+// its record (stage 1), then its segment's screen endpoints and material
+// (stage 2), which put_shadow_ref turns into the batch's LDS records.
+struct ShadowRefAddr {
+  sr_fine_seg_ref sr;
+  uint32_t w;
+  bool has;
+};
+struct ShadowRefData {
+  float4 a, c;
+  float minr, maxr, alpha;
+  uint32_t w;
+  bool has;
+};
+
 template <int TPX>
-__device__ void load_shadow_ref(const FrameParams& fp, uint32_t fi, const RunTable<TPX>& t, uint32_t i,
-                                ShadowBatch<TPX>& b) {
+__device__ __forceinline__ ShadowRefAddr shadow_ref_addr(const FrameParams& fp, const RunTable<TPX>& t, uint32_t i,
+                                                         uint32_t total) {
+  ShadowRefAddr r{sr_fine_seg_ref{0u, 0u, 0u}, 0u, i < total};
+  if (r.has) {
+    r.w = item_of(t, i);
+    r.sr = fp.refs[t.base[r.w] + (i - t.pref[r.w])];
+  }
+  return r;
+}
+
+__device__ __forceinline__ ShadowRefData shadow_ref_data(const FrameParams& fp, uint32_t fi, const ShadowRefAddr& r) {
+  ShadowRefData d{make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0.0f, 0.0f, r.w,
+                  r.has};
+  if (r.has) {
+    const InstRec& in = fp.inst[r.sr.inst_id];
+    ref_screen(fp, fi, in, r.sr.seg_id, &d.a, &d.c);
+    const sr_strand_material& m = in.materials[r.sr.material_idx];
+    d.minr = m.min_radius_pixels;
+    d.maxr = m.max_radius_pixels;
+    d.alpha = m.absorption_color[3];
+  }
+  return d;
+}
+
+template <int TPX>
+__device__ __forceinline__ void put_shadow_ref(const ShadowRefData& d, ShadowBatch<TPX>& b) {
   const uint32_t k = threadIdx.x;
-  const uint32_t w = item_of(t, i);
-  const sr_fine_seg_ref sr = fp.refs[t.base[w] + (i - t.pref[w])];
-  const InstRec& in = fp.inst[sr.inst_id];
-  float4 ra, rc;
-  ref_screen(fp, fi, in, sr.seg_id, &ra, &rc);
-  const sr_strand_material& m = in.materials[sr.material_idx];
-  float minr = fmaxf(m.min_radius_pixels, 1e-4f);
-  float z0 = normalize_depth01(ra.z), z1 = normalize_depth01(rc.z);
-  float abx = rc.x - ra.x, aby = rc.y - ra.y;
+  float minr = fmaxf(d.minr, 1e-4f);
+  float z0 = normalize_depth01(d.a.z), z1 = normalize_depth01(d.c.z);
+  float abx = d.c.x - d.a.x, aby = d.c.y - d.a.y;
   float l2 = abx * abx + aby * aby;  // divided per pixel: t = dot / l2 as the reference
-  b.a[k] = make_float4(ra.x, ra.y, z0, l2);
-  b.c[k] = make_float4(rc.x, rc.y, z1, m.absorption_color[3]);
-  b.r[k] = make_float4(minr, fmaxf(m.max_radius_pixels, minr), abx, aby);
-  b.item[k] = w;
+  b.a[k] = make_float4(d.a.x, d.a.y, z0, l2);
+  b.c[k] = make_float4(d.c.x, d.c.y, z1, d.alpha);
+  b.r[k] = make_float4(minr, fmaxf(d.maxr, minr), abx, aby);
+  b.item[k] = d.w;
 }
 
 // fragment_position_line_relative + coverage (strand_rasterizer.wgsl:138-147,
@@ -287,9 +321,12 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     uint32_t cur = 0, kept = 0;
     // (references a pixel's column leaves out do not cover it: skipping them
     // changes no result, item boundaries included)
+    ShadowRefAddr ra = shadow_ref_addr(fp, t, threadIdx.x, total);
+    ShadowRefData rd = shadow_ref_data(fp, fi, ra);
+    ra = shadow_ref_addr(fp, t, TPX + threadIdx.x, total);
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
-      if (bs + threadIdx.x < total) load_shadow_ref(fp, fi, t, bs + threadIdx.x, b);
+      if (rd.has) put_shadow_ref(rd, b);
       __syncthreads();
       const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
       shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
@@ -299,6 +336,8 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
         for (uint32_t cw = 0; cw < kWords; ++cw) keep[((size_t)bi * TPX + pidx) * kWords + cw] = cols[pidx][cw];
         kept = bi + 1u;
       }
+      rd = shadow_ref_data(fp, fi, ra);                             // the next batch
+      ra = shadow_ref_addr(fp, t, bs + 2u * TPX + threadIdx.x, total);  // the one after it
       for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u && !found; ++cw) {
         for (unsigned long long bits = cols[pidx][cw]; bits;) {
           const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
@@ -320,17 +359,26 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     for (uint32_t i = 0; i < L; ++i) part_px[i * TPX] = 0.0f;
     const float inv_span = 1.0f / fmaxf(1e-6f, found ? z0 : 1.0f);
     if (__syncthreads_count(found) != 0) {
+      ShadowRefAddr ra = shadow_ref_addr(fp, t, threadIdx.x, total);
+      ShadowRefData rd = shadow_ref_data(fp, fi, ra);
+      ra = shadow_ref_addr(fp, t, TPX + threadIdx.x, total);
       for (uint32_t bs = 0; bs < total; bs += TPX) {
         __syncthreads();
-        if (bs + threadIdx.x < total) load_shadow_ref(fp, fi, t, bs + threadIdx.x, b);
+        if (rd.has) put_shadow_ref(rd, b);
         __syncthreads();
         const uint32_t nb = min((uint32_t)TPX, total - bs), bi = bs / TPX;
         if (bi >= kept) shadow_columns<TPX>(b, nb, tx * fx, ty * fy, fx, fy, cols);
         __syncthreads();
+        unsigned long long kw[kWords];  // this pixel's column words, read before the next batch's loads
+#pragma unroll
+        for (uint32_t cw = 0; cw < kWords; ++cw)
+          kw[cw] = bi < kept ? keep[((size_t)bi * TPX + pidx) * kWords + cw] : cols[pidx][cw];
+        rd = shadow_ref_data(fp, fi, ra);
+        ra = shadow_ref_addr(fp, t, bs + 2u * TPX + threadIdx.x, total);
         if (found) {
-          for (uint32_t cw = 0; cw < (uint32_t)TPX / 64u; ++cw)
-          for (unsigned long long bits = bi < kept ? keep[((size_t)bi * TPX + pidx) * kWords + cw] : cols[pidx][cw];
-               bits;) {
+#pragma unroll
+          for (uint32_t cw = 0; cw < kWords; ++cw)
+          for (unsigned long long bits = kw[cw]; bits;) {
             const uint32_t k = cw * 64u + (uint32_t)(__ffsll((long long)bits) - 1);
             bits &= bits - 1ull;
             float pz, cov;
