@@ -257,9 +257,20 @@ SR_HD int32_t sgn_i32(float f) { return f > 1e-6f ? 1 : (f < -1e-6f ? -1 : 0); }
 
 struct Clipped { bool ok; V3 p0, p1; };
 
-SR_HD bool clip_axis(float p, float d, float mn, float mx, float& tmin, float& tmax) {
+// A segment's direction and its componentwise reciprocal, computed once for
+// every box the segment is clipped against (the reciprocal of a component
+// below 1e-6 in magnitude is never used).
+struct SegDir { V3 d, inv; };
+
+SR_HD SegDir seg_dir(V3 p0, V3 p1) {
+  SegDir s;
+  s.d = V3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+  s.inv = V3{1.0f / s.d.x, 1.0f / s.d.y, 1.0f / s.d.z};
+  return s;
+}
+
+SR_HD bool clip_axis(float p, float d, float inv, float mn, float mx, float& tmin, float& tmax) {
   if (fabsf(d) < 1e-6f) return !(p < mn || p > mx);
-  float inv = 1.0f / d;
   float a = (mn - p) * inv, b = (mx - p) * inv;
   if (a > b) { float t = a; a = b; b = t; }
   float omin = fmaxf(tmin, a), omax = fminf(tmax, b);
@@ -267,18 +278,22 @@ SR_HD bool clip_axis(float p, float d, float mn, float mx, float& tmin, float& t
   return !(omin > omax);
 }
 
-SR_HD Clipped clip_segment_to_box(V3 p0, V3 p1, V3 bmin, V3 bmax) {
-  V3 d{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+SR_HD Clipped clip_segment_dir(V3 p0, V3 p1, const SegDir& s, V3 bmin, V3 bmax) {
+  const V3 d = s.d;
   float tmin = 0.0f, tmax = 1.0f;
   Clipped c;
   c.ok = false; c.p0 = p0; c.p1 = p1;
-  if (!clip_axis(p0.x, d.x, bmin.x, bmax.x, tmin, tmax)) return c;
-  if (!clip_axis(p0.y, d.y, bmin.y, bmax.y, tmin, tmax)) return c;
-  if (!clip_axis(p0.z, d.z, bmin.z, bmax.z, tmin, tmax)) return c;
+  if (!clip_axis(p0.x, d.x, s.inv.x, bmin.x, bmax.x, tmin, tmax)) return c;
+  if (!clip_axis(p0.y, d.y, s.inv.y, bmin.y, bmax.y, tmin, tmax)) return c;
+  if (!clip_axis(p0.z, d.z, s.inv.z, bmin.z, bmax.z, tmin, tmax)) return c;
   c.ok = true;
   c.p0 = V3{p0.x + d.x * tmin, p0.y + d.y * tmin, p0.z + d.z * tmin};
   c.p1 = V3{p0.x + d.x * tmax, p0.y + d.y * tmax, p0.z + d.z * tmax};
   return c;
+}
+
+SR_HD Clipped clip_segment_to_box(V3 p0, V3 p1, V3 bmin, V3 bmax) {
+  return clip_segment_dir(p0, p1, seg_dir(p0, p1), bmin, bmax);
 }
 
 SR_HD uint32_t pack_binning_field(uint32_t is_shadow, uint32_t frustum) {

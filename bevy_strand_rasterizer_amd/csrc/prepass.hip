@@ -334,6 +334,14 @@ __device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, float za,
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
 
+// x / n for a tile edge n in pixels: a power of two divides as the product
+// with its exact reciprocal 2^-k (both are x / n correctly rounded), any
+// other n by the division.
+__device__ __forceinline__ float div_by(float x, uint32_t n) {
+  if (n != 0u && (n & (n - 1u)) == 0u) return x * ldexpf(1.0f, -(int)(31u - (uint32_t)__clz(n)));
+  return x / (float)n;
+}
+
 // The fine tiles of one coarse tile as fine_tile_occluded_by_opaque
 // (strand_prepass.wgsl:551-573) sees them: a camera fine tile fully covered
 // by opaque depth hides a slice whose nearest depth (z_max, reverse-Z) lies
@@ -370,10 +378,9 @@ template <class Emit>
 __device__ __forceinline__ void fine_dda(V3 q0, V3 q1, V2 origin, float zmin, float zmax, const sr_frustum_desc& fd,
                                          const Occluder& occ, Emit&& emit) {
   float zspan = fmaxf(zmax - zmin, 1.0f / (float)kQuant);
-  float fxs = (float)fd.froxel_size_x, fys = (float)fd.froxel_size_y;
-  float pa[3] = {(q0.x - origin.x) / fxs, (q0.y - origin.y) / fys,
+  float pa[3] = {div_by(q0.x - origin.x, fd.froxel_size_x), div_by(q0.y - origin.y, fd.froxel_size_y),
                  wclamp((q0.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
-  float pb[3] = {(q1.x - origin.x) / fxs, (q1.y - origin.y) / fys,
+  float pb[3] = {div_by(q1.x - origin.x, fd.froxel_size_x), div_by(q1.y - origin.y, fd.froxel_size_y),
                  wclamp((q1.z - zmin) / zspan, 0.0f, 0.999999f) * (float)kSlices};
   const int32_t hi[3] = {(int32_t)kExt - 1, (int32_t)kExt - 1, (int32_t)kSlices - 1};
   int32_t f[3], f1[3], step[3];
@@ -443,24 +450,25 @@ __device__ __forceinline__ void write_hit(const FrameParams& fp, uint64_t idx, u
 // cells of every touched page; SCATTER appends the slot to each touched
 // page's candidate list.
 template <int MODE>
-__device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32_t tile,
-                                  uint32_t local, uint32_t cx, uint32_t cy, const sr_fine_seg_ref& ref, uint32_t task,
-                                  HitRec& rec) {
+__device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const SegDir& sd, const sr_frustum_desc& fd,
+                                  uint32_t tile, uint32_t local, uint32_t cx, uint32_t cy, const sr_fine_seg_ref& ref,
+                                  uint32_t task, HitRec& rec) {
   V2 origin;
   V3 bmax;
   coarse_tile_box(fd, cx, cy, &origin, &bmax);
-  Clipped c = clip_segment_to_box(p0, p1, V3{origin.x, origin.y, 0.0f}, bmax);
+  Clipped c = clip_segment_dir(p0, p1, sd, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
   const uint2* lz = fp.lut_z + (size_t)tile * kSlices;
   uint32_t z0s, z1s;
   lut_slices_for_depths(lz, c.p0.z, c.p1.z, &z0s, &z1s);
   if (z0s == kInvalid || z1s == kInvalid) return;
+  const SegDir cd = seg_dir(c.p0, c.p1);  // the tile's piece, clipped against each slice
   uint32_t zlo = min(z0s, z1s), zhi = max(z0s, z1s);
   for (uint32_t cz = zlo; cz <= zhi; ++cz) {
     const uint2 e = lz[cz];
     if (e.x > e.y) continue;  // unused slice
     float zmin = dequantize_depth01(e.x), zmax = dequantize_depth01(e.y);
-    Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
+    Clipped dc = clip_segment_dir(c.p0, c.p1, cd, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
     if (!dc.ok) continue;
     uint32_t pti = tile * kSlices + cz;
     if (MODE == MODE_MARK) {
@@ -523,10 +531,11 @@ __device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V
   if (max_cx < min_cx || max_cy < min_cy) return;
   V3 bmin{(float)(min_cx * cpx), (float)(min_cy * cpy), 0.0f};
   V3 bmax{fminf((float)((max_cx + 1u) * cpx), W) - 1e-3f, fminf((float)((max_cy + 1u) * cpy), H) - 1e-3f, 1.0f};
-  Clipped c = clip_segment_to_box(p0, p1, bmin, bmax);
+  const SegDir sd = seg_dir(p0, p1);  // shared with every coarse tile's clip
+  Clipped c = clip_segment_dir(p0, p1, sd, bmin, bmax);
   if (!c.ok) return;
-  float pa[2] = {c.p0.x / (float)cpx, c.p0.y / (float)cpy};
-  float pb[2] = {c.p1.x / (float)cpx, c.p1.y / (float)cpy};
+  float pa[2] = {div_by(c.p0.x, cpx), div_by(c.p0.y, cpy)};
+  float pb[2] = {div_by(c.p1.x, cpx), div_by(c.p1.y, cpy)};
   const int32_t lo[2] = {(int32_t)min_cx, (int32_t)min_cy}, hi[2] = {(int32_t)max_cx, (int32_t)max_cy};
   int32_t f[2], f1[2], step[2];
   float delta[2], tmax[2];
@@ -548,8 +557,8 @@ __device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V
     if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
     uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
     if (local < fd.coarse_depth_tile_count)
-      trace_coarse_tile<MODE>(fp, p0, p1, fd, fd.coarse_depth_tile_base + local, local, (uint32_t)f[0], (uint32_t)f[1],
-                              ref, task, rec);
+      trace_coarse_tile<MODE>(fp, p0, p1, sd, fd, fd.coarse_depth_tile_base + local, local, (uint32_t)f[0],
+                              (uint32_t)f[1], ref, task, rec);
     if (f[0] == f1[0] && f[1] == f1[1]) break;
     if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
     else { f[1] += step[1]; tmax[1] += delta[1]; }
