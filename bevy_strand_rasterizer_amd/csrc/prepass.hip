@@ -1082,7 +1082,8 @@ __global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32
   __shared__ uint32_t cur[256];       // output position of each cell's next reference
   __shared__ uint32_t wcnt[4][256];   // per wave: digit counts, then its start per cell in the step
   __shared__ uint32_t lstart[256], gbase[256];
-  __shared__ uint32_t lcell[kPageStep], lval[kPageStep];  // the step's hits in cell order
+  __shared__ uint32_t lval[kPageStep];  // the step's hits in cell order
+  __shared__ uint8_t lcell[kPageStep];   // and their cells
   __shared__ uint32_t lds4[4];
   __shared__ uint32_t bound[4];  // lo of both searches, then hi
   __shared__ uint32_t colmask;
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32
       for (uint32_t r = 0; r < kPageRounds; ++r) {
         if (chunk + r * 64u + lane >= end) continue;
         const uint32_t d = key[r] & 255u, l = wcnt[w][d] + rank[r];
-        lcell[l] = d;
+        lcell[l] = (uint8_t)d;
         lval[l] = val[r];
       }
       __syncthreads();
