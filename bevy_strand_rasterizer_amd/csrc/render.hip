@@ -715,14 +715,15 @@ __global__ __launch_bounds__(128) void k_shading(FrameParams fp) {
 }
 
 // --------------------------------------------------- K28 camera raster
+// One reference of a camera batch as 16-byte LDS records, in the order a
+// pixel's coverage test reads them (one load per record).
 template <int TPX>
 struct ColorBatch {
-  float p0x[TPX], p0y[TPX], abx[TPX], aby[TPX], inv_l2[TPX];
-  float d0[TPX], d1[TPX], aw0[TPX], aw1[TPX];
-  float rmin[TPX], rmax[TPX];
-  float mnx[TPX], mny[TPX], mxx[TPX], mxy[TPX];
+  float4 box[TPX];  // screen box grown by the max radius: min x, min y, max x, max y
+  float4 seg[TPX];  // p0.x, p0.y, p1.x - p0.x, p1.y - p0.y
+  float4 geo[TPX];  // 1 / |p1 - p0|^2, raw depths at p0 and p1, min radius
+  float4 wgt[TPX];  // max radius, |w| at p0 and p1, shadow visibility
   float4 s0[TPX], s1[TPX];
-  float vis[TPX];
   uint32_t item[TPX];  // work item, with bit 31 set when the reference is invalid
 };
 
@@ -743,16 +744,13 @@ __device__ void load_color_ref(const FrameParams& fp, const FrustumRec& fr, uint
   const size_t lt = ((size_t)fr.cam_index * fp.NI + sr.inst_id) * fp.atlas_texels;
   uint32_t i0 = sr.seg_id, i1 = min(i0 + 1u, fp.atlas_texels - 1u);
   float abx = c.x - a.x, aby = c.y - a.y;
-  b.p0x[k] = a.x; b.p0y[k] = a.y; b.abx[k] = abx; b.aby[k] = aby;
-  b.inv_l2[k] = 1.0f / (abx * abx + aby * aby);
-  b.d0[k] = a.z; b.d1[k] = c.z;
-  b.aw0[k] = fmaxf(fabsf(a.w), 1e-6f); b.aw1[k] = fmaxf(fabsf(c.w), 1e-6f);
-  b.rmin[k] = minr; b.rmax[k] = maxr;
-  b.mnx[k] = fminf(a.x, c.x) - maxr; b.mny[k] = fminf(a.y, c.y) - maxr;
-  b.mxx[k] = fmaxf(a.x, c.x) + maxr; b.mxy[k] = fmaxf(a.y, c.y) + maxr;
+  b.box[k] = make_float4(fminf(a.x, c.x) - maxr, fminf(a.y, c.y) - maxr, fmaxf(a.x, c.x) + maxr,
+                         fmaxf(a.y, c.y) + maxr);
+  b.seg[k] = make_float4(a.x, a.y, abx, aby);
+  b.geo[k] = make_float4(1.0f / (abx * abx + aby * aby), a.z, c.z, minr);
+  b.wgt[k] = make_float4(maxr, fmaxf(fabsf(a.w), 1e-6f), fmaxf(fabsf(c.w), 1e-6f), fp.history[lt + i0].x);
   b.s0[k] = fp.atlas[lt + i0];
   b.s1[k] = fp.atlas[lt + i1];
-  b.vis[k] = fp.history[lt + i0].x;
 }
 
 template <int TPX>
@@ -787,9 +785,10 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
         const bool valid = k < nb && !(b.item[k] & 0x80000000u);
         float x0 = 0.0f, y0 = 0.0f, ax = 0.0f, ay = 0.0f, rmx = 0.0f;
         if (valid) {
-          x0 = b.p0x[k]; y0 = b.p0y[k]; ax = b.abx[k]; ay = b.aby[k];
-          const float minr = fmaxf(b.rmin[k], 1e-4f);
-          rmx = span_radius(minr, fmaxf(b.rmax[k], minr), span_depth01_max(b.d0[k], b.d1[k]));
+          const float4 sg = b.seg[k], ge = b.geo[k];
+          x0 = sg.x; y0 = sg.y; ax = sg.z; ay = sg.w;
+          const float minr = fmaxf(ge.w, 1e-4f);
+          rmx = span_radius(minr, fmaxf(b.wgt[k].x, minr), span_depth01_max(ge.y, ge.z));
         }
         batch_columns<TPX>(valid, x0, y0, ax, ay, rmx, tx * fx, ty * fy, fx, fy, cols);
       }
@@ -810,22 +809,25 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
             }
           }
           if (item & 0x80000000u) continue;
-          if (cx < b.mnx[k] || cy < b.mny[k] || cx > b.mxx[k] || cy > b.mxy[k]) continue;
-          float tt = ((cx - b.p0x[k]) * b.abx[k] + (cy - b.p0y[k]) * b.aby[k]) * b.inv_l2[k];
+          const float4 bx = b.box[k];
+          if (cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) continue;
+          const float4 sg = b.seg[k], ge = b.geo[k];
+          float tt = ((cx - sg.x) * sg.z + (cy - sg.y) * sg.w) * ge.x;
           if (!(tt >= 0.0f && tt <= 1.0f)) continue;
-          float qx = b.p0x[k] + b.abx[k] * tt, qy = b.p0y[k] + b.aby[k] * tt;
-          float qz = wmix(b.d0[k], b.d1[k], tt);
-          float minr = fmaxf(b.rmin[k], 1e-4f);
-          float r = wmix(minr, fmaxf(b.rmax[k], minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
+          float qx = sg.x + sg.z * tt, qy = sg.y + sg.w * tt;
+          float qz = wmix(ge.y, ge.z, tt);
+          const float4 wg = b.wgt[k];
+          float minr = fmaxf(ge.w, 1e-4f);
+          float r = wmix(minr, fmaxf(wg.x, minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
           float dx = cx - qx, dy = cy - qy;
           float d2 = dx * dx + dy * dy;
           if (!(d2 < r * r)) continue;
           float cov = wclamp(1.0f - sqrtf(d2) / r, 0.0f, 1.0f);
-          float aw0 = b.aw0[k], aw1 = b.aw1[k];
+          float aw0 = wg.y, aw1 = wg.z;
           float tw = (tt * aw0) / ((1.0f - tt) * aw1 + tt * aw0);
           float4 s0 = b.s0[k], s1 = b.s1[k];
           float alpha = wmix(s0.w, s1.w, tw) * cov;
-          float vis = b.vis[k];
+          float vis = wg.w;
           float om = 1.0f - fc3;
           fc0 = fc0 + (wmix(s0.x, s1.x, tw) * vis * alpha) * om;
           fc1 = fc1 + (wmix(s0.y, s1.y, tw) * vis * alpha) * om;
