@@ -60,10 +60,13 @@ def refs_by_kind(r, w):
 
 def stage_bytes(ctr, w, refs=None):
     """Algorithmic bytes per stage: every ABI record read or written once.
-    T tasks, C = 256 P cells, H references, N sorted hits, W work items,
-    R runs; raster stages: per reference its 12-byte record, two vertex
-    indices and two float4 vertices (+ two atlas texels and a history texel
-    for the camera), plus the framebuffers written once."""
+    T tasks (valid slots), C = 256 P cells, H references, N sorted hits,
+    W work items, R runs.  Projection and marking: per valid slot its
+    float4 vertex and index, the 12-byte projected segment, the two float4
+    raw screen endpoints and the hit count; per hit its 4-byte cell record.
+    Raster stages: per reference its 12-byte record and the segment's two
+    float4 screen endpoints (+ two atlas texels and a history texel for the
+    camera), plus the framebuffers written once."""
     T, P, H = ctr["binning_tasks"], ctr["allocated_pages"], ctr["fine_refs"]
     C = 256 * P
     W = ctr["work_items"]
@@ -74,7 +77,7 @@ def stage_bytes(ctr, w, refs=None):
     page_cap = min(ptab, max(4096, ptab // 2))  # the runtime's first-frame page capacity
     passes = (min(max(page_cap.bit_length(), 1), 24) + 7) // 8  # radix passes over the page bits
     out = {
-        "project_mark": 28 * T,
+        "project_mark": T * (16 + 4 + 12 + 32 + 4) + 4 * N,
         "count": 28 * T + 4 * C,
         # page CSR: (key, slot) pairs; each radix pass over the page bits
         # reads keys for the digit histogram, key + slot for the scatter and
@@ -97,8 +100,8 @@ def stage_bytes(ctr, w, refs=None):
         L = int(w.settings["dom_layers"])
         light_px = sum(int(f["screen_width"]) * int(f["screen_height"]) for f in w.frusta if f["kind"] == 1)
         cam_px = sum(int(f["screen_width"]) * int(f["screen_height"]) for f in w.frusta if f["kind"] == 0)
-        out["shadow"] = refs[1] * (12 + 8 + 32) + light_px * 4 * (1 + L)
-        out["raster"] = refs[0] * (12 + 8 + 32 + 32 + 8) + cam_px * (16 + 4)
+        out["shadow"] = refs[1] * (12 + 32) + light_px * 4 * (1 + L)
+        out["raster"] = refs[0] * (12 + 32 + 32 + 8) + cam_px * (16 + 4)
     return out
 
 
@@ -120,6 +123,9 @@ def profile_row(kernel: str):
 
 STAGE_KERNEL = {"shadow": "k_shadow_raster", "raster": "k_camera_raster", "shading": "k_shading",
                 "project_mark": "k_project_mark", "project_write": "k_project_write"}
+STAGE_WORK = {"shadow": "exact per-pixel coverage arithmetic", "raster": "exact per-pixel coverage arithmetic",
+              "shading": "hair BSDF per camera segment",
+              "project_mark": "exact projection, clipping and cell walks with correctly rounded divisions"}
 
 
 def roofline_for(stage, ms, ctr, w, refs=None):
@@ -138,7 +144,7 @@ def roofline_for(stage, ms, ctr, w, refs=None):
         if valu is not None:
             out["valu_busy"] = valu
             if valu > 0.6:
-                out["note"] = "VALU-bound (exact per-pixel coverage arithmetic); traffic and VALU busy from profiles/"
+                out["note"] = f"VALU-bound ({STAGE_WORK.get(stage, 'exact arithmetic')}); traffic and VALU busy from profiles/"
     return out
 
 
