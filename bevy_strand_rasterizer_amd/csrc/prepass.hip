@@ -301,35 +301,41 @@ __global__ void k_lut(FrameParams fp) {
 // quantised depth, else the first nearest one.  Both endpoint depths are
 // looked up in one pass over the tile's 16 (z_min_q, z_max_q) pairs, loaded
 // together (a used slice has z_min_q <= z_max_q; unused ones are (~0, 0)).
-struct SliceSearch {
-  uint32_t zq, hit, best, best_d;
-  __device__ __forceinline__ void see(uint32_t i, uint2 e) {
-    if (hit != kInvalid || e.x > e.y) return;
-    if (zq >= e.x && zq <= e.y) { hit = i; return; }
-    const uint32_t d = zq > e.y ? zq - e.y : e.x - zq;
-    if (d < best_d) { best_d = d; best = i; }
+// The LUT slice of a quantised depth: the first used slice whose range holds
+// it, else the used slice nearest to it (first on ties), else invalid.  The
+// holding slices come from one mask per depth; the nearest-slice scan runs
+// only for a depth no slice holds.
+__device__ __forceinline__ uint32_t nearest_slice(const uint4 (&q)[kSlices / 2], uint32_t zq) {
+  uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
+#pragma unroll
+  for (uint32_t i = 0; i < kSlices; ++i) {
+    const uint32_t ex = (i & 1u) ? q[i / 2].z : q[i / 2].x, ey = (i & 1u) ? q[i / 2].w : q[i / 2].y;
+    if (ex > ey) continue;
+    const uint32_t d = zq > ey ? zq - ey : ex - zq;
+    if (d < best_d) {
+      best_d = d;
+      best = i;
+    }
   }
-  __device__ __forceinline__ uint32_t result() const { return hit != kInvalid ? hit : best; }
-};
+  return best;
+}
 
 __device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, float za, float zb, uint32_t* sa,
                                                       uint32_t* sb) {
-  SliceSearch a{quantize_depth01(za), kInvalid, kInvalid, 0xFFFFFFFFu};
-  SliceSearch b{quantize_depth01(zb), kInvalid, kInvalid, 0xFFFFFFFFu};
+  const uint32_t qa = quantize_depth01(za), qb = quantize_depth01(zb);
   const uint4* l4 = reinterpret_cast<const uint4*>(lz);
   uint4 q[kSlices / 2];
 #pragma unroll
   for (uint32_t i = 0; i < kSlices / 2; ++i) q[i] = l4[i];
+  uint32_t ma = 0u, mb = 0u;  // slices holding qa / qb (a used slice has x <= y)
 #pragma unroll
-  for (uint32_t i = 0; i < kSlices / 2; ++i) {
-    const uint2 e0 = make_uint2(q[i].x, q[i].y), e1 = make_uint2(q[i].z, q[i].w);
-    a.see(2 * i, e0);
-    b.see(2 * i, e0);
-    a.see(2 * i + 1, e1);
-    b.see(2 * i + 1, e1);
+  for (uint32_t i = 0; i < kSlices; ++i) {
+    const uint32_t ex = (i & 1u) ? q[i / 2].z : q[i / 2].x, ey = (i & 1u) ? q[i / 2].w : q[i / 2].y;
+    ma |= (qa >= ex && qa <= ey) ? (1u << i) : 0u;
+    mb |= (qb >= ex && qb <= ey) ? (1u << i) : 0u;
   }
-  *sa = a.result();
-  *sb = b.result();
+  *sa = ma ? (uint32_t)(__ffs(ma) - 1) : nearest_slice(q, qa);
+  *sb = mb ? (uint32_t)(__ffs(mb) - 1) : nearest_slice(q, qb);
 }
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
