@@ -61,6 +61,15 @@ void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, 
 void launch_shading(const FrameParams& fp, hipStream_t s);
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
-void launch_fill_u32(uint32_t* dst, uint64_t count, uint32_t value, hipStream_t s);
+// Up to kFillSpans buffers filled with one 32-bit value by one launch.
+constexpr uint32_t kFillSpans = 10;
+struct FillSpans {
+  uint32_t* dst[kFillSpans] = {};
+  uint64_t n[kFillSpans] = {};          // words
+  uint64_t start[kFillSpans + 1] = {};  // first 16-byte chunk of each span
+  uint32_t count = 0, value = 0;
+  void add(uint32_t* p, uint64_t words);  // ignored when full, empty or null
+};
+void launch_fill_spans(const FillSpans& f, hipStream_t s);
 
 }  // namespace srk

@@ -26,16 +26,38 @@ namespace srk {
 
 constexpr uint32_t kBatch = 256;
 
-__global__ void k_fill_u32(uint32_t* dst, uint64_t n, uint32_t v) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    dst[i] = v;
+// Fills every span of a FillSpans in one launch: a flat grid-stride loop
+// over the spans' 16-byte chunks (span bases are allocation bases, so
+// 16-byte aligned; a span's last chunk may be partial).
+__global__ void k_fill_spans(FillSpans f) {
+  const uint64_t total = f.start[f.count];
+  const uint4 v4 = make_uint4(f.value, f.value, f.value, f.value);
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t k = 0;
+    while (k + 1u < f.count && q >= f.start[k + 1u]) ++k;
+    const uint64_t w = (q - f.start[k]) * 4u;
+    uint32_t* d = f.dst[k];
+    if (w + 4u <= f.n[k]) {
+      *reinterpret_cast<uint4*>(d + w) = v4;
+    } else {
+      for (uint64_t j = w; j < f.n[k]; ++j) d[j] = f.value;
+    }
+  }
 }
 
-void launch_fill_u32(uint32_t* dst, uint64_t count, uint32_t value, hipStream_t s) {
-  if (count == 0) return;
-  uint64_t g = (count + 255) / 256;
-  if (g > 8192) g = 8192;
-  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)g), dim3(256), 0, s, dst, count, value);
+void FillSpans::add(uint32_t* p, uint64_t words) {
+  if (!p || words == 0 || count == kFillSpans) return;
+  dst[count] = p;
+  n[count] = words;
+  start[count + 1] = start[count] + (words + 3u) / 4u;
+  ++count;
+}
+
+void launch_fill_spans(const FillSpans& f, hipStream_t s) {
+  if (f.count == 0) return;
+  uint64_t g = (f.start[f.count] + 255u) / 256u;
+  if (g > 16384) g = 16384;
+  hipLaunchKernelGGL(k_fill_spans, dim3((uint32_t)g), dim3(256), 0, s, f);
 }
 
 __device__ __forceinline__ V3 world_v3(const InstRec& in, uint32_t vi) {

@@ -755,17 +755,19 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
 
   auto launches = [&]() {
     sl.timer.mark(s, "clear");
-    launch_fill_u32(fp.counters, C_COUNT, 0u, s);
-    if (fp.tel) launch_fill_u32(fp.tel, tel_words, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.image), img_px * 4, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.depth), img_px, 0u, s);
+    FillSpans clear;  // every frame buffer that starts at zero, in one launch
+    clear.add(fp.counters, C_COUNT);
+    if (fp.tel) clear.add(fp.tel, tel_words);
+    clear.add(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px);
+    clear.add(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers);
+    clear.add(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px);
+    clear.add(reinterpret_cast<uint32_t*>(fp.image), img_px * 4);
+    clear.add(reinterpret_cast<uint32_t*>(fp.depth), img_px);
     // texels no segment writes this frame read as zero (the reference leaves
     // them stale; dumps must be deterministic)
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4, 0u, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2, 0u, s);
+    clear.add(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4);
+    clear.add(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2);
+    launch_fill_spans(clear, s);
     launch_prepass(fp, ps, s, &sl.timer);
     sl.timer.mark(s, "shadow");
     launch_shadow(fp, s, max_light_tile_px);
