@@ -136,7 +136,7 @@ enum BufId {
   B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS, B_CAM_ORDER, B_SHADOW_ORDER, B_RUN_HIST,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SHADOW_COLS, B_SLOT_PROJ, B_SLOT_SCREEN,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
-  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_FRUSTA, B_INSTS, B_LIGHTS,
+  B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_TABLES,
   B_COUNT
 };
 
@@ -595,6 +595,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const int hist_next = hist_valid ? (sl.hist_cur ^ 1) : 0;
 
   struct Need { BufId id; uint64_t bytes; };
+  // the per-frame tables share one buffer (one upload): frusta, instances,
+  // lights, each section 256-byte aligned
+  const uint64_t tab_inst = align_up(sizeof(FrustumRec) * F, 256);
+  const uint64_t tab_light = tab_inst + align_up(sizeof(InstRec) * std::max(NI, 1u), 256);
+  const uint64_t tab_bytes = tab_light + sizeof(sr_directional_light) * std::max(NL, 1u);
   const Need needs[] = {
       {B_COUNTERS, 4ull * C_COUNT}, {B_BROAD, 16ull * NI + 16}, {B_KEEP, 4ull * NI * F + 4},
       {B_RANGES, 32ull * NI * F + 32}, {B_SLOT_MASKS, 8ull * task_cap / 64}, {B_WAVE_OFF, 4ull * task_cap / 64},
@@ -616,8 +621,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
-      {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_FRUSTA, sizeof(FrustumRec) * F},
-      {B_INSTS, sizeof(InstRec) * std::max(NI, 1u)}, {B_LIGHTS, sizeof(sr_directional_light) * std::max(NL, 1u)}};
+      {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_TABLES, tab_bytes}};
   if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
     uint64_t held = sl.held_bytes();
     for (const Need& n : needs) {
@@ -638,20 +642,15 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     if (!sl.buf[n.id].ensure(n.bytes)) return efail(err, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
   }
   // ---- upload per-frame tables through pinned staging
-  const uint64_t tbytes = sizeof(FrustumRec) * F + sizeof(InstRec) * NI + sizeof(sr_directional_light) * NL;
+  const uint64_t tbytes = tab_light + sizeof(sr_directional_light) * NL;
   int slot = 0;
-  char* stg = next_staging(sl, tbytes + 256, &slot);
+  char* stg = next_staging(sl, tab_bytes + 256, &slot);
   if (!stg) return efail(err, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
+  std::memset(stg, 0, tbytes);
   std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
-  if (NI) std::memcpy(stg + sizeof(FrustumRec) * F, ir.data(), sizeof(InstRec) * NI);
-  if (NL) std::memcpy(stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI, fd->lights, sizeof(sr_directional_light) * NL);
-  HIP_OKS(hipMemcpyAsync(sl.buf[B_FRUSTA].p, stg, sizeof(FrustumRec) * F, hipMemcpyHostToDevice, s), "upload frusta");
-  if (NI)
-    HIP_OKS(hipMemcpyAsync(sl.buf[B_INSTS].p, stg + sizeof(FrustumRec) * F, sizeof(InstRec) * NI,
-                          hipMemcpyHostToDevice, s), "upload instances");
-  if (NL)
-    HIP_OKS(hipMemcpyAsync(sl.buf[B_LIGHTS].p, stg + sizeof(FrustumRec) * F + sizeof(InstRec) * NI,
-                          sizeof(sr_directional_light) * NL, hipMemcpyHostToDevice, s), "upload lights");
+  if (NI) std::memcpy(stg + tab_inst, ir.data(), sizeof(InstRec) * NI);
+  if (NL) std::memcpy(stg + tab_light, fd->lights, sizeof(sr_directional_light) * NL);
+  HIP_OKS(hipMemcpyAsync(sl.buf[B_TABLES].p, stg, tbytes, hipMemcpyHostToDevice, s), "upload frame tables");
   if (opaque_floats) {  // the opaque depth travels through its own staging entry
     int oslot = 0;
     char* ostg = next_staging(sl, 4ull * opaque_floats, &oslot);
@@ -667,9 +666,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
 
   FrameParams fp;
   std::memset(&fp, 0, sizeof(fp));
-  fp.fr = sl.buf[B_FRUSTA].as<FrustumRec>();
-  fp.inst = sl.buf[B_INSTS].as<InstRec>();
-  fp.lights = sl.buf[B_LIGHTS].as<sr_directional_light>();
+  char* tab = sl.buf[B_TABLES].as<char>();
+  fp.fr = reinterpret_cast<FrustumRec*>(tab);
+  fp.inst = reinterpret_cast<InstRec*>(tab + tab_inst);
+  fp.lights = reinterpret_cast<sr_directional_light*>(tab + tab_light);
   fp.F = F; fp.NI = NI; fp.NL = NL; fp.cams = cams;
   fp.slots = slots; fp.coarse_total = coarse_total;
   fp.task_cap = task_cap; fp.ref_cap = ref_cap; fp.work_cap = work_cap; fp.atlas_texels = atlas_texels;
