@@ -1014,7 +1014,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
   if (base >= n) return;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t nvalid = min(kRadixBlock, n - base);
-  const uint32_t nb = (n + kRadixBlock - 1u) / kRadixBlock;
   const unsigned long long lower = (1ull << lane) - 1ull;
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[q][t] = 0u;
@@ -1613,7 +1612,6 @@ __global__ __launch_bounds__(256) void k_telemetry(FrameParams fp) {
     for (uint32_t fi = 0; fi < fp.F; ++fi)
       tel_add(fp, fi, TF_EMITTED_SEGMENTS, fp.frustum_task_base[fi + 1] - fp.frustum_task_base[fi]);
   }
-  const bool csr = fp.st.fine_binning_backend != 0u;
   for (uint32_t p = gid; p < P; p += stride) {
     const sr_fine_page_meta pm = fp.page_meta[p];
     const uint32_t fi = frustum_for_coarse_tile(fp, pm.coarse_tile_id);

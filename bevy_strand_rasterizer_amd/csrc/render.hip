@@ -24,8 +24,6 @@
 
 namespace srk {
 
-constexpr uint32_t kBatch = 256;
-
 // Fills every span of a FillSpans in one launch: a flat grid-stride loop
 // over the spans' 16-byte chunks (span bases are allocation bases, so
 // 16-byte aligned; a span's last chunk may be partial).
