@@ -430,13 +430,38 @@ constexpr uint32_t kHitOverflow = 0xFFu;
 // Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst;
 // n counts every visible cell, also past cap).  SCATTER: the slot's next hit
 // goes to base + n.
+// Marking defers each (slot, page) fine walk to a queue of its wave: the
+// lanes walk their segments' coarse tiles and slices, then the wave runs the
+// queued fine walks one per lane, so a slot with many pages does not hold the
+// other 63 lanes idle.  A slot's cells are counted and recorded through its
+// lane's LDS counter; their order in the record does not matter (a slot's
+// cells are distinct (page, cell) pairs, and the sort orders hits by page,
+// cell and slot).  A full queue walks in place.
+struct MarkJob {
+  float q0x, q0y, q0z, q1x, q1y, q1z, zmin, zmax;  // the segment clipped to the page
+  uint32_t pti, cxy, lo;  // page-table index; cx | cy << 16; local tile << 6 | owner lane
+};
+constexpr uint32_t kMarkQ = 128;  // queued walks per wave
+struct MarkQueue {
+  MarkJob* q;
+  uint32_t* n;     // walks queued (may pass kMarkQ: the rest walked in place)
+  uint32_t* nrec;  // per lane: cells of its slot so far
+  uint32_t owner;  // the lane whose slot the walk belongs to
+};
+
 struct HitRec {
   uint32_t* dst;  // null: not recording; hit j at dst[j * stride]
   uint32_t n;
   uint32_t cap;
   uint32_t base, slot;
   uint32_t stride;
+  MarkQueue* mq = nullptr;  // MARK with the CSR backend: the wave's queue
 };
+
+__device__ __forceinline__ void mark_cell(HitRec& rec, uint32_t pti, uint32_t cell) {
+  const uint32_t n = rec.mq ? atomicAdd(&rec.mq->nrec[rec.mq->owner], 1u) : rec.n++;
+  if (rec.dst && n < rec.cap) rec.dst[(size_t)n * rec.stride] = (pti << 8) | cell;
+}
 
 // The sort key of a hit: page * 256 + cell; a dropped page's hits carry the
 // page id page_cap and sort after every kept one.
@@ -479,11 +504,18 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
     uint32_t pti = tile * kSlices + cz;
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
-      if (fp.st.fine_binning_backend != 0u)  // the visible cells: recorded (up to cap) and counted
-        fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local), [&](uint32_t cell) {
-          if (rec.dst && rec.n < rec.cap) rec.dst[(size_t)rec.n * rec.stride] = (pti << 8) | cell;
-          ++rec.n;
-        });
+      if (fp.st.fine_binning_backend != 0u) {  // the visible cells: recorded (up to cap) and counted
+        if (rec.mq) {
+          const uint32_t j = atomicAdd(rec.mq->n, 1u);
+          if (j < kMarkQ) {
+            rec.mq->q[j] = MarkJob{dc.p0.x, dc.p0.y, dc.p0.z, dc.p1.x, dc.p1.y, dc.p1.z, zmin, zmax, pti,
+                                   cx | (cy << 16), (local << 6) | rec.mq->owner};
+            continue;
+          }
+        }
+        fine_dda(dc.p0, dc.p1, origin, zmin, zmax, fd, occluder_for(fp, fd, local),
+                 [&](uint32_t cell) { mark_cell(rec, pti, cell); });
+      }
       continue;
     }
     uint32_t h = fp.page_table[pti];
@@ -610,10 +642,16 @@ __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
 // slots also mark their pages (the LUT is built before projection) and, for
 // the CSR backend, record their fine cells and count page candidates.
 __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
+  __shared__ MarkJob jobs[4][kMarkQ];
+  __shared__ uint32_t jn[4], nrec[4][64];
   const uint64_t total = (uint64_t)fp.F * fp.slots;
   const bool csr = fp.st.fine_binning_backend != 0u;
   const bool record = csr && fp.hit_k != 0u;
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
+    if (lane == 0u) jn[w] = 0u;
+    nrec[w][lane] = 0u;
+    MarkQueue mq{jobs[w], &jn[w], nrec[w], lane};
     uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
@@ -629,11 +667,27 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
       V3 p0, p1;
       task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
       HitRec rec{record ? fp.slot_hits + s : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
+      if (csr) rec.mq = &mq;
       visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
-      if (record) fp.slot_hit_count[s] = (uint8_t)(rec.n > rec.cap ? kHitOverflow : rec.n);
-      hits = rec.n;
     }
-    if (csr) fp.slot_hit_total[s] = hits;  // offsets by a scan: no device atomic
+    if (csr) {  // the wave's queued fine walks, one per lane (the wave is converged here)
+      const sr_frustum_desc& fd = fp.fr[fi].d;
+      const uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
+      const uint32_t nj = min(jn[w], kMarkQ);
+      for (uint32_t j = lane; j < nj; j += 64u) {
+        const MarkJob jb = jobs[w][j];
+        const uint32_t owner = jb.lo & 63u, local = jb.lo >> 6;
+        MarkQueue omq{nullptr, nullptr, nrec[w], owner};
+        HitRec orec{record ? fp.slot_hits + (s - lane + owner) : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
+        orec.mq = &omq;
+        const V2 origin{(float)((jb.cxy & 0xFFFFu) * cpx), (float)((jb.cxy >> 16) * cpy)};
+        fine_dda(V3{jb.q0x, jb.q0y, jb.q0z}, V3{jb.q1x, jb.q1y, jb.q1z}, origin, jb.zmin, jb.zmax, fd,
+                 occluder_for(fp, fd, local), [&](uint32_t cell) { mark_cell(orec, jb.pti, cell); });
+      }
+      hits = ok ? nrec[w][lane] : 0u;
+      if (record && ok) fp.slot_hit_count[s] = (uint8_t)(hits > fp.hit_k ? kHitOverflow : hits);
+      fp.slot_hit_total[s] = hits;  // offsets by a scan: no device atomic
+    }
     unsigned long long m = __ballot(ok);
     if ((threadIdx.x & 63) == 0) fp.slot_masks[s >> 6] = m;
     const FrustumRec& fr = fp.fr[fi];
@@ -694,10 +748,10 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       uint32_t nh = kHitOverflow, base = 0u, mat = 0u;
       uint32_t hv[kPre], he[kPre];
       const uint32_t* h = fp.slot_hits + s;
+      const uint32_t pt_last = fp.coarse_total * kSlices - 1u;
       if (csr) {
         nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
         base = fp.hit_base[s];
-        const uint32_t pt_last = fp.coarse_total * kSlices - 1u;
 #pragma unroll
         for (uint32_t k = 0; k < kPre; ++k) hv[k] = k < fp.hit_k ? h[(size_t)k * total] : 0u;
 #pragma unroll
@@ -709,27 +763,29 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       if (fr.d.kind == 0u) fp.shading_queue[fp.frustum_cam_base[fi] + (t - fp.frustum_task_base[fi])] = t;
       if (csr) {  // K17: the slot's hits at its scanned offset
         fp.slot_material[pos] = mat;  // the same value from every frustum
-        if (nh != kHitOverflow) {  // the recorded cells (one page's cells are consecutive)
-          uint32_t prev = kInvalid, entry = 0u;
+        if (nh != kHitOverflow) {  // the recorded cells, in any order
 #pragma unroll
           for (uint32_t k = 0; k < kPre; ++k) {
             if (k >= nh) break;
-            const uint32_t pti = hv[k] >> 8;
-            if (pti != prev) {
-              prev = pti;
-              entry = he[k];
-              if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
-            }
-            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, hv[k] & 255u), pos);
+            write_hit(fp, (uint64_t)base + k, hit_key(fp, he[k], hv[k] & 255u), pos);
           }
-          for (uint32_t k = kPre; k < nh; ++k) {
-            const uint32_t v = h[(size_t)k * total], pti = v >> 8;
-            if (pti != prev) {
-              prev = pti;
-              entry = fp.page_table[pti];
-              if (fp.tel && entry != 0u && entry != kMarked) atomicAdd(&fp.tel[kTelPageOff + entry - 1u], 1u);
+          for (uint32_t k0 = kPre; k0 < nh; k0 += kPre) {  // the rest, kPre at a time, loads first
+#pragma unroll
+            for (uint32_t k = 0; k < kPre; ++k) hv[k] = k0 + k < nh ? h[(size_t)(k0 + k) * total] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kPre; ++k) he[k] = fp.page_table[min(hv[k] >> 8, pt_last)];
+#pragma unroll
+            for (uint32_t k = 0; k < kPre; ++k)
+              if (k0 + k < nh) write_hit(fp, (uint64_t)base + k0 + k, hit_key(fp, he[k], hv[k] & 255u), pos);
+          }
+          if (fp.tel) {  // (segment, page) candidates: each recorded page once
+            for (uint32_t k = 0; k < nh; ++k) {
+              const uint32_t pti = h[(size_t)k * total] >> 8;
+              bool seen = false;
+              for (uint32_t k2 = 0; k2 < k && !seen; ++k2) seen = (h[(size_t)k2 * total] >> 8) == pti;
+              const uint32_t e = seen ? 0u : fp.page_table[pti];
+              if (e != 0u && e != kMarked) atomicAdd(&fp.tel[kTelPageOff + e - 1u], 1u);
             }
-            write_hit(fp, (uint64_t)base + k, hit_key(fp, entry, v & 255u), pos);
           }
         } else {  // too many cells to record: trace again
           V3 p0, p1;
