@@ -439,9 +439,9 @@ constexpr uint32_t kHitOverflow = 0xFFu;
 // cell and slot).  A full queue walks in place.
 struct MarkJob {
   float q0x, q0y, q0z, q1x, q1y, q1z, zmin, zmax;  // the segment clipped to the page
-  uint32_t pti, cxy, lo;  // page-table index; cx | cy << 16; local tile << 6 | owner lane
+  uint32_t pti, lo;  // page-table index; frustum-local coarse tile << 6 | owner lane
 };
-constexpr uint32_t kMarkQ = 128;  // queued walks per wave
+constexpr uint32_t kMarkQ = 192;  // queued walks per wave (4 waves: 30 KB of LDS)
 struct MarkQueue {
   MarkJob* q;
   uint32_t* n;     // walks queued (may pass kMarkQ: the rest walked in place)
@@ -509,7 +509,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
           const uint32_t j = atomicAdd(rec.mq->n, 1u);
           if (j < kMarkQ) {
             rec.mq->q[j] = MarkJob{dc.p0.x, dc.p0.y, dc.p0.z, dc.p1.x, dc.p1.y, dc.p1.z, zmin, zmax, pti,
-                                   cx | (cy << 16), (local << 6) | rec.mq->owner};
+                                   (local << 6) | rec.mq->owner};
             continue;
           }
         }
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
         MarkQueue omq{nullptr, nullptr, nrec[w], owner};
         HitRec orec{record ? fp.slot_hits + (s - lane + owner) : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
         orec.mq = &omq;
-        const V2 origin{(float)((jb.cxy & 0xFFFFu) * cpx), (float)((jb.cxy >> 16) * cpy)};
+        const V2 origin{(float)((local % fd.coarse_tiles_x) * cpx), (float)((local / fd.coarse_tiles_x) * cpy)};
         fine_dda(V3{jb.q0x, jb.q0y, jb.q0z}, V3{jb.q1x, jb.q1y, jb.q1z}, origin, jb.zmin, jb.zmax, fd,
                  occluder_for(fp, fd, local), [&](uint32_t cell) { mark_cell(orec, jb.pti, cell); });
       }
