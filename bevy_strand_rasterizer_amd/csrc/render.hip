@@ -532,11 +532,11 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
 }
 
 // Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)) over the
-// window k in kc +- kmax around the nearest k; the caller sizes the window so
-// that every term left out is below e^-16 (1e-7 of the largest term).
-__device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, int kmax) {
-  float kc = rintf(pd * (float)(1.0 / (2.0 * 3.14159265359)));
-  int k0 = max(-10, (int)kc - kmax), k1 = min(10, (int)kc + kmax);
+// k with |d - 2 pi k| <= r = sqrt(32 var): every term left out is below
+// e^-16 (1e-7 of the largest term).  For r < pi that is at most one term.
+__device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, float r) {
+  const float inv_2pi = (float)(1.0 / (2.0 * 3.14159265359));
+  int k0 = max(-10, (int)ceilf((pd - r) * inv_2pi)), k1 = min(10, (int)floorf((pd + r) * inv_2pi));
   float total = 0.0f;
   for (int k = k0; k <= k1; ++k) {
     float ang = pd - TWO_PI_F * (float)k;
@@ -574,9 +574,8 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
   bool narrow = var < 1e-6f;
   float norm = __builtin_amdgcn_rcpf(SQRT_2PI_F * beta);
   float neg_inv_2var = -__builtin_amdgcn_rcpf(2.0f * var);
-  // a term with |d - 2 pi k| above r = sqrt(16 * 2 var) is below e^-16; those
-  // within r lie at |k - kc| <= (r + pi) / (2 pi)
-  int kmax = (int)ceilf((sqrtf(32.0f * var) + kPiF) / TWO_PI_F);
+  // a term with |d - 2 pi k| above r = sqrt(16 * 2 var) is below e^-16
+  const float r_win = sqrtf(32.0f * var);
   V3 acc[3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
   for (int i = 0; i < 10; ++i) {
     const LegPoint lp = leg[i];
@@ -606,7 +605,7 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
       float pd = phi - shift;
       float det;
       if (narrow) det = fabsf(fmodf(pd, TWO_PI_F)) < 1e-3f ? 0.0f : 1e6f;
-      else det = norm * wrapped_gaussian(pd, neg_inv_2var, kmax);
+      else det = norm * wrapped_gaussian(pd, neg_inv_2var, r_win);
       acc[p].x = acc[p].x + att.x * det * wgt;
       acc[p].y = acc[p].y + att.y * det * wgt;
       acc[p].z = acc[p].z + att.z * det * wgt;
