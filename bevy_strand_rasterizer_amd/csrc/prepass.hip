@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       // every load first (recorded cells up to kPre and their page-table
       // entries, index clamped; values past the slot's count are unused),
       // then the stores, so the latencies overlap
-      constexpr uint32_t kPre = 8u;
+      constexpr uint32_t kPre = 6u;
       uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
       uint32_t ii = find_instance(fp, pos);
       const InstRec& in = fp.inst[ii];
