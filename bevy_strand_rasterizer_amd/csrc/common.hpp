@@ -169,7 +169,8 @@ struct FrameParams {
   const sr_projected_segment* slot_proj;  // [F * slots]
   uint32_t* slot_hits;
   uint8_t* slot_hit_count;
-  uint32_t hit_k, pad2;
+  uint32_t hit_k;
+  uint32_t mark_queue;  // fine walks a wave queues while marking (<= kMarkQ)
   // shadow raster: span columns of each workgroup's first batches, kept
   // from its first pass for the second (kShadowColBytes per frame)
   unsigned long long* shadow_cols;

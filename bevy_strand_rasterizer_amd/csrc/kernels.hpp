@@ -52,6 +52,10 @@ struct PrepassScratch {
   uint32_t radix_blocks, radix_passes;
 };
 
+// Fine walks each wave of the marking pass can queue (40 B each; four waves
+// per workgroup: 30 KB of LDS); FrameParams::mark_queue may lower it.
+constexpr uint32_t kMarkQ = 192;
+
 // Bytes of the shadow raster's column cache (FrameParams::shadow_cols).
 constexpr uint64_t kShadowColBytes = 64ull << 20;
 

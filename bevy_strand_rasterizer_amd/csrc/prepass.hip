@@ -441,10 +441,10 @@ struct MarkJob {
   float q0x, q0y, q0z, q1x, q1y, q1z, zmin, zmax;  // the segment clipped to the page
   uint32_t pti, lo;  // page-table index; frustum-local coarse tile << 6 | owner lane
 };
-constexpr uint32_t kMarkQ = 192;  // queued walks per wave (4 waves: 30 KB of LDS)
+
 struct MarkQueue {
   MarkJob* q;
-  uint32_t* n;     // walks queued (may pass kMarkQ: the rest walked in place)
+  uint32_t* n;     // walks queued (may pass the capacity: the rest walked in place)
   uint32_t* nrec;  // per lane: cells of its slot so far
   uint32_t owner;  // the lane whose slot the walk belongs to
 };
@@ -507,7 +507,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
       if (fp.st.fine_binning_backend != 0u) {  // the visible cells: recorded (up to cap) and counted
         if (rec.mq) {
           const uint32_t j = atomicAdd(rec.mq->n, 1u);
-          if (j < kMarkQ) {
+          if (j < fp.mark_queue) {
             rec.mq->q[j] = MarkJob{dc.p0.x, dc.p0.y, dc.p0.z, dc.p1.x, dc.p1.y, dc.p1.z, zmin, zmax, pti,
                                    (local << 6) | rec.mq->owner};
             continue;
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_project
     if (csr) {  // the wave's queued fine walks, one per lane (the wave is converged here)
       const sr_frustum_desc& fd = fp.fr[fi].d;
       const uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
-      const uint32_t nj = min(jn[w], kMarkQ);
+      const uint32_t nj = min(jn[w], fp.mark_queue);
       for (uint32_t j = lane; j < nj; j += 64u) {
         const MarkJob jb = jobs[w][j];
         const uint32_t owner = jb.lo & 63u, local = jb.lo >> 6;
