@@ -427,9 +427,6 @@ __device__ __forceinline__ void coarse_tile_box(const sr_frustum_desc& fd, uint3
 
 constexpr uint32_t kHitOverflow = 0xFFu;
 
-// Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst;
-// n counts every visible cell, also past cap).  SCATTER: the slot's next hit
-// goes to base + n.
 // Marking defers each (slot, page) fine walk to a queue of its wave: the
 // lanes walk their segments' coarse tiles and slices, then the wave runs the
 // queued fine walks one per lane, so a slot with many pages does not hold the
@@ -449,6 +446,9 @@ struct MarkQueue {
   uint32_t owner;  // the lane whose slot the walk belongs to
 };
 
+// Per-slot state of a walk (CSR backend).  MARK: the cells recorded (dst;
+// every visible cell is counted, also past cap: in n, or with a queue in the
+// owner lane's LDS counter).  SCATTER: the slot's next hit goes to base + n.
 struct HitRec {
   uint32_t* dst;  // null: not recording; hit j at dst[j * stride]
   uint32_t n;
