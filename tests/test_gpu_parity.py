@@ -68,6 +68,9 @@ def compare(w, ctr, gpu, ref):
 CASES = {
     "tiny-f8": lambda: scene.config_tiny(),
     "tiny-f16": lambda: scene.config_tiny(froxel=16, res=96, light_res=80),
+    # 12-px froxels: tile edges that are not powers of two (divided, not
+    # multiplied by a reciprocal, in the walks) and 144-pixel raster tiles
+    "tiny-f12": lambda: scene.config_tiny(froxel=12, res=96, light_res=84),
     "tiny-2lights-8layers": lambda: scene.config_tiny(strands=40, lights=2, dom_layers=8, res=72),
     "small-f16": lambda: scene.config_small(froxel=16),
     "small-f8": lambda: scene.config_small(froxel=8),
@@ -98,6 +101,7 @@ CSR_CASES = {
     "tiny-2lights-8layers-csr": lambda: csr(scene.config_tiny(strands=40, lights=2, dom_layers=8, res=72)),
     "small-f16-csr": lambda: csr(scene.config_small(froxel=16)),
     "small-f8-csr": lambda: csr(scene.config_small(froxel=8)),
+    "tiny-f12-csr": lambda: csr(scene.config_tiny(froxel=12, res=96, light_res=84)),
     # thousands of strands over one coarse tile: pages of thousands of hits
     # (several steps of the per-page cell sort)
     "dense-one-tile-csr": lambda: csr(scene.config_tiny(strands=3000, points=8, res=32, light_res=32)),
