@@ -155,8 +155,9 @@ __device__ __forceinline__ uint32_t find_instance(const FrameParams& fp, uint32_
 
 // Evaluates one (frustum, index position) slot.  Returns true and the
 // packed endpoints when the segment survives culling and clipping.
-// scr receives the endpoints' raw screen coordinates as the rasterizers
-// compute them (world point with w = 1 through world_to_screen_raw).
+// scr receives the endpoints' raw screen coordinates, which the rasterizers
+// use as they are: the world point (w included) through world_to_screen_raw
+// (get_segment_vertices_from_index_unchecked, strand_rasterizer.wgsl:289-303).
 __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, uint32_t* ii_out, uint32_t* strand_out,
                              uint32_t* seg_out, sr_projected_segment* out, float4* scr) {
   if (fp.NI == 0u) return false;
@@ -191,10 +192,8 @@ __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, u
   out->p0_xy = pack2x16unorm(wclamp(c.p0.x / ssx, 0.0f, 1.0f), wclamp(c.p0.y / ssy, 0.0f, 1.0f));
   out->p1_xy = pack2x16unorm(wclamp(c.p1.x / ssx, 0.0f, 1.0f), wclamp(c.p1.y / ssy, 0.0f, 1.0f));
   out->depths = pack2x16unorm(wclamp(c.p0.z, 0.0f, 1.0f), wclamp(c.p1.z, 0.0f, 1.0f));
-  const V4 s0 = p0w.w == 1.0f ? r0 : world_to_screen_raw(V4{p0w.x, p0w.y, p0w.z, 1.0f}, fr.clip, vw, vh);
-  const V4 s1 = p1w.w == 1.0f ? r1 : world_to_screen_raw(V4{p1w.x, p1w.y, p1w.z, 1.0f}, fr.clip, vw, vh);
-  scr[0] = make_float4(s0.x, s0.y, s0.z, s0.w);
-  scr[1] = make_float4(s1.x, s1.y, s1.z, s1.w);
+  scr[0] = make_float4(r0.x, r0.y, r0.z, r0.w);
+  scr[1] = make_float4(r1.x, r1.y, r1.z, r1.w);
   *ii_out = ii;
   *strand_out = strand;
   *seg_out = p;

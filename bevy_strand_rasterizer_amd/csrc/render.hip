@@ -163,8 +163,9 @@ struct ShadowBatch {
 };
 
 // The endpoints' raw screen coordinates of a reference's segment in frustum
-// fi (world_v3 then world_to_screen_raw, w = 1), as the projection pass
-// stored them for the segment's slot.
+// fi (world_from_local times the vertex with w = 1, then world_to_screen_raw
+// of that world point, w included), as the projection pass stored them for
+// the segment's slot.
 __device__ __forceinline__ void ref_screen(const FrameParams& fp, uint32_t fi, const InstRec& in, uint32_t seg,
                                            float4* a, float4* c) {
   const float4* sc = fp.slot_screen + 2ull * ((size_t)fi * fp.slots + in.pos_base + seg);

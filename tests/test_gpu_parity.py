@@ -167,6 +167,34 @@ def test_parity_multiview():
     compare(w, ctr, gpu, ref)
 
 
+def instance_transform(kind: str) -> np.ndarray:
+    """world_from_local (column-major) turning the scalp 0.3 rad about its
+    centre, scaled by 1.1 and shifted; "projective" adds 0.05 x to w, so world
+    points have w != 1, which the rasterizers keep when they project
+    (strand_rasterizer.wgsl:289-303)."""
+    c = np.array(scene.HEAD_CENTER, np.float64)
+    a = 0.3
+    rot = np.array([[np.cos(a), 0.0, np.sin(a)], [0.0, 1.0, 0.0], [-np.sin(a), 0.0, np.cos(a)]]) * 1.1
+    m = np.eye(4)
+    m[:3, :3] = rot
+    m[:3, 3] = c - rot @ c + np.array([0.01, -0.005, 0.02])
+    if kind == "projective":
+        m[3, 0] = 0.05
+    return m.T.reshape(-1).astype(np.float32)
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("kind", ["affine", "projective"])
+def test_parity_instance_transform(kind, backend):
+    """A transformed instance (affine, and with a projective row)."""
+    w = scene.config_tiny(strands=60, res=80, light_res=72)
+    w.instances["world_from_local"][0] = instance_transform(kind)
+    w.settings["fine_binning_backend"] = backend
+    ref = oracle.render(w)
+    ctr, gpu = gpu_frame(w)
+    compare(w, ctr, gpu, ref)
+
+
 def test_backends_agree_multiframe():
     """Both fine-binning backends give byte-identical dumps, frame after
     frame, on the medium workload."""
