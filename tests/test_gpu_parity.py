@@ -5,6 +5,8 @@ the canonical dumps is compared byte for byte); deep-shadow layers, shading
 atlas and images are within 1e-3 max abs in linear FP32, images reach
 PSNR >= 50 dB.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -189,6 +191,32 @@ def test_parity_instance_transform(kind, backend):
     """A transformed instance (affine, and with a projective row)."""
     w = scene.config_tiny(strands=60, res=80, light_res=72)
     w.instances["world_from_local"][0] = instance_transform(kind)
+    w.settings["fine_binning_backend"] = backend
+    ref = oracle.render(w)
+    ctr, gpu = gpu_frame(w)
+    compare(w, ctr, gpu, ref)
+
+
+def three_instances():
+    """Three instances in one frame: the tiny scalp, a second scalp asset
+    shifted right, and the first asset again shifted left (per-instance
+    index positions, asset ranges and materials)."""
+    w = scene.config_tiny(strands=40, points=6, res=80, light_res=72)
+    second = scene.scalp_asset(30, 5, 19, 0.004, 2.0)
+    extra = []
+    for asset_id, shift in ((1, (0.03, 0.0, 0.01)), (0, (-0.03, 0.01, -0.01))):
+        inst = scene.identity_instance(asset_id)
+        m = np.eye(4)
+        m[:3, 3] = shift
+        inst["world_from_local"] = m.T.reshape(-1).astype(np.float32)
+        extra.append(inst)
+    return dataclasses.replace(w, name="tiny-3-instances", assets=[w.assets[0], second],
+                               instances=np.concatenate([w.instances] + extra))
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_parity_three_instances(backend):
+    w = three_instances()
     w.settings["fine_binning_backend"] = backend
     ref = oracle.render(w)
     ctr, gpu = gpu_frame(w)
