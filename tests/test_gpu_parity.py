@@ -223,6 +223,25 @@ def test_parity_three_instances(backend):
     compare(w, ctr, gpu, ref)
 
 
+CULL_SETTINGS = {
+    "no-stochastic-cull": {"stochastic_cull_enabled": 0},
+    "half-shadow-keep": {"shadow_keep_probability": 0.5, "target_strands_per_pixel": 0.5},
+}
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+@pytest.mark.parametrize("cull", list(CULL_SETTINGS))
+def test_parity_cull_settings(cull, backend):
+    """Stochastic culling off, and a shadow keep probability below one with
+    fewer strands per pixel (camera and shadow slots keep different strands)."""
+    w = scene.config_tiny(strands=60, res=80, light_res=72)
+    w.settings.update(CULL_SETTINGS[cull])
+    w.settings["fine_binning_backend"] = backend
+    ref = oracle.render(w)
+    ctr, gpu = gpu_frame(w)
+    compare(w, ctr, gpu, ref)
+
+
 def test_backends_agree_multiframe():
     """Both fine-binning backends give byte-identical dumps, frame after
     frame, on the medium workload."""
