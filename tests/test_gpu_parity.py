@@ -98,12 +98,24 @@ def cache_workload():
 
 CASES["cache-asset"] = cache_workload
 
+
+def rect_froxels(w, fx: int, fy: int):
+    """The workload with every frustum's froxels fx x fy pixels."""
+    cfgs = [(int(f["screen_width"]), int(f["screen_height"]), fx, fy, int(f["depth_slices"])) for f in w.frusta]
+    return dataclasses.replace(w, name=f"{w.name}-f{fx}x{fy}", frusta=scene.frustum_table(cfgs, list(w.frusta["kind"])),
+                               settings=dict(w.settings))
+
+
+# non-square froxels: 16 x 8 pixel tiles
+CASES["tiny-f16x8"] = lambda: rect_froxels(scene.config_tiny(strands=60, res=96, light_res=80), 16, 8)
+
 CSR_CASES = {
     "tiny-f8-csr": lambda: csr(scene.config_tiny()),
     "tiny-2lights-8layers-csr": lambda: csr(scene.config_tiny(strands=40, lights=2, dom_layers=8, res=72)),
     "small-f16-csr": lambda: csr(scene.config_small(froxel=16)),
     "small-f8-csr": lambda: csr(scene.config_small(froxel=8)),
     "tiny-f12-csr": lambda: csr(scene.config_tiny(froxel=12, res=96, light_res=84)),
+    "tiny-f16x8-csr": lambda: csr(rect_froxels(scene.config_tiny(strands=60, res=96, light_res=80), 16, 8)),
     # thousands of strands over one coarse tile: pages of thousands of hits
     # (several steps of the per-page cell sort)
     "dense-one-tile-csr": lambda: csr(scene.config_tiny(strands=3000, points=8, res=32, light_res=32)),
