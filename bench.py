@@ -381,7 +381,7 @@ def main():
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent frames and views shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
