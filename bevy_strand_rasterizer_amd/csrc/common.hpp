@@ -141,6 +141,9 @@ struct FrameParams {
   uint32_t* cam_order;               // [coarse_total * 16] camera runs, heaviest first
   uint32_t* shadow_order;            // [coarse_total * 16] shadow runs, heaviest first
   uint32_t* run_hist;                // [2 * kOrderBlocks * 32] run-order bucket counts
+  uint32_t* page_bounds;             // [page_cap + 1] first sorted hit of each page (CSR)
+  uint32_t* page_order;              // [page_cap] pages, heaviest first (CSR)
+  uint32_t* page_hist;               // [kOrderBlocks * 32] page-order bucket counts
   // page CSR backend
   // page CSR backend: every (page, cell, slot) hit is written in slot order
   // at scanned offsets, then one stable radix sort by (page, cell) puts the

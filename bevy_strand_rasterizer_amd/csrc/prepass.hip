@@ -1130,11 +1130,74 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
 // strand_prepass.wgsl:1313-1327) at its final position.  Hits arrive in slot
 // order; kPageStep per step, wave w ranks the w-th quarter by ballots in
 // rounds of 64, so each cell keeps slot order = (inst, seg) order; the step
-// is reordered by cell in LDS before the references are written.  A page's
-// hits [start, end) are found first: the lower bounds of (page << 8) and
-// ((page + 1) << 8) in the sorted keys, each searched by half the workgroup
-// in rounds of a 128-way split (four rounds cover 2^28 keys).  Dropped
-// pages' hits (page >= page_cap) sort after every kept page.
+// is reordered by cell in LDS before the references are written.  Pages are
+// taken heaviest first (page_order), their hits from page_bounds.
+// Each page's hits [bounds[p], bounds[p + 1]) in the page-sorted keys: the
+// lower bound of p << 8, one thread per page boundary (dropped pages' hits,
+// page >= page_cap, sort after every kept page).
+__global__ __launch_bounds__(256) void k_page_bounds(FrameParams fp, const uint32_t* keys) {
+  const uint32_t P = fp.counters[C_PAGES];
+  const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= P; p += gridDim.x * blockDim.x) {
+    const uint64_t target = (uint64_t)p << 8;
+    uint32_t lo = 0u, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if ((uint64_t)keys[mid] < target) lo = mid + 1u; else hi = mid;
+    }
+    fp.page_bounds[p] = lo;
+  }
+}
+
+// Page order for the per-page cell sort: pages by descending log2 of their
+// hit count (a counting sort over 32 buckets, kOrderBlocks blocks counting
+// their chunk of pages, then placing them), so the heaviest pages start
+// first and none is left for the end.  The order inside a bucket does not
+// matter: every page writes only its own references and cells.
+__device__ __forceinline__ uint32_t page_bucket(const FrameParams& fp, uint32_t p) {
+  return 31u - (uint32_t)__clz((fp.page_bounds[p + 1u] - fp.page_bounds[p]) | 1u);
+}
+
+__device__ __forceinline__ uint2 page_chunk(const FrameParams& fp) {
+  const uint32_t P = fp.counters[C_PAGES], chunk = (P + kOrderBlocks - 1u) / kOrderBlocks;
+  const uint32_t lo = min(P, blockIdx.x * chunk);
+  return make_uint2(lo, min(P, lo + chunk));
+}
+
+__global__ __launch_bounds__(256) void k_page_order_hist(FrameParams fp) {
+  __shared__ uint32_t cnt[32];
+  const uint2 c = page_chunk(fp);
+  if (threadIdx.x < 32u) cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  for (uint32_t p = c.x + threadIdx.x; p < c.y; p += blockDim.x) atomicAdd(&cnt[page_bucket(fp, p)], 1u);
+  __syncthreads();
+  if (threadIdx.x < 32u) fp.page_hist[blockIdx.x * 32u + threadIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_page_order_scatter(FrameParams fp) {
+  __shared__ uint32_t cur[32], tot[32];
+  const uint2 c = page_chunk(fp);
+  if (threadIdx.x < 32u) {  // bucket totals and this block's start inside its bucket
+    uint32_t t = 0u, before = 0u;
+#pragma unroll 8
+    for (uint32_t k = 0; k < kOrderBlocks; ++k) {
+      const uint32_t v = fp.page_hist[k * 32u + threadIdx.x];
+      t += v;
+      before += k < blockIdx.x ? v : 0u;
+    }
+    tot[threadIdx.x] = t;
+    cur[threadIdx.x] = before;
+  }
+  __syncthreads();
+  if (threadIdx.x < 32u) {  // heavier buckets come first
+    uint32_t heavier = 0u;
+    for (uint32_t k = threadIdx.x + 1u; k < 32u; ++k) heavier += tot[k];
+    cur[threadIdx.x] += heavier;
+  }
+  __syncthreads();
+  for (uint32_t p = c.x + threadIdx.x; p < c.y; p += blockDim.x) fp.page_order[atomicAdd(&cur[page_bucket(fp, p)], 1u)] = p;
+}
+
 constexpr uint32_t kPageRounds = 8u;                    // rounds of 64 per wave and step
 constexpr uint32_t kPageStep = 4u * 64u * kPageRounds;  // hits per step
 
@@ -1145,46 +1208,13 @@ __global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32
   __shared__ uint32_t lval[kPageStep];  // the step's hits in cell order
   __shared__ uint8_t lcell[kPageStep];   // and their cells
   __shared__ uint32_t lds4[4];
-  __shared__ uint32_t bound[4];  // lo of both searches, then hi
   __shared__ uint32_t colmask;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const unsigned long long lower = (1ull << lane) - 1ull;
   const uint32_t P = fp.counters[C_PAGES];
-  const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
-  const uint32_t half = t >> 7, j = t & 127u;  // search half, its sample
-  for (uint32_t page = blockIdx.x; page < P; page += gridDim.x) {
-    // the search keeps the lower bound in [lo, hi]; samples j < 127 split
-    // [lo, hi) evenly, below are monotone in j, and the last sample below
-    // and the first not below close the interval in
-    const uint64_t target = (uint64_t)(page + half) << 8;
-    if (t < 2u) {
-      bound[t] = 0u;
-      bound[2u + t] = n;
-    }
-    __syncthreads();
-    for (;;) {
-      const uint32_t lo = bound[half], hi = bound[2u + half];
-      const bool more = bound[0] != bound[2] || bound[1] != bound[3];
-      const bool sample = hi > lo && j < 127u;
-      uint32_t m = lo;
-      bool below = false;
-      if (sample) {
-        m = lo + (uint32_t)(((uint64_t)(hi - lo) * (j + 1u)) >> 7);
-        below = (uint64_t)keys[m] < target;
-      }
-      __syncthreads();
-      if (!more) break;
-      wcnt[0][t] = below ? 1u : 0u;
-      __syncthreads();
-      if (sample) {
-        const bool next_below = j < 126u && wcnt[0][t + 1u] != 0u;
-        const bool prev_below = j == 0u || wcnt[0][t - 1u] != 0u;
-        if (below && !next_below) bound[half] = m + 1u;
-        if (!below && prev_below) bound[2u + half] = m;
-      }
-      __syncthreads();
-    }
-    const uint32_t start = bound[0], end = bound[1];
+  for (uint32_t pi = blockIdx.x; pi < P; pi += gridDim.x) {
+    const uint32_t page = fp.page_order[pi];
+    const uint32_t start = fp.page_bounds[page], end = fp.page_bounds[page + 1u];
     cur[t] = 0u;
     if (t == 0) colmask = 0u;
     __syncthreads();
@@ -1763,6 +1793,10 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                          ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift, ps.radix_base);
     }
     timer->mark(s, "page_cells");
+    hipLaunchKernelGGL(k_page_bounds, dim3(grid_for((uint64_t)fp.page_cap + 1u, 256, 1u << 16)), dim3(256), 0, s, fp,
+                       ps.hkeys[0]);
+    hipLaunchKernelGGL(k_page_order_hist, dim3(kOrderBlocks), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_page_order_scatter, dim3(kOrderBlocks), dim3(256), 0, s, fp);
     hipLaunchKernelGGL(k_page_cells, dim3(grid_for(fp.page_cap, 1, 1u << 20)), dim3(256), 0, s, fp, ps.hkeys[0],
                        ps.hvals[0]);
     timer->mark(s, "page_bases");

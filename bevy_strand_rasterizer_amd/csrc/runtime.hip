@@ -133,7 +133,7 @@ struct DBuf {
 enum BufId {
   B_COUNTERS, B_BROAD, B_KEEP, B_RANGES, B_SLOT_MASKS, B_WAVE_OFF, B_FR_TASK_BASE, B_FR_CAM_BASE, B_SHADED,
   B_TASKS, B_PROJ, B_SHADING_Q, B_LUT, B_LUT_Z, B_PAGE_TABLE, B_PAGE_META, B_PAGE_TOTALS, B_CELL_COUNTS, B_CELL_OFFSETS,
-  B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS, B_CAM_ORDER, B_SHADOW_ORDER, B_RUN_HIST,
+  B_CELL_CURSORS, B_REFS, B_ACTIVE_FLAGS, B_ACTIVE_Q, B_TILE_WORK, B_TILE_BASES, B_WORK, B_CAM_RUNS, B_SHADOW_RUNS, B_CAM_ORDER, B_SHADOW_ORDER, B_RUN_HIST, B_PAGE_BOUNDS, B_PAGE_ORDER, B_PAGE_HIST,
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SHADOW_COLS, B_SLOT_PROJ, B_SLOT_SCREEN,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
   B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_TABLES,
@@ -610,7 +610,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_CELL_OFFSETS, 4ull * page_cap * kPage}, {B_CELL_CURSORS, 4ull * page_cap * kPage},
       {B_REFS, 12ull * ref_cap}, {B_ACTIVE_FLAGS, 4ull * tiles16}, {B_ACTIVE_Q, 4ull * tiles16},
       {B_TILE_WORK, 16ull * tiles16}, {B_TILE_BASES, 16ull * tiles16}, {B_WORK, 32ull * work_cap},
-      {B_CAM_RUNS, 32ull * tiles16}, {B_SHADOW_RUNS, 32ull * tiles16}, {B_CAM_ORDER, 4ull * tiles16}, {B_SHADOW_ORDER, 4ull * tiles16}, {B_RUN_HIST, 4ull * 2 * kOrderBlocks * 32}, {B_DOM_DEPTH, 4ull * dom_px},
+      {B_CAM_RUNS, 32ull * tiles16}, {B_SHADOW_RUNS, 32ull * tiles16}, {B_CAM_ORDER, 4ull * tiles16}, {B_SHADOW_ORDER, 4ull * tiles16}, {B_RUN_HIST, 4ull * 2 * kOrderBlocks * 32}, {B_PAGE_BOUNDS, csr ? 4ull * (page_cap + 1) : 0ull},
+      {B_PAGE_ORDER, csr ? 4ull * page_cap : 0ull}, {B_PAGE_HIST, csr ? 4ull * kOrderBlocks * 32 : 0ull}, {B_DOM_DEPTH, 4ull * dom_px},
       {B_DOM_OPACITY, 4ull * dom_px * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
       {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
       {B_COMPOSITE, 16ull * img_px}, {B_SHADOW_COLS, kShadowColBytes}, {B_SLOT_PROJ, 12ull * task_cap}, {B_SLOT_SCREEN, 32ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
@@ -707,6 +708,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.cam_order = sl.buf[B_CAM_ORDER].as<uint32_t>();
   fp.shadow_order = sl.buf[B_SHADOW_ORDER].as<uint32_t>();
   fp.run_hist = sl.buf[B_RUN_HIST].as<uint32_t>();
+  fp.page_bounds = sl.buf[B_PAGE_BOUNDS].as<uint32_t>();
+  fp.page_order = sl.buf[B_PAGE_ORDER].as<uint32_t>();
+  fp.page_hist = sl.buf[B_PAGE_HIST].as<uint32_t>();
   fp.dom_depth = sl.buf[B_DOM_DEPTH].as<float>();
   fp.dom_opacity = sl.buf[B_DOM_OPACITY].as<float>();
   fp.stamp = sl.buf[B_STAMP].as<float>();
