@@ -1123,15 +1123,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
   }
 }
 
-// K14 + the final order for the page-sorted hits, one workgroup per page:
-// the page's cell histogram, the page-local exclusive scan (cell counts,
-// offsets, page total, mask of fine tiles with work), then a stable counting
-// sort by cell that writes each hit's reference (make_fine_seg_ref,
-// strand_prepass.wgsl:1313-1327) at its final position.  Hits arrive in slot
-// order; kPageStep per step, wave w ranks the w-th quarter by ballots in
-// rounds of 64, so each cell keeps slot order = (inst, seg) order; the step
-// is reordered by cell in LDS before the references are written.  Pages are
-// taken heaviest first (page_order), their hits from page_bounds.
 // Each page's hits [bounds[p], bounds[p + 1]) in the page-sorted keys: the
 // lower bound of p << 8, one thread per page boundary (dropped pages' hits,
 // page >= page_cap, sort after every kept page).
@@ -1198,6 +1189,15 @@ __global__ __launch_bounds__(256) void k_page_order_scatter(FrameParams fp) {
   for (uint32_t p = c.x + threadIdx.x; p < c.y; p += blockDim.x) fp.page_order[atomicAdd(&cur[page_bucket(fp, p)], 1u)] = p;
 }
 
+// K14 + the final order for the page-sorted hits, one workgroup per page:
+// the page's cell histogram, the page-local exclusive scan (cell counts,
+// offsets, page total, mask of fine tiles with work), then a stable counting
+// sort by cell that writes each hit's reference (make_fine_seg_ref,
+// strand_prepass.wgsl:1313-1327) at its final position.  Hits arrive in slot
+// order; kPageStep per step, wave w ranks the w-th quarter by ballots in
+// rounds of 64, so each cell keeps slot order = (inst, seg) order; the step
+// is reordered by cell in LDS before the references are written.  Pages are
+// taken heaviest first (page_order), their hits from page_bounds.
 constexpr uint32_t kPageRounds = 8u;                    // rounds of 64 per wave and step
 constexpr uint32_t kPageStep = 4u * 64u * kPageRounds;  // hits per step
 
