@@ -1773,7 +1773,9 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                           nullptr, ps.partials, s);
   }
   timer->mark(s, "project_write");
-  hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  // one slot per thread: the pass is latency-bound, and a thread that owns
+  // one slot has no later slot waiting behind its loads
+  hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, 1u << 20)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
     timer->mark(s, "sort_hits");
     // stable LSD passes over the page bits of (key, position) pairs; the last
