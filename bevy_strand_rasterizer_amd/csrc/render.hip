@@ -894,7 +894,9 @@ void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
 }
 
 void launch_shading(const FrameParams& fp, hipStream_t s) {
-  hipLaunchKernelGGL(k_shading, dim3(256u * 16u), dim3(128), 0, s, fp);
+  // 16k workgroups: each thread shades a few segments, so the uneven per-segment
+  // cost (lights, lobe windows) leaves a short tail
+  hipLaunchKernelGGL(k_shading, dim3(256u * 64u), dim3(128), 0, s, fp);
 }
 
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
