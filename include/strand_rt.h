@@ -441,14 +441,14 @@ typedef enum sr_dump_kind {
   SR_DUMP_DEPTH = 22,        /* float per camera frustum W*H */
   SR_DUMP_COMPOSITE = 23,    /* float4 per camera frustum W*H */
   SR_DUMP_STAMP = 24,        /* float per light frustum stamp target */
-  SR_DUMP_OPAQUE_TILES = 26, /* uint32 (covered, quantised farthest depth) per fine tile when opaque depth was given */
   SR_DUMP_TELEMETRY = 25,    /* uint32 telemetry when settings.telemetry_enabled (finalize_telemetry layout):
                                 [0] pages [1] tasks [2] page candidates [3] fine refs [4] pages with candidates
                                 [5] max candidates per page, [8..39] log2 histogram of candidates per page,
                                 [40 .. 40+page_cap) candidates per page, then 11 words per frustum: visible
                                 instances, visible strands, retained strands, emitted segments, pages,
                                 candidates, refs, raster runs, raster load, max page candidates, max raster load */
-  SR_DUMP_KIND_COUNT = 25
+  SR_DUMP_OPAQUE_TILES = 26, /* uint32 (covered, quantised farthest depth) per fine tile when opaque depth was given */
+  SR_DUMP_KIND_COUNT = 27     /* one past the largest kind */
 } sr_dump_kind;
 
 /* ------------------------------------------------------------------------ */

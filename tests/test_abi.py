@@ -132,3 +132,19 @@ def test_frustum_table_builder():
     assert t[1]["fine_depth_tile_base"] == 240 * 135
     assert t[1]["bucket_base"] == 240 * 135 * 16
     assert t[1]["kind"] == 1 and t[0]["kind"] == 0
+
+
+def dump_kinds():
+    """sr_dump_kind enumerators of the header (name -> value)."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    body = re.search(r"typedef enum sr_dump_kind \{(.*?)\} sr_dump_kind;", text, flags=re.S).group(1)
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"SR_DUMP_([A-Z_]+)\s*=\s*(\d+)", body)}
+
+
+def test_dump_kind_count_exceeds_every_kind():
+    kinds = dump_kinds()
+    count = kinds.pop("KIND_COUNT")
+    assert count == max(kinds.values()) + 1
+    assert sorted(kinds.values()) == list(range(count)), "dump kinds are dense and listed once"
+    assert list(kinds.values()) == sorted(kinds.values()), "enumerators in numeric order"
+    assert {k.lower(): v for k, v in kinds.items()} == abi.DUMP
