@@ -25,6 +25,7 @@ BINNING_FRUSTUM_SHIFT = 3
 CONFIG_CSR_FEW_HITS = 2  # sr_device_config.flags
 CONFIG_HIP_GRAPHS = 4
 CONFIG_TIGHT_CAPACITY = 8
+NOALLOC_SCRATCH_BYTES = 256 << 20  # SR_NOALLOC_SCRATCH_BYTES
 
 # sr_status
 SR_OK = 0
@@ -190,6 +191,8 @@ _SIGNATURES = {
     "sr_asset_upload": (C.c_int, [C.c_void_p, C.POINTER(AssetSlabs), C.POINTER(AssetInfo)]),
     "sr_asset_release": (C.c_int, [C.c_void_p, C.c_uint32]),
     "sr_frame_submit": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_void_p]),
+    "sr_frame_submit_noalloc": (C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
     "sr_frame_wait": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sr_frame_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sr_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -39,6 +39,35 @@ void set_global_error(const std::string& e) {
 
 inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+// ------------------------------------------------------------ no-allocation scratch
+// Every intermediate buffer of a frame drawn by sr_frame_submit_noalloc lives
+// in this array.  Each GPU the library runs on holds its own copy until the
+// process ends; host code never passes the array's name to a kernel (on the
+// host it is a stand-in, not a device address) but the address that
+// hipGetSymbolAddress returned on that device, kept in g_scratch_ptr.
+constexpr uint64_t kScratchBytes = SR_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's device address on `dev`, looked up once per device
+// with that device current.
+hipError_t scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 // ------------------------------------------------------------ slabs
 struct SlabAllocator {
   struct Slab {
@@ -107,25 +136,32 @@ struct AssetRec {
   uint32_t vertex_count = 0, material_count = 0;
 };
 
-// grow-only device buffer
+// grow-only device buffer, or (borrowed) a span of memory the buffer does
+// not own: the no-allocation scratch or a caller's output
 struct DBuf {
   void* p = nullptr;
   uint64_t bytes = 0;
+  bool borrowed = false;
   bool ensure(uint64_t need) {
     need = std::max<uint64_t>(need, 256);
-    if (need <= bytes) return true;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
+    if (need <= bytes && !borrowed) return true;
+    free_();
     uint64_t b = align_up(need + need / 4, 4096);
     if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; return false; }
     bytes = b;
     return true;
   }
+  void borrow(void* q, uint64_t n) {
+    free_();
+    p = q;
+    bytes = n;
+    borrowed = true;
+  }
   void free_() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -235,6 +271,11 @@ struct sr_context {
   uint32_t flags = 0;             // SR_CONFIG_* bits
   uint64_t hbm_budget = 0;
   FrameSlot& main() { return *slots[0]; }
+  // sr_frame_submit_noalloc: a slot whose buffers are carved out of this
+  // device's scratch array (set up at sr_create)
+  FrameSlot* na = nullptr;
+  void* scratch = nullptr;
+  std::string scratch_err;
 };
 
 namespace {
@@ -327,6 +368,15 @@ int sr_create(const sr_device_config* cfg, sr_context** out) {
     return fail(nullptr, SR_ERR_DEVICE, "sr_create: " + err);
   }
   ctx->slots.push_back(sl);
+  // the no-allocation path: this device's scratch address and its own slot
+  // (stream and completion event), so that sr_frame_submit_noalloc creates
+  // nothing itself
+  if ((e = scratch_address(dev, &ctx->scratch)) != hipSuccess) {
+    ctx->scratch = nullptr;
+    ctx->scratch_err = std::string("scratch address: ") + hipGetErrorString(e);
+  } else if (!(ctx->na = new_slot(&err))) {
+    ctx->scratch_err = "no-allocation slot: " + err;
+  }
   *out = ctx;
   return SR_OK;
 }
@@ -336,6 +386,8 @@ int sr_destroy(sr_context* ctx) {
   (void)hipSetDevice(ctx->device);
   for (FrameSlot* sl : ctx->slots) free_slot(sl);
   ctx->slots.clear();
+  free_slot(ctx->na);  // its buffers are borrowed: nothing is freed but the stream and events
+  ctx->na = nullptr;
   delete ctx;
   return SR_OK;
 }
@@ -408,6 +460,7 @@ int sr_asset_release(sr_context* ctx, uint32_t id) {
     return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_asset_release: unknown asset");
   (void)hipSetDevice(ctx->device);
   for (FrameSlot* sl : ctx->slots) (void)hipStreamSynchronize(sl->stream);  // frames in flight may read it
+  if (ctx->na) (void)hipStreamSynchronize(ctx->na->stream);
   AssetRec& r = ctx->assets[id];
   const sr_device_ptr* owned[6] = {&r.info.vertices, &r.info.indices, &r.info.meta,
                                    &r.info.geo,      &r.info.materials, &r.pos_strand};
@@ -430,12 +483,76 @@ int sr_set_profiling(sr_context* ctx, int32_t enabled) {
 
 namespace {
 
+// Where a no-allocation frame puts its buffers and outputs: every buffer is a
+// 256-byte aligned span of `base` (the scratch array, `bytes` long), except
+// the composite and depth, which go straight to the caller's device memory;
+// counters (optional) receives the frame's sr_frame_counters on the device.
+struct NoAllocTarget {
+  char* base;
+  uint64_t bytes;
+  float4* composite;
+  float* depth;
+  sr_frame_counters* counters;
+};
+
+// Table bytes handed to k_put_words per launch (kernel arguments, so that no
+// staging memory is needed).
+constexpr uint32_t kPutWords = 480;
+struct PutWords {
+  uint32_t w[kPutWords];
+  uint32_t n;
+};
+
+__global__ void k_put_words(uint32_t* dst, PutWords c) {
+  for (uint32_t i = threadIdx.x; i < c.n; i += blockDim.x) dst[i] = c.w[i];
+}
+
+// sr_frame_counters of a frame from its device counter words and the
+// capacities it ran with (the same rule as frame_wait_slot).
+struct CounterCaps {
+  uint32_t coarse_total, task_cap, ref_cap, work_cap, hit_cap;
+};
+
+__host__ __device__ inline sr_frame_counters counters_from_words(const uint32_t* c, CounterCaps k) {
+  sr_frame_counters fc;
+  fc.binning_tasks = c[C_TASKS];
+  fc.camera_tasks = c[C_CAM_TASKS];
+  fc.allocated_pages = c[C_PAGES];
+  fc.fine_refs = c[C_REFS];
+  fc.active_tiles = c[C_ACTIVE];
+  fc.work_items = c[C_WORK];
+  fc.camera_runs = c[C_CAM_RUNS];
+  fc.shadow_runs = c[C_SHADOW_RUNS];
+  fc.overflow_pages = c[C_OVERFLOW];
+  fc.coarse_tiles = k.coarse_total;
+  fc.task_capacity = k.task_cap;
+  fc.ref_capacity = k.ref_cap;
+  fc.page_drops = c[C_PAGE_DROPS];
+  fc.page_candidates = k.hit_cap ? c[C_CANDS] : 0u;
+  fc.capacity_exceeded = (c[C_REFS] > k.ref_cap || c[C_PAGE_DROPS] > 0u || c[C_WORK] > k.work_cap ||
+                          c[C_OVERFLOW] > 0u || (k.hit_cap && c[C_CANDS] > k.hit_cap))
+                             ? 1u
+                             : 0u;
+  fc.pad = 0u;
+  return fc;
+}
+
+__global__ void k_counters_out(const uint32_t* c, CounterCaps k, sr_frame_counters* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = counters_from_words(c, k);
+}
+
 // Plans and launches one frame on `sl`.  need_bytes (optional) receives the
 // device bytes the slot holds once its buffers fit this frame; dry_run stops
-// after planning.  Asset records are read under ctx->mu.
+// after planning.  With `na`, the slot's buffers are carved out of na->base
+// (nothing is allocated; a frame that does not fit fails with
+// SR_ERR_CAPACITY), the per-frame tables travel as kernel arguments, no graph
+// is captured and shadow history is off.  Asset records are read under
+// ctx->mu.
 int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, hipStream_t stream_in,
-                      std::string& err, uint64_t* need_bytes, bool dry_run) {
+                      std::string& err, uint64_t* need_bytes, bool dry_run, const NoAllocTarget* na = nullptr) {
   if (!fd) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: NULL argument");
+  if (na && fd->opaque_depth)
+    return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: opaque depth needs a staging upload; use sr_frame_submit");
   if (fd->frustum_count == 0 || !fd->frusta || !fd->views)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no frusta");
   if (fd->instance_count && !fd->instances) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
@@ -590,7 +707,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
                                                  csr ? kDigitChunks * 256u : 0u);
 
   // history: keep the previous buffer when the atlas layout is unchanged
-  const bool hist_valid = st.shadow_history_valid && sl.shape.valid && sl.shape.cams == cams &&
+  const bool hist_valid = !na && st.shadow_history_valid && sl.shape.valid && sl.shape.cams == cams &&
                           sl.shape.NI == NI && sl.shape.atlas_texels == atlas_texels;
   const int hist_next = hist_valid ? (sl.hist_cur ^ 1) : 0;
 
@@ -632,26 +749,61 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     *need_bytes = held;
   }
   if (dry_run) return SR_OK;
-  bool grow = false;
-  for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > sl.buf[n.id].bytes;
-  if (grow && sl.submitted) {  // in-flight frames may still use the buffers
-    (void)hipStreamSynchronize(s);
-    (void)hipStreamSynchronize(sl.stream);
-  }
-  if (grow) sl.drop_graphs();  // captured launches hold the old buffer addresses
-  for (const Need& n : needs) {
-    if (!sl.buf[n.id].ensure(n.bytes)) return efail(err, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
+  const uint64_t tbytes = tab_light + sizeof(sr_directional_light) * NL;
+  if (na) {
+    // carve every buffer out of the scratch array; the composite and depth
+    // are the caller's
+    uint64_t off = 0;
+    for (const Need& n : needs) {
+      if (n.id == B_COMPOSITE || n.id == B_DEPTH) continue;
+      off = align_up(off, 256);
+      const uint64_t b = std::max<uint64_t>(n.bytes, 256);
+      if (off + b <= na->bytes) sl.buf[n.id].borrow(na->base + off, b);
+      off += b;
+    }
+    if (off > na->bytes)
+      return efail(err, SR_ERR_CAPACITY,
+                   "sr_frame_submit_noalloc: the frame needs " + std::to_string(off) +
+                       " bytes of scratch; the no-allocation path holds at most " + std::to_string(na->bytes) +
+                       " (SR_NOALLOC_SCRATCH_BYTES)");
+    sl.buf[B_COMPOSITE].borrow(na->composite, 16ull * img_px);
+    sl.buf[B_DEPTH].borrow(na->depth, 4ull * img_px);
+    // per-frame tables as kernel arguments, a few hundred words per launch
+    std::vector<uint32_t> words((tbytes + 3) / 4, 0u);
+    char* wb = reinterpret_cast<char*>(words.data());
+    std::memcpy(wb, fr.data(), sizeof(FrustumRec) * F);
+    if (NI) std::memcpy(wb + tab_inst, ir.data(), sizeof(InstRec) * NI);
+    if (NL) std::memcpy(wb + tab_light, fd->lights, sizeof(sr_directional_light) * NL);
+    uint32_t* dst = sl.buf[B_TABLES].as<uint32_t>();
+    for (size_t i = 0; i < words.size(); i += kPutWords) {
+      PutWords c;
+      c.n = (uint32_t)std::min<size_t>(kPutWords, words.size() - i);
+      std::memcpy(c.w, words.data() + i, 4ull * c.n);
+      k_put_words<<<1, 256, 0, s>>>(dst + i, c);
+    }
+  } else {
+    bool grow = false;
+    for (const Need& n : needs) grow = grow || std::max<uint64_t>(n.bytes, 256) > sl.buf[n.id].bytes || sl.buf[n.id].borrowed;
+    if (grow && sl.submitted) {  // in-flight frames may still use the buffers
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamSynchronize(sl.stream);
+    }
+    if (grow) sl.drop_graphs();  // captured launches hold the old buffer addresses
+    for (const Need& n : needs) {
+      if (!sl.buf[n.id].ensure(n.bytes)) return efail(err, SR_ERR_OUT_OF_MEMORY, "sr_frame_submit: device allocation failed");
+    }
   }
   // ---- upload per-frame tables through pinned staging
-  const uint64_t tbytes = tab_light + sizeof(sr_directional_light) * NL;
   int slot = 0;
-  char* stg = next_staging(sl, tab_bytes + 256, &slot);
-  if (!stg) return efail(err, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
-  std::memset(stg, 0, tbytes);
-  std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
-  if (NI) std::memcpy(stg + tab_inst, ir.data(), sizeof(InstRec) * NI);
-  if (NL) std::memcpy(stg + tab_light, fd->lights, sizeof(sr_directional_light) * NL);
-  HIP_OKS(hipMemcpyAsync(sl.buf[B_TABLES].p, stg, tbytes, hipMemcpyHostToDevice, s), "upload frame tables");
+  char* stg = na ? nullptr : next_staging(sl, tab_bytes + 256, &slot);
+  if (!na && !stg) return efail(err, SR_ERR_OUT_OF_MEMORY, "staging allocation failed");
+  if (stg) {
+    std::memset(stg, 0, tbytes);
+    std::memcpy(stg, fr.data(), sizeof(FrustumRec) * F);
+    if (NI) std::memcpy(stg + tab_inst, ir.data(), sizeof(InstRec) * NI);
+    if (NL) std::memcpy(stg + tab_light, fd->lights, sizeof(sr_directional_light) * NL);
+    HIP_OKS(hipMemcpyAsync(sl.buf[B_TABLES].p, stg, tbytes, hipMemcpyHostToDevice, s), "upload frame tables");
+  }
   if (opaque_floats) {  // the opaque depth travels through its own staging entry
     int oslot = 0;
     char* ostg = next_staging(sl, 4ull * opaque_floats, &oslot);
@@ -662,8 +814,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     HIP_OKS(hipEventRecord(sl.staging_ev[oslot], s), "staging event");
     sl.staging_used[oslot] = true;
   }
-  HIP_OKS(hipEventRecord(sl.staging_ev[slot], s), "staging event");
-  sl.staging_used[slot] = true;
+  if (stg) {
+    HIP_OKS(hipEventRecord(sl.staging_ev[slot], s), "staging event");
+    sl.staging_used[slot] = true;
+  }
 
   FrameParams fp;
   std::memset(&fp, 0, sizeof(fp));
@@ -788,7 +942,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     sl.timer.mark(s, "end");
   };
   sl.timer.begin();
-  if ((ctx->flags & SR_CONFIG_HIP_GRAPHS) && !sl.timer.enabled) {
+  if ((ctx->flags & SR_CONFIG_HIP_GRAPHS) && !sl.timer.enabled && !na) {
     // The sequence reads every per-frame table from device memory, so one
     // capture serves every frame whose launches bake in the same values:
     // FrameParams (buffers, capacities, settings), the launch extents and
@@ -840,6 +994,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   } else {
     launches();
   }
+  if (na && na->counters) {
+    const CounterCaps caps{coarse_total, task_cap, ref_cap, work_cap, hit_cap};
+    k_counters_out<<<1, 64, 0, s>>>(fp.counters, caps, na->counters);
+  }
   HIP_OKS(hipGetLastError(), "sr_frame_submit: launch");
   HIP_OKS(hipEventRecord(sl.done, s), "sr_frame_submit: event");
 
@@ -865,31 +1023,13 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   uint32_t c[C_COUNT];
   HIP_OKS(hipMemcpy(c, sl.buf[B_COUNTERS].p, sizeof(c), hipMemcpyDeviceToHost), "sr_frame_wait: counters");
   const FrameShape& sh = sl.shape;
-  sr_frame_counters fc;
-  std::memset(&fc, 0, sizeof(fc));
-  fc.binning_tasks = c[C_TASKS];
-  fc.camera_tasks = c[C_CAM_TASKS];
-  fc.allocated_pages = c[C_PAGES];
-  fc.fine_refs = c[C_REFS];
-  fc.active_tiles = c[C_ACTIVE];
-  fc.work_items = c[C_WORK];
-  fc.camera_runs = c[C_CAM_RUNS];
-  fc.shadow_runs = c[C_SHADOW_RUNS];
-  fc.overflow_pages = c[C_OVERFLOW];
-  fc.coarse_tiles = sh.coarse_total;
-  fc.task_capacity = sh.task_cap;
-  fc.ref_capacity = sh.ref_cap;
+  const sr_frame_counters fc =
+      counters_from_words(c, CounterCaps{sh.coarse_total, sh.task_cap, sh.ref_cap, sh.work_cap, sh.hit_cap});
   // grow-only capacity planning for the next frame
   if (c[C_REFS] > sh.ref_cap) sl.ref_cap_hint = next_pow2(c[C_REFS]);
   if (c[C_PAGE_DROPS] > 0) sl.page_cap_hint = next_pow2(c[C_MARKED]);
   if (c[C_WORK] > sh.work_cap) sl.work_cap_hint = next_pow2(c[C_WORK]);
   if (sh.hit_cap && c[C_CANDS] > sh.hit_cap) sl.ref_cap_hint = std::max(sl.ref_cap_hint, next_pow2(c[C_CANDS]));
-  fc.page_drops = c[C_PAGE_DROPS];
-  fc.page_candidates = sh.hit_cap ? c[C_CANDS] : 0u;
-  fc.capacity_exceeded = (c[C_REFS] > sh.ref_cap || c[C_PAGE_DROPS] > 0 || c[C_WORK] > sh.work_cap ||
-                          c[C_OVERFLOW] > 0 || (sh.hit_cap && c[C_CANDS] > sh.hit_cap))
-                             ? 1u
-                             : 0u;
   if (out) *out = fc;
   return SR_OK;
 }
@@ -903,6 +1043,20 @@ int sr_frame_submit(sr_context* ctx, const sr_frame_desc* fd, void* stream_ptr) 
   (void)hipSetDevice(ctx->device);
   std::string err;
   int rc = frame_submit_slot(ctx, ctx->main(), fd, (hipStream_t)stream_ptr, err, nullptr, false);
+  return rc == SR_OK ? rc : fail(ctx, rc, err);
+}
+
+int sr_frame_submit_noalloc(sr_context* ctx, const sr_frame_desc* fd, float* composite, float* depth,
+                            sr_frame_counters* counters, void* stream_ptr) {
+  if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+  if (!composite || !depth) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit_noalloc: NULL output");
+  if (!ctx->scratch || !ctx->na)
+    return fail(ctx, SR_ERR_DEVICE, "sr_frame_submit_noalloc: no scratch on this device (" + ctx->scratch_err + ")");
+  (void)hipSetDevice(ctx->device);
+  const NoAllocTarget na{static_cast<char*>(ctx->scratch), kScratchBytes, reinterpret_cast<float4*>(composite), depth,
+                         counters};
+  std::string err;
+  int rc = frame_submit_slot(ctx, *ctx->na, fd, (hipStream_t)stream_ptr, err, nullptr, false, &na);
   return rc == SR_OK ? rc : fail(ctx, rc, err);
 }
 

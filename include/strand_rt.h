@@ -501,6 +501,29 @@ int sr_asset_release(sr_context* ctx, uint32_t geo_id);
  * stream).  All work is enqueued; nothing is read back. */
 int sr_frame_submit(sr_context* ctx, const sr_frame_desc* frame, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the frame of sr_frame_submit, with every intermediate
+ * buffer carved out of one file-scope device array of
+ * SR_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends.  The call allocates nothing: the per-frame tables travel as
+ * kernel arguments, no graph is captured, no staging memory is used.
+ *   composite  device memory, float4 per camera pixel in frustum order (the
+ *              layout of SR_DUMP_COMPOSITE)
+ *   depth      device memory, float per camera pixel (SR_DUMP_DEPTH)
+ *   counters   device memory for one sr_frame_counters, or NULL
+ *   stream     hipStream_t; NULL = the context's no-allocation stream
+ * A frame whose buffers do not fit the array fails with SR_ERR_CAPACITY (at
+ * fixed capacities: a frame reporting capacity_exceeded is incomplete and is
+ * not regrown); opaque depth is not supported and shadow history is off.
+ * Every call on one GPU shares that GPU's array (the last call's contents
+ * stay there; what must start at zero is cleared by each call): calls that
+ * overlap on one GPU race, so serialise them on one stream or with events.
+ * sr_frame_wait and sr_frame_dump do not see these frames. */
+#define SR_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int sr_frame_submit_noalloc(sr_context* ctx, const sr_frame_desc* frame, float* composite, float* depth,
+                            sr_frame_counters* counters, void* stream);
+
 /* Wait for the last submitted frame and fetch its counters. */
 int sr_frame_wait(sr_context* ctx, sr_frame_counters* out_counters);
 
