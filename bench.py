@@ -260,21 +260,27 @@ def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool 
     return out
 
 
-def run_stress(views: int, views_per_frame: int, backend: int, repeats: int = 2):
-    """The stress configuration in frames of `views_per_frame` 4K views plus
-    the 4 lights: each frame shape first renders until its capacities fit
-    (the runtime grows them after a frame reports capacity_exceeded), then
-    `repeats` passes over all views are timed."""
+def run_stress(views: int, views_per_frame: int, first_frame_views: int, backend: int, repeats: int = 2):
+    """The stress configuration as batches of `views` 4K views: the first
+    frame of a batch draws `first_frame_views` cameras and builds the 4
+    lights' 8-layer shadow maps, every later frame draws `views_per_frame`
+    cameras with those maps (SR_FRAME_REUSE_LIGHT_DOMS).  Every frame shape
+    first renders until its capacities fit (the runtime grows them after a
+    frame reports capacity_exceeded), then `repeats` batches are timed, each
+    building its lights again."""
     import torch
 
     from bevy_strand_rasterizer_amd import scene
     from bevy_strand_rasterizer_amd.runtime import StrandRenderer
 
-    ws = [scene.retile(w, 0, 0, backend) for w in scene.config_stress_chunks(views=views, views_per_frame=views_per_frame)]
+    ws = [scene.retile(w, 0, 0, backend)
+          for w in scene.config_stress_chunks(views=views, views_per_frame=views_per_frame,
+                                              first_frame_views=first_frame_views)]
     r = StrandRenderer(device=0)
     r.upload(ws[0].assets[0])
     grow = 0
-    for w in ws[:1]:
+    per_frame = []
+    for w in ws:
         for _ in range(6):
             ctr = r.render(w)
             print(f"bench: stress warm {w.name}: refs {ctr['fine_refs']} / {ctr['ref_capacity']}, "
@@ -282,9 +288,11 @@ def run_stress(views: int, views_per_frame: int, backend: int, repeats: int = 2)
             if not ctr["capacity_exceeded"]:
                 break
             grow += 1
-    if ctr["capacity_exceeded"]:
-        r.close()
-        return {"error": "capacity not reached", "views_per_frame": views_per_frame, "counters": ctr}
+        if ctr["capacity_exceeded"]:
+            r.close()
+            return {"error": "capacity not reached", "frame": w.name, "counters": ctr}
+        per_frame.append({"frame": w.name, "cameras": w.camera_count, "fine_refs": ctr["fine_refs"],
+                          "binning_tasks": ctr["binning_tasks"], "shadow_runs": ctr["shadow_runs"]})
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     exceeded = 0
@@ -298,12 +306,13 @@ def run_stress(views: int, views_per_frame: int, backend: int, repeats: int = 2)
     r.close()
     segs = ws[0].input_segments
     return {"workload": f"stress-1000k-x64-4k-{views}views-4lights-8layers", "views": views,
-            "views_per_frame": views_per_frame, "frames": len(ws) * repeats,
+            "frames_per_batch": len(ws), "views_per_frame": [w.camera_count for w in ws],
+            "lights_built_per_batch": 1, "batches": repeats,
             "fine_binning_backend": ["scatter", "csr"][backend],
             "frames_per_s": round(views * repeats / dt, 3), "ms_per_view": round(dt * 1e3 / (views * repeats), 3),
+            "ms_per_batch": round(dt * 1e3 / repeats, 2),
             "msegments_per_s": round(segs * views * repeats / dt / 1e6, 2), "warm_growths": grow,
-            "frames_over_capacity": exceeded,
-            "counters_last_frame": {k: v for k, v in c.items() if not k.startswith("pad")}}
+            "frames_over_capacity": exceeded, "frames": per_frame}
 
 
 def run_serving(seconds: float, rate: float, slots: int, max_views: int, budget_gib: float, seed: int = 7):
@@ -375,7 +384,8 @@ def main():
     ap.add_argument("--report-frames", type=int, default=10)
     ap.add_argument("--stress", action="store_true",
                     help="also run the stress configuration (1M x 64, 16 4K views, 4 lights x 8 layers)")
-    ap.add_argument("--stress-views-per-frame", type=int, default=2)
+    ap.add_argument("--stress-views-per-frame", type=int, default=4)
+    ap.add_argument("--stress-first-frame-views", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a plain copy kernel of 1 GiB read + 1 GiB written (torch.neg), the reference "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -453,7 +463,8 @@ def main():
 
     stress = None
     if args.stress:
-        stress = run_stress(16, args.stress_views_per_frame, 1 if args.backend == "csr" else 0)
+        stress = run_stress(16, args.stress_views_per_frame, args.stress_first_frame_views,
+                            1 if args.backend == "csr" else 0)
         print(f"bench: stress {stress.get('frames_per_s')} frames/s", file=sys.stderr, flush=True)
 
     cpu = None

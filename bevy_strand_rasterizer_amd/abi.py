@@ -26,6 +26,7 @@ CONFIG_CSR_FEW_HITS = 2  # sr_device_config.flags
 CONFIG_HIP_GRAPHS = 4
 CONFIG_TIGHT_CAPACITY = 8
 NOALLOC_SCRATCH_BYTES = 256 << 20  # SR_NOALLOC_SCRATCH_BYTES
+FRAME_REUSE_LIGHT_DOMS = 1  # sr_frame_settings.flags
 
 # sr_status
 SR_OK = 0
@@ -112,7 +113,7 @@ class FrameSettings(C.Structure):
                 ("dom_layers", C.c_uint32), ("shadow_history_valid", C.c_uint32),
                 ("ambient_color", C.c_float * 4), ("scene_clear_color", C.c_float * 4),
                 ("scene_clear_depth", C.c_float), ("stamp_target_width", C.c_uint32),
-                ("stamp_target_height", C.c_uint32), ("pad", C.c_uint32)]
+                ("stamp_target_height", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class FrameDesc(C.Structure):
@@ -170,7 +171,7 @@ request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel
                                                  ("dom_layers", "<u4"), ("shadow_history_valid", "<u4"),
                                                  ("ambient_color", "<f4", 4), ("scene_clear_color", "<f4", 4),
                                                  ("scene_clear_depth", "<f4"), ("stamp_target_width", "<u4"),
-                                                 ("stamp_target_height", "<u4"), ("pad", "<u4")])),
+                                                 ("stamp_target_height", "<u4"), ("flags", "<u4")])),
                           ("out_rgba", "<u8"), ("out_depth", "<u8"), ("tag", "<u8")], align=True)
 request_result_dtype = np.dtype([("ticket", "<u8"), ("tag", "<u8"), ("status", "<i4"), ("frame_views", "<u4"),
                                  ("submit_ms", "<f8"), ("launch_ms", "<f8"), ("done_ms", "<f8"),

@@ -214,6 +214,11 @@ struct FrameSlot {
   };
   std::vector<Graph> graphs;
   uint64_t graph_clock = 0, graph_captures = 0, graph_replays = 0;
+  // deep shadow maps left by the last frame that built its lights: what they
+  // were built from (light frusta and views, lights, instances, culling
+  // settings), and whether that frame completed without overflow
+  std::vector<uint8_t> dom_key;
+  bool dom_pending = false, dom_ready = false;
 
   void drop_graphs() {
     for (Graph& g : graphs)
@@ -229,6 +234,7 @@ struct FrameSlot {
     drop_graphs();  // they point into the buffers
     for (DBuf& d : buf) d.free_();
     shape.valid = false;
+    dom_pending = dom_ready = false;
   }
 };
 
@@ -562,6 +568,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
+  if (st.flags & ~SR_FRAME_REUSE_LIGHT_DOMS) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
+  const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
+  if (reuse && na)
+    return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: the scratch array keeps no shadow maps between calls");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = stream_in ? stream_in : sl.stream;
 
@@ -606,7 +616,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       r.image_off = img_px;
       img_px += (uint64_t)d.screen_width * d.screen_height;
     }
-    r.enabled = (d.kind == 0u) || (d.kind == 1u && r.light_layer != kInvalid);
+    r.enabled = (d.kind == 0u) || (d.kind == 1u && r.light_layer != kInvalid && !reuse);
     if (d.kind == 0u) max_cam_tile_px = std::max(max_cam_tile_px, d.froxel_size_x * d.froxel_size_y);
     if (d.kind == 1u) max_light_tile_px = std::max(max_light_tile_px, d.froxel_size_x * d.froxel_size_y);
     coarse_total = std::max(coarse_total, d.coarse_depth_tile_base + d.coarse_depth_tile_count);
@@ -654,6 +664,37 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     pos += r.index_count;
     max_idx = std::max(max_idx, r.index_count);
   }
+  // what this frame's shadow maps are built from: each light frustum's
+  // tiling, view and DOM placement (not its table bases, which follow the
+  // cameras before it), the lights, the instances and the culling settings
+  std::vector<uint8_t> dom_key;
+  auto put_key = [&dom_key](const void* p, size_t n) {
+    const uint8_t* b = static_cast<const uint8_t*>(p);
+    dom_key.insert(dom_key.end(), b, b + n);
+  };
+  for (uint32_t i = 0; i < F; ++i) {
+    const FrustumRec& r = fr[i];
+    if (r.d.kind != 1u) continue;
+    const uint32_t k[11] = {r.d.screen_width, r.d.screen_height, r.d.froxel_size_x, r.d.froxel_size_y,
+                            r.d.depth_slices, r.d.coarse_tiles_x, r.d.coarse_tiles_y, r.d.cascade_index,
+                            r.light_layer, r.stamp_w, r.stamp_h};
+    put_key(k, sizeof(k));
+    put_key(r.clip, sizeof(r.clip));
+    const uint64_t o[3] = {r.dom_depth_off, r.dom_opacity_off, r.stamp_off};
+    put_key(o, sizeof(o));
+  }
+  if (NL) put_key(fd->lights, sizeof(sr_directional_light) * NL);
+  if (NI) put_key(ir.data(), sizeof(InstRec) * NI);
+  const uint32_t ks[5] = {st.stochastic_cull_enabled, st.dom_layers, NL, NI, lights_seen};
+  put_key(ks, sizeof(ks));
+  const float kf[3] = {st.target_strands_per_pixel, st.min_keep_probability, st.shadow_keep_probability};
+  put_key(kf, sizeof(kf));
+  if (reuse && !(sl.dom_ready && sl.dom_key == dom_key))
+    return efail(err, SR_ERR_INVALID_ARGUMENT,
+                 sl.dom_ready ? "sr_frame_submit: SR_FRAME_REUSE_LIGHT_DOMS with lights, instances or culling "
+                                "settings other than those the shadow maps were built from"
+                              : "sr_frame_submit: SR_FRAME_REUSE_LIGHT_DOMS before a light-building frame "
+                                "completed without overflow (wait for it first)");
   const uint32_t slots = (uint32_t)align_up(std::max<uint64_t>(pos, 64), 64);
   if ((uint64_t)F * slots > 0x7FFFFFFFull) return efail(err, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
   const uint32_t task_cap = F * slots;
@@ -917,9 +958,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     FillSpans clear;  // every frame buffer that starts at zero, in one launch
     clear.add(fp.counters, C_COUNT);
     if (fp.tel) clear.add(fp.tel, tel_words);
-    clear.add(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px);
-    clear.add(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers);
-    clear.add(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px);
+    if (!reuse) {  // reused shadow maps stay as their frame left them
+      clear.add(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px);
+      clear.add(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers);
+      clear.add(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px);
+    }
     clear.add(reinterpret_cast<uint32_t*>(fp.image), img_px * 4);
     clear.add(reinterpret_cast<uint32_t*>(fp.depth), img_px);
     // texels no segment writes this frame read as zero (the reference leaves
@@ -929,9 +972,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     launch_fill_spans(clear, s);
     launch_prepass(fp, ps, s, &sl.timer);
     sl.timer.mark(s, "shadow");
-    launch_shadow(fp, s, max_light_tile_px);
+    if (!reuse) launch_shadow(fp, s, max_light_tile_px);
     sl.timer.mark(s, "stampback");
-    for (uint32_t i = 0; i < F; ++i)
+    for (uint32_t i = 0; i < F && !reuse; ++i)
       if (fr[i].d.kind == 1u) launch_stampback(fp, i, fr[i].stamp_w * fr[i].stamp_h, s);
     sl.timer.mark(s, "shading");
     launch_shading(fp, s);
@@ -1013,6 +1056,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.hist_cur = hist_next;
   sl.hist_cur = hist_next;
   sl.submitted = true;
+  if (!reuse) {  // this frame built the slot's shadow maps; usable once it completes without overflow
+    sl.dom_key = std::move(dom_key);
+    sl.dom_pending = true;
+    sl.dom_ready = false;
+  }
   return SR_OK;
 }
 
@@ -1030,6 +1078,10 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   if (c[C_PAGE_DROPS] > 0) sl.page_cap_hint = next_pow2(c[C_MARKED]);
   if (c[C_WORK] > sh.work_cap) sl.work_cap_hint = next_pow2(c[C_WORK]);
   if (sh.hit_cap && c[C_CANDS] > sh.hit_cap) sl.ref_cap_hint = std::max(sl.ref_cap_hint, next_pow2(c[C_CANDS]));
+  if (sl.dom_pending) {
+    sl.dom_ready = fc.capacity_exceeded == 0u;
+    sl.dom_pending = false;
+  }
   if (out) *out = fc;
   return SR_OK;
 }
@@ -1538,6 +1590,8 @@ int sr_server_submit(sr_server* srv, const sr_request* req, uint64_t* ticket) {
   if (!srv || !req) return SR_ERR_INVALID_ARGUMENT;
   if (req->light_count > SR_SERVER_MAX_LIGHTS || req->camera.screen_width == 0 || req->camera.screen_height == 0)
     return fail(srv->ctx, SR_ERR_INVALID_ARGUMENT, "sr_server_submit: bad request");
+  if (req->settings.flags != 0u)  // served frames build their own lights
+    return fail(srv->ctx, SR_ERR_UNSUPPORTED, "sr_server_submit: frame flags are not served");
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->stopping) return fail(srv->ctx, SR_ERR_NOT_READY, "sr_server_submit: server is stopping");
   PendingRequest p{*req, srv->next_ticket++, srv->now_ms()};

@@ -87,6 +87,7 @@ class Workload:
         st.scene_clear_depth = d["scene_clear_depth"]
         st.stamp_target_width = int(d["stamp_target_width"])
         st.stamp_target_height = int(d["stamp_target_height"])
+        st.flags = int(d["frame_flags"])
         return st
 
     def frame_desc(self, keep: list | None = None) -> abi.FrameDesc:
@@ -127,6 +128,7 @@ def default_settings() -> dict:
         "scene_clear_depth": 0.0,
         "stamp_target_width": 0,
         "stamp_target_height": 0,
+        "frame_flags": 0,  # SR_FRAME_* bits
     }
 
 
@@ -368,6 +370,37 @@ def retile(w: Workload, camera_froxel: int = 0, light_froxel: int = 0, backend: 
                     views=w.views, lights=w.lights, settings=st, opaque=w.opaque)
 
 
+def select_views(w: Workload, cameras: list, name: str | None = None) -> Workload:
+    """The workload with only the camera frusta `cameras` (indices into its
+    frustum table, kept in the order given) and every light frustum after
+    them, the frustum table rebuilt with the same froxel sizes: one view of a
+    batched frame drawn on its own."""
+    fr = w.frusta
+    keep = list(cameras) + [i for i in range(len(fr)) if int(fr[i]["kind"]) == 1]
+    for i in cameras:
+        if int(fr[i]["kind"]) != 0:
+            raise ValueError(f"frustum {i} is not a camera")
+    cfgs = [(int(fr[i]["screen_width"]), int(fr[i]["screen_height"]), int(fr[i]["froxel_size_x"]),
+             int(fr[i]["froxel_size_y"]), int(fr[i]["depth_slices"])) for i in keep]
+    kinds = [int(fr[i]["kind"]) for i in keep]
+    return Workload(name=name or f"{w.name}-views{'-'.join(map(str, cameras))}", assets=w.assets,
+                    instances=w.instances, frusta=frustum_table(cfgs, kinds),
+                    views=np.ascontiguousarray(w.views[keep]), lights=w.lights, settings=dict(w.settings),
+                    opaque=None)
+
+
+def camera_pixel_offsets(w: Workload) -> list:
+    """(first pixel, width, height) of every camera frustum in the frame's
+    concatenated image / composite / depth outputs, in frustum order."""
+    out, off = [], 0
+    for f in w.frusta:
+        if int(f["kind"]) == 0:
+            cw, ch = int(f["screen_width"]), int(f["screen_height"])
+            out.append((off, cw, ch))
+            off += cw * ch
+    return out
+
+
 def opaque_disc(w: Workload, depth: float, radius_frac: float = 0.3, samples: int = 4, seed: int = 1) -> np.ndarray:
     """A synthetic opaque scene depth for every camera of `w`: a disc (an
     occluder in front of or behind the hair) of constant reverse-Z `depth`
@@ -470,21 +503,34 @@ def config_batched_named(views: int = 8, seed: int = 0x5EEDCAFE) -> Workload:
                           resolutions=((1920, 1080),))
 
 
-def config_stress_chunks(views: int = 16, views_per_frame: int = 2, seed: int = 0x5EEDCAFE, strands: int = 1_000_000,
-                         points: int = 64, lights: int = 4, light_res: int = 2048, dom_layers: int = 8,
-                         width: int = 3840, height: int = 2160) -> list:
+def config_stress_chunks(views: int = 16, views_per_frame: int = 4, first_frame_views: int = 2,
+                         seed: int = 0x5EEDCAFE, strands: int = 1_000_000, points: int = 64, lights: int = 4,
+                         light_res: int = 2048, dom_layers: int = 8, width: int = 3840,
+                         height: int = 2160) -> list:
     """The stress configuration: 1M strands x 64 points (63M segments), curl
     radius up to 0.02 m, `views` 3840x2160 orbit cameras and 4 lights with
-    2048^2 x 8-layer deep shadow maps.  One frame of all views would need
-    more fine references than 32-bit offsets address, so the views come as
-    frames of `views_per_frame` cameras, each with every light; all frames
-    draw the same asset (id 0, upload it once)."""
+    2048^2 x 8-layer deep shadow maps, as a batch of frames that share one
+    build of the lights.  One frame of every view would need more fine
+    references than 32-bit offsets address, so the first frame draws
+    `first_frame_views` cameras and builds the lights, and every later frame
+    draws `views_per_frame` cameras with the shadow maps the first one left
+    (SR_FRAME_REUSE_LIGHT_DOMS; the same lights, instance and settings).  All
+    frames draw the same asset (id 0, upload it once)."""
     dist = 0.6
     asset = scalp_asset(strands, points, seed, 0.0006, radius_px_for(0.0006, height, dist), curl_max=0.02)
     cams = [(orbit_eye(i, views, dist), width, height) for i in range(views)]
-    return [build_workload(f"stress-{strands // 1000}k-x{points}-views{i}-{i + views_per_frame - 1}", asset,
-                           cams[i:i + views_per_frame], lights, light_res, dom_layers)
-            for i in range(0, views, views_per_frame)]
+    bounds = [0, min(first_frame_views, views)]
+    while bounds[-1] < views:
+        bounds.append(min(bounds[-1] + views_per_frame, views))
+    frames = []
+    for k in range(len(bounds) - 1):
+        a, b = bounds[k], bounds[k + 1]
+        w = build_workload(f"stress-{strands // 1000}k-x{points}-views{a}-{b - 1}", asset, cams[a:b], lights,
+                           light_res, dom_layers)
+        if k > 0:
+            w.settings["frame_flags"] = abi.FRAME_REUSE_LIGHT_DOMS
+        frames.append(w)
+    return frames
 
 
 SERVING_SIZES = (10_000, 25_000, 50_000, 100_000, 150_000, 200_000, 300_000, 500_000)

@@ -336,9 +336,22 @@ typedef struct sr_frame_settings {
   float scene_clear_depth;       /* headless scene depth (reverse-Z, 0 = far) */
   uint32_t stamp_target_width;   /* light depth-map size for the stampback */
   uint32_t stamp_target_height;
-  uint32_t pad;
+  uint32_t flags;                /* SR_FRAME_* bits */
 } sr_frame_settings;
 SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 80, "sr_frame_settings is 80 bytes");
+
+/* sr_frame_settings.flags: draw the frame's cameras with the deep shadow maps
+ * that the context's last light-building frame left (no counterpart in the
+ * reference, which renders every light in every frame).  The frame lists the
+ * same light frusta, views and lights, the same instances and the same
+ * culling settings as that frame (checked: SR_ERR_INVALID_ARGUMENT
+ * otherwise), which must have been waited for and must not have reported
+ * capacity_exceeded.  Its light frusta are then neither binned nor
+ * rasterised, the DOMs and stampback targets stay as they are, and every
+ * camera's shading reads them: a camera drawn this way equals the same
+ * camera drawn in a frame that builds the lights.  Lets a batch of views too
+ * large for one frame share one DOM build. */
+#define SR_FRAME_REUSE_LIGHT_DOMS 1u
 
 typedef struct sr_frame_desc {
   const sr_strand_instance* instances;
