@@ -148,3 +148,12 @@ def test_dump_kind_count_exceeds_every_kind():
     assert sorted(kinds.values()) == list(range(count)), "dump kinds are dense and listed once"
     assert list(kinds.values()) == sorted(kinds.values()), "enumerators in numeric order"
     assert {k.lower(): v for k, v in kinds.items()} == abi.DUMP
+
+
+def test_integration_binding_declares_every_entry_point():
+    """The Rust extern block a maintainer would add names
+    every function the header exports."""
+    text = (ROOT / "INTEGRATION.md").read_text()
+    block = re.search(r'#\[link\(name = "strand_rt"\)\]\s*extern "C" \{(.*?)\n\}', text, flags=re.S).group(1)
+    bound = set(re.findall(r"pub fn (sr_[a-z_0-9]+)\(", block))
+    assert bound == set(declared_functions())
