@@ -16,7 +16,11 @@ and so do the two system patterns:
              cameras over {720p, 1080p, 1440p} with 4 lights of 2048^2, and
              the named asset from 8 1080p cameras sharing its light;
   serving -- the request loop over 8 resident assets (10k..500k strands x
-             32 points) under a Poisson stream, p50/p99 latency and frames/s.
+             32 points) under a Poisson stream, p50/p99 latency and frames/s;
+  stress  -- 1M strands x 64 points from 16 4K views with 4 lights of
+             2048^2 x 8 layers, in frames that share one build of the lights.
+The CPU baseline is the oracle (the C++ restatement) on the host cores over
+full frames of the named workload and of the 2,000 x 16 scalp.
 """
 from __future__ import annotations
 
@@ -105,6 +109,41 @@ def stage_bytes(ctr, w, refs=None):
     return out
 
 
+# Prepass stages that bin (marking through the last page pass) per backend,
+# as named by the library's stage timer.
+BINNING_STAGES = {
+    "csr": ("project_mark", "project_compact", "allocate", "hit_offsets", "project_write", "sort_hits",
+            "page_cells", "page_bases"),
+    "scatter": ("project_mark", "project_compact", "allocate", "project_write", "count", "prefix", "fill",
+                "sort_cells"),
+}
+
+
+def binning_headline(stage_acc, ctr, backend: str):
+    """Count + scan + fill algorithmic bytes (every ABI record read or written
+    once: count 28 T + 4 C, scan 8 C + 32 P, fill 28 T + 4 C + 12 H, with T
+    binning tasks, P pages, C = 256 P cells, H fine references) over the summed
+    time of every binning stage of the backend, marking and writing included,
+    against the 8 TB/s HBM peak; and the same with the projection's bytes
+    (per task a float4 vertex, an index, the projected segment and the two raw
+    screen endpoints, fused into the marking and write passes) added."""
+    T, P, H = ctr["binning_tasks"], ctr["allocated_pages"], ctr["fine_refs"]
+    C = 256 * P
+    count, scan, fill = 28 * T + 4 * C, 8 * C + 32 * P, 28 * T + 4 * C + 12 * H
+    proj = T * (16 + 4 + 12 + 32)
+    stages = [k for k in BINNING_STAGES[backend] if k in stage_acc]
+    ms = sum(stage_acc[k] for k in stages)
+    b = count + scan + fill
+    gbs = b / (ms * 1e-3) / 1e9
+    gbs_p = (b + proj) / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_frame": int(b),
+            "bytes": {"count": int(count), "scan": int(scan), "fill": int(fill)}, "ms": round(ms, 4),
+            "stages": {k: round(stage_acc[k], 4) for k in stages},
+            "with_projection": {"bytes_per_frame": int(b + proj), "achieved": round(gbs_p, 1),
+                                "frac": round(gbs_p / HBM_PEAK_GBS, 4)}}
+
+
 def profile_row(kernel: str):
     """(HBM bytes, VALU busy) per launch of `kernel` from the committed
     rocprofv3 summary of the named workload, or (None, None)."""
@@ -148,24 +187,53 @@ def roofline_for(stage, ms, ctr, w, refs=None):
     return out
 
 
-def cpu_baseline(threads: int, fraction: float = 0.25):
-    """The oracle (C++ restatement) on a bounded sample: the same workload
-    with `fraction` of its strands, one frame; frames/s scaled by `fraction`
-    to the full workload (work is linear in segments)."""
-    from bevy_strand_rasterizer_amd import scene
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_frames(w, threads: int, warmup: int, samples: int) -> dict:
+    """Whole frames of workload `w` through the oracle (the C++ restatement,
+    `threads` worker threads): per-frame wall times in the reference bench's
+    summary (examples/strand_bench.rs:931-947) after `warmup` untimed frames."""
+    from bevy_strand_rasterizer_amd import report
     from oracle import oracle
 
-    strands = int(100_000 * fraction)
-    w = scene.config_named(strands=strands)
-    oracle.render(scene.config_tiny(), threads=threads)  # load + warm
-    t0 = time.perf_counter()
-    f = oracle.render(w, threads=threads)
-    dt = time.perf_counter() - t0
-    f.close()
-    return {"value": round(fraction / dt, 5), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"one frame of the named workload with {strands} of 100000 strands "
-                      f"({w.input_segments} segments), {dt:.1f} s, scaled by {fraction}",
-            "sample_seconds": round(dt, 2), "msegments_per_s": round(w.input_segments / dt / 1e6, 4)}
+    for _ in range(warmup):
+        oracle.render(w, threads=threads).close()
+    ms = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        f = oracle.render(w, threads=threads)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        f.close()
+    st = report.summarize(ms)
+    return {"workload": w.name, "segments": w.input_segments, "views": w.camera_count,
+            "warmup_frames": warmup, "sample_frames": samples, "frame_ms": {k: round(v, 3) if isinstance(v, float)
+                                                                            else v for k, v in st.items()},
+            "frames_per_s": round(w.camera_count * 1e3 / st["median"], 5),
+            "msegments_per_s": round(w.input_segments * w.camera_count / st["median"] / 1e3, 4)}
+
+
+def cpu_baseline(threads: int, named_samples: int = 3, small_samples: int = 20):
+    """The oracle (C++ restatement of the reference's pipeline) on the box's
+    host cores: the full named frame (1 warmup + `named_samples` timed frames)
+    and the small configuration (2,000 x 16 at 512^2 with its 512^2 light,
+    3 warmup + `small_samples` timed frames).  `value` is the named frame's
+    frames/s from the median frame time."""
+    from bevy_strand_rasterizer_amd import scene
+
+    named = cpu_frames(scene.config_named(), threads, 1, named_samples)
+    small = cpu_frames(scene.config_small(froxel=16), threads, 3, small_samples)
+    return {"value": named["frames_per_s"], "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{named_samples} full frames of the named workload after 1 warmup frame (median), "
+                      f"and {small_samples} frames of the 2,000 x 16 scalp after 3",
+            "cpu": cpu_model(), "named": named, "small": small}
 
 
 def percentile_ms(xs):
@@ -229,6 +297,33 @@ def write_report(args, r, w):
         stages, tele)
     with open(args.report, "w") as f:
         json.dump(rep, f, indent=1)
+
+
+def plain_views_fps(w, repeats: int = 2) -> float:
+    """Frames/s of the plain per-frame path over the same views as the batched
+    frame `w`: every camera drawn alone in a frame with all of w's lights
+    (which that frame renders again), no graph, w's own froxel sizes and
+    binning backend."""
+    import torch
+
+    from bevy_strand_rasterizer_amd import scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    r = StrandRenderer(device=0)
+    r.upload_workload(w)
+    ones = [scene.select_views(w, [i]) for i in range(len(w.frusta)) if int(w.frusta[i]["kind"]) == 0]
+    for one in ones:  # each shape once, so buffers have grown before timing
+        r.render(one)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        for one in ones:
+            r.submit(one)
+    r.wait()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r.close()
+    return len(ones) * repeats / dt
 
 
 def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool = True):
@@ -382,8 +477,8 @@ def main():
     ap.add_argument("--report", default="", help="also write the reference bench's report (schema_version 1) here, "
                                                   "from --report-frames frames with telemetry on")
     ap.add_argument("--report-frames", type=int, default=10)
-    ap.add_argument("--stress", action="store_true",
-                    help="also run the stress configuration (1M x 64, 16 4K views, 4 lights x 8 layers)")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the stress configuration (1M x 64, 16 4K views, 4 lights x 8 layers)")
     ap.add_argument("--stress-views-per-frame", type=int, default=4)
     ap.add_argument("--stress-first-frame-views", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -436,8 +531,11 @@ def main():
 
     dom = max((k for k in stage_acc if k not in ("end",)), key=lambda k: stage_acc[k])
     roof = roofline_for(dom, stage_acc[dom], ctr, w, refs)
+    binning = binning_headline(stage_acc, ctr, args.backend)
+    # the sort kernels against the bytes they move themselves (radix passes,
+    # per-page cell sort), beside the algorithmic headline above
     bstages = ("sort_hits", "page_cells") if args.backend == "csr" else ("count", "prefix", "fill")
-    binning = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w, refs) for k in bstages}
+    binning_kernels = {k: roofline_for(k, stage_acc.get(k, 0.0), ctr, w, refs) for k in bstages}
     if roof is None:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4)}
@@ -447,7 +545,15 @@ def main():
         batched = {}
         for name, wb in (("64views_4lights_2kx16", scene.config_batched()),
                          ("named_8views_1light", scene.config_batched_named(8))):
-            batched[name] = run_batched(wb, steps=max(3, args.steps // 4), warmup=2, plain_fps=fps)
+            wb.settings["fine_binning_backend"] = 1 if args.backend == "csr" else 0
+            if name == "named_8views_1light":  # each view alone is the named plain frame
+                plain, plain_what = fps, "the named plain frame (one view, its light rendered per frame)"
+            else:
+                plain = plain_views_fps(wb)
+                plain_what = "each view alone with the 4 lights, one plain frame per view"
+            batched[name] = run_batched(wb, steps=max(3, args.steps // 4), warmup=2, plain_fps=plain)
+            batched[name]["plain_frames_per_s"] = round(plain, 3)
+            batched[name]["plain"] = plain_what
             print(f"bench: batched {name} {batched[name]['frames_per_s']} frames/s", file=sys.stderr, flush=True)
 
     serving = None
@@ -462,7 +568,7 @@ def main():
                   f"{serving['below_saturation']['latency_ms']}", file=sys.stderr, flush=True)
 
     stress = None
-    if args.stress:
+    if not args.no_stress:
         stress = run_stress(16, args.stress_views_per_frame, args.stress_first_frame_views,
                             1 if args.backend == "csr" else 0)
         print(f"bench: stress {stress.get('frames_per_s')} frames/s", file=sys.stderr, flush=True)
@@ -499,6 +605,7 @@ def main():
         "refs_by_kind": {"camera": refs[0], "light": refs[1]},
         "roofline": roof,
         "binning_roofline": binning,
+        "binning_kernel_traffic": binning_kernels,
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

@@ -556,6 +556,39 @@ def serving_request(assets: list, asset_ids: list, rng: np.random.Generator, tag
     return request_from_workload(wl, tag=tag), wl
 
 
+def config_two_strands(light_offset: float = 0.0, dom_layers: int = 4, res: int = 64,
+                       near=(0.9, 0.1, 0.1, 0.6), far=(0.1, 0.2, 0.9, 0.7)) -> Workload:
+    """Two straight one-segment strands on pixel row 20 of a res x res view,
+    seen by a camera and a light whose clip_from_world is the identity
+    (screen x = (x / 2 + 1 / 2) res, y = (1 / 2 - y / 2) res, depth = z): the
+    near strand at depth 0.8 from x = -0.5 to 0.5, the far one at depth 0.5
+    from x = -0.4 to 0.4, both 1.5 px in radius, so pixel (res / 2, 20) is
+    covered fully by both.  ambient_factor 1 with no ambient light makes the
+    shaded colour each strand's absorption colour.  light_offset shifts the
+    light's view sideways (clip x + offset): an offset of 4 puts both strands
+    outside the light's map, so nothing shadows them.  Stochastic culling is
+    off."""
+    y = 1.0 - 2.0 * 20.5 / res  # pixel-centre row 20
+    asset = asset_from_strands([[(-0.5, y, 0.8), (0.5, y, 0.8)], [(-0.4, y, 0.5), (0.4, y, 0.5)]],
+                               material={"ambient_factor": 1.0}, radius_px=1.5)
+    asset.materials["absorption_color"][0] = near
+    asset.materials["absorption_color"][1] = far
+    eye = np.eye(4, dtype=np.float32)
+    views = np.zeros(2, abi.view_dtype)
+    views["clip_from_world"][0] = eye.T.reshape(-1)
+    views["world_position"][0] = (0.0, 0.0, 5.0, 1.0)
+    light = eye.copy()
+    light[0, 3] = light_offset
+    views["clip_from_world"][1] = light.T.reshape(-1)
+    views["world_position"][1] = (0.0, 0.0, 5.0, 1.0)
+    frusta = frustum_table([(res, res, 8, 8, 16), (res, res, 8, 8, 16)], [0, 1])
+    return Workload(name=f"two-strands-{res}-light{light_offset:g}-L{dom_layers}", assets=[asset],
+                    instances=identity_instance(0), frusta=frusta, views=views,
+                    lights=np.ascontiguousarray(make_light((0.0, 0.0, 1.0))),
+                    settings={"dom_layers": dom_layers, "stochastic_cull_enabled": 0,
+                              "ambient_color": (0.0, 0.0, 0.0, 1.0)})
+
+
 def config_tiny(strands: int = 24, points: int = 6, res: int = 64, froxel: int = 8, seed: int = 7,
                 light_res: int = 64, dom_layers: int = 4, lights: int = 1) -> Workload:
     """Tiny scalp for exhaustive parity checks."""

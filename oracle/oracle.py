@@ -55,6 +55,9 @@ def load() -> C.CDLL:
         ("oracle_mp", C.c_float, [C.c_float] * 4),
         ("oracle_np", None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_float,
                              C.c_void_p]),
+        ("oracle_phi", C.c_float, [C.c_uint32, C.c_float, C.c_float]),
+        ("oracle_a", None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p]),
+        ("oracle_marschner", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res

@@ -1695,5 +1695,17 @@ void oracle_np(uint32_t p, float phi, float ti, float tr, float eta, const float
   V3 r = Np(p, phi, ti, tr, eta, V3{mu[0], mu[1], mu[2]}, beta);
   out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
 }
+float oracle_phi(uint32_t p, float h, float eta_prime) { return Phi(p, h, eta_prime); }
+void oracle_a(uint32_t p, float h, float eta, float eta_prime, const float* mu, float theta_d, float* out3) {
+  V3 r = A_term(p, h, eta, eta_prime, V3{mu[0], mu[1], mu[2]}, theta_d);
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+// marschner(direction, view_normal, light_normal, material, occlusion)
+void oracle_marschner(const float* dir3, const float* view3, const float* light3, const sr_strand_material* m,
+                      float occlusion, float* out3) {
+  V3 r = marschner(V3{dir3[0], dir3[1], dir3[2]}, V3{view3[0], view3[1], view3[2]},
+                   V3{light3[0], light3[1], light3[2]}, *m, occlusion);
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
 
 }  // extern "C"

@@ -108,6 +108,10 @@ def rect_froxels(w, fx: int, fy: int):
 
 # non-square froxels: 16 x 8 pixel tiles
 CASES["tiny-f16x8"] = lambda: rect_froxels(scene.config_tiny(strands=60, res=96, light_res=80), 16, 8)
+# two strands whose DOM layers and blended pixel have closed forms
+# (tests/test_oracle_closed_forms.py): lit, and with the light moved off them
+CASES["two-strands"] = lambda: scene.config_two_strands()
+CASES["two-strands-unlit"] = lambda: scene.config_two_strands(light_offset=4.0)
 
 CSR_CASES = {
     "tiny-f8-csr": lambda: csr(scene.config_tiny()),
@@ -168,6 +172,25 @@ def test_parity(case):
     compare(w, ctr, gpu, ref)
     if w.settings.get("fine_binning_backend", 0):
         assert ctr["page_candidates"] > 0
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_two_strands_closed_forms(backend):
+    """The HIP path gives the hand-derived values of the two-strand scene:
+    DOM layers 0.3, 0.4225, 0.5235625, 0.5235625 at depth 0.8, and the near
+    strand over the far one, unpremultiplied."""
+    lit, unlit = scene.config_two_strands(), scene.config_two_strands(light_offset=4.0)
+    for w in (lit, unlit):
+        w.settings["fine_binning_backend"] = backend
+    _, g = gpu_frame(lit)
+    np.testing.assert_allclose(g["dom_opacity"].reshape(4, 64, 64)[:, 20, 32], [0.3, 0.4225, 0.5235625, 0.5235625],
+                               atol=1e-6)
+    assert g["dom_depth"].reshape(64, 64)[20, 32] == np.float32(0.8)
+    _, g = gpu_frame(unlit)
+    near, far = np.array([0.9, 0.1, 0.1, 0.6]), np.array([0.1, 0.2, 0.9, 0.7])
+    a = near[3] + far[3] * (1 - near[3])
+    rgb = (near[:3] * near[3] + far[:3] * far[3] * (1 - near[3])) / a
+    np.testing.assert_allclose(g["image"].reshape(64, 64, 4)[20, 32], np.r_[rgb, a], atol=1e-6)
 
 
 def test_parity_multiview():
