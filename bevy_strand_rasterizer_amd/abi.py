@@ -27,6 +27,9 @@ CONFIG_HIP_GRAPHS = 4
 CONFIG_TIGHT_CAPACITY = 8
 NOALLOC_SCRATCH_BYTES = 256 << 20  # SR_NOALLOC_SCRATCH_BYTES
 FRAME_REUSE_LIGHT_DOMS = 1  # sr_frame_settings.flags
+FRAME_RGBA8_OUTPUTS = 2
+REQUEST_RGBA8 = 1  # sr_request.flags
+REQUEST_PINNED_OUTPUTS = 2
 
 # sr_status
 SR_OK = 0
@@ -42,7 +45,7 @@ DUMP = {
     "lut": 7, "page_table": 8, "page_meta": 9, "cell_counts": 10, "cell_offsets": 11, "refs": 12,
     "active_tiles": 13, "work_items": 14, "camera_runs": 15, "shadow_runs": 16, "dom_depth": 17,
     "dom_opacity": 18, "atlas": 19, "history": 20, "image": 21, "depth": 22, "composite": 23, "stamp": 24,
-    "telemetry": 25, "opaque_tiles": 26,
+    "telemetry": 25, "opaque_tiles": 26, "image_rgba8": 27, "composite_rgba8": 28,
 }
 
 
@@ -102,6 +105,7 @@ DUMP_DTYPES = {
     "dom_opacity": np.dtype("<f4"), "atlas": np.dtype("<f4"), "history": np.dtype("<f4"),
     "image": np.dtype("<f4"), "depth": np.dtype("<f4"), "composite": np.dtype("<f4"), "stamp": np.dtype("<f4"),
     "telemetry": np.dtype("<u4"), "opaque_tiles": np.dtype([("covered", "<u4"), ("farthest_q", "<u4")]),
+    "image_rgba8": np.dtype("u1"), "composite_rgba8": np.dtype("u1"),
 }
 
 
@@ -162,7 +166,7 @@ tune_result_dtype = np.dtype([("best", tune_trial_dtype), ("trial_count", "<u4")
 
 SERVER_MAX_LIGHTS = 4
 request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel_config_dtype),
-                          ("camera_view", view_dtype), ("light_count", "<u4"), ("pad", "<u4"),
+                          ("camera_view", view_dtype), ("light_count", "<u4"), ("flags", "<u4"),
                           ("light_frusta", froxel_config_dtype, SERVER_MAX_LIGHTS),
                           ("light_views", view_dtype, SERVER_MAX_LIGHTS), ("lights", light_dtype, SERVER_MAX_LIGHTS),
                           ("settings", np.dtype([("stochastic_cull_enabled", "<u4"), ("target_strands_per_pixel", "<f4"),
@@ -178,7 +182,8 @@ request_result_dtype = np.dtype([("ticket", "<u8"), ("tag", "<u8"), ("status", "
                                  ("counters", frame_counters_dtype)], align=True)
 server_config_dtype = np.dtype([("slots", "<u4"), ("max_views_per_frame", "<u4"), ("hbm_budget_bytes", "<u8")])
 server_stats_dtype = np.dtype([("submitted", "<u8"), ("completed", "<u8"), ("failed", "<u8"), ("frames", "<u8"),
-                               ("peak_held_bytes", "<u8"), ("budget_waits", "<u8"), ("max_frame_views", "<u4"),
+                               ("peak_held_bytes", "<u8"), ("budget_waits", "<u8"), ("groups_split", "<u8"),
+                               ("max_frame_views", "<u4"),
                                ("max_concurrent_frames", "<u4")])
 
 strand_cache_info_dtype = np.dtype([("version", "<u4"), ("scale", "<f4"), ("vertex_count", "<u8"),

@@ -187,6 +187,8 @@ struct FrameParams {
   float4* image;
   float* depth;
   float4* composite;
+  uint32_t* image8;      // rgba8unorm image, or null
+  uint32_t* composite8;  // rgba8unorm composite, or null
 };
 
 // ------------------------------------------------------------- vectors

@@ -65,6 +65,9 @@ void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, 
 void launch_shading(const FrameParams& fp, hipStream_t s);
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
+// rgba8unorm copies of the image and the composite (FrameParams::image8 /
+// composite8; nothing when they are null).
+void launch_pack_rgba8(const FrameParams& fp, uint64_t pixels, hipStream_t s);
 // Up to kFillSpans buffers filled with one 32-bit value by one launch.
 constexpr uint32_t kFillSpans = 10;
 struct FillSpans {

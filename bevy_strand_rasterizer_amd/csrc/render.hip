@@ -885,6 +885,23 @@ __global__ void k_composite(FrameParams fp, uint64_t pixels) {
   }
 }
 
+// ------------------------------------------- reference framebuffer format
+// rgba8unorm stores of the strand image and the composite (the reference's
+// render target, src/pipelines/raster.rs:500-513): each channel clamped to
+// [0, 1], scaled by 255 and rounded to nearest, ties to even (as WGSL
+// pack4x8unorm), red in the lowest byte.
+__device__ __forceinline__ uint32_t unorm8(float x) { return (uint32_t)rintf(wclamp(x, 0.0f, 1.0f) * 255.0f); }
+__device__ __forceinline__ uint32_t pack_rgba8(float4 c) {
+  return unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (unorm8(c.w) << 24);
+}
+
+__global__ void k_pack_rgba8(FrameParams fp, uint64_t pixels) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pixels; i += (uint64_t)gridDim.x * blockDim.x) {
+    fp.image8[i] = pack_rgba8(fp.image[i]);
+    fp.composite8[i] = pack_rgba8(fp.composite[i]);
+  }
+}
+
 // ------------------------------------------------------------ launchers
 void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
   if (max_tile_px <= 64)
@@ -911,6 +928,13 @@ void launch_stampback(const FrameParams& fp, uint32_t fi, uint32_t texels, hipSt
   uint32_t g = (texels + 255) / 256;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_stampback, dim3(g), dim3(256), 0, s, fp, fi);
+}
+
+void launch_pack_rgba8(const FrameParams& fp, uint64_t pixels, hipStream_t s) {
+  if (pixels == 0 || !fp.image8 || !fp.composite8) return;
+  uint64_t g = (pixels + 255) / 256;
+  if (g > 8192) g = 8192;
+  hipLaunchKernelGGL(k_pack_rgba8, dim3((uint32_t)g), dim3(256), 0, s, fp, pixels);
 }
 
 void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s) {

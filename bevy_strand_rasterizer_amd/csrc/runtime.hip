@@ -173,6 +173,7 @@ enum BufId {
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SHADOW_COLS, B_SLOT_PROJ, B_SLOT_SCREEN,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
   B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_TABLES,
+  B_IMAGE8, B_COMPOSITE8,
   B_COUNT
 };
 
@@ -183,6 +184,7 @@ struct FrameShape {
   uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, hit_cap = 0, atlas_texels = 0, dom_layers = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0, opaque_tiles = 0;
   int hist_cur = 0;
+  bool rgba8 = false;  // the frame stored rgba8unorm outputs
 };
 
 // Everything one in-flight frame owns: its stream, grow-only buffers, the
@@ -555,7 +557,8 @@ __global__ void k_counters_out(const uint32_t* c, CounterCaps k, sr_frame_counte
 // is captured and shadow history is off.  Asset records are read under
 // ctx->mu.
 int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, hipStream_t stream_in,
-                      std::string& err, uint64_t* need_bytes, bool dry_run, const NoAllocTarget* na = nullptr) {
+                      std::string& err, uint64_t* need_bytes, bool dry_run, const NoAllocTarget* na = nullptr,
+                      uint64_t* own_bytes = nullptr) {
   if (!fd) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: NULL argument");
   if (na && fd->opaque_depth)
     return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: opaque depth needs a staging upload; use sr_frame_submit");
@@ -568,8 +571,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
-  if (st.flags & ~SR_FRAME_REUSE_LIGHT_DOMS) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
+  if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS))
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
   const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
+  const bool rgba8 = (st.flags & SR_FRAME_RGBA8_OUTPUTS) != 0u;
   if (reuse && na)
     return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: the scratch array keeps no shadow maps between calls");
   (void)hipSetDevice(ctx->device);
@@ -780,7 +785,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_SLOT_HITS, hit_k ? hit_bytes : 0ull},
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
-      {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_TABLES, tab_bytes}};
+      {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_TABLES, tab_bytes},
+      {B_IMAGE8, rgba8 ? 4ull * img_px : 0ull}, {B_COMPOSITE8, rgba8 ? 4ull * img_px : 0ull}};
   if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
     uint64_t held = sl.held_bytes();
     for (const Need& n : needs) {
@@ -788,6 +794,14 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       if (want > sl.buf[n.id].bytes) held += align_up(want + want / 4, 4096) - sl.buf[n.id].bytes;
     }
     *need_bytes = held;
+  }
+  if (own_bytes) {  // what the frame alone would make an empty slot allocate
+    uint64_t own = 0;
+    for (const Need& n : needs) {
+      const uint64_t want = std::max<uint64_t>(n.bytes, 256);
+      own += align_up(want + want / 4, 4096);
+    }
+    *own_bytes = own;
   }
   if (dry_run) return SR_OK;
   const uint64_t tbytes = tab_light + sizeof(sr_directional_light) * NL;
@@ -915,6 +929,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.image = sl.buf[B_IMAGE].as<float4>();
   fp.depth = sl.buf[B_DEPTH].as<float>();
   fp.composite = sl.buf[B_COMPOSITE].as<float4>();
+  fp.image8 = rgba8 ? sl.buf[B_IMAGE8].as<uint32_t>() : nullptr;
+  fp.composite8 = rgba8 ? sl.buf[B_COMPOSITE8].as<uint32_t>() : nullptr;
   fp.shadow_cols = sl.buf[B_SHADOW_COLS].as<unsigned long long>();
   // page_cap bounds the page arrays; the allocation scan drops pages past it
   PrepassScratch ps{};
@@ -982,6 +998,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     launch_camera_raster(fp, s, max_cam_tile_px);
     sl.timer.mark(s, "composite");
     launch_composite(fp, img_px, s);
+    if (rgba8) {
+      sl.timer.mark(s, "rgba8");
+      launch_pack_rgba8(fp, img_px, s);
+    }
     sl.timer.mark(s, "end");
   };
   sl.timer.begin();
@@ -1053,6 +1073,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.opaque_tiles = opaque_floats ? fine_total : 0ull;
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
+  sh.rgba8 = rgba8;
   sh.hist_cur = hist_next;
   sl.hist_cur = hist_next;
   sl.submitted = true;
@@ -1167,6 +1188,8 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
     case SR_DUMP_STAMP: src = buf[B_STAMP].p; bytes = 4ull * sh.stamp_px; break;
     case SR_DUMP_TELEMETRY: src = buf[B_TELEMETRY].p; bytes = 4ull * sh.tel_words; break;
     case SR_DUMP_OPAQUE_TILES: src = buf[B_OPAQUE_TILES].p; bytes = 8ull * sh.opaque_tiles; break;
+    case SR_DUMP_IMAGE_RGBA8: src = buf[B_IMAGE8].p; bytes = sh.rgba8 ? 4ull * sh.img_px : 0ull; break;
+    case SR_DUMP_COMPOSITE_RGBA8: src = buf[B_COMPOSITE8].p; bytes = sh.rgba8 ? 4ull * sh.img_px : 0ull; break;
     default: return fail(ctx, SR_ERR_INVALID_ARGUMENT, "sr_frame_dump: unknown kind");
   }
   *out_bytes = bytes;
@@ -1232,7 +1255,7 @@ struct ServerSlot {
   FrameSlot* fs = nullptr;
   bool busy = false;
   std::vector<PendingRequest> reqs;  // cameras of the frame, in frustum order
-  std::vector<uint64_t> out_off;     // float offset of each request's composite in staging
+  std::vector<uint64_t> out_off;     // byte offset of each request's composite in staging
   void* out = nullptr;               // pinned composite + depth of every camera
   uint64_t out_bytes = 0;
   hipEvent_t out_ev = nullptr;
@@ -1289,6 +1312,8 @@ int build_group_frame(const std::vector<PendingRequest>& g, GroupFrame& out) {
   out.fd.lights = nl ? out.lights.data() : nullptr;
   out.fd.light_count = nl;
   out.fd.settings = h.settings;
+  for (const PendingRequest& p : g)  // rgba8unorm outputs when any request of the frame asks for them
+    if (p.req.flags & SR_REQUEST_RGBA8) out.fd.settings.flags |= SR_FRAME_RGBA8_OUTPUTS;
   return SR_OK;
 }
 
@@ -1326,11 +1351,12 @@ struct sr_server {
     const double t = now_ms();
     for (size_t i = 0; i < s.reqs.size(); ++i) {
       const PendingRequest& p = s.reqs[i];
-      if (status == SR_OK) {
+      if (status == SR_OK && !(p.req.flags & SR_REQUEST_PINNED_OUTPUTS)) {  // pinned outputs arrived directly
         const uint64_t px = (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height;
-        const float* base = static_cast<const float*>(s.out) + s.out_off[i];
-        if (p.req.out_rgba) std::memcpy(p.req.out_rgba, base, px * 16u);
-        if (p.req.out_depth) std::memcpy(p.req.out_depth, base + px * 4u, px * 4u);
+        const uint64_t cb = px * ((p.req.flags & SR_REQUEST_RGBA8) ? 4u : 16u);
+        const char* base = static_cast<const char*>(s.out) + s.out_off[i];
+        if (p.req.out_rgba) std::memcpy(p.req.out_rgba, base, cb);
+        if (p.req.out_depth) std::memcpy(p.req.out_depth, base + cb, px * 4u);
       }
       sr_request_result r{};
       r.ticket = p.ticket;
@@ -1396,35 +1422,51 @@ struct sr_server {
     return any;
   }
 
+  // Bytes of a request's outputs in the slot's staging: its composite (float
+  // RGBA or rgba8unorm) and depth; none when the caller's buffers are pinned.
+  static uint64_t staged_bytes(const sr_request& r) {
+    if (r.flags & SR_REQUEST_PINNED_OUTPUTS) return 0u;
+    const uint64_t px = (uint64_t)r.camera.screen_width * r.camera.screen_height;
+    return px * ((r.flags & SR_REQUEST_RGBA8) ? 4u : 16u) + px * 4u;
+  }
+
   // Launches one group on slot s: frame, then the copies of every camera's
-  // composite and depth into the slot's pinned staging.
+  // composite and depth, into the slot's pinned staging or straight into
+  // the caller's pinned buffers.
   int launch(ServerSlot& s, std::vector<PendingRequest>& g, GroupFrame& gf, std::string& err) {
-    uint64_t floats = 0;
+    uint64_t bytes = 0;
     s.out_off.clear();
     for (auto& p : g) {
-      s.out_off.push_back(floats);
-      floats += (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height * 5u;
+      s.out_off.push_back(bytes);
+      bytes += staged_bytes(p.req);
     }
-    if (floats * 4u > s.out_bytes) {
+    if (bytes > s.out_bytes) {
       if (s.out) (void)hipHostFree(s.out);
       s.out = nullptr;
       s.out_bytes = 0;
-      if (hipHostMalloc(&s.out, floats * 4u, hipHostMallocDefault) != hipSuccess)
+      if (hipHostMalloc(&s.out, bytes, hipHostMallocDefault) != hipSuccess)
         return efail(err, SR_ERR_OUT_OF_MEMORY, "server: pinned output staging");
-      s.out_bytes = floats * 4u;
+      s.out_bytes = bytes;
     }
     int rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, nullptr, false);
     if (rc != SR_OK) return rc;
     const FrameSlot& fs = *s.fs;
     const float4* comp = fs.buf[B_COMPOSITE].as<float4>();
+    const uint32_t* comp8 = fs.buf[B_COMPOSITE8].as<uint32_t>();
     const float* dep = fs.buf[B_DEPTH].as<float>();
     uint64_t img_off = 0;
     for (size_t i = 0; i < g.size(); ++i) {
-      const uint64_t px = (uint64_t)g[i].req.camera.screen_width * g[i].req.camera.screen_height;
-      float* dst = static_cast<float*>(s.out) + s.out_off[i];
-      HIP_OKS(hipMemcpyAsync(dst, comp + img_off, px * 16u, hipMemcpyDeviceToHost, fs.stream), "server: copy image");
-      HIP_OKS(hipMemcpyAsync(dst + px * 4u, dep + img_off, px * 4u, hipMemcpyDeviceToHost, fs.stream),
-              "server: copy depth");
+      const sr_request& q = g[i].req;
+      const uint64_t px = (uint64_t)q.camera.screen_width * q.camera.screen_height;
+      const bool r8 = (q.flags & SR_REQUEST_RGBA8) != 0u;
+      const void* csrc = r8 ? static_cast<const void*>(comp8 + img_off) : static_cast<const void*>(comp + img_off);
+      const uint64_t cb = px * (r8 ? 4u : 16u);
+      char* stg = static_cast<char*>(s.out) + s.out_off[i];
+      const bool pinned = (q.flags & SR_REQUEST_PINNED_OUTPUTS) != 0u;
+      void* cdst = pinned ? static_cast<void*>(q.out_rgba) : static_cast<void*>(stg);
+      void* ddst = pinned ? static_cast<void*>(q.out_depth) : static_cast<void*>(stg + cb);
+      if (cdst) HIP_OKS(hipMemcpyAsync(cdst, csrc, cb, hipMemcpyDeviceToHost, fs.stream), "server: copy image");
+      if (ddst) HIP_OKS(hipMemcpyAsync(ddst, dep + img_off, px * 4u, hipMemcpyDeviceToHost, fs.stream), "server: copy depth");
       img_off += px;
     }
     HIP_OKS(hipEventRecord(s.out_ev, fs.stream), "server: event");
@@ -1473,14 +1515,33 @@ struct sr_server {
         GroupFrame gf;
         int rc = build_group_frame(g, gf);
         std::string err;
-        uint64_t need = 0;
-        if (rc == SR_OK) rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, &need, true);
+        uint64_t need = 0, own = 0;
+        if (rc == SR_OK) rc = frame_submit_slot(ctx, *s.fs, &gf.fd, nullptr, err, &need, true, nullptr, &own);
         if (rc != SR_OK) {
           fail_requests(g, rc);
           break;
         }
+        // a multi-view frame stays within one slot's share of the budget, so
+        // every slot can hold a frame at once and none gives its buffers
+        // back: larger groups are halved, the tail goes back to the queue
+        if (budget && g.size() > 1) {
+          uint64_t assets;
+          {
+            std::lock_guard<std::mutex> alk(ctx->mu);
+            assets = ctx->slabs.capacity();
+          }
+          const uint64_t share = budget > assets ? (budget - assets) / slots.size() : 0u;
+          if (own > share) {
+            std::lock_guard<std::mutex> lk(mu);
+            const size_t keep = g.size() / 2;
+            for (size_t k = g.size(); k-- > keep;) queue.push_front(g[k]);
+            g.resize(keep);
+            ++stats.groups_split;
+            continue;
+          }
+        }
         uint64_t outb = 0;
-        for (auto& p : g) outb += (uint64_t)p.req.camera.screen_width * p.req.camera.screen_height * 20u;
+        for (auto& p : g) outb += staged_bytes(p.req);
         auto total = [&]() { return held_bytes() - s.fs->held_bytes() - s.out_bytes + need + std::max(s.out_bytes, outb); };
         if (budget && total() > budget) {
           for (auto& o : slots)  // idle slots give their buffers back
@@ -1590,8 +1651,10 @@ int sr_server_submit(sr_server* srv, const sr_request* req, uint64_t* ticket) {
   if (!srv || !req) return SR_ERR_INVALID_ARGUMENT;
   if (req->light_count > SR_SERVER_MAX_LIGHTS || req->camera.screen_width == 0 || req->camera.screen_height == 0)
     return fail(srv->ctx, SR_ERR_INVALID_ARGUMENT, "sr_server_submit: bad request");
-  if (req->settings.flags != 0u)  // served frames build their own lights
+  if (req->settings.flags != 0u)  // served frames build their own lights; outputs follow the request's flags
     return fail(srv->ctx, SR_ERR_UNSUPPORTED, "sr_server_submit: frame flags are not served");
+  if (req->flags & ~(SR_REQUEST_RGBA8 | SR_REQUEST_PINNED_OUTPUTS))
+    return fail(srv->ctx, SR_ERR_INVALID_ARGUMENT, "sr_server_submit: unknown request flags");
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->stopping) return fail(srv->ctx, SR_ERR_NOT_READY, "sr_server_submit: server is stopping");
   PendingRequest p{*req, srv->next_ticket++, srv->now_ms()};

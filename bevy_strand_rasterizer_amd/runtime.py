@@ -146,13 +146,30 @@ class StrandServer:
         self.srv = h
         self._pending = {}
 
-    def submit(self, request: np.ndarray, outputs: bool = True) -> int:
+    def submit(self, request: np.ndarray, outputs: bool = True, rgba8: bool = False, out=None,
+               pinned: bool = False) -> int:
+        """Queues one request.  outputs=False asks for none; otherwise the
+        composite (float32 H x W x 4, or with rgba8 uint8 H x W x 4 in the
+        reference's rgba8unorm format) and depth (float32 H x W) land in
+        arrays allocated here, or in `out` = (rgba, depth) given by the
+        caller; pinned=True says those are pinned host memory (the frame
+        copies straight into them)."""
         req = np.array(request, dtype=abi.request_dtype).reshape(1)
         w, h = int(req["camera"]["screen_width"][0]), int(req["camera"]["screen_height"][0])
-        rgba = np.zeros((h, w, 4), np.float32) if outputs else None
-        depth = np.zeros((h, w), np.float32) if outputs else None
-        req["out_rgba"] = rgba.ctypes.data if outputs else 0
-        req["out_depth"] = depth.ctypes.data if outputs else 0
+        if out is not None:
+            rgba, depth = out
+            assert rgba.shape == (h, w, 4) and rgba.dtype == (np.uint8 if rgba8 else np.float32)
+            assert depth.shape == (h, w) and depth.dtype == np.float32
+            assert rgba.flags["C_CONTIGUOUS"] and depth.flags["C_CONTIGUOUS"]
+        elif outputs:
+            rgba = np.empty((h, w, 4), np.uint8 if rgba8 else np.float32)
+            depth = np.empty((h, w), np.float32)
+        else:
+            rgba = depth = None
+        req["flags"] = (abi.REQUEST_RGBA8 if rgba8 else 0) | (abi.REQUEST_PINNED_OUTPUTS if pinned and rgba is not None
+                                                              else 0)
+        req["out_rgba"] = rgba.ctypes.data if rgba is not None else 0
+        req["out_depth"] = depth.ctypes.data if depth is not None else 0
         t = C.c_uint64()
         abi.check(self.lib, self.lib.sr_server_submit(self.srv, abi.ptr(req), C.byref(t)), self.r.ctx,
                   "sr_server_submit")

@@ -543,16 +543,17 @@ def serving_assets(sizes=SERVING_SIZES, points: int = 32, seed: int = 0x5EEDCAFE
     return [scalp_asset(n, points, seed + i, 0.0006, radius_px_for(0.0006, 1080, 0.6)) for i, n in enumerate(sizes)]
 
 
-def serving_request(assets: list, asset_ids: list, rng: np.random.Generator, tag: int = 0):
+def serving_request(assets: list, asset_ids: list, rng: np.random.Generator, tag: int = 0, backend: int = 0):
     """One request of the serving stream: a uniform-random asset, a random
     orbit camera and a resolution drawn from SERVING_RESOLUTIONS, lit by one
-    2048^2 x 4-layer light (shared by requests of the same asset).  Returns
+    2048^2 x 4-layer light (shared by requests of the same asset), binned
+    with fine-binning `backend` (0 segment scatter, 1 page CSR).  Returns
     (request record, workload of the same frame drawn alone)."""
     k = int(rng.integers(0, len(assets)))
     w, h = SERVING_RESOLUTIONS[int(rng.integers(0, len(SERVING_RESOLUTIONS)))]
     eye = orbit_eye(int(rng.integers(0, 360)), 360, 0.6)
-    wl = with_asset_id(build_workload(f"serve-{k}-{w}x{h}", assets[k], [(eye, w, h)], 1, 2048, 4, 8, 8),
-                       asset_ids[k])
+    wl = with_asset_id(build_workload(f"serve-{k}-{w}x{h}", assets[k], [(eye, w, h)], 1, 2048, 4, 8, 8,
+                                      settings={"fine_binning_backend": backend}), asset_ids[k])
     return request_from_workload(wl, tag=tag), wl
 
 

@@ -352,6 +352,12 @@ SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 80, "sr_frame_settings is 80 bytes
  * camera drawn in a frame that builds the lights.  Lets a batch of views too
  * large for one frame share one DOM build. */
 #define SR_FRAME_REUSE_LIGHT_DOMS 1u
+/* sr_frame_settings.flags: also store the strand image and the composite in
+ * the reference's framebuffer format, rgba8unorm (src/pipelines/raster.rs:
+ * 500-513): each channel clamped to [0, 1] and rounded to the nearest of the
+ * 255 steps, ties to even (WGSL pack4x8unorm), red in the lowest byte
+ * (SR_DUMP_IMAGE_RGBA8, SR_DUMP_COMPOSITE_RGBA8). */
+#define SR_FRAME_RGBA8_OUTPUTS 2u
 
 typedef struct sr_frame_desc {
   const sr_strand_instance* instances;
@@ -461,7 +467,9 @@ typedef enum sr_dump_kind {
                                 instances, visible strands, retained strands, emitted segments, pages,
                                 candidates, refs, raster runs, raster load, max page candidates, max raster load */
   SR_DUMP_OPAQUE_TILES = 26, /* uint32 (covered, quantised farthest depth) per fine tile when opaque depth was given */
-  SR_DUMP_KIND_COUNT = 27     /* one past the largest kind */
+  SR_DUMP_IMAGE_RGBA8 = 27,  /* uint8 x 4 per camera frustum W*H when SR_FRAME_RGBA8_OUTPUTS (the image, rgba8unorm) */
+  SR_DUMP_COMPOSITE_RGBA8 = 28, /* uint8 x 4 per camera frustum W*H when SR_FRAME_RGBA8_OUTPUTS (the composite) */
+  SR_DUMP_KIND_COUNT = 29     /* one past the largest kind */
 } sr_dump_kind;
 
 /* ------------------------------------------------------------------------ */
@@ -608,15 +616,24 @@ typedef struct sr_request {
   sr_froxel_config camera;      /* resolution and froxel size of the view */
   sr_view camera_view;
   uint32_t light_count;         /* 0..SR_SERVER_MAX_LIGHTS */
-  uint32_t pad;
+  uint32_t flags;               /* SR_REQUEST_* bits */
   sr_froxel_config light_frusta[SR_SERVER_MAX_LIGHTS];
   sr_view light_views[SR_SERVER_MAX_LIGHTS];
   sr_directional_light lights[SR_SERVER_MAX_LIGHTS];
   sr_frame_settings settings;
-  float* out_rgba;              /* host, width * height * 4 (composite, straight alpha) or NULL */
+  float* out_rgba;              /* host, width * height * 4 floats (composite, straight alpha), or with
+                                   SR_REQUEST_RGBA8 width * height * 4 bytes (rgba8unorm); or NULL */
   float* out_depth;             /* host, width * height or NULL */
   uint64_t tag;                 /* returned with the result */
 } sr_request;
+
+/* sr_request.flags: the composite arrives as rgba8unorm (4 bytes a pixel,
+ * SR_FRAME_RGBA8_OUTPUTS) instead of float RGBA. */
+#define SR_REQUEST_RGBA8 1u
+/* sr_request.flags: out_rgba and out_depth are pinned host memory
+ * (hipHostMalloc): the frame copies its outputs straight into them, with no
+ * staging copy on the host. */
+#define SR_REQUEST_PINNED_OUTPUTS 2u
 
 typedef struct sr_request_result {
   uint64_t ticket;
@@ -633,6 +650,7 @@ typedef struct sr_server_stats {
   uint64_t submitted, completed, failed, frames;
   uint64_t peak_held_bytes;     /* assets + slot buffers, the most seen at an admission */
   uint64_t budget_waits;        /* admissions deferred by the budget */
+  uint64_t groups_split;        /* multi-view groups halved to fit one slot's share of the budget */
   uint32_t max_frame_views, max_concurrent_frames;
 } sr_server_stats;
 

@@ -41,7 +41,7 @@ STRUCTS = [
     ("sr_request", abi.request_dtype, 904, "serving request (new)"),
     ("sr_request_result", abi.request_result_dtype, 112, "serving result (new)"),
     ("sr_server_config", abi.server_config_dtype, 16, "serving config (new)"),
-    ("sr_server_stats", abi.server_stats_dtype, 56, "serving stats (new)"),
+    ("sr_server_stats", abi.server_stats_dtype, 64, "serving stats (new)"),
     ("sr_tune_trial", abi.tune_trial_dtype, 32, "autotuner trial (new)"),
     ("sr_tune_result", abi.tune_result_dtype, 552, "autotuner result (new)"),
 ]
@@ -138,7 +138,7 @@ def dump_kinds():
     """sr_dump_kind enumerators of the header (name -> value)."""
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
     body = re.search(r"typedef enum sr_dump_kind \{(.*?)\} sr_dump_kind;", text, flags=re.S).group(1)
-    return {m.group(1): int(m.group(2)) for m in re.finditer(r"SR_DUMP_([A-Z_]+)\s*=\s*(\d+)", body)}
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"SR_DUMP_([A-Z0-9_]+)\s*=\s*(\d+)", body)}
 
 
 def test_dump_kind_count_exceeds_every_kind():

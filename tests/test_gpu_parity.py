@@ -193,6 +193,37 @@ def test_two_strands_closed_forms(backend):
     np.testing.assert_allclose(g["image"].reshape(64, 64, 4)[20, 32], np.r_[rgb, a], atol=1e-6)
 
 
+def unorm8(x):
+    """rgba8unorm of float channels: clamp, x 255, round half to even."""
+    return np.rint(np.clip(x, 0.0, 1.0) * np.float32(255.0)).astype(np.uint8)
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_rgba8_outputs(backend):
+    """SR_FRAME_RGBA8_OUTPUTS: the image and composite in the reference's
+    rgba8unorm framebuffer format (src/pipelines/raster.rs:500-513) are the
+    FP32 outputs rounded to 8 bits, and within one step (1/255) of the
+    oracle's FP32 image."""
+    w = scene.config_small(froxel=8)
+    w.settings["fine_binning_backend"] = backend
+    w.settings["frame_flags"] = abi.FRAME_RGBA8_OUTPUTS
+    ref = oracle.render(w)
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        r.render(w)
+        img, comp = r.dump("image"), r.dump("composite")
+        img8, comp8 = r.dump("image_rgba8"), r.dump("composite_rgba8")
+        w.settings["frame_flags"] = 0
+        r.render(w)
+        assert r.dump("image_rgba8").size == 0, "rgba8 outputs only when asked for"
+        assert r.dump("image").tobytes() == img.tobytes()
+    assert img8.tobytes() == unorm8(img).tobytes() and comp8.tobytes() == unorm8(comp).tobytes()
+    for got, want in ((img8, ref.get("image")), (comp8, ref.get("composite"))):
+        d = np.abs(got.astype(np.float64) / 255.0 - np.clip(want, 0.0, 1.0)).max()
+        assert d <= 1.0 / 255.0 + 1e-6, d
+    assert (img8.reshape(-1, 4)[:, 3] > 0).sum() > 1000
+
+
 def test_parity_multiview():
     """Two cameras of different resolution and two lights in one frame."""
     base = scene.config_small(froxel=8)

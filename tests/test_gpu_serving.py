@@ -10,6 +10,10 @@ from bevy_strand_rasterizer_amd.runtime import StrandRenderer, StrandServer
 
 pytestmark = pytest.mark.gpu
 
+# torch (pinned host buffers) is imported before any test loads the native
+# library, so that its device runtime comes up in this process
+torch = pytest.importorskip("torch")
+
 
 def assets():
     a = scene.scalp_asset(300, 8, 11, 0.002, 2.0)
@@ -101,3 +105,33 @@ def test_budget_admission():
             srv.submit(scene.request_from_workload(wls[0]))
             (rec,) = srv.drain(1)
         assert rec["status"] == abi.STATUS_CODES["SR_ERR_CAPACITY"]
+
+
+def test_rgba8_pinned_outputs_equal_alone():
+    """Requests asking for the reference's rgba8unorm composite, copied
+    straight into pinned host buffers, equal the same requests drawn alone
+    (their FP32 composite rounded to 8 bits) and their depth is exact."""
+    torch.cuda.init()
+    a = assets()
+    mix = MIX[:6]
+    with StrandRenderer() as r:
+        for x in a:
+            r.upload(x)
+        wls = [request_workload(a[aid], aid, i, 10, w, h) for aid, i, w, h in mix]
+        ref = [alone(r, wl) for wl in wls]
+        outs = []
+        for aid, i, w, h in mix:
+            rgba = torch.empty((h, w, 4), dtype=torch.uint8, pin_memory=True).numpy()
+            depth = torch.empty((h, w), dtype=torch.float32, pin_memory=True).numpy()
+            outs.append((rgba, depth))
+        with StrandServer(r, slots=2, max_views=3) as srv:
+            for k, wl in enumerate(wls):
+                srv.submit(scene.request_from_workload(wl, tag=k), rgba8=True, out=outs[k], pinned=True)
+            res = srv.drain(len(wls))
+    for rec in res:
+        k = rec["tag"]
+        assert rec["status"] == 0
+        comp, depth = ref[k]
+        want8 = np.rint(np.clip(comp, 0.0, 1.0) * np.float32(255.0)).astype(np.uint8)
+        assert outs[k][0].reshape(-1, 4).tobytes() == want8.tobytes(), f"request {k}: rgba8 composite"
+        assert outs[k][1].reshape(-1).tobytes() == depth.tobytes(), f"request {k}: depth"
