@@ -338,7 +338,7 @@ def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool 
     if tune:
         res = r.autotune(w, frames_per_trial=2, min_psnr_db=50.0)
         best, trials = res["best"], res["trials"]
-        w = scene.retile(w, best["camera_froxel"], best["light_froxel"], best["fine_binning_backend"])
+        w = scene.apply_tuning(w, best)
     ctr, lat, dt = time_frames(r, w, steps, warmup)
     r.close()
     views = w.camera_count

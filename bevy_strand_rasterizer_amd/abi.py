@@ -123,8 +123,8 @@ class FrameSettings(C.Structure):
 class FrameDesc(C.Structure):
     _fields_ = [("instances", C.c_void_p), ("instance_count", C.c_uint32), ("frustum_count", C.c_uint32),
                 ("frusta", C.c_void_p), ("views", C.c_void_p), ("lights", C.c_void_p),
-                ("light_count", C.c_uint32), ("pad", C.c_uint32), ("settings", FrameSettings),
-                ("opaque_depth", C.c_void_p), ("opaque_samples", C.c_uint32), ("pad2", C.c_uint32)]
+                ("light_count", C.c_uint32), ("mark_waves", C.c_uint32), ("settings", FrameSettings),
+                ("opaque_depth", C.c_void_p), ("opaque_samples", C.c_uint32), ("raster_waves", C.c_uint32)]
 
 
 class AssetSlabs(C.Structure):
@@ -160,9 +160,10 @@ class ScalpParams(C.Structure):
 # Every symbol the header declares, with its ctypes signature.
 tune_trial_dtype = np.dtype([("fine_binning_backend", "<u4"), ("camera_froxel", "<u4"), ("light_froxel", "<u4"),
                              ("hip_graphs", "<u4"), ("ms_per_frame", "<f4"), ("psnr_db", "<f4"), ("accepted", "<u4"),
-                             ("pad", "<u4")])
+                             ("mark_waves", "<u4"), ("raster_waves", "<u4"), ("pad", "<u4")])
+TUNE_MAX_TRIALS = 24
 tune_result_dtype = np.dtype([("best", tune_trial_dtype), ("trial_count", "<u4"), ("pad", "<u4"),
-                              ("trials", tune_trial_dtype, 16)])
+                              ("trials", tune_trial_dtype, TUNE_MAX_TRIALS)])
 
 SERVER_MAX_LIGHTS = 4
 request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel_config_dtype),

@@ -103,7 +103,8 @@ struct FrameParams {
   uint32_t coarse_total;  // coarse tiles over all frusta
   uint32_t task_cap, ref_cap, work_cap, atlas_texels;
   uint32_t dom_layers, page_cap;
-  uint32_t pad0, pad00;
+  uint32_t mark_waves;     // waves per workgroup of the projection / marking pass (1, 2 or 4)
+  uint32_t raster_waves;   // waves per CU of the persistent rasterisers (0 = 16)
   sr_frame_settings st;
 
   uint32_t* counters;  // Counter words

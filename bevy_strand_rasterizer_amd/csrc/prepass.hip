@@ -640,9 +640,10 @@ __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
 // Pass 1: per slot validity ballot + packed endpoints in slot layout; valid
 // slots also mark their pages (the LUT is built before projection) and, for
 // the CSR backend, record their fine cells and count page candidates.
-__global__ __launch_bounds__(256) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
-  __shared__ MarkJob jobs[4][kMarkQ];
-  __shared__ uint32_t jn[4], nrec[4][64];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
+  __shared__ MarkJob jobs[WAVES][kMarkQ];
+  __shared__ uint32_t jn[WAVES], nrec[WAVES][64];
   const uint64_t total = (uint64_t)fp.F * fp.slots;
   const bool csr = fp.st.fine_binning_backend != 0u;
   const bool record = csr && fp.hit_k != 0u;
@@ -1759,7 +1760,20 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   timer->mark(s, "depth_lut");
   hipLaunchKernelGGL(k_lut, dim3(grid_for(fp.coarse_total, 64, 1u << 20)), dim3(64), 0, s, fp);
   timer->mark(s, "project_mark");
-  hipLaunchKernelGGL(k_project_mark, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp, ps.slot_proj);
+  // one to four waves per workgroup, the same number of threads in flight
+  switch (fp.mark_waves) {
+    case 1u:
+      hipLaunchKernelGGL(k_project_mark<1>, dim3(grid_for(slots_total, 64, 4u * kMaxGrid)), dim3(64), 0, s, fp,
+                         ps.slot_proj);
+      break;
+    case 2u:
+      hipLaunchKernelGGL(k_project_mark<2>, dim3(grid_for(slots_total, 128, 2u * kMaxGrid)), dim3(128), 0, s, fp,
+                         ps.slot_proj);
+      break;
+    default:
+      hipLaunchKernelGGL(k_project_mark<4>, dim3(grid_for(slots_total, 256, kMaxGrid)), dim3(256), 0, s, fp,
+                         ps.slot_proj);
+  }
   timer->mark(s, "project_compact");
   launch_scan<uint32_t>(MaskSrc{fp.slot_masks}, WaveOffsetDst{fp.wave_offsets, fp.counters},
                         (uint32_t)(slots_total / 64u), nullptr, ps.partials, s);

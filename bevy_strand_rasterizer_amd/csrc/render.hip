@@ -903,11 +903,18 @@ __global__ void k_pack_rgba8(FrameParams fp, uint64_t pixels) {
 }
 
 // ------------------------------------------------------------ launchers
+// Persistent raster workgroups: raster_waves waves per CU (256 CUs, 16 by
+// default) in workgroups of `waves` waves.
+static inline uint32_t raster_grid(const FrameParams& fp, uint32_t waves) {
+  const uint32_t per_cu = fp.raster_waves ? fp.raster_waves : 16u;
+  return 256u * (per_cu > waves ? per_cu / waves : 1u);
+}
+
 void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
   if (max_tile_px <= 64)
-    hipLaunchKernelGGL(k_shadow_raster<64>, dim3(256u * 16u), dim3(64), 64u * 4u * fp.dom_layers, s, fp);
+    hipLaunchKernelGGL(k_shadow_raster<64>, dim3(raster_grid(fp, 1u)), dim3(64), 64u * 4u * fp.dom_layers, s, fp);
   else
-    hipLaunchKernelGGL(k_shadow_raster<256>, dim3(256u * 4u), dim3(256), 256u * 4u * fp.dom_layers, s, fp);
+    hipLaunchKernelGGL(k_shadow_raster<256>, dim3(raster_grid(fp, 4u)), dim3(256), 256u * 4u * fp.dom_layers, s, fp);
 }
 
 void launch_shading(const FrameParams& fp, hipStream_t s) {
@@ -918,9 +925,9 @@ void launch_shading(const FrameParams& fp, hipStream_t s) {
 
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
   if (max_tile_px <= 64)
-    hipLaunchKernelGGL(k_camera_raster<64>, dim3(256u * 16u), dim3(64), 0, s, fp);
+    hipLaunchKernelGGL(k_camera_raster<64>, dim3(raster_grid(fp, 1u)), dim3(64), 0, s, fp);
   else
-    hipLaunchKernelGGL(k_camera_raster<256>, dim3(256u * 4u), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_camera_raster<256>, dim3(raster_grid(fp, 4u)), dim3(256), 0, s, fp);
 }
 
 void launch_stampback(const FrameParams& fp, uint32_t fi, uint32_t texels, hipStream_t s) {

@@ -567,6 +567,9 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (fd->instance_count && !fd->instances) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no instances");
   if (fd->instance_count >= (1u << 24)) return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit: at most 2^24 instances");
   if (fd->light_count && !fd->lights) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: no lights");
+  if (!(fd->mark_waves == 0u || fd->mark_waves == 1u || fd->mark_waves == 2u || fd->mark_waves == 4u) ||
+      fd->raster_waves > 32u)
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: mark_waves must be 1, 2 or 4 and raster_waves at most 32");
   const sr_frame_settings& st = fd->settings;
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
@@ -884,6 +887,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.slots = slots; fp.coarse_total = coarse_total;
   fp.task_cap = task_cap; fp.ref_cap = ref_cap; fp.work_cap = work_cap; fp.atlas_texels = atlas_texels;
   fp.dom_layers = st.dom_layers;
+  fp.mark_waves = fd->mark_waves ? fd->mark_waves : 4u;
+  fp.raster_waves = fd->raster_waves;  // 0: the kernel's default
   fp.st = st;
   fp.st.shadow_history_valid = hist_valid ? 1u : 0u;
   fp.counters = sl.buf[B_COUNTERS].as<uint32_t>();
@@ -1708,12 +1713,15 @@ int sr_server_stop(sr_server* srv) {
 }  // extern "C"
 
 // ==================================================================== autotune
-// Times candidate configurations of one frame on the context's main slot:
-// fine-binning backend x camera froxel size x light froxel size, with the
-// launch sequence in a HIP graph, plus the frame as given without graphs
-// (the quality reference).  A candidate is accepted when its composite is
-// within min_psnr_db of the reference (tile size changes the order in which
-// overlapping fragments blend); the fastest accepted one is `best`.
+// Times candidate configurations of one frame on the context's main slot,
+// with the launch sequence in a HIP graph, against the frame as given
+// without graphs (the quality reference): fine-binning backend x camera
+// froxel size x light froxel size, then for the fastest accepted of those
+// the launch shape (marking waves per workgroup x raster waves per CU).  A
+// candidate is accepted when its composite is within min_psnr_db of the
+// reference (tile size changes the order in which overlapping fragments
+// blend; the launch shape changes no result); the fastest accepted one is
+// `best`.
 namespace {
 
 struct TuneFrame {
@@ -1721,7 +1729,8 @@ struct TuneFrame {
   sr_frame_desc fd;
 };
 
-int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint32_t light_fx, TuneFrame& out) {
+int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint32_t light_fx, uint32_t mark_waves,
+               uint32_t raster_waves, TuneFrame& out) {
   const uint32_t F = src->frustum_count;
   std::vector<sr_froxel_config> cfg(F);
   std::vector<uint32_t> is_light(F), cascade(F);
@@ -1739,6 +1748,8 @@ int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint
   out.fd = *src;
   out.fd.frusta = out.frusta.data();
   out.fd.settings.fine_binning_backend = backend;
+  out.fd.mark_waves = mark_waves;
+  out.fd.raster_waves = raster_waves;
   return SR_OK;
 }
 
@@ -1758,12 +1769,14 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
     if (fd->frusta[i].kind == 0u && !cam0) cam0 = fd->frusta[i].froxel_size_x;
     if (fd->frusta[i].kind == 1u && !light0) light0 = fd->frusta[i].froxel_size_x;
   }
-  struct Cand { uint32_t backend, cam, light, graphs; };
+  struct Cand { uint32_t backend, cam, light, graphs, mark_waves, raster_waves; };
   std::vector<Cand> cands;
-  cands.push_back({fd->settings.fine_binning_backend, cam0, light0, 0u});  // the reference
+  cands.push_back({fd->settings.fine_binning_backend, cam0, light0, 0u, fd->mark_waves, fd->raster_waves});  // the reference
   for (uint32_t b : {0u, 1u})
     for (uint32_t cf : {8u, 16u})
-      for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, 1u});
+      for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, 1u, 0u, 0u});
+  const size_t stage1 = cands.size();
+  bool stage2 = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<float4> ref_img, img;
   int rc = SR_OK;
@@ -1771,10 +1784,19 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SR_ERR_DEVICE;
   const uint32_t frames = frames_per_trial ? frames_per_trial : 3;
   float best_ms = 0.0f;
-  for (size_t c = 0; c < cands.size() && rc == SR_OK && out->trial_count < 16; ++c) {
-    const Cand& k = cands[c];
+  for (size_t c = 0; rc == SR_OK && out->trial_count < SR_TUNE_MAX_TRIALS; ++c) {
+    if (c == stage1 && !stage2 && best_ms > 0.0f) {  // launch shapes of the best configuration so far
+      stage2 = true;
+      const sr_tune_trial b = out->best;
+      for (uint32_t mw : {1u, 2u, 4u})
+        for (uint32_t rw : {8u, 16u, 24u})
+          if (!(mw == 4u && rw == 16u))  // the defaults, timed in the first stage
+            cands.push_back({b.fine_binning_backend, b.camera_froxel, b.light_froxel, 1u, mw, rw});
+    }
+    if (c >= cands.size()) break;
+    const Cand k = cands[c];
     TuneFrame tf;
-    rc = tune_frame(fd, k.backend, k.cam, k.light, tf);
+    rc = tune_frame(fd, k.backend, k.cam, k.light, k.mark_waves, k.raster_waves, tf);
     if (rc != SR_OK) break;
     ctx->flags = k.graphs ? (saved_flags | SR_CONFIG_HIP_GRAPHS) : (saved_flags & ~SR_CONFIG_HIP_GRAPHS);
     sr_frame_counters fc;
@@ -1814,6 +1836,8 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
     t.camera_froxel = k.cam;
     t.light_froxel = k.light;
     t.hip_graphs = k.graphs;
+    t.mark_waves = k.mark_waves;
+    t.raster_waves = k.raster_waves;
     t.ms_per_frame = ms / (float)frames;
     t.psnr_db = mse > 0.0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.0f;
     t.accepted = t.psnr_db >= min_psnr_db ? 1u : 0u;

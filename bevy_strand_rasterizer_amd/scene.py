@@ -103,6 +103,8 @@ class Workload:
         fd.lights = abi.ptr(self.lights) if len(self.lights) else None
         fd.light_count = len(self.lights)
         fd.settings = self.frame_settings()
+        fd.mark_waves = int(self.settings.get("mark_waves", 0))  # launch shape (0 = defaults)
+        fd.raster_waves = int(self.settings.get("raster_waves", 0))
         if self.opaque is not None:
             assert self.opaque.dtype == np.float32 and self.opaque.flags["C_CONTIGUOUS"] and self.opaque.ndim == 2
             fd.opaque_depth = abi.ptr(self.opaque)
@@ -368,6 +370,16 @@ def retile(w: Workload, camera_froxel: int = 0, light_froxel: int = 0, backend: 
         st["fine_binning_backend"] = int(backend)
     return Workload(name=w.name, assets=w.assets, instances=w.instances, frusta=frustum_table(cfgs, kinds),
                     views=w.views, lights=w.lights, settings=st, opaque=w.opaque)
+
+
+def apply_tuning(w: Workload, best: dict) -> Workload:
+    """The workload with an autotuner result applied: froxel sizes and
+    binning backend (retile) and the launch shape (marking waves per
+    workgroup, raster waves per CU)."""
+    t = retile(w, best["camera_froxel"], best["light_froxel"], best["fine_binning_backend"])
+    t.settings["mark_waves"] = int(best.get("mark_waves", 0))
+    t.settings["raster_waves"] = int(best.get("raster_waves", 0))
+    return t
 
 
 def select_views(w: Workload, cameras: list, name: str | None = None) -> Workload:

@@ -367,7 +367,9 @@ typedef struct sr_frame_desc {
   const sr_view* views;          /* [frustum_count] */
   const sr_directional_light* lights;
   uint32_t light_count;
-  uint32_t pad;
+  /* Launch shape (no reference counterpart; results do not depend on it):
+   * waves per workgroup of the projection / marking pass (1, 2 or 4; 0 = 4). */
+  uint32_t mark_waves;
   sr_frame_settings settings;
   /* Opaque scene depth per camera (the reference's prepass depth texture,
    * src/pipelines/prepass.rs:780-816), host memory: for every camera frustum
@@ -376,7 +378,10 @@ typedef struct sr_frame_desc {
    * opaque geometry, nothing is occluded. */
   const float* opaque_depth;
   uint32_t opaque_samples; /* 1..4 (MSAA samples) */
-  uint32_t pad2;
+  /* Launch shape: waves per CU of the persistent shadow and camera
+   * rasterisers (1..32; 0 = 16): one-wave workgroups for tiles of up to 64
+   * pixels, four-wave ones (raster_waves / 4 of them) above. */
+  uint32_t raster_waves;
 } sr_frame_desc;
 
 /* Host-side view of one asset's slabs in the reference layout
@@ -564,13 +569,17 @@ const char* sr_last_error(sr_context* ctx);
 /* ------------------------------------------------------------------------ */
 /* Autotuner (no reference counterpart: the reference fixes its tile sizes and
  * binning backend per run, src/resources.rs).  Times one frame under the
- * candidate configurations -- fine-binning backend (segment scatter / page
- * CSR) x camera froxel size {8, 16} x light froxel size {8, 16}, replayed from
- * a HIP graph -- against the frame as given without graphs (the quality
- * reference), on the context's main slot.  A candidate whose composite stays
- * within min_psnr_db of the reference is accepted; `best` is the fastest
- * accepted one.  The caller rebuilds its frustum table with best's froxel
- * sizes (sr_build_frustum_table) and sets the backend and SR_CONFIG_HIP_GRAPHS. */
+ * candidate configurations, replayed from a HIP graph, against the frame as
+ * given without graphs (the quality reference), on the context's main slot:
+ * first fine-binning backend (segment scatter / page CSR) x camera froxel size
+ * {8, 16} x light froxel size {8, 16}; then, for the fastest accepted of
+ * those, the launch shape: marking workgroups of 1, 2 or 4 waves x 8, 16 or
+ * 24 persistent raster waves per CU.  A candidate whose composite stays
+ * within min_psnr_db of the reference is accepted (tile size changes the
+ * order in which overlapping fragments blend; the launch shape changes no
+ * result); `best` is the fastest accepted one.  The caller rebuilds its
+ * frustum table with best's froxel sizes (sr_build_frustum_table), sets the
+ * backend, SR_CONFIG_HIP_GRAPHS and the frame's mark_waves / raster_waves. */
 typedef struct sr_tune_trial {
   uint32_t fine_binning_backend;
   uint32_t camera_froxel;       /* 0 when the frame has no camera */
@@ -579,14 +588,17 @@ typedef struct sr_tune_trial {
   float ms_per_frame;
   float psnr_db;                /* composite against the reference configuration */
   uint32_t accepted;
+  uint32_t mark_waves;          /* sr_frame_desc.mark_waves of the trial (0 = default) */
+  uint32_t raster_waves;        /* sr_frame_desc.raster_waves of the trial (0 = default) */
   uint32_t pad;
 } sr_tune_trial;
 
+#define SR_TUNE_MAX_TRIALS 24u
 typedef struct sr_tune_result {
   sr_tune_trial best;
   uint32_t trial_count;
   uint32_t pad;
-  sr_tune_trial trials[16];
+  sr_tune_trial trials[SR_TUNE_MAX_TRIALS];
 } sr_tune_result;
 
 int sr_autotune(sr_context* ctx, const sr_frame_desc* frame, uint32_t frames_per_trial, float min_psnr_db,

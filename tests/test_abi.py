@@ -42,8 +42,8 @@ STRUCTS = [
     ("sr_request_result", abi.request_result_dtype, 112, "serving result (new)"),
     ("sr_server_config", abi.server_config_dtype, 16, "serving config (new)"),
     ("sr_server_stats", abi.server_stats_dtype, 64, "serving stats (new)"),
-    ("sr_tune_trial", abi.tune_trial_dtype, 32, "autotuner trial (new)"),
-    ("sr_tune_result", abi.tune_result_dtype, 552, "autotuner result (new)"),
+    ("sr_tune_trial", abi.tune_trial_dtype, 40, "autotuner trial (new)"),
+    ("sr_tune_result", abi.tune_result_dtype, 40 + 8 + 24 * 40, "autotuner result (new)"),
 ]
 
 

@@ -67,7 +67,7 @@ def test_batched_views_equal_alone_and_oracle():
         r.upload_workload(base)
         tuned = r.autotune(base, frames_per_trial=1, min_psnr_db=50.0)["best"]
     print(f"[configs] tuned {tuned}", flush=True)
-    w = scene.retile(base, tuned["camera_froxel"], tuned["light_froxel"], tuned["fine_binning_backend"])
+    w = scene.apply_tuning(base, tuned)
     flags = abi.CONFIG_HIP_GRAPHS if tuned["hip_graphs"] else 0
     with StrandRenderer(flags=flags) as r:
         r.upload_workload(w)
@@ -100,12 +100,14 @@ def test_serving_mix_on_serving_assets():
     rng = np.random.default_rng(2024)
     with StrandRenderer() as r:
         ids = [r.upload(a).geo_id for a in assets]
-        reqs = [scene.serving_request(assets, ids, rng, tag=k) for k in range(12)]
+        # both binning backends (served frames of one backend are merged)
+        reqs = [scene.serving_request(assets, ids, rng, tag=k, backend=k % 2) for k in range(12)]
         # the 500k-strand asset at 3840x2160, and the 500k asset again at 1080p
         for k, (asset, res) in enumerate(((7, (3840, 2160)), (7, (1920, 1080)), (0, (3840, 2160)))):
             eye = scene.orbit_eye(40 * k, 360, 0.6)
             wl = scene.with_asset_id(scene.build_workload(f"serve-{asset}-{res[0]}x{res[1]}", assets[asset],
-                                                          [(eye, *res)], 1, 2048, 4, 8, 8), ids[asset])
+                                                          [(eye, *res)], 1, 2048, 4, 8, 8,
+                                                          settings={"fine_binning_backend": 1}), ids[asset])
             reqs.append((scene.request_from_workload(wl, tag=len(reqs)), wl))
         want = []
         for req, wl in reqs:

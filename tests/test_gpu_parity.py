@@ -362,16 +362,24 @@ def test_autotune_picks_an_accepted_configuration():
         res = r.autotune(w, frames_per_trial=2, min_psnr_db=40.0)
     print(f"[autotune] best {res['best']}", flush=True)
     trials = res["trials"]
-    assert len(trials) == 9
+    assert len(trials) == 17  # reference, 8 backend x froxel candidates, 8 launch shapes of the best
     assert trials[0]["hip_graphs"] == 0 and trials[0]["psnr_db"] > 900  # the reference itself
     assert res["best"]["accepted"] == 1 and res["best"]["ms_per_frame"] > 0
     assert res["best"]["ms_per_frame"] == min(t["ms_per_frame"] for t in trials if t["accepted"])
     # same froxel sizes and backend as the reference: the graph replay is exact
-    same = [t for t in trials[1:] if t["camera_froxel"] == 8 and t["light_froxel"] == 8
+    same = [t for t in trials[1:9] if t["camera_froxel"] == 8 and t["light_froxel"] == 8
             and t["fine_binning_backend"] == trials[0]["fine_binning_backend"]]
     assert same and same[0]["psnr_db"] > 900
-    tuned = scene.retile(w, res["best"]["camera_froxel"], res["best"]["light_froxel"],
-                         res["best"]["fine_binning_backend"])
+    # the launch shapes keep the stage-one winner's configuration and change no result
+    first = min((t for t in trials[1:9] if t["accepted"]), key=lambda t: t["ms_per_frame"])
+    shapes = trials[9:]
+    assert {(t["mark_waves"], t["raster_waves"]) for t in shapes} == {
+        (m, r) for m in (1, 2, 4) for r in (8, 16, 24)} - {(4, 16)}
+    for t in shapes:
+        assert (t["fine_binning_backend"], t["camera_froxel"], t["light_froxel"]) == (
+            first["fine_binning_backend"], first["camera_froxel"], first["light_froxel"])
+        assert t["psnr_db"] == first["psnr_db"]
+    tuned = scene.apply_tuning(w, res["best"])
     ref = oracle.render(tuned)
     ctr, gpu = gpu_frame(tuned)
     compare(tuned, ctr, gpu, ref)
