@@ -35,6 +35,16 @@ SR_HD float span_depth01_max(float za, float zb) {
   return lo < 0.0f ? fmaxf(n, 0.5f) : n;
 }
 
+// Reciprocal and square root of the span setup.  The spans only need to be
+// conservative, and their pad (0.03 px plus 4e-6 and 1e-6 relative terms)
+// exceeds a few ulp of error in either, so the rasterisers use the
+// hardware's (within 1 ulp) and the host check uses the correctly rounded
+// ones and ones perturbed by a few ulp (tests/span_check.cpp).
+struct ExactMath {
+  SR_HD static float rcp(float x) { return 1.0f / x; }
+  SR_HD static float sqrt(float x) { return sqrtf(x); }
+};
+
 // The rectangle's interval on pixel row y of a tile is
 //   x in [max(box_lo, al0 + y al1, cl0 + y cl1), min(box_hi, ah0 + y al1, ch0 + y cl1)]
 // (x, y relative to the tile's first pixel centre): the along and across
@@ -49,11 +59,12 @@ struct SpanRef {
   int32_t y0, y1;        // rows that can meet the rectangle
 };
 
+template <class M = ExactMath>
 SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, uint32_t tx0, uint32_t ty0,
                          uint32_t fy) {
   SpanRef s;
   const float l2 = abx * abx + aby * aby;
-  const float len = sqrtf(l2);
+  const float len = M::sqrt(l2);
   const float pad = 0.03f + 4e-6f * (fabsf(x0) + fabsf(y0) + len + rmax);
   const float rp = rmax + pad;
   const float dlo = -pad * len, dhi = l2 + pad * len, c = rp * len;
@@ -65,7 +76,7 @@ SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, u
   const float inf = __builtin_huge_valf();
   float m = fabsf(c0) + fabsf(s.box_lo) + fabsf(s.box_hi);
   if (abx != 0.0f) {  // u abx in [dlo - v aby, dhi - v aby]
-    const float ia = 1.0f / abx;
+    const float ia = M::rcp(abx);
     const float e0 = (dlo - d0 * aby) * ia - c0, e1 = (dhi - d0 * aby) * ia - c0;
     s.al0 = fminf(e0, e1);
     s.ah0 = fmaxf(e0, e1);
@@ -77,13 +88,14 @@ SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, u
     s.ah0 = inf;
     s.al1 = 0.0f;
     if (aby != 0.0f) {
-      const float w0 = dlo / aby, w1 = dhi / aby;
+      const float iy = M::rcp(aby);
+      const float w0 = dlo * iy, w1 = dhi * iy;
       vlo = fmaxf(vlo, fminf(w0, w1));
       vhi = fminf(vhi, fmaxf(w0, w1));
     }
   }
   if (aby != 0.0f) {  // u aby in [v abx - c, v abx + c]
-    const float ib = 1.0f / aby;
+    const float ib = M::rcp(aby);
     const float e0 = (d0 * abx - c) * ib - c0, e1 = (d0 * abx + c) * ib - c0;
     s.cl0 = fminf(e0, e1);
     s.ch0 = fmaxf(e0, e1);
@@ -95,7 +107,7 @@ SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, u
     s.ch0 = inf;
     s.cl1 = 0.0f;
     if (abx != 0.0f) {
-      const float w = c / fabsf(abx);
+      const float w = c * M::rcp(fabsf(abx));
       vlo = fmaxf(vlo, -w);
       vhi = fminf(vhi, w);
     }

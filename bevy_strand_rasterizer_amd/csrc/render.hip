@@ -159,6 +159,7 @@ struct ShadowBatch {
   float4 a[TPX];  // p0.x, p0.y, p0.z (depth01), |p1 - p0|^2 in xy
   float4 c[TPX];  // p1.x, p1.y, p1.z, absorption alpha
   float4 r[TPX];  // min radius, max radius, p1.x - p0.x, p1.y - p0.y
+  float inv[TPX]; // 1 / |p1 - p0|^2, correctly rounded (0 when the length is 0)
   uint32_t item[TPX];
 };
 
@@ -224,6 +225,7 @@ __device__ __forceinline__ void put_shadow_ref(const ShadowRefData& d, ShadowBat
   b.a[k] = make_float4(d.a.x, d.a.y, z0, l2);
   b.c[k] = make_float4(d.c.x, d.c.y, z1, d.alpha);
   b.r[k] = make_float4(minr, fmaxf(d.maxr, minr), abx, aby);
+  b.inv[k] = l2 == 0.0f ? 0.0f : 1.0f / l2;
   b.item[k] = d.w;
 }
 
@@ -234,7 +236,17 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
                                              float* cov) {
   const float4 A = b.a[k], Cc = b.c[k], Rr = b.r[k];
   float t = 0.0f;
-  if (!(A.w == 0.0f)) t = ((px - A.x) * Rr.z + (py - A.y) * Rr.w) / A.w;
+  if (!(A.w == 0.0f)) {
+    // t = num / l2 correctly rounded, from the reference's reciprocal: q is
+    // within an ulp of the quotient, r = num - q l2 exactly (FMA) and q + r /
+    // l2 rounds to the correctly rounded quotient (Markstein).  A q far
+    // outside [0, 1] decides the test without the correction.
+    const float num = (px - A.x) * Rr.z + (py - A.y) * Rr.w;
+    const float inv = b.inv[k];
+    const float q = num * inv;
+    if (q < -0.5f || q > 1.5f) return false;
+    t = __builtin_fmaf(__builtin_fmaf(-q, A.w, num), inv, q);
+  }
   if (t < 0.0f || t > 1.0f) return false;
   float qx = wmix(A.x, Cc.x, t), qy = wmix(A.y, Cc.y, t), qz = wmix(A.z, Cc.z, t);
   float r = wmix(Rr.x, Rr.y, wclamp(qz, 0.0f, 1.0f));
@@ -268,6 +280,12 @@ __device__ __forceinline__ unsigned long long wave_transpose64(unsigned long lon
   return x;
 }
 
+// The hardware reciprocal and square root (within 1 ulp) for the span setup.
+struct HwMath {
+  __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  __device__ static float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
 // Per pixel of the tile, one bit per reference of the batch that may cover
 // it (cols[p][w]: references w*64 .. w*64+63): each lane builds its
 // reference's conservative pixel mask from row spans, and a wave transpose
@@ -280,7 +298,7 @@ __device__ __forceinline__ void batch_columns(bool valid, float x0, float y0, fl
   const uint32_t k = threadIdx.x, w = k >> 6, l = k & 63u;
   uint64_t m[W];
   if (valid) {
-    span_mask<W>(span_setup(x0, y0, abx, aby, rmax, tx0, ty0, fy), fx, m);
+    span_mask<W>(span_setup<HwMath>(x0, y0, abx, aby, rmax, tx0, ty0, fy), fx, m);
   } else {
 #pragma unroll
     for (int j = 0; j < W; ++j) m[j] = 0ull;

@@ -6,11 +6,29 @@
 // 553-566).  Prints "cases misses kept covered"; exits 1 on a miss.
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <random>
 
 #include "../bevy_strand_rasterizer_amd/csrc/coverage.hpp"
 
 using namespace srk;
+
+// The kernels' span setup takes the hardware reciprocal and square root
+// (within 1 ulp); the check runs it with results moved 2 ulp down and up.
+template <int ULPS>
+struct UlpMath {
+  static float move(float x) {
+    for (int i = 0; i < (ULPS < 0 ? -ULPS : ULPS); ++i) x = nextafterf(x, ULPS < 0 ? -INFINITY : INFINITY);
+    return x;
+  }
+  static float rcp(float x) { return move(1.0f / x); }
+  static float sqrt(float x) { return move(sqrtf(x)); }
+};
+
+template <class M>
+static SpanRef setup(float x0, float y0, float abx, float aby, float rmax, uint32_t tx0, uint32_t ty0, uint32_t fy) {
+  return span_setup<M>(x0, y0, abx, aby, rmax, tx0, ty0, fy);
+}
 
 static bool shadow_exact(float x0, float y0, float z0, float x1, float y1, float z1, float minr, float maxr, float px,
                          float py) {
@@ -41,7 +59,7 @@ static bool camera_exact(float x0, float y0, float z0, float x1, float y1, float
   return dx * dx + dy * dy < r * r;
 }
 
-template <int W>
+template <int W, class M>
 static void run(std::mt19937& g, long cases, uint32_t fx, uint32_t fy, long* misses, long* kept, long* covered) {
   std::uniform_real_distribution<float> U(0.0f, 1.0f);
   for (long i = 0; i < cases; ++i) {
@@ -82,10 +100,10 @@ static void run(std::mt19937& g, long cases, uint32_t fx, uint32_t fy, long* mis
     const float abx = x1 - x0, aby = y1 - y0;
     uint64_t ms[W], mc[W];
     // the kernels' span radii: the exact radius at the deeper endpoint
-    span_mask<W>(span_setup(x0, y0, abx, aby, span_radius(minr_s, maxr_s, fmaxf(z0, z1)), tx0, ty0, fy), fx, ms);
+    span_mask<W>(setup<M>(x0, y0, abx, aby, span_radius(minr_s, maxr_s, fmaxf(z0, z1)), tx0, ty0, fy), fx, ms);
     const float minr_c = fmaxf(minr, 1e-4f), maxr_c = fmaxf(maxr, minr_c);
     const float rcam = span_radius(minr_c, maxr_c, span_depth01_max(z0, z1));
-    span_mask<W>(span_setup(x0, y0, abx, aby, rcam, tx0, ty0, fy), fx, mc);
+    span_mask<W>(setup<M>(x0, y0, abx, aby, rcam, tx0, ty0, fy), fx, mc);
     for (uint32_t y = 0; y < fy; ++y)
       for (uint32_t x = 0; x < fx; ++x) {
         const uint32_t p = y * fx + x;
@@ -109,10 +127,18 @@ int main(int argc, char** argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 200000;
   std::mt19937 g(12345);
   long misses = 0, kept = 0, covered = 0;
-  run<1>(g, n, 8, 8, &misses, &kept, &covered);
-  run<1>(g, n / 4, 4, 4, &misses, &kept, &covered);
-  run<4>(g, n / 4, 16, 16, &misses, &kept, &covered);
-  run<2>(g, n / 4, 16, 8, &misses, &kept, &covered);
-  std::printf("%ld %ld %ld %ld\n", n + 3 * (n / 4), misses, kept, covered);
+  long cases = 0;
+  auto all = [&](auto tag) {
+    using M = decltype(tag);
+    run<1, M>(g, n, 8, 8, &misses, &kept, &covered);
+    run<1, M>(g, n / 4, 4, 4, &misses, &kept, &covered);
+    run<4, M>(g, n / 4, 16, 16, &misses, &kept, &covered);
+    run<2, M>(g, n / 4, 16, 8, &misses, &kept, &covered);
+    cases += n + 3 * (n / 4);
+  };
+  all(ExactMath{});
+  all(UlpMath<-2>{});
+  all(UlpMath<2>{});
+  std::printf("%ld %ld %ld %ld\n", cases, misses, kept, covered);
   return misses ? 1 : 0;
 }
