@@ -145,19 +145,23 @@ def binning_headline(stage_acc, ctr, backend: str):
 
 
 def profile_row(kernel: str):
-    """(HBM bytes, VALU busy) per launch of `kernel` from the committed
-    rocprofv3 summary of the named workload, or (None, None)."""
+    """(HBM bytes, VALU issue, lane use) per launch of `kernel` from the
+    committed rocprofv3 summary of the named workload, or (None, None, None).
+    VALU issue is non-transcendental wave64 VALU instructions against one per
+    4 cycles per SIMD at the measured clock (0.95 is what a pure FMA stream
+    reaches); lane use is the fraction of a wave's lanes active per VALU
+    instruction."""
     f = Path(__file__).resolve().parent / "profiles" / "named_1080p_csr_kernels.md"
     if not f.exists():
-        return None, None
+        return None, None, None
     for line in f.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0].strip("`").startswith(kernel):
+        if len(cells) == 10 and cells[0].strip("`").startswith(kernel):
             try:
-                return int((float(cells[3]) + float(cells[4])) * 1e6), float(cells[6])
+                return int((float(cells[3]) + float(cells[4])) * 1e6), float(cells[7]), float(cells[9])
             except ValueError:
-                return None, None
-    return None, None
+                return None, None, None
+    return None, None, None
 
 
 STAGE_KERNEL = {"shadow": "k_shadow_raster", "raster": "k_camera_raster", "shading": "k_shading",
@@ -177,13 +181,16 @@ def roofline_for(stage, ms, ctr, w, refs=None):
            "bytes_per_launch": int(b), "avg_ms": round(ms, 4)}
     kern = STAGE_KERNEL.get(stage)
     if kern:
-        traffic, valu = profile_row(kern)
+        traffic, valu, lanes = profile_row(kern)
         out["kernel"] = kern
         out["traffic"] = traffic
         if valu is not None:
-            out["valu_busy"] = valu
+            out["valu_issue"] = valu
+            out["valu_lane_use"] = lanes
             if valu > 0.6:
-                out["note"] = f"VALU-bound ({STAGE_WORK.get(stage, 'exact arithmetic')}); traffic and VALU busy from profiles/"
+                out["note"] = (f"VALU-issue bound ({STAGE_WORK.get(stage, 'exact arithmetic')}): {valu:.2f} of one "
+                               "wave64 instruction per 4 cycles per SIMD, against 0.95 for a pure FMA stream, "
+                               f"with {lanes:.2f} of the lanes active; traffic and VALU figures from profiles/")
     return out
 
 
