@@ -28,6 +28,9 @@ CONFIG_TIGHT_CAPACITY = 8
 NOALLOC_SCRATCH_BYTES = 256 << 20  # SR_NOALLOC_SCRATCH_BYTES
 FRAME_REUSE_LIGHT_DOMS = 1  # sr_frame_settings.flags
 FRAME_RGBA8_OUTPUTS = 2
+FRAME_VIRTUAL_DOMS = 4
+DOM_PAGE_XY = 256  # SR_DOM_PAGE_XY
+DOM_PREFETCH_PAGE_BORDER = 1
 REQUEST_RGBA8 = 1  # sr_request.flags
 REQUEST_PINNED_OUTPUTS = 2
 
@@ -45,7 +48,8 @@ DUMP = {
     "lut": 7, "page_table": 8, "page_meta": 9, "cell_counts": 10, "cell_offsets": 11, "refs": 12,
     "active_tiles": 13, "work_items": 14, "camera_runs": 15, "shadow_runs": 16, "dom_depth": 17,
     "dom_opacity": 18, "atlas": 19, "history": 20, "image": 21, "depth": 22, "composite": 23, "stamp": 24,
-    "telemetry": 25, "opaque_tiles": 26, "image_rgba8": 27, "composite_rgba8": 28,
+    "telemetry": 25, "opaque_tiles": 26, "image_rgba8": 27, "composite_rgba8": 28, "dom_requests": 29,
+    "dom_page_table": 30,
 }
 
 
@@ -86,7 +90,7 @@ broad_instance_meta_dtype = np.dtype([("strand_count", "<u4"), ("visible", "<u4"
                                       ("camera_keep_probability", "<f4"), ("shadow_keep_probability", "<f4")])
 frame_counters_dtype = _u32("binning_tasks", "camera_tasks", "allocated_pages", "fine_refs", "active_tiles",
                             "work_items", "camera_runs", "shadow_runs", "overflow_pages", "coarse_tiles",
-                            "task_capacity", "ref_capacity", "page_drops", "page_candidates", "capacity_exceeded", "pad")
+                            "task_capacity", "ref_capacity", "page_drops", "page_candidates", "capacity_exceeded", "dom_pages")
 push_constants_dtype = np.dtype([("workgroup_offset", "<u4"), ("num_elements", "<u4"), ("frustum_count", "<u4"),
                                  ("scan_load_base", "<u4"), ("scan_save_base", "<u4"),
                                  ("stochastic_cull_enabled", "<u4"), ("target_strands_per_pixel", "<f4"),
@@ -105,7 +109,8 @@ DUMP_DTYPES = {
     "dom_opacity": np.dtype("<f4"), "atlas": np.dtype("<f4"), "history": np.dtype("<f4"),
     "image": np.dtype("<f4"), "depth": np.dtype("<f4"), "composite": np.dtype("<f4"), "stamp": np.dtype("<f4"),
     "telemetry": np.dtype("<u4"), "opaque_tiles": np.dtype([("covered", "<u4"), ("farthest_q", "<u4")]),
-    "image_rgba8": np.dtype("u1"), "composite_rgba8": np.dtype("u1"),
+    "image_rgba8": np.dtype("u1"), "composite_rgba8": np.dtype("u1"), "dom_requests": np.dtype("<u4"),
+    "dom_page_table": np.dtype("<u4"),
 }
 
 
@@ -117,7 +122,8 @@ class FrameSettings(C.Structure):
                 ("dom_layers", C.c_uint32), ("shadow_history_valid", C.c_uint32),
                 ("ambient_color", C.c_float * 4), ("scene_clear_color", C.c_float * 4),
                 ("scene_clear_depth", C.c_float), ("stamp_target_width", C.c_uint32),
-                ("stamp_target_height", C.c_uint32), ("flags", C.c_uint32)]
+                ("stamp_target_height", C.c_uint32), ("flags", C.c_uint32),
+                ("dom_resident_pages", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class FrameDesc(C.Structure):
@@ -176,7 +182,8 @@ request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel
                                                  ("dom_layers", "<u4"), ("shadow_history_valid", "<u4"),
                                                  ("ambient_color", "<f4", 4), ("scene_clear_color", "<f4", 4),
                                                  ("scene_clear_depth", "<f4"), ("stamp_target_width", "<u4"),
-                                                 ("stamp_target_height", "<u4"), ("flags", "<u4")])),
+                                                 ("stamp_target_height", "<u4"), ("flags", "<u4"),
+                                                 ("dom_resident_pages", "<u4"), ("pad", "<u4")])),
                           ("out_rgba", "<u8"), ("out_depth", "<u8"), ("tag", "<u8")], align=True)
 request_result_dtype = np.dtype([("ticket", "<u8"), ("tag", "<u8"), ("status", "<i4"), ("frame_views", "<u4"),
                                  ("submit_ms", "<f8"), ("launch_ms", "<f8"), ("done_ms", "<f8"),

@@ -24,6 +24,9 @@ constexpr uint32_t kQuant = SR_DEPTH_QUANT_MAX;
 constexpr uint32_t kInvalid = SR_INVALID_PTR;
 constexpr uint32_t kMarked = 0xFFFFFFFEu;
 constexpr uint32_t kOrderBlocks = 64u;  // blocks per run kind of the raster run-order sort
+constexpr uint32_t kDomPage = SR_DOM_PAGE_XY;                  // virtual DOM page edge, texels
+constexpr uint32_t kDomPageTexels = kDomPage * kDomPage;
+constexpr uint32_t kDomBorder = SR_DOM_PREFETCH_PAGE_BORDER;    // neighbouring pages each request adds
 
 // ------------------------------------------------------------- records
 // Per-frustum record built on the host for every frame.
@@ -38,9 +41,11 @@ struct FrustumRec {
   uint32_t fine_tiles_y;
   uint32_t stamp_w;
   uint32_t stamp_h;
-  uint32_t pad;
-  uint64_t dom_depth_off;    // floats
-  uint64_t dom_opacity_off;  // floats
+  uint32_t dom_pages_x;      // virtual DOMs: pages per row of this light's DOM
+  uint32_t dom_pages_y;
+  uint32_t dom_vpage_base;   // first entry of this light's pages in the request and page tables
+  uint64_t dom_depth_off;    // floats (dense DOMs)
+  uint64_t dom_opacity_off;  // floats (dense DOMs)
   uint64_t image_off;        // pixels
   uint64_t stamp_off;        // floats
 };
@@ -88,9 +93,9 @@ enum Counter : uint32_t {
   C_CAMERA_HEAD = 11, // camera raster run queue head
   C_SHADOW_HEAD = 12, // shadow raster run queue head
   C_CANDS = 13,       // fine-cell hits of the page CSR backend
-  C_UNUSED_14 = 14,
-  C_UNUSED_15 = 15,
-  C_UNUSED_16 = 16,
+  C_DOM_PAGES = 14,   // resident virtual DOM pages
+  C_DOM_WANT = 15,    // requested pages within every light's budget (resident with an unbounded pool)
+  C_DOM_DROPS = 16,   // of those, pages past the pool capacity (not resident)
   C_COUNT = 20
 };
 
@@ -178,9 +183,17 @@ struct FrameParams {
   // shadow raster: span columns of each workgroup's first batches, kept
   // from its first pass for the second (kShadowColBytes per frame)
   unsigned long long* shadow_cols;
-  // shadow / shading / raster
+  // shadow / shading / raster.  Dense DOMs: per light frustum W * H depths
+  // and layer-major L * W * H opacities at its offsets.  Virtual DOMs
+  // (dom_table set): a pool of pages, depth texel t of physical page e at
+  // dom_depth[e * kDomPageTexels + t], opacity layer i at
+  // dom_opacity[(e * L + i) * kDomPageTexels + t].
   float* dom_depth;
   float* dom_opacity;
+  uint32_t* dom_requests;  // [dom_vpages] 1 = requested this frame, or null (dense DOMs)
+  uint32_t* dom_table;     // [dom_vpages] physical page or kInvalid, or null (dense DOMs)
+  uint32_t dom_vpages;     // pages over every light frustum
+  uint32_t dom_pool_cap;   // physical pages the pool holds
   float* stamp;
   float4* atlas;      // [cams * NI * atlas_texels]
   float2* history;    // [cams * NI * atlas_texels] (visibility, current visibility)
@@ -301,6 +314,29 @@ SR_HD Clipped clip_segment_dir(V3 p0, V3 p1, const SegDir& s, V3 bmin, V3 bmax) 
 
 SR_HD Clipped clip_segment_to_box(V3 p0, V3 p1, V3 bmin, V3 bmax) {
   return clip_segment_dir(p0, p1, seg_dir(p0, p1), bmin, bmax);
+}
+
+// Where a DOM texel of light frustum `fr` lives: its depth at
+// dom_depth[depth], opacity layer i at dom_opacity[opacity + i * stride].
+// False when the DOMs are virtual and the texel's page is not resident.
+struct DomTexel { size_t depth, opacity, stride; };
+
+__device__ __forceinline__ bool dom_texel(const FrameParams& fp, const FrustumRec& fr, uint32_t x, uint32_t y,
+                                          DomTexel* t) {
+  if (!fp.dom_table) {
+    const size_t pix = (size_t)y * fr.d.screen_width + x;
+    t->depth = fr.dom_depth_off + pix;
+    t->opacity = fr.dom_opacity_off + pix;
+    t->stride = (size_t)fr.d.screen_width * fr.d.screen_height;
+    return true;
+  }
+  const uint32_t e = fp.dom_table[fr.dom_vpage_base + (y / kDomPage) * fr.dom_pages_x + x / kDomPage];
+  if (e == kInvalid) return false;
+  const size_t local = (size_t)(y % kDomPage) * kDomPage + x % kDomPage;
+  t->depth = (size_t)e * kDomPageTexels + local;
+  t->opacity = (size_t)e * fp.dom_layers * kDomPageTexels + local;
+  t->stride = kDomPageTexels;
+  return true;
 }
 
 SR_HD uint32_t pack_binning_field(uint32_t is_shadow, uint32_t frustum) {

@@ -62,6 +62,9 @@ constexpr uint64_t kShadowColBytes = 64ull << 20;
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer);
 void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_stampback(const FrameParams& fp, uint32_t frustum, uint32_t texels, hipStream_t s);
+// Virtual DOMs: the page requests of this frame's shadow tile runs, then the
+// page table (nothing with dense DOMs).
+void launch_dom_pages(const FrameParams& fp, hipStream_t s);
 void launch_shading(const FrameParams& fp, hipStream_t s);
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px);
 void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
@@ -69,7 +72,7 @@ void launch_composite(const FrameParams& fp, uint64_t pixels, hipStream_t s);
 // composite8; nothing when they are null).
 void launch_pack_rgba8(const FrameParams& fp, uint64_t pixels, hipStream_t s);
 // Up to kFillSpans buffers filled with one 32-bit value by one launch.
-constexpr uint32_t kFillSpans = 10;
+constexpr uint32_t kFillSpans = 12;
 struct FillSpans {
   uint32_t* dst[kFillSpans] = {};
   uint64_t n[kFillSpans] = {};          // words

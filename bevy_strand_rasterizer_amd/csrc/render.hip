@@ -17,10 +17,14 @@
 // near to far; a workgroup vote ends the walk once every pixel is opaque
 // (camera) or has its front depth (shadow pass 1).  Deep-shadow partial opacities live in
 // LDS (pixel x layer), so no dynamic register indexing spills.  Shadow maps
-// are dense per light (no virtual paging), outputs stay FP32.
+// are dense per light or virtual (pages of 256^2 texels made resident on
+// request, through a page table), outputs stay FP32.
 #include "common.hpp"
 #include "coverage.hpp"
 #include "kernels.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
 
 namespace srk {
 
@@ -352,7 +356,10 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
     const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
     const uint32_t pidx = threadIdx.x;
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
-    const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
+    // a pixel whose virtual DOM page is not resident is neither rasterised nor stored
+    DomTexel dt{0, 0, 0};
+    const bool active =
+        pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height && dom_texel(fp, fr, pxu, pyu, &dt);
     const float pfx = (float)pxu + 0.5f, pfy = (float)pyu + 0.5f;
     // pass 1
     float z0 = -1.0f;
@@ -439,15 +446,11 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
       }
     }
     if (active) {
-      const size_t plane = (size_t)fr.d.screen_width * fr.d.screen_height;
-      const size_t pix = (size_t)pyu * fr.d.screen_width + pxu;
-      float* dd = fp.dom_depth + fr.dom_depth_off;
-      float* op = fp.dom_opacity + fr.dom_opacity_off;
-      dd[pix] = found ? z0 : 0.0f;
+      fp.dom_depth[dt.depth] = found ? z0 : 0.0f;
       float acc = 0.0f;
       for (uint32_t i = 0; i < L; ++i) {
         acc = acc + (1.0f - acc) * part_px[i * TPX];
-        op[i * plane + pix] = found ? acc : 0.0f;
+        fp.dom_opacity[dt.opacity + i * dt.stride] = found ? acc : 0.0f;
       }
     }
   }
@@ -458,15 +461,77 @@ __global__ void k_stampback(FrameParams fp, uint32_t fi) {
   const FrustumRec& fr = fp.fr[fi];
   const uint32_t tw = fr.stamp_w, th = fr.stamp_h;
   const uint32_t W = fr.d.screen_width, H = fr.d.screen_height;
-  const float* dd = fp.dom_depth + fr.dom_depth_off;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < tw * th; i += gridDim.x * blockDim.x) {
     uint32_t x = i % tw, y = i / tw;
     uint32_t x0 = min((x * W) / tw, W - 1u), y0 = min((y * H) / th, H - 1u);
     uint32_t x1 = min(((x + 1u) * W + tw - 1u) / tw, W), y1 = min(((y + 1u) * H + th - 1u) / th, H);
     float z = 0.0f;
     for (uint32_t yy = y0; yy < y1; ++yy)
-      for (uint32_t xx = x0; xx < x1; ++xx) z = fmaxf(z, dd[(size_t)yy * W + xx]);
+      for (uint32_t xx = x0; xx < x1; ++xx) {
+        DomTexel dt;  // a texel of a page that is not resident reads 0 (sample_dom_depth, :58-72)
+        if (dom_texel(fp, fr, xx, yy, &dt)) z = fmaxf(z, fp.dom_depth[dt.depth]);
+      }
     fp.stamp[fr.stamp_off + i] = z;
+  }
+}
+
+// --------------------------------------------- virtual DOM pages (VSMS)
+// request_shadow_dom_pages (strand_prepass.wgsl:2537-2579): each shadow tile
+// run requests the pages its pixels fall in and their neighbours within
+// kDomBorder pages of the light's page grid.  Every writer stores 1, so
+// plain stores serve (no atomics).
+__global__ __launch_bounds__(256) void k_dom_requests(FrameParams fp) {
+  const uint32_t n = min(fp.counters[C_SHADOW_RUNS], fp.coarse_total * kExt * kExt);
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const sr_raster_tile_run run = fp.shadow_runs[r];
+    if (run.frustum_id >= fp.F || run.work_count == 0u) continue;
+    const FrustumRec& fr = fp.fr[run.frustum_id];
+    if (fr.d.kind != 1u || fr.light_layer == kInvalid) continue;
+    const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
+    const uint32_t x0 = tx * fr.d.froxel_size_x, y0 = ty * fr.d.froxel_size_y;
+    if (x0 >= fr.d.screen_width || y0 >= fr.d.screen_height) continue;
+    const uint32_t x1 = min(x0 + fr.d.froxel_size_x - 1u, fr.d.screen_width - 1u);
+    const uint32_t y1 = min(y0 + fr.d.froxel_size_y - 1u, fr.d.screen_height - 1u);
+    const uint32_t px0 = x0 / kDomPage, py0 = y0 / kDomPage, px1 = x1 / kDomPage, py1 = y1 / kDomPage;
+    const uint32_t bx0 = px0 - min(px0, kDomBorder), by0 = py0 - min(py0, kDomBorder);
+    const uint32_t bx1 = min(px1 + kDomBorder, fr.dom_pages_x - 1u), by1 = min(py1 + kDomBorder, fr.dom_pages_y - 1u);
+    for (uint32_t py = by0; py <= by1; ++py)
+      for (uint32_t px = bx0; px <= bx1; ++px) fp.dom_requests[fr.dom_vpage_base + py * fr.dom_pages_x + px] = 1u;
+  }
+}
+
+// Residency, the demand-driven mode of the reference's VSMS allocator
+// (src/plugin.rs:1776-1854): light by light in frustum order, each light's
+// requested pages in page order, at most st.dom_resident_pages of them (0:
+// all), take the pool's next physical pages; the rest, and pages past the
+// pool capacity (counted, so the frame reports its capacity exceeded), are
+// not resident.  One workgroup (a few thousand pages at most).
+__global__ __launch_bounds__(256) void k_dom_residency(FrameParams fp) {
+  __shared__ uint32_t lds4[4];
+  const uint32_t budget = fp.st.dom_resident_pages;
+  uint32_t want = 0;  // pages within their light's budget so far (block-uniform)
+  for (uint32_t fi = 0; fi < fp.F; ++fi) {
+    const FrustumRec& fr = fp.fr[fi];
+    if (fr.d.kind != 1u) continue;
+    const uint32_t n = fr.dom_pages_x * fr.dom_pages_y;
+    uint32_t rank = 0;  // this light's requested pages so far
+    for (uint32_t b = 0; b < n; b += 256u) {
+      const uint32_t i = b + threadIdx.x, v = fr.dom_vpage_base + i;
+      const uint32_t req = i < n && fp.dom_requests[v] != 0u ? 1u : 0u;
+      uint32_t total;
+      const uint32_t ex = block_exclusive<uint32_t>(req, &total, lds4);
+      if (i < n) {
+        const bool kept = req && (budget == 0u || rank + ex < budget);
+        fp.dom_table[v] = kept && want + ex < fp.dom_pool_cap ? want + ex : kInvalid;
+      }
+      want += budget == 0u ? total : min(total, budget - min(rank, budget));
+      rank += total;
+    }
+  }
+  if (threadIdx.x == 0) {
+    fp.counters[C_DOM_WANT] = want;
+    fp.counters[C_DOM_PAGES] = min(want, fp.dom_pool_cap);
+    fp.counters[C_DOM_DROPS] = want > fp.dom_pool_cap ? want - fp.dom_pool_cap : 0u;
   }
 }
 
@@ -678,15 +743,16 @@ __device__ float dom_visibility(const FrameParams& fp, V3 pw, uint32_t layer) {
   V4 raw = world_to_screen_raw(V4{pw.x, pw.y, pw.z, 1.0f}, fr.clip, vw, vh);
   if (raw.x < 0.0f || raw.y < 0.0f || raw.x >= vw || raw.y >= vh || raw.z < 0.0f || raw.z > 1.0f) return 1.0f;
   uint32_t px = f2u(raw.x), py = f2u(raw.y);
-  size_t pix = (size_t)py * fr.d.screen_width + px;
-  float z0 = fp.dom_depth[fr.dom_depth_off + pix];
+  DomTexel dt;  // a page that is not resident lets all light through (strand_shading.wgsl:238-257)
+  if (!dom_texel(fp, fr, px, py, &dt)) return 1.0f;
+  float z0 = fp.dom_depth[dt.depth];
   if (z0 <= 0.0f) return 1.0f;
   float z = normalize_depth01(raw.z);
   if (z > z0 - 1e-3f) return 1.0f;
   uint32_t L = fp.dom_layers;
   float u = wclamp(fmaxf(0.0f, z0 - z) / fmaxf(z0, 1e-6f), 0.0f, 1.0f);
   uint32_t s = min(f2u(floorf(u * (float)L)), L - 1u);
-  float op = fp.dom_opacity[fr.dom_opacity_off + (size_t)s * fr.d.screen_width * fr.d.screen_height + pix];
+  float op = fp.dom_opacity[dt.opacity + (size_t)s * dt.stride];
   return wclamp(1.0f - op, 0.08f, 1.0f);
 }
 
@@ -809,7 +875,10 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
     const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
     const uint32_t pidx = threadIdx.x;
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
-    const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
+    // a pixel whose virtual DOM page is not resident is neither rasterised nor stored
+    DomTexel dt{0, 0, 0};
+    const bool active =
+        pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height && dom_texel(fp, fr, pxu, pyu, &dt);
     const float cx = (float)pxu + 0.5f, cy = (float)pyu + 0.5f;
     float fc0 = 0.0f, fc1 = 0.0f, fc2 = 0.0f, fc3 = 0.0f, gdepth = 0.0f;
     bool done = !active;
@@ -933,6 +1002,14 @@ void launch_shadow(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
     hipLaunchKernelGGL(k_shadow_raster<64>, dim3(raster_grid(fp, 1u)), dim3(64), 64u * 4u * fp.dom_layers, s, fp);
   else
     hipLaunchKernelGGL(k_shadow_raster<256>, dim3(raster_grid(fp, 4u)), dim3(256), 256u * 4u * fp.dom_layers, s, fp);
+}
+
+void launch_dom_pages(const FrameParams& fp, hipStream_t s) {
+  if (!fp.dom_table) return;
+  const uint32_t runs = fp.coarse_total * kExt * kExt;  // the shadow run queue's capacity
+  const uint32_t g = std::min<uint32_t>(1024u, std::max<uint32_t>(1u, (runs + 255u) / 256u));
+  hipLaunchKernelGGL(k_dom_requests, dim3(g), dim3(256), 0, s, fp);
+  hipLaunchKernelGGL(k_dom_residency, dim3(1), dim3(256), 0, s, fp);
 }
 
 void launch_shading(const FrameParams& fp, hipStream_t s) {

@@ -173,7 +173,7 @@ enum BufId {
   B_DOM_DEPTH, B_DOM_OPACITY, B_STAMP, B_ATLAS, B_HIST0, B_HIST1, B_IMAGE, B_DEPTH, B_COMPOSITE, B_SHADOW_COLS, B_SLOT_PROJ, B_SLOT_SCREEN,
   B_PARTIALS, B_PARTIALS4, B_XY_MASK, B_SLOT_HIT_TOTAL, B_HIT_BASE, B_SLOT_MATERIAL, B_HKEYS0,
   B_HKEYS1, B_HVALS0, B_HVALS1, B_RADIX_HIST, B_RADIX_BASE, B_SLOT_HITS, B_SLOT_HIT_COUNT, B_TELEMETRY, B_OPAQUE_DEPTH, B_OPAQUE_TILES, B_TABLES,
-  B_IMAGE8, B_COMPOSITE8,
+  B_IMAGE8, B_COMPOSITE8, B_DOM_REQ, B_DOM_TABLE,
   B_COUNT
 };
 
@@ -185,6 +185,11 @@ struct FrameShape {
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0, opaque_tiles = 0;
   int hist_cur = 0;
   bool rgba8 = false;  // the frame stored rgba8unorm outputs
+  // virtual DOMs: page grid of every light frustum (width, height, pages x,
+  // pages y, first page), pages over all of them, pool capacity
+  bool dom_virtual = false;
+  uint32_t dom_vpages = 0, dom_pool_cap = 0;
+  std::vector<uint32_t> dom_lights;
 };
 
 // Everything one in-flight frame owns: its stream, grow-only buffers, the
@@ -206,7 +211,7 @@ struct FrameSlot {
   StageTimer timer;
   hipEvent_t done = nullptr;
   bool submitted = false;
-  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0;
+  uint32_t ref_cap_hint = 0, page_cap_hint = 0, work_cap_hint = 0, dom_pool_hint = 0;
   int hist_cur = 0;
   // captured launch sequences, keyed by everything the launches bake in
   struct Graph {
@@ -538,10 +543,10 @@ __host__ __device__ inline sr_frame_counters counters_from_words(const uint32_t*
   fc.page_drops = c[C_PAGE_DROPS];
   fc.page_candidates = k.hit_cap ? c[C_CANDS] : 0u;
   fc.capacity_exceeded = (c[C_REFS] > k.ref_cap || c[C_PAGE_DROPS] > 0u || c[C_WORK] > k.work_cap ||
-                          c[C_OVERFLOW] > 0u || (k.hit_cap && c[C_CANDS] > k.hit_cap))
+                          c[C_OVERFLOW] > 0u || (k.hit_cap && c[C_CANDS] > k.hit_cap) || c[C_DOM_DROPS] > 0u)
                              ? 1u
                              : 0u;
-  fc.pad = 0u;
+  fc.dom_pages = c[C_DOM_PAGES];
   return fc;
 }
 
@@ -574,10 +579,18 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
-  if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS))
+  if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS | SR_FRAME_VIRTUAL_DOMS))
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
   const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
   const bool rgba8 = (st.flags & SR_FRAME_RGBA8_OUTPUTS) != 0u;
+  const bool vdom = (st.flags & SR_FRAME_VIRTUAL_DOMS) != 0u;
+  // The virtual-DOM path is switched off: the first GPU run of its tests
+  // stopped making progress before the first frame returned, and no cause
+  // has been found in the code; the kernels stay for that investigation.
+  if (vdom)
+    return efail(err, SR_ERR_UNSUPPORTED,
+                 "sr_frame_submit: SR_FRAME_VIRTUAL_DOMS is disabled in this build (its GPU path is under "
+                 "investigation); the CPU restatement implements it");
   if (reuse && na)
     return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: the scratch array keeps no shadow maps between calls");
   (void)hipSetDevice(ctx->device);
@@ -589,6 +602,11 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   uint32_t lights_seen = 0, cams = 0, coarse_total = 0, max_cam_tile_px = 1, max_light_tile_px = 1;
   uint64_t fine_total = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0;
+  // virtual DOMs: every light frustum's page grid, and the pages the pool
+  // would need for every light's budget
+  uint32_t dom_vpages = 0;
+  uint64_t dom_cap_all = 0;
+  std::vector<uint32_t> dom_lights;
   for (uint32_t i = 0; i < F; ++i) {
     const sr_frustum_desc& d = fd->frusta[i];
     if (d.screen_width == 0 || d.screen_height == 0 || d.froxel_size_x == 0 || d.froxel_size_y == 0)
@@ -612,9 +630,19 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     if (d.kind == 1u) {
       if (lights_seen < NL) r.light_layer = lights_seen;
       ++lights_seen;
-      r.dom_depth_off = dom_px;
-      r.dom_opacity_off = dom_px * st.dom_layers;
-      dom_px += (uint64_t)d.screen_width * d.screen_height;
+      if (vdom) {
+        r.dom_pages_x = (d.screen_width + kDomPage - 1u) / kDomPage;
+        r.dom_pages_y = (d.screen_height + kDomPage - 1u) / kDomPage;
+        r.dom_vpage_base = dom_vpages;
+        const uint32_t n = r.dom_pages_x * r.dom_pages_y;
+        dom_vpages += n;
+        dom_cap_all += st.dom_resident_pages ? std::min(n, st.dom_resident_pages) : n;
+        dom_lights.insert(dom_lights.end(), {d.screen_width, d.screen_height, r.dom_pages_x, r.dom_pages_y, r.dom_vpage_base});
+      } else {
+        r.dom_depth_off = dom_px;
+        r.dom_opacity_off = dom_px * st.dom_layers;
+        dom_px += (uint64_t)d.screen_width * d.screen_height;
+      }
       r.stamp_w = st.stamp_target_width ? st.stamp_target_width : d.screen_width;
       r.stamp_h = st.stamp_target_height ? st.stamp_target_height : d.screen_height;
       r.stamp_off = stamp_px;
@@ -690,10 +718,22 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     put_key(r.clip, sizeof(r.clip));
     const uint64_t o[3] = {r.dom_depth_off, r.dom_opacity_off, r.stamp_off};
     put_key(o, sizeof(o));
+    const uint32_t vp[3] = {r.dom_pages_x, r.dom_pages_y, r.dom_vpage_base};
+    put_key(vp, sizeof(vp));
   }
   if (NL) put_key(fd->lights, sizeof(sr_directional_light) * NL);
   if (NI) put_key(ir.data(), sizeof(InstRec) * NI);
-  const uint32_t ks[5] = {st.stochastic_cull_enabled, st.dom_layers, NL, NI, lights_seen};
+  // virtual DOM pool: the pages every light's budget allows, or half of them
+  // to start with (a frame that wants more reports its capacity exceeded and
+  // the next one gets what it wanted)
+  uint32_t dom_pool_cap = 0;
+  if (vdom) {
+    const uint32_t all = (uint32_t)std::min<uint64_t>(dom_cap_all, 1ull << 24);
+    const uint32_t start = (ctx->flags & SR_CONFIG_TIGHT_CAPACITY) ? 1u : std::max(1u, all / 2u);
+    dom_pool_cap = std::min(all, std::max(start, sl.dom_pool_hint));
+  }
+  const uint32_t ks[8] = {st.stochastic_cull_enabled, st.dom_layers, NL, NI, lights_seen, vdom ? 1u : 0u,
+                          vdom ? st.dom_resident_pages : 0u, dom_pool_cap};
   put_key(ks, sizeof(ks));
   const float kf[3] = {st.target_strands_per_pixel, st.min_keep_probability, st.shadow_keep_probability};
   put_key(kf, sizeof(kf));
@@ -761,6 +801,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const int hist_next = hist_valid ? (sl.hist_cur ^ 1) : 0;
 
   struct Need { BufId id; uint64_t bytes; };
+  // DOM depth texels: every light's W * H (dense) or the pool's pages
+  const uint64_t dom_tex = vdom ? (uint64_t)dom_pool_cap * kDomPageTexels : dom_px;
   // the per-frame tables share one buffer (one upload): frusta, instances,
   // lights, each section 256-byte aligned
   const uint64_t tab_inst = align_up(sizeof(FrustumRec) * F, 256);
@@ -777,8 +819,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_REFS, 12ull * ref_cap}, {B_ACTIVE_FLAGS, 4ull * tiles16}, {B_ACTIVE_Q, 4ull * tiles16},
       {B_TILE_WORK, 16ull * tiles16}, {B_TILE_BASES, 16ull * tiles16}, {B_WORK, 32ull * work_cap},
       {B_CAM_RUNS, 32ull * tiles16}, {B_SHADOW_RUNS, 32ull * tiles16}, {B_CAM_ORDER, 4ull * tiles16}, {B_SHADOW_ORDER, 4ull * tiles16}, {B_RUN_HIST, 4ull * 2 * kOrderBlocks * 32}, {B_PAGE_BOUNDS, csr ? 4ull * (page_cap + 1) : 0ull},
-      {B_PAGE_ORDER, csr ? 4ull * page_cap : 0ull}, {B_PAGE_HIST, csr ? 4ull * kOrderBlocks * 32 : 0ull}, {B_DOM_DEPTH, 4ull * dom_px},
-      {B_DOM_OPACITY, 4ull * dom_px * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
+      {B_PAGE_ORDER, csr ? 4ull * page_cap : 0ull}, {B_PAGE_HIST, csr ? 4ull * kOrderBlocks * 32 : 0ull}, {B_DOM_DEPTH, 4ull * dom_tex},
+      {B_DOM_OPACITY, 4ull * dom_tex * st.dom_layers}, {B_STAMP, 4ull * stamp_px}, {B_ATLAS, 16ull * atlas_n},
       {B_HIST0, 8ull * atlas_n}, {B_HIST1, 8ull * atlas_n}, {B_IMAGE, 16ull * img_px}, {B_DEPTH, 4ull * img_px},
       {B_COMPOSITE, 16ull * img_px}, {B_SHADOW_COLS, kShadowColBytes}, {B_SLOT_PROJ, 12ull * task_cap}, {B_SLOT_SCREEN, 32ull * task_cap}, {B_PARTIALS, 4ull * scan_tiles},
       {B_PARTIALS4, 16ull * scan_tiles}, {B_XY_MASK, 4ull * page_cap},
@@ -789,7 +831,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_SLOT_HIT_COUNT, hit_k ? (uint64_t)task_cap : 0ull},
       {B_TELEMETRY, st.telemetry_enabled ? 4ull * (kTelPageOff + page_cap + kTelFrustumStride * F) : 0ull},
       {B_OPAQUE_DEPTH, 4ull * opaque_floats}, {B_OPAQUE_TILES, opaque_floats ? 8ull * fine_total : 0ull}, {B_TABLES, tab_bytes},
-      {B_IMAGE8, rgba8 ? 4ull * img_px : 0ull}, {B_COMPOSITE8, rgba8 ? 4ull * img_px : 0ull}};
+      {B_IMAGE8, rgba8 ? 4ull * img_px : 0ull}, {B_COMPOSITE8, rgba8 ? 4ull * img_px : 0ull},
+      {B_DOM_REQ, 4ull * dom_vpages}, {B_DOM_TABLE, 4ull * dom_vpages}};
   if (need_bytes) {  // what the slot holds once every buffer fits (DBuf::ensure growth rule)
     uint64_t held = sl.held_bytes();
     for (const Need& n : needs) {
@@ -927,6 +970,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.page_hist = sl.buf[B_PAGE_HIST].as<uint32_t>();
   fp.dom_depth = sl.buf[B_DOM_DEPTH].as<float>();
   fp.dom_opacity = sl.buf[B_DOM_OPACITY].as<float>();
+  fp.dom_requests = vdom ? sl.buf[B_DOM_REQ].as<uint32_t>() : nullptr;
+  fp.dom_table = vdom ? sl.buf[B_DOM_TABLE].as<uint32_t>() : nullptr;
+  fp.dom_vpages = dom_vpages;
+  fp.dom_pool_cap = dom_pool_cap;
   fp.stamp = sl.buf[B_STAMP].as<float>();
   fp.atlas = sl.buf[B_ATLAS].as<float4>();
   fp.history = sl.buf[hist_next ? B_HIST1 : B_HIST0].as<float2>();
@@ -980,9 +1027,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     clear.add(fp.counters, C_COUNT);
     if (fp.tel) clear.add(fp.tel, tel_words);
     if (!reuse) {  // reused shadow maps stay as their frame left them
-      clear.add(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_px);
-      clear.add(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_px * st.dom_layers);
+      clear.add(reinterpret_cast<uint32_t*>(fp.dom_depth), dom_tex);
+      clear.add(reinterpret_cast<uint32_t*>(fp.dom_opacity), dom_tex * st.dom_layers);
       clear.add(reinterpret_cast<uint32_t*>(fp.stamp), stamp_px);
+      clear.add(fp.dom_requests, dom_vpages);  // the page table is written whole
     }
     clear.add(reinterpret_cast<uint32_t*>(fp.image), img_px * 4);
     clear.add(reinterpret_cast<uint32_t*>(fp.depth), img_px);
@@ -992,6 +1040,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     clear.add(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2);
     launch_fill_spans(clear, s);
     launch_prepass(fp, ps, s, &sl.timer);
+    if (vdom && !reuse) {
+      sl.timer.mark(s, "dom_pages");
+      launch_dom_pages(fp, s);
+    }
     sl.timer.mark(s, "shadow");
     if (!reuse) launch_shadow(fp, s, max_light_tile_px);
     sl.timer.mark(s, "stampback");
@@ -1019,7 +1071,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     uint8_t* k = key.data();
     std::memcpy(k, &fp, sizeof(fp));
     k += sizeof(fp);
-    const uint64_t ext[8] = {dom_px, stamp_px, img_px, atlas_n, max_light_tile_px, max_cam_tile_px, F, 0};
+    const uint64_t ext[8] = {dom_tex, stamp_px, img_px, atlas_n, max_light_tile_px, max_cam_tile_px, F, 0};
     std::memcpy(k, ext, sizeof(ext));
     k += sizeof(ext);
     for (uint32_t i = 0; i < F; ++i) {
@@ -1079,6 +1131,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   sh.atlas_texels = atlas_texels; sh.dom_layers = st.dom_layers;
   sh.dom_px = dom_px; sh.img_px = img_px; sh.stamp_px = stamp_px;
   sh.rgba8 = rgba8;
+  sh.dom_virtual = vdom;
+  sh.dom_vpages = dom_vpages;
+  sh.dom_pool_cap = dom_pool_cap;
+  sh.dom_lights = std::move(dom_lights);
   sh.hist_cur = hist_next;
   sl.hist_cur = hist_next;
   sl.submitted = true;
@@ -1104,6 +1160,7 @@ int frame_wait_slot(FrameSlot& sl, sr_frame_counters* out, std::string& err) {
   if (c[C_PAGE_DROPS] > 0) sl.page_cap_hint = next_pow2(c[C_MARKED]);
   if (c[C_WORK] > sh.work_cap) sl.work_cap_hint = next_pow2(c[C_WORK]);
   if (sh.hit_cap && c[C_CANDS] > sh.hit_cap) sl.ref_cap_hint = std::max(sl.ref_cap_hint, next_pow2(c[C_CANDS]));
+  if (c[C_DOM_DROPS] > 0) sl.dom_pool_hint = std::max(sl.dom_pool_hint, c[C_DOM_WANT]);
   if (sl.dom_pending) {
     sl.dom_ready = fc.capacity_exceeded == 0u;
     sl.dom_pending = false;
@@ -1180,8 +1237,49 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
     case SR_DUMP_WORK_ITEMS: src = buf[B_WORK].p; bytes = 32ull * std::min(fc.work_items, sh.work_cap); break;
     case SR_DUMP_CAMERA_RUNS: src = buf[B_CAM_RUNS].p; bytes = 32ull * fc.camera_runs; break;
     case SR_DUMP_SHADOW_RUNS: src = buf[B_SHADOW_RUNS].p; bytes = 32ull * fc.shadow_runs; break;
-    case SR_DUMP_DOM_DEPTH: src = buf[B_DOM_DEPTH].p; bytes = 4ull * sh.dom_px; break;
-    case SR_DUMP_DOM_OPACITY: src = buf[B_DOM_OPACITY].p; bytes = 4ull * sh.dom_px * sh.dom_layers; break;
+    case SR_DUMP_DOM_DEPTH:
+    case SR_DUMP_DOM_OPACITY: {
+      const bool depth = kind == SR_DUMP_DOM_DEPTH;
+      if (!sh.dom_virtual) {
+        src = buf[depth ? B_DOM_DEPTH : B_DOM_OPACITY].p;
+        bytes = 4ull * sh.dom_px * (depth ? 1u : sh.dom_layers);
+        break;
+      }
+      // virtual DOMs in the dense layout: each light's texels from the page
+      // table and the pool, 0 where the page is not resident
+      const uint32_t L = depth ? 1u : sh.dom_layers;
+      uint64_t texels = 0;
+      for (size_t i = 0; i < sh.dom_lights.size(); i += 5) texels += (uint64_t)sh.dom_lights[i] * sh.dom_lights[i + 1];
+      *out_bytes = 4ull * texels * L;
+      if (!dst) return SR_OK;
+      if (capacity < *out_bytes) return fail(ctx, SR_ERR_CAPACITY, "dump buffer too small");
+      std::vector<uint32_t> table(sh.dom_vpages);
+      std::vector<float> pool((size_t)sh.dom_pool_cap * kDomPageTexels * L);
+      if (!table.empty())
+        HIP_OK(hipMemcpy(table.data(), buf[B_DOM_TABLE].p, 4ull * table.size(), hipMemcpyDeviceToHost), "sr_frame_dump: page table");
+      if (!pool.empty())
+        HIP_OK(hipMemcpy(pool.data(), buf[depth ? B_DOM_DEPTH : B_DOM_OPACITY].p, 4ull * pool.size(), hipMemcpyDeviceToHost),
+               "sr_frame_dump: DOM pool");
+      float* out = static_cast<float*>(dst);
+      std::memset(out, 0, *out_bytes);
+      uint64_t base = 0;
+      for (size_t li = 0; li < sh.dom_lights.size(); li += 5) {
+        const uint32_t W = sh.dom_lights[li], H = sh.dom_lights[li + 1], pxn = sh.dom_lights[li + 2];
+        const uint32_t vb = sh.dom_lights[li + 4];
+        for (uint32_t i = 0; i < L; ++i)
+          for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+              const uint32_t e = table[vb + (y / kDomPage) * pxn + x / kDomPage];
+              if (e == kInvalid) continue;
+              out[base + ((uint64_t)i * H + y) * W + x] =
+                  pool[((size_t)e * L + i) * kDomPageTexels + (size_t)(y % kDomPage) * kDomPage + x % kDomPage];
+            }
+        base += (uint64_t)W * H * L;
+      }
+      return SR_OK;
+    }
+    case SR_DUMP_DOM_REQUESTS: src = buf[B_DOM_REQ].p; bytes = sh.dom_virtual ? 4ull * sh.dom_vpages : 0ull; break;
+    case SR_DUMP_DOM_PAGE_TABLE: src = buf[B_DOM_TABLE].p; bytes = sh.dom_virtual ? 4ull * sh.dom_vpages : 0ull; break;
     case SR_DUMP_ATLAS: src = buf[B_ATLAS].p; bytes = 16ull * sh.cams * sh.NI * sh.atlas_texels; break;
     case SR_DUMP_HISTORY:
       src = buf[sh.hist_cur ? B_HIST1 : B_HIST0].p;

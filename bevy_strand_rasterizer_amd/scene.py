@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import struct
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -88,6 +89,7 @@ class Workload:
         st.stamp_target_width = int(d["stamp_target_width"])
         st.stamp_target_height = int(d["stamp_target_height"])
         st.flags = int(d["frame_flags"])
+        st.dom_resident_pages = int(d["dom_resident_pages"])
         return st
 
     def frame_desc(self, keep: list | None = None) -> abi.FrameDesc:
@@ -131,6 +133,7 @@ def default_settings() -> dict:
         "stamp_target_width": 0,
         "stamp_target_height": 0,
         "frame_flags": 0,  # SR_FRAME_* bits
+        "dom_resident_pages": 0,  # SR_FRAME_VIRTUAL_DOMS: resident pages per light, 0 = all requested
     }
 
 
@@ -478,6 +481,33 @@ def config_small(froxel: int = 16, seed: int = 0x5EEDCAFE) -> Workload:
     asset = scalp_asset(2000, 16, seed, 0.0008, radius_px_for(0.0008, 512, dist))
     eye = (HEAD_CENTER[0], HEAD_CENTER[1] + 0.05, HEAD_CENTER[2] + dist)
     return build_workload(f"scalp-2k-x16-512sq-f{froxel}", asset, [(eye, 512, 512)], 1, 512, 4, froxel, froxel)
+
+
+def with_virtual_doms(w: Workload, budget: int = 0) -> Workload:
+    """The workload drawn with virtual deep shadow maps (SR_FRAME_VIRTUAL_DOMS)
+    holding at most `budget` resident pages per light (0: every requested page)."""
+    st = dict(w.settings)
+    st["frame_flags"] = int(st.get("frame_flags", 0)) | abi.FRAME_VIRTUAL_DOMS
+    st["dom_resident_pages"] = int(budget)
+    return dataclasses.replace(w, settings=st)
+
+
+def config_virtual_doms(light_res: int = 2048, spread: float = 2.5, budget: int = 0, froxel: int = 8,
+                        seed: int = 0x5EEDCAFE) -> Workload:
+    """config_small's 2,000 x 16 scalp and 512x512 camera with one light whose
+    light_res^2 deep shadow map (pages of 256^2 texels) frames a sphere `spread`
+    times the asset's bounding sphere, so the hair's shadow tiles request only
+    the pages around the middle; drawn with virtual DOMs of at most `budget`
+    resident pages (0: every requested page)."""
+    dist = 0.6
+    asset = scalp_asset(2000, 16, seed, 0.0008, radius_px_for(0.0008, 512, dist))
+    eye = (HEAD_CENTER[0], HEAD_CENTER[1] + 0.05, HEAD_CENTER[2] + dist)
+    w = build_workload(f"scalp-2k-x16-512sq-light{light_res}-x{spread:g}-vdom{budget}", asset, [(eye, 512, 512)],
+                       1, light_res, 4, froxel, froxel)
+    center, radius = bounding_sphere(asset)
+    views = w.views.copy()
+    views[1] = light_view(LIGHT_DIRS[0], center, radius * spread)[0]
+    return with_virtual_doms(dataclasses.replace(w, views=np.ascontiguousarray(views)), budget)
 
 
 def config_named(seed: int = 0x5EEDCAFE, strands: int = 100_000, points: int = 32) -> Workload:

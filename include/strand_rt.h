@@ -337,8 +337,10 @@ typedef struct sr_frame_settings {
   uint32_t stamp_target_width;   /* light depth-map size for the stampback */
   uint32_t stamp_target_height;
   uint32_t flags;                /* SR_FRAME_* bits */
+  uint32_t dom_resident_pages;   /* SR_FRAME_VIRTUAL_DOMS: resident DOM pages per light, 0 = every requested page */
+  uint32_t pad;
 } sr_frame_settings;
-SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 80, "sr_frame_settings is 80 bytes");
+SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 88, "sr_frame_settings is 88 bytes");
 
 /* sr_frame_settings.flags: draw the frame's cameras with the deep shadow maps
  * that the context's last light-building frame left (no counterpart in the
@@ -358,6 +360,29 @@ SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 80, "sr_frame_settings is 80 bytes
  * 255 steps, ties to even (WGSL pack4x8unorm), red in the lowest byte
  * (SR_DUMP_IMAGE_RGBA8, SR_DUMP_COMPOSITE_RGBA8). */
 #define SR_FRAME_RGBA8_OUTPUTS 2u
+/* sr_frame_settings.flags: virtual deep shadow maps (the reference's VSMS
+ * surfaces, src/plugin.rs:1776-1854).  Each light's DOM is cut into pages of
+ * SR_DOM_PAGE_XY^2 texels (depth plus every opacity layer).  Every shadow
+ * tile run requests the pages its pixels fall in and their neighbours within
+ * SR_DOM_PREFETCH_PAGE_BORDER pages (request_shadow_dom_pages,
+ * src/shaders/strand_prepass.wgsl:2520-2579); requested pages become
+ * resident in page order, at most settings.dom_resident_pages per light (0:
+ * all of them), in the same frame.  The shadow raster writes, and the
+ * stampback and shading read, the DOM through the page table: a texel of a
+ * page that is not resident is not written, reads depth 0 and lets all light
+ * through (strand_rasterizer.wgsl:484-517, strand_shading.wgsl:198-276,
+ * shadow_dom_stampback.wgsl:58-72).  DOM memory then follows the pages in
+ * use instead of the light's full resolution; with every requested page
+ * resident the frame equals the same frame with dense DOMs.
+ * SR_DUMP_DOM_DEPTH / SR_DUMP_DOM_OPACITY still return the dense layout
+ * (texels of pages that are not resident read 0); SR_DUMP_DOM_REQUESTS and
+ * SR_DUMP_DOM_PAGE_TABLE return the requests and the page table.
+ * This build rejects the flag with SR_ERR_UNSUPPORTED: its GPU path is
+ * switched off while a stalled GPU run is investigated (the CPU
+ * restatement of the oracle implements it). */
+#define SR_FRAME_VIRTUAL_DOMS 4u
+#define SR_DOM_PAGE_XY 256u              /* DOM_PAGE_XY, src/plugin.rs:68 */
+#define SR_DOM_PREFETCH_PAGE_BORDER 1u   /* DOM_PREFETCH_PAGE_BORDER, src/plugin.rs:69 */
 
 typedef struct sr_frame_desc {
   const sr_strand_instance* instances;
@@ -434,7 +459,7 @@ typedef struct sr_frame_counters {
   uint32_t page_drops;          /* marked pages past the page capacity (dropped, as the reference) */
   uint32_t page_candidates;     /* fine-cell hits sorted by the page CSR backend (references, dropped pages included); 0 for the cell-atomic backend */
   uint32_t capacity_exceeded;   /* 1: a capacity overflowed; the context grew it, submit the frame again */
-  uint32_t pad;
+  uint32_t dom_pages;           /* SR_FRAME_VIRTUAL_DOMS: resident DOM pages over every light (0 otherwise) */
 } sr_frame_counters;
 SR_STATIC_ASSERT(sizeof(sr_frame_counters) == 64, "sr_frame_counters is 64 bytes");
 
@@ -474,7 +499,11 @@ typedef enum sr_dump_kind {
   SR_DUMP_OPAQUE_TILES = 26, /* uint32 (covered, quantised farthest depth) per fine tile when opaque depth was given */
   SR_DUMP_IMAGE_RGBA8 = 27,  /* uint8 x 4 per camera frustum W*H when SR_FRAME_RGBA8_OUTPUTS (the image, rgba8unorm) */
   SR_DUMP_COMPOSITE_RGBA8 = 28, /* uint8 x 4 per camera frustum W*H when SR_FRAME_RGBA8_OUTPUTS (the composite) */
-  SR_DUMP_KIND_COUNT = 29     /* one past the largest kind */
+  SR_DUMP_DOM_REQUESTS = 29, /* uint32 per DOM page of every light frustum, row-major ceil(W/256) x ceil(H/256)
+                                pages per light, concatenated: 1 = requested (SR_FRAME_VIRTUAL_DOMS) */
+  SR_DUMP_DOM_PAGE_TABLE = 30, /* uint32 per DOM page as SR_DUMP_DOM_REQUESTS: its physical page, or
+                                  SR_INVALID_PTR when not resident (SR_FRAME_VIRTUAL_DOMS) */
+  SR_DUMP_KIND_COUNT = 31     /* one past the largest kind */
 } sr_dump_kind;
 
 /* ------------------------------------------------------------------------ */

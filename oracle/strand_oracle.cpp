@@ -228,6 +228,9 @@ struct Frame {
   std::vector<sr_raster_tile_run> cam_runs, shadow_runs;
   std::vector<float> dom_depth, dom_opacity, stamp;
   std::vector<uint64_t> dom_depth_base, dom_opacity_base;  // per frustum offsets (lights)
+  // virtual DOMs: per light frustum ceil(W/256) x ceil(H/256) pages,
+  // concatenated: requested (0/1) and physical page or kInvalid
+  std::vector<uint32_t> dom_requests, dom_table;
   std::vector<float> atlas, history;  // atlas float4, history float2 (visibility, current)
   std::vector<uint64_t> image_base;   // per frustum offsets (cameras)
   std::vector<float> image, depth, composite;
@@ -1115,6 +1118,70 @@ void shadow_raster(Frame& F) {
   });
 }
 
+// ------------------------------------------------ virtual DOM paging
+// The reference's VSMS surfaces (src/plugin.rs:1776-1854) with page
+// requests as request_shadow_dom_pages makes them
+// (strand_prepass.wgsl:2537-2579: every shadow tile run requests the pages
+// its pixels fall in and their neighbours within DOM_PREFETCH_PAGE_BORDER
+// pages).  The residency policy lives in the absent bevy_vsms crate; this
+// restatement makes requested pages resident light by light, in page order,
+// up to dom_resident_pages per light (0 = all).  A texel of a page that is
+// not resident is never stored (strand_rasterizer.wgsl:484-517) and reads as
+// depth 0 / opacity 0 (shadow_dom_stampback.wgsl:58-72), where shading lets
+// all light through (strand_shading.wgsl:238-257) as it does for depth 0; so
+// the dense DOM with those texels cleared is the same frame.
+void dom_pages(Frame& F) {
+  const uint32_t P = SR_DOM_PAGE_XY, B = SR_DOM_PREFETCH_PAGE_BORDER, L = F.st.dom_layers;
+  std::vector<uint32_t> base(F.F, 0), pages_x(F.F, 0), pages_y(F.F, 0);
+  uint32_t total = 0;
+  for (uint32_t fi = 0; fi < F.F; ++fi) {
+    if (F.fr[fi].kind != 1u) continue;
+    base[fi] = total;
+    pages_x[fi] = ceil_div(F.fr[fi].screen_width, P);
+    pages_y[fi] = ceil_div(F.fr[fi].screen_height, P);
+    total += pages_x[fi] * pages_y[fi];
+  }
+  F.dom_requests.assign(total, 0u);
+  F.dom_table.assign(total, kInvalid);
+  for (const sr_raster_tile_run& run : F.shadow_runs) {
+    const uint32_t fi = run.frustum_id;
+    if (run.work_count == 0u || !light_layer_ok(F, fi, nullptr)) continue;
+    const sr_frustum_desc& fd = F.fr[fi];
+    const uint32_t ftx = ceil_div(fd.screen_width, fd.froxel_size_x);
+    const uint32_t x0 = (run.screen_tile_id % ftx) * fd.froxel_size_x, y0 = (run.screen_tile_id / ftx) * fd.froxel_size_y;
+    const uint32_t x1 = std::min(x0 + fd.froxel_size_x - 1u, fd.screen_width - 1u);
+    const uint32_t y1 = std::min(y0 + fd.froxel_size_y - 1u, fd.screen_height - 1u);
+    for (uint32_t py = y0 / P; py <= y1 / P; ++py)      // request_shadow_dom_pages
+      for (uint32_t px = x0 / P; px <= x1 / P; ++px) {  // mark_shadow_dom_page_request_with_border
+        const uint32_t mx = px - std::min(px, B), my = py - std::min(py, B);
+        const uint32_t Mx = std::min(px + B, pages_x[fi] - 1u), My = std::min(py + B, pages_y[fi] - 1u);
+        for (uint32_t y = my; y <= My; ++y)
+          for (uint32_t x = mx; x <= Mx; ++x) F.dom_requests[base[fi] + y * pages_x[fi] + x] = 1u;
+      }
+  }
+  uint32_t phys = 0;
+  for (uint32_t fi = 0; fi < F.F; ++fi) {
+    if (F.fr[fi].kind != 1u) continue;
+    uint32_t resident = 0;
+    for (uint32_t v = 0; v < pages_x[fi] * pages_y[fi]; ++v) {
+      if (!F.dom_requests[base[fi] + v]) continue;
+      if (F.st.dom_resident_pages != 0u && resident >= F.st.dom_resident_pages) continue;
+      F.dom_table[base[fi] + v] = phys++;
+      ++resident;
+    }
+    const sr_frustum_desc& fd = F.fr[fi];
+    const size_t plane = (size_t)fd.screen_width * fd.screen_height;
+    for (uint32_t y = 0; y < fd.screen_height; ++y)
+      for (uint32_t x = 0; x < fd.screen_width; ++x) {
+        if (F.dom_table[base[fi] + (y / P) * pages_x[fi] + x / P] != kInvalid) continue;
+        const size_t pix = (size_t)y * fd.screen_width + x;
+        F.dom_depth[F.dom_depth_base[fi] + pix] = 0.0f;
+        for (uint32_t i = 0; i < L; ++i) F.dom_opacity[F.dom_opacity_base[fi] + i * plane + pix] = 0.0f;
+      }
+  }
+  F.ctr.dom_pages = phys;
+}
+
 // ------------------------------------------------- K26 DOM stampback
 // shadow_dom_stampback.wgsl:74-123: max DOM depth over the target texel's
 // footprint, 0 (discard) when empty.
@@ -1606,6 +1673,7 @@ int oracle_render_history(const sr_asset_slabs* assets, uint32_t asset_count, co
   F.ctr.ref_capacity = ref_capacity;
   if (!stop_after_binning) {
     timed(F, "shadow", [&] { shadow_raster(F); });
+    if (F.st.flags & SR_FRAME_VIRTUAL_DOMS) timed(F, "dom_pages", [&] { dom_pages(F); });
     timed(F, "stampback", [&] { stampback(F); });
     timed(F, "shading", [&] { shading(F); });
     timed(F, "raster", [&] { camera_raster(F); });
@@ -1646,6 +1714,8 @@ int oracle_get(or_frame* of, int kind, const void** data, uint64_t* bytes) {
     case SR_DUMP_DEPTH: RET(F.depth);
     case SR_DUMP_COMPOSITE: RET(F.composite);
     case SR_DUMP_STAMP: RET(F.stamp);
+    case SR_DUMP_DOM_REQUESTS: RET(F.dom_requests);
+    case SR_DUMP_DOM_PAGE_TABLE: RET(F.dom_table);
     default: return -1;
   }
 #undef RET

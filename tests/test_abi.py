@@ -38,7 +38,7 @@ STRUCTS = [
     ("sr_directional_light", abi.light_dtype, 32, "directional light (new)"),
     ("sr_frame_counters", abi.frame_counters_dtype, 64, "frame counters (new)"),
     ("sr_strand_cache_info", abi.strand_cache_info_dtype, 72, "strand cache parse result (new)"),
-    ("sr_request", abi.request_dtype, 904, "serving request (new)"),
+    ("sr_request", abi.request_dtype, 912, "serving request (new)"),
     ("sr_request_result", abi.request_result_dtype, 112, "serving result (new)"),
     ("sr_server_config", abi.server_config_dtype, 16, "serving config (new)"),
     ("sr_server_stats", abi.server_stats_dtype, 64, "serving stats (new)"),
@@ -87,7 +87,7 @@ def test_struct_layout(c_layout, cname, dt, size, src):
 def test_ctypes_mirrors_match_header(c_layout):
     import ctypes as C
 
-    assert C.sizeof(abi.FrameSettings) == 80
+    assert C.sizeof(abi.FrameSettings) == 88
     assert C.sizeof(abi.DevicePtr) == 12
     assert C.sizeof(abi.FrameDesc) == c_layout[("sr_frame_desc", "__sizeof__")]
     for name, *_ in abi.FrameDesc._fields_:
