@@ -875,10 +875,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
     const uint32_t tx = run.screen_tile_id % fr.fine_tiles_x, ty = run.screen_tile_id / fr.fine_tiles_x;
     const uint32_t pidx = threadIdx.x;
     const uint32_t pxu = tx * fx + pidx % fx, pyu = ty * fy + pidx / fx;
-    // a pixel whose virtual DOM page is not resident is neither rasterised nor stored
-    DomTexel dt{0, 0, 0};
-    const bool active =
-        pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height && dom_texel(fp, fr, pxu, pyu, &dt);
+    const bool active = pidx < fx * fy && pxu < fr.d.screen_width && pyu < fr.d.screen_height;
     const float cx = (float)pxu + 0.5f, cy = (float)pyu + 0.5f;
     float fc0 = 0.0f, fc1 = 0.0f, fc2 = 0.0f, fc3 = 0.0f, gdepth = 0.0f;
     bool done = !active;

@@ -584,13 +584,6 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
   const bool rgba8 = (st.flags & SR_FRAME_RGBA8_OUTPUTS) != 0u;
   const bool vdom = (st.flags & SR_FRAME_VIRTUAL_DOMS) != 0u;
-  // The virtual-DOM path is switched off: the first GPU run of its tests
-  // stopped making progress before the first frame returned, and no cause
-  // has been found in the code; the kernels stay for that investigation.
-  if (vdom)
-    return efail(err, SR_ERR_UNSUPPORTED,
-                 "sr_frame_submit: SR_FRAME_VIRTUAL_DOMS is disabled in this build (its GPU path is under "
-                 "investigation); the CPU restatement implements it");
   if (reuse && na)
     return efail(err, SR_ERR_UNSUPPORTED, "sr_frame_submit_noalloc: the scratch array keeps no shadow maps between calls");
   (void)hipSetDevice(ctx->device);

@@ -376,10 +376,7 @@ SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 88, "sr_frame_settings is 88 bytes
  * resident the frame equals the same frame with dense DOMs.
  * SR_DUMP_DOM_DEPTH / SR_DUMP_DOM_OPACITY still return the dense layout
  * (texels of pages that are not resident read 0); SR_DUMP_DOM_REQUESTS and
- * SR_DUMP_DOM_PAGE_TABLE return the requests and the page table.
- * This build rejects the flag with SR_ERR_UNSUPPORTED: its GPU path is
- * switched off while a stalled GPU run is investigated (the CPU
- * restatement of the oracle implements it). */
+ * SR_DUMP_DOM_PAGE_TABLE return the requests and the page table. */
 #define SR_FRAME_VIRTUAL_DOMS 4u
 #define SR_DOM_PAGE_XY 256u              /* DOM_PAGE_XY, src/plugin.rs:68 */
 #define SR_DOM_PREFETCH_PAGE_BORDER 1u   /* DOM_PREFETCH_PAGE_BORDER, src/plugin.rs:69 */

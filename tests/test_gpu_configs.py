@@ -20,6 +20,7 @@ from bevy_strand_rasterizer_amd.runtime import StrandRenderer, StrandServer
 from oracle import oracle
 
 from test_gpu_parity import EXACT, FLOAT, compare
+from bufcmp import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -85,8 +86,8 @@ def test_batched_views_equal_alone_and_oracle():
         for k, fi in enumerate(cams):
             one = scene.select_views(w, [fi])
             render_until_fits(r, one)
-            assert r.dump("composite").tobytes() == comp[k].tobytes(), f"view {k}: composite differs from alone"
-            assert r.dump("depth").tobytes() == depth[k].tobytes(), f"view {k}: depth differs from alone"
+            assert_same(r.dump("composite"), comp[k], f"view {k}: composite differs from alone")
+            assert_same(r.dump("depth"), depth[k], f"view {k}: depth differs from alone")
             if k in picks:
                 print(f"[configs] oracle view {k} ({heights[k]}p)", flush=True)
                 ctr = r.render(one)
@@ -124,5 +125,5 @@ def test_serving_mix_on_serving_assets():
     for rec in res:
         k = rec["tag"]
         assert rec["status"] == 0
-        assert rec["rgba"].tobytes() == want[k][0], f"request {k}: composite differs from alone"
-        assert rec["depth"].tobytes() == want[k][1], f"request {k}: depth differs from alone"
+        assert_same(rec["rgba"], want[k][0], f"request {k}: composite differs from alone")
+        assert_same(rec["depth"], want[k][1], f"request {k}: depth differs from alone")

@@ -13,6 +13,7 @@ import pytest
 from bevy_strand_rasterizer_amd import abi, scene
 from bevy_strand_rasterizer_amd.runtime import StrandRenderer
 from oracle import oracle
+from bufcmp import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -216,8 +217,9 @@ def test_rgba8_outputs(backend):
         w.settings["frame_flags"] = 0
         r.render(w)
         assert r.dump("image_rgba8").size == 0, "rgba8 outputs only when asked for"
-        assert r.dump("image").tobytes() == img.tobytes()
-    assert img8.tobytes() == unorm8(img).tobytes() and comp8.tobytes() == unorm8(comp).tobytes()
+        assert_same(r.dump("image"), img, "float image with rgba8 outputs on")
+    assert_same(img8, unorm8(img), "rgba8 image")
+    assert_same(comp8, unorm8(comp), "rgba8 composite")
     for got, want in ((img8, ref.get("image")), (comp8, ref.get("composite"))):
         d = np.abs(got.astype(np.float64) / 255.0 - np.clip(want, 0.0, 1.0)).max()
         assert d <= 1.0 / 255.0 + 1e-6, d
@@ -317,7 +319,7 @@ def test_backends_agree_multiframe():
     ctr, b = gpu_frame(w1, frames=2)
     assert ctr["overflow_pages"] == 0
     for k in EXACT + FLOAT:
-        assert a[k].tobytes() == b[k].tobytes(), k
+        assert_same(a[k], b[k], k)
 
 
 def test_deterministic_repeat():
@@ -330,7 +332,7 @@ def test_deterministic_repeat():
         r.render(w)
         b = {k: r.dump(k) for k in EXACT + FLOAT}
     for k in EXACT + FLOAT:
-        assert a[k].tobytes() == b[k].tobytes(), k
+        assert_same(a[k], b[k], k)
 
 
 def test_graph_replays_match_eager():
@@ -352,7 +354,7 @@ def test_graph_replays_match_eager():
                 got = {k: graph.dump(k) for k in EXACT + FLOAT}
                 eager.render(w2 if name == "w1" else w1)  # eager work between replays
                 for k in EXACT + FLOAT:
-                    assert got[k].tobytes() == want[name][k].tobytes(), f"replay {rep} {name}: {k} differs"
+                    assert_same(got[k], want[name][k], f"replay {rep} {name}: {k} differs")
 
 
 def test_autotune_picks_an_accepted_configuration():

@@ -7,6 +7,7 @@ import pytest
 
 from bevy_strand_rasterizer_amd import abi, scene
 from bevy_strand_rasterizer_amd.runtime import StrandRenderer, StrandServer
+from bufcmp import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -58,8 +59,8 @@ def test_alone_vs_mix():
         k = rec["tag"]
         assert rec["status"] == 0 and tickets[rec["ticket"]] == k
         comp, depth = ref[k]
-        assert rec["rgba"].reshape(-1, 4).tobytes() == comp.tobytes(), f"request {k}: composite differs from alone"
-        assert rec["depth"].reshape(-1).tobytes() == depth.tobytes(), f"request {k}: depth differs from alone"
+        assert_same(rec["rgba"].reshape(-1, 4), comp, f"request {k}: composite differs from alone")
+        assert_same(rec["depth"].reshape(-1), depth, f"request {k}: depth differs from alone")
         assert rec["latency_ms"] >= 0.0 and rec["done_ms"] >= rec["launch_ms"] >= rec["submit_ms"]
 
 
@@ -73,8 +74,8 @@ def test_one_slot_one_view_equals_frame_path():
             srv.submit(scene.request_from_workload(wl))
             (rec,) = srv.drain(1)
     assert rec["frame_views"] == 1
-    assert rec["rgba"].reshape(-1, 4).tobytes() == comp.tobytes()
-    assert rec["depth"].reshape(-1).tobytes() == depth.tobytes()
+    assert_same(rec["rgba"].reshape(-1, 4), comp, "composite")
+    assert_same(rec["depth"].reshape(-1), depth, "depth")
 
 
 def test_budget_admission():
@@ -99,7 +100,7 @@ def test_budget_admission():
         assert st["peak_held_bytes"] <= int(one * 1.3)
         for rec in res:
             comp, depth = ref[rec["tag"]]
-            assert rec["rgba"].reshape(-1, 4).tobytes() == comp.tobytes()
+            assert_same(rec["rgba"].reshape(-1, 4), comp, "composite")
         # a budget below what the assets alone hold rejects the request
         with StrandServer(r, slots=1, max_views=1, hbm_budget=1 << 20) as srv:
             srv.submit(scene.request_from_workload(wls[0]))
@@ -133,5 +134,5 @@ def test_rgba8_pinned_outputs_equal_alone():
         assert rec["status"] == 0
         comp, depth = ref[k]
         want8 = np.rint(np.clip(comp, 0.0, 1.0) * np.float32(255.0)).astype(np.uint8)
-        assert outs[k][0].reshape(-1, 4).tobytes() == want8.tobytes(), f"request {k}: rgba8 composite"
-        assert outs[k][1].reshape(-1).tobytes() == depth.tobytes(), f"request {k}: depth"
+        assert_same(outs[k][0].reshape(-1, 4), want8, f"request {k}: rgba8 composite")
+        assert_same(outs[k][1].reshape(-1), depth, f"request {k}: depth")

@@ -14,6 +14,7 @@ import pytest
 
 from bevy_strand_rasterizer_amd import abi, scene
 from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+from bufcmp import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -63,9 +64,9 @@ def test_reused_shadow_maps_equal_built_ones(backend):
         atlas = r.dump("atlas").reshape(2, -1)
         offs = scene.camera_pixel_offsets(later)
         for k, (off, cw, ch) in enumerate(offs):
-            assert comp[off * 4:(off + cw * ch) * 4].tobytes() == want[k + 1][0], f"camera {k + 1}: composite"
-            assert depth[off:off + cw * ch].tobytes() == want[k + 1][1], f"camera {k + 1}: depth"
-            assert atlas[k].tobytes() == want[k + 1][2], f"camera {k + 1}: shading atlas"
+            assert_same(comp[off * 4:(off + cw * ch) * 4], want[k + 1][0], f"camera {k + 1}: composite")
+            assert_same(depth[off:off + cw * ch], want[k + 1][1], f"camera {k + 1}: depth")
+            assert_same(atlas[k], want[k + 1][2], f"camera {k + 1}: shading atlas")
 
 
 def test_reuse_refused_for_other_lights():
@@ -131,6 +132,6 @@ def test_stress_batch_properties():
         render_until_fits(r, alone)
         off, cw, ch = scene.camera_pixel_offsets(batch)[k]
         comp = np.frombuffer(first[0], np.float32)
-        assert r.dump("composite").tobytes() == comp[off * 4:(off + cw * ch) * 4].tobytes(), "view alone vs batch"
+        assert_same(r.dump("composite"), comp[off * 4:(off + cw * ch) * 4], "view alone vs batch")
         img = comp[off * 4:(off + cw * ch) * 4].reshape(-1, 4)
         assert np.isfinite(img).all() and (img[:, 3] > 0).mean() > 0.05, "the view shows hair"

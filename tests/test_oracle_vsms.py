@@ -24,6 +24,7 @@ import pytest
 
 from bevy_strand_rasterizer_amd import abi, scene
 from oracle import oracle
+from bufcmp import assert_same
 
 P = abi.DOM_PAGE_XY
 
@@ -85,7 +86,7 @@ def test_every_requested_page_resident_equals_dense(frames):
     _, v = frames[0]
     _, d = frames["dense"]
     for k in ("dom_depth", "dom_opacity", "stamp", "atlas", "history", "image", "depth", "composite"):
-        assert v.get(k).tobytes() == d.get(k).tobytes(), k
+        assert_same(v.get(k), d.get(k), k)
 
 
 @pytest.mark.parametrize("budget", [7, 1])

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from bevy_strand_rasterizer_amd import scene
+from bufcmp import assert_same
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -83,11 +84,11 @@ def test_json_equals_binary():
     b = scene.parse_strand_cache(kat())
     assert int(a[0]["format"]) == 1
     for x, y in zip(a[1:], b[1:]):
-        assert x.tobytes() == y.tobytes()
+        assert_same(x, y, "JSON and binary cache slabs")
     # scale defaults to 1 (unit_scale, :42-44); source may be null
     doc.pop("scale")
     doc["source"] = None
-    assert scene.parse_strand_cache(json.dumps(doc).encode())[1].tobytes() == b[1].tobytes()
+    assert_same(scene.parse_strand_cache(json.dumps(doc).encode())[1], b[1], "vertices without scale or source")
     with pytest.raises(scene.StrandCacheError, match="missing field `strands`"):
         scene.parse_strand_cache(b'{"version": 1, "vertices": []}')
 

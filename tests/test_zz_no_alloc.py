@@ -10,6 +10,7 @@ import pytest
 
 from bevy_strand_rasterizer_amd import abi, scene
 from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+from bufcmp import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -61,8 +62,8 @@ def test_noalloc_matches_frame_submit(backend):
             want_comp, want_depth, want_ctr = plain(r, w)
             assert ctr == want_ctr
             assert ctr["capacity_exceeded"] == 0 and ctr["fine_refs"] > 0
-            assert comp.tobytes() == want_comp.tobytes(), f"call {k}: composite differs"
-            assert depth.tobytes() == want_depth.tobytes(), f"call {k}: depth differs"
+            assert_same(comp, want_comp, f"call {k}: composite differs")
+            assert_same(depth, want_depth, f"call {k}: depth differs")
 
 
 def test_noalloc_refuses_what_it_cannot_hold():
@@ -83,4 +84,5 @@ def test_noalloc_refuses_what_it_cannot_hold():
         rc, comp, depth, ctr = noalloc(r, small)
         assert rc == 0 and ctr["capacity_exceeded"] == 0
         want_comp, want_depth, _ = plain(r, small)
-        assert comp.tobytes() == want_comp.tobytes() and depth.tobytes() == want_depth.tobytes()
+        assert_same(comp, want_comp, "composite")
+        assert_same(depth, want_depth, "depth")
