@@ -438,6 +438,40 @@ class PinnedPool:
         self.free.setdefault((depth.shape[1], depth.shape[0]), []).append(out)
 
 
+def run_virtual_doms(backend: int, steps: int = 10):
+    """Dense against virtual deep shadow maps (SR_FRAME_VIRTUAL_DOMS, every
+    requested page resident) on the named frame and on the 2,000-strand scalp
+    under a 2048^2 light framed 2.5x wider than the asset: frames/s of both,
+    the pages made resident and the DOM bytes each holds."""
+    import dataclasses
+
+    from bevy_strand_rasterizer_amd import abi, scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    out = {}
+    for name, wv in (("named", scene.with_virtual_doms(scene.config_named())),
+                     ("scalp_2k_wide_light", scene.config_virtual_doms())):
+        wv.settings["fine_binning_backend"] = backend
+        wd = dataclasses.replace(wv, settings={k: v for k, v in wv.settings.items()
+                                               if k not in ("frame_flags", "dom_resident_pages")})
+        L = int(wv.settings["dom_layers"])
+        lights = [f for f in wv.frusta if int(f["kind"]) == 1]
+        dense_px = sum(int(f["screen_width"]) * int(f["screen_height"]) for f in lights)
+        rec = {"lights": len(lights), "light_pages": sum(-(-int(f["screen_width"]) // abi.DOM_PAGE_XY) *
+                                                         -(-int(f["screen_height"]) // abi.DOM_PAGE_XY) for f in lights)}
+        with StrandRenderer(device=0) as r:
+            r.upload_workload(wv)
+            for label, w in (("dense", wd), ("virtual", wv)):
+                ctr, lat, dt = time_frames(r, w, steps, 3)
+                rec[f"{label}_frames_per_s"] = round(w.camera_count * steps / dt, 3)
+                if label == "virtual":
+                    rec["resident_pages"] = int(ctr["dom_pages"])
+        rec["dom_bytes_dense"] = 4 * dense_px * (1 + L)
+        rec["dom_bytes_virtual"] = 4 * rec["resident_pages"] * abi.DOM_PAGE_XY ** 2 * (1 + L)
+        out[name] = rec
+    return out
+
+
 def run_serving(seconds: float, rate: float, slots: int, max_views: int, budget_gib: float, seed: int = 7,
                 backend: int = 1):
     """Poisson arrivals at `rate` req/s for `seconds` over the 8 resident
@@ -516,6 +550,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-virtual-doms", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-peak-rate", type=float, default=400.0,
@@ -605,6 +640,11 @@ def main():
             batched[name]["plain"] = plain_what
             print(f"bench: batched {name} {batched[name]['frames_per_s']} frames/s", file=sys.stderr, flush=True)
 
+    vdoms = None
+    if not args.no_virtual_doms:
+        vdoms = run_virtual_doms(1 if args.backend == "csr" else 0)
+        print(f"bench: virtual DOMs {vdoms}", file=sys.stderr, flush=True)
+
     serving = None
     if not args.no_serving:
         bk = 1 if args.backend == "csr" else 0
@@ -658,6 +698,7 @@ def main():
         "binning_kernel_traffic": binning_kernels,
         "cpu_baseline": cpu,
         "batched": batched,
+        "virtual_doms": vdoms,
         "serving": serving,
         "stress": stress,
     }
