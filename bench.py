@@ -472,6 +472,29 @@ def run_virtual_doms(backend: int, steps: int = 10):
     return out
 
 
+def run_sorted_blend(backend: int, steps: int = 10):
+    """The named frame with depth-sorted blending inside each work item
+    (SR_FRAME_DEPTH_SORTED_BLEND) against the default (inst, seg) order:
+    frames/s and the camera raster's stage time of each."""
+    from bevy_strand_rasterizer_amd import abi, scene
+    from bevy_strand_rasterizer_amd.runtime import StrandRenderer
+
+    w = scene.config_named()
+    w.settings["fine_binning_backend"] = backend
+    out = {"workload": w.name, "fragments_held_per_pixel": abi.BLEND_SORT_FRAGMENTS}
+    with StrandRenderer(device=0) as r:
+        r.upload_workload(w)
+        for label, flags in (("default", 0), ("sorted", abi.FRAME_DEPTH_SORTED_BLEND)):
+            w.settings["frame_flags"] = flags
+            ctr, lat, dt = time_frames(r, w, steps, 3)
+            out[f"{label}_frames_per_s"] = round(steps / dt, 3)
+            r.set_profiling(True)
+            r.render(w)
+            out[f"{label}_raster_ms"] = round(r.stage_times().get("raster", 0.0), 4)
+            r.set_profiling(False)
+    return out
+
+
 def run_serving(seconds: float, rate: float, slots: int, max_views: int, budget_gib: float, seed: int = 7,
                 backend: int = 1):
     """Poisson arrivals at `rate` req/s for `seconds` over the 8 resident
@@ -551,6 +574,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-virtual-doms", action="store_true")
+    ap.add_argument("--no-sorted-blend", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-peak-rate", type=float, default=400.0,
@@ -645,6 +669,11 @@ def main():
         vdoms = run_virtual_doms(1 if args.backend == "csr" else 0)
         print(f"bench: virtual DOMs {vdoms}", file=sys.stderr, flush=True)
 
+    sorted_blend = None
+    if not args.no_sorted_blend:
+        sorted_blend = run_sorted_blend(1 if args.backend == "csr" else 0)
+        print(f"bench: depth-sorted blending {sorted_blend}", file=sys.stderr, flush=True)
+
     serving = None
     if not args.no_serving:
         bk = 1 if args.backend == "csr" else 0
@@ -699,6 +728,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "virtual_doms": vdoms,
+        "depth_sorted_blend": sorted_blend,
         "serving": serving,
         "stress": stress,
     }

@@ -29,6 +29,8 @@ NOALLOC_SCRATCH_BYTES = 256 << 20  # SR_NOALLOC_SCRATCH_BYTES
 FRAME_REUSE_LIGHT_DOMS = 1  # sr_frame_settings.flags
 FRAME_RGBA8_OUTPUTS = 2
 FRAME_VIRTUAL_DOMS = 4
+FRAME_DEPTH_SORTED_BLEND = 8
+BLEND_SORT_FRAGMENTS = 4  # SR_BLEND_SORT_FRAGMENTS
 DOM_PAGE_XY = 256  # SR_DOM_PAGE_XY
 DOM_PREFETCH_PAGE_BORDER = 1
 REQUEST_RGBA8 = 1  # sr_request.flags

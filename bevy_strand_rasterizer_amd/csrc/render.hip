@@ -857,7 +857,62 @@ __device__ void load_color_ref(const FrameParams& fp, const FrustumRec& fr, uint
   b.s1[k] = fp.atlas[lt + i1];
 }
 
-template <int TPX>
+// Depth-sorted blending (SR_FRAME_DEPTH_SORTED_BLEND): a pixel's fragments
+// of the current work item, nearest first (larger depth), premultiplied
+// colour and alpha, in registers (each lane owns its pixel; indices are
+// compile-time, so nothing spills).  One slot more than the list holds
+// takes the arriving fragment before the nearest is blended.
+constexpr int kSortK = (int)SR_BLEND_SORT_FRAGMENTS;
+
+struct FragList {
+  float z[kSortK + 1], r[kSortK + 1], g[kSortK + 1], b[kSortK + 1], a[kSortK + 1];
+  int n;
+};
+
+__device__ __forceinline__ void blend_under(float& f0, float& f1, float& f2, float& f3, float r, float g, float b,
+                                            float a) {
+  const float om = 1.0f - f3;
+  f0 = f0 + r * om;
+  f1 = f1 + g * om;
+  f2 = f2 + b * om;
+  f3 = f3 + a * om;
+}
+
+__device__ __forceinline__ void frag_push(FragList& L, float z, float r, float g, float b, float a, float& f0,
+                                          float& f1, float& f2, float& f3) {
+#pragma unroll
+  for (int i = 0; i <= kSortK; ++i)
+    if (i == L.n) { L.z[i] = z; L.r[i] = r; L.g[i] = g; L.b[i] = b; L.a[i] = a; }
+  // the new fragment moves ahead of every farther one (equal depths keep arrival order)
+#pragma unroll
+  for (int i = kSortK; i >= 1; --i) {
+    if (i <= L.n && L.z[i] > L.z[i - 1]) {
+      float t;
+      t = L.z[i]; L.z[i] = L.z[i - 1]; L.z[i - 1] = t;
+      t = L.r[i]; L.r[i] = L.r[i - 1]; L.r[i - 1] = t;
+      t = L.g[i]; L.g[i] = L.g[i - 1]; L.g[i - 1] = t;
+      t = L.b[i]; L.b[i] = L.b[i - 1]; L.b[i - 1] = t;
+      t = L.a[i]; L.a[i] = L.a[i - 1]; L.a[i - 1] = t;
+    }
+  }
+  if (++L.n > kSortK) {  // full: the nearest goes first
+    blend_under(f0, f1, f2, f3, L.r[0], L.g[0], L.b[0], L.a[0]);
+#pragma unroll
+    for (int i = 0; i < kSortK; ++i) {
+      L.z[i] = L.z[i + 1]; L.r[i] = L.r[i + 1]; L.g[i] = L.g[i + 1]; L.b[i] = L.b[i + 1]; L.a[i] = L.a[i + 1];
+    }
+    L.n = kSortK;
+  }
+}
+
+__device__ __forceinline__ void frag_flush(FragList& L, float& f0, float& f1, float& f2, float& f3) {
+#pragma unroll
+  for (int i = 0; i < kSortK; ++i)
+    if (i < L.n) blend_under(f0, f1, f2, f3, L.r[i], L.g[i], L.b[i], L.a[i]);
+  L.n = 0;
+}
+
+template <int TPX, bool SORTED>
 __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
   __shared__ RunTable<TPX> t;
   __shared__ ColorBatch<TPX> b;
@@ -880,6 +935,8 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
     float fc0 = 0.0f, fc1 = 0.0f, fc2 = 0.0f, fc3 = 0.0f, gdepth = 0.0f;
     bool done = !active;
     uint32_t cur = 0;
+    FragList fl;
+    fl.n = 0;
     for (uint32_t bs = 0; bs < total; bs += TPX) {
       __syncthreads();
       if (bs + threadIdx.x < total) load_color_ref(fp, fr, fi, t, bs + threadIdx.x, b);
@@ -905,6 +962,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           const uint32_t item = b.item[k];
           const uint32_t w = item & 0x7FFFFFFFu;
           if (w != cur) {  // end of a work item: early out above 0.98 alpha
+            if (SORTED) frag_flush(fl, fc0, fc1, fc2, fc3);
             cur = w;
             if (fc3 > 0.98f) {
               fc0 = fc0 / fc3; fc1 = fc1 / fc3; fc2 = fc2 / fc3; fc3 = 1.0f;
@@ -932,16 +990,22 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           float4 s0 = b.s0[k], s1 = b.s1[k];
           float alpha = wmix(s0.w, s1.w, tw) * cov;
           float vis = wg.w;
-          float om = 1.0f - fc3;
-          fc0 = fc0 + (wmix(s0.x, s1.x, tw) * vis * alpha) * om;
-          fc1 = fc1 + (wmix(s0.y, s1.y, tw) * vis * alpha) * om;
-          fc2 = fc2 + (wmix(s0.z, s1.z, tw) * vis * alpha) * om;
-          fc3 = fc3 + alpha * om;
+          if (SORTED) {
+            frag_push(fl, qz, wmix(s0.x, s1.x, tw) * vis * alpha, wmix(s0.y, s1.y, tw) * vis * alpha,
+                      wmix(s0.z, s1.z, tw) * vis * alpha, alpha, fc0, fc1, fc2, fc3);
+          } else {
+            float om = 1.0f - fc3;
+            fc0 = fc0 + (wmix(s0.x, s1.x, tw) * vis * alpha) * om;
+            fc1 = fc1 + (wmix(s0.y, s1.y, tw) * vis * alpha) * om;
+            fc2 = fc2 + (wmix(s0.z, s1.z, tw) * vis * alpha) * om;
+            fc3 = fc3 + alpha * om;
+          }
           gdepth = fmaxf(gdepth, qz);
         }
       }
       if (__syncthreads_count(!done) == 0) break;
     }
+    if (SORTED && !done) frag_flush(fl, fc0, fc1, fc2, fc3);
     if (!done && fc3 > 0.98f) {  // end of the last work item
       fc0 = fc0 / fc3; fc1 = fc1 / fc3; fc2 = fc2 / fc3; fc3 = 1.0f;
     }
@@ -1016,10 +1080,18 @@ void launch_shading(const FrameParams& fp, hipStream_t s) {
 }
 
 void launch_camera_raster(const FrameParams& fp, hipStream_t s, uint32_t max_tile_px) {
-  if (max_tile_px <= 64)
-    hipLaunchKernelGGL(k_camera_raster<64>, dim3(raster_grid(fp, 1u)), dim3(64), 0, s, fp);
-  else
-    hipLaunchKernelGGL(k_camera_raster<256>, dim3(raster_grid(fp, 4u)), dim3(256), 0, s, fp);
+  const bool sorted = (fp.st.flags & SR_FRAME_DEPTH_SORTED_BLEND) != 0u;
+  if (max_tile_px <= 64) {
+    if (sorted)
+      hipLaunchKernelGGL((k_camera_raster<64, true>), dim3(raster_grid(fp, 1u)), dim3(64), 0, s, fp);
+    else
+      hipLaunchKernelGGL((k_camera_raster<64, false>), dim3(raster_grid(fp, 1u)), dim3(64), 0, s, fp);
+  } else {
+    if (sorted)
+      hipLaunchKernelGGL((k_camera_raster<256, true>), dim3(raster_grid(fp, 4u)), dim3(256), 0, s, fp);
+    else
+      hipLaunchKernelGGL((k_camera_raster<256, false>), dim3(raster_grid(fp, 4u)), dim3(256), 0, s, fp);
+  }
 }
 
 void launch_stampback(const FrameParams& fp, uint32_t fi, uint32_t texels, hipStream_t s) {

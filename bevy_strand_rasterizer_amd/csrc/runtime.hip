@@ -579,7 +579,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
-  if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS | SR_FRAME_VIRTUAL_DOMS))
+  if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS | SR_FRAME_VIRTUAL_DOMS |
+                   SR_FRAME_DEPTH_SORTED_BLEND))
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
   const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
   const bool rgba8 = (st.flags & SR_FRAME_RGBA8_OUTPUTS) != 0u;

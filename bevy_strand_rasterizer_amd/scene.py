@@ -632,6 +632,28 @@ def config_two_strands(light_offset: float = 0.0, dom_layers: int = 4, res: int 
                               "ambient_color": (0.0, 0.0, 0.0, 1.0)})
 
 
+def config_one_cell_pair(swap: bool = False, res: int = 64) -> Workload:
+    """Two straight one-segment strands on pixel row 20 of a res x res view
+    (the identity view of config_two_strands, no shadowing light), one at
+    depth 0.6 (colour (0.9, 0.1, 0.1), alpha 0.6) and one a single float step
+    nearer (colour (0.1, 0.2, 0.9), alpha 0.7): both depths quantise to the
+    same 24-bit value, so every pixel they share holds both fragments in one
+    cell, where the (inst, seg) order and the depth order disagree.  swap
+    lists the nearer strand first."""
+    y = 1.0 - 2.0 * 20.5 / res
+    z0 = np.float32(0.6)
+    z1 = np.nextafter(z0, np.float32(1.0))
+    strands = [[(-0.5, y, float(z0)), (0.5, y, float(z0))], [(-0.5, y, float(z1)), (0.5, y, float(z1))]]
+    colors = [(0.9, 0.1, 0.1, 0.6), (0.1, 0.2, 0.9, 0.7)]
+    if swap:
+        strands, colors = strands[::-1], colors[::-1]
+    w = config_two_strands(light_offset=4.0, res=res)
+    asset = asset_from_strands(strands, material={"ambient_factor": 1.0}, radius_px=1.5)
+    for i, c in enumerate(colors):
+        asset.materials["absorption_color"][i] = c
+    return dataclasses.replace(w, name=f"one-cell-pair-{res}{'-swapped' if swap else ''}", assets=[asset])
+
+
 def config_tiny(strands: int = 24, points: int = 6, res: int = 64, froxel: int = 8, seed: int = 7,
                 light_res: int = 64, dom_layers: int = 4, lights: int = 1) -> Workload:
     """Tiny scalp for exhaustive parity checks."""

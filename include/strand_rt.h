@@ -378,6 +378,17 @@ SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 88, "sr_frame_settings is 88 bytes
  * (texels of pages that are not resident read 0); SR_DUMP_DOM_REQUESTS and
  * SR_DUMP_DOM_PAGE_TABLE return the requests and the page table. */
 #define SR_FRAME_VIRTUAL_DOMS 4u
+/* sr_frame_settings.flags: depth-sorted blending inside a work item.  The
+ * reference blends a pixel's fragments of one cell in reference order
+ * (strand_rasterizer.wgsl:998-1020, an order its atomics leave open; the
+ * default here is (inst, seg)).  With this flag each pixel keeps the
+ * fragments of the current work item in a list of SR_BLEND_SORT_FRAGMENTS,
+ * nearest first (larger reverse-Z depth; equal depths in arrival order):
+ * when one more arrives, the nearest of them is blended, and the rest are
+ * blended nearest first when the work item ends.  A cell with at most that
+ * many fragments on a pixel is blended in exact depth order. */
+#define SR_FRAME_DEPTH_SORTED_BLEND 8u
+#define SR_BLEND_SORT_FRAGMENTS 4u
 #define SR_DOM_PAGE_XY 256u              /* DOM_PAGE_XY, src/plugin.rs:68 */
 #define SR_DOM_PREFETCH_PAGE_BORDER 1u   /* DOM_PREFETCH_PAGE_BORDER, src/plugin.rs:69 */
 

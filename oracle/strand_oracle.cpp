@@ -1483,6 +1483,7 @@ void shading(Frame& F) {
 // cell order, premultiplied front-to-back over, early out at alpha > 0.98
 // after a work item.  Every pixel of a tile is rasterised.
 void camera_raster(Frame& F) {
+  const bool sorted = (F.st.flags & SR_FRAME_DEPTH_SORTED_BLEND) != 0u;
   F.image_base.assign(F.F, 0);
   uint64_t sz = 0;
   for (uint32_t fi = 0; fi < F.F; ++fi)
@@ -1546,6 +1547,17 @@ void camera_raster(Frame& F) {
         float fc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float gdepth = 0.0f;
         bool done = false;
+        // final = blend_over_premult(final, src), strand_rasterizer.wgsl:156-162
+        auto blend = [&fc](const float* src) {
+          float om = 1.0f - fc[3];
+          for (int c = 0; c < 3; ++c) fc[c] = fc[c] + src[c] * om;
+          fc[3] = fc[3] + src[3] * om;
+        };
+        // SR_FRAME_DEPTH_SORTED_BLEND: the work item's fragments nearest
+        // first (larger depth; equal depths in arrival order), at most
+        // SR_BLEND_SORT_FRAGMENTS held; one more blends the nearest held
+        struct Frag { float z, c[4]; };
+        std::vector<Frag> held;
         for (uint32_t w = 0; w < run.work_count && !done; ++w) {
           for (uint32_t k = wstart[w]; k < wstart[w + 1]; ++k) {
             const Rec& rc = recs[k];
@@ -1568,12 +1580,22 @@ void camera_raster(Frame& F) {
             for (int c = 0; c < 4; ++c) sh[c] = wmix(rc.s0[c], rc.s1[c], tw);
             float alpha = sh[3] * cov;
             float src[4] = {sh[0] * rc.vis * alpha, sh[1] * rc.vis * alpha, sh[2] * rc.vis * alpha, alpha};
-            // final = blend_over_premult(final, src), strand_rasterizer.wgsl:156-162
-            float om = 1.0f - fc[3];
-            for (int c = 0; c < 3; ++c) fc[c] = fc[c] + src[c] * om;
-            fc[3] = fc[3] + src[3] * om;
+            if (sorted) {
+              Frag f{fz, {src[0], src[1], src[2], src[3]}};
+              size_t at = held.size();  // ahead of every farther fragment
+              while (at > 0 && fz > held[at - 1].z) --at;
+              held.insert(held.begin() + (long)at, f);
+              if (held.size() > SR_BLEND_SORT_FRAGMENTS) {
+                blend(held.front().c);
+                held.erase(held.begin());
+              }
+            } else {
+              blend(src);
+            }
             gdepth = std::fmax(gdepth, fz);
           }
+          for (const Frag& f : held) blend(f.c);
+          held.clear();
           if (fc[3] > 0.98f) {
             for (int c = 0; c < 3; ++c) fc[c] = fc[c] / fc[3];
             fc[3] = 1.0f;
