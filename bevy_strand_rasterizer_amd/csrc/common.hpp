@@ -242,9 +242,16 @@ SR_HD uint32_t pack2x16unorm(float a, float b) {
   uint32_t hi = (uint32_t)floorf(0.5f + 65535.0f * wclamp(b, 0.0f, 1.0f));
   return lo | (hi << 16);
 }
-SR_HD V2 unpack2x16unorm(uint32_t v) {
-  return V2{(float)(v & 0xFFFFu) / 65535.0f, (float)(v >> 16) / 65535.0f};
+// i / 65535 for a 16-bit i: the product with the rounded reciprocal,
+// corrected once by an FMA, equals the correctly rounded quotient for every
+// i (tests/unorm_check.cpp checks all 65,536) at a quarter of the cost of
+// the division sequence.
+SR_HD float unorm16_to_float(uint32_t i) {
+  const float x = (float)i, r = 1.0f / 65535.0f;
+  const float q = x * r;
+  return fmaf(fmaf(-q, 65535.0f, x), r, q);
 }
+SR_HD V2 unpack2x16unorm(uint32_t v) { return V2{unorm16_to_float(v & 0xFFFFu), unorm16_to_float(v >> 16)}; }
 
 SR_HD uint32_t wang_hash(uint32_t seed) {
   uint32_t x = seed;

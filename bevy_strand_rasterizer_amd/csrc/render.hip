@@ -238,34 +238,28 @@ __device__ __forceinline__ void put_shadow_ref(const ShadowRefData& d, ShadowBat
 template <int TPX>
 __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t k, float px, float py, float* pz,
                                              float* cov) {
+  // t = num / l2 correctly rounded, from the reference's reciprocal: q0 is
+  // within an ulp of the quotient, num - q0 l2 is exact (FMA) and q0 + that
+  // remainder / l2 rounds to the correctly rounded quotient (Markstein).  A
+  // zero-length segment (inv = 0) gets t = 0, as in the reference.  Every
+  // candidate runs the whole test and one predicate decides it: the lanes
+  // of a wave test different references, so early exits only added branches.
   const float4 A = b.a[k], Cc = b.c[k], Rr = b.r[k];
-  float t = 0.0f;
-  if (!(A.w == 0.0f)) {
-    // t = num / l2 correctly rounded, from the reference's reciprocal: q is
-    // within an ulp of the quotient, r = num - q l2 exactly (FMA) and q + r /
-    // l2 rounds to the correctly rounded quotient (Markstein).  A q far
-    // outside [0, 1] decides the test without the correction.
-    const float num = (px - A.x) * Rr.z + (py - A.y) * Rr.w;
-    const float inv = b.inv[k];
-    const float q = num * inv;
-    if (q < -0.5f || q > 1.5f) return false;
-    t = __builtin_fmaf(__builtin_fmaf(-q, A.w, num), inv, q);
-  }
-  if (t < 0.0f || t > 1.0f) return false;
+  const float num = (px - A.x) * Rr.z + (py - A.y) * Rr.w;
+  const float inv = b.inv[k];
+  const float q0 = num * inv;
+  const float t = __builtin_fmaf(__builtin_fmaf(-q0, A.w, num), inv, q0);
+  const bool in = t >= 0.0f && t <= 1.0f;
   float qx = wmix(A.x, Cc.x, t), qy = wmix(A.y, Cc.y, t), qz = wmix(A.z, Cc.z, t);
   float r = wmix(Rr.x, Rr.y, wclamp(qz, 0.0f, 1.0f));
   float dx = px - qx, dy = py - qy;
-  // d / r from the hardware square root and reciprocal (a few ulp): outside
-  // [1 - 1e-5, 1 + 1e-5] it decides coverage as the correctly rounded
-  // quotient would; inside, the quotient is computed exactly.  Coverage is a
-  // DOM value (1e-3 tolerance); t, q and the depth stay exact.
   const float s2 = dx * dx + dy * dy;
   float q = __builtin_amdgcn_sqrtf(s2) * __builtin_amdgcn_rcpf(r);
-  if (!(q < 0.99999f || q > 1.00001f)) q = sqrtf(s2) / r;
+  if (in && !(q < 0.99999f || q > 1.00001f)) q = sqrtf(s2) / r;
   float c = wclamp(1.0f - q, 0.0f, 1.0f);
   *pz = qz;
   *cov = c;
-  return c > 0.0f;
+  return in && c > 0.0f;
 }
 
 // 64 x 64 bit-matrix transpose across the wave: lane l holds row l on entry
@@ -970,12 +964,9 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
               break;
             }
           }
-          if (item & 0x80000000u) continue;
           const float4 bx = b.box[k];
-          if (cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) continue;
           const float4 sg = b.seg[k], ge = b.geo[k];
           float tt = ((cx - sg.x) * sg.z + (cy - sg.y) * sg.w) * ge.x;
-          if (!(tt >= 0.0f && tt <= 1.0f)) continue;
           float qx = sg.x + sg.z * tt, qy = sg.y + sg.w * tt;
           float qz = wmix(ge.y, ge.z, tt);
           const float4 wg = b.wgt[k];
@@ -983,7 +974,11 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           float r = wmix(minr, fmaxf(wg.x, minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
           float dx = cx - qx, dy = cy - qy;
           float d2 = dx * dx + dy * dy;
-          if (!(d2 < r * r)) continue;
+          // one predicate for the box, projection and radius tests (the
+          // records of an invalid reference are stale but never used)
+          const bool hit = !(item & 0x80000000u) && !(cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) &&
+                           (tt >= 0.0f && tt <= 1.0f) && d2 < r * r;
+          if (!hit) continue;
           float cov = wclamp(1.0f - sqrtf(d2) / r, 0.0f, 1.0f);
           float aw0 = wg.y, aw1 = wg.z;
           float tw = (tt * aw0) / ((1.0f - tt) * aw1 + tt * aw0);

@@ -5,7 +5,8 @@ over random and adversarial segments: zero, sub-pixel and very long
 lengths, axis-aligned and nearly axis-aligned directions, endpoints and
 radii on pixel centres, tiles near 16k pixels, 4x4 / 8x8 / 16x8 / 16x16
 tiles.  Every pixel that the shadow or camera exact coverage test accepts
-must be in the span mask (a miss would change rendered results).
+must be in the span mask (a miss would change rendered results).  Also
+checks the kernels' exact 16-bit unorm decode (tests/unorm_check.cpp).
 """
 import shutil
 import subprocess
@@ -37,3 +38,15 @@ def test_spans_keep_every_covered_pixel(span_check):
     assert covered > 3_000_000
     # the spans are tight: at most twice the covered pixels are kept
     assert kept < 2 * covered
+
+
+def test_unorm16_matches_division(tmp_path):
+    """The kernels' i / 65535 (an FMA-corrected product) equals the correctly
+    rounded quotient for all 65,536 values of a 16-bit unorm."""
+    if not Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    exe = tmp_path / "unorm_check"
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", str(HERE / "unorm_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.split() == ["0"], out.stdout
