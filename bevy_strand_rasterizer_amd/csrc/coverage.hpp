@@ -127,17 +127,20 @@ SR_HD SpanRef span_setup(float x0, float y0, float abx, float aby, float rmax, u
 template <int W>
 SR_HD void span_mask(const SpanRef& s, uint32_t fx, uint64_t (&m)[W]) {
   for (int j = 0; j < W; ++j) m[j] = 0ull;
-  for (int32_t y = s.y0; y <= s.y1; ++y) {
+  // every lane walks all rows of the tile (a row outside [y0, y1] or with an
+  // empty interval adds no bits), so a wave's lanes stay converged
+  const int32_t rows = (int32_t)((64u * W) / fx);
+  for (int32_t y = 0; y < rows; ++y) {
     const float fy = (float)y;
     const float lo = fmaxf(s.box_lo, fmaxf(s.al0 + fy * s.al1, s.cl0 + fy * s.cl1));
     const float hi = fminf(s.box_hi, fminf(s.ah0 + fy * s.al1, s.ch0 + fy * s.cl1));
     const float cl = fmaxf(ceilf(lo - s.eps), 0.0f), ch = fminf(floorf(hi + s.eps), (float)(fx - 1u));
-    if (!(cl <= ch)) continue;
-    const uint32_t b0 = (uint32_t)y * fx + (uint32_t)cl, b1 = (uint32_t)y * fx + (uint32_t)ch + 1u;  // bits [b0, b1)
+    const bool row = y >= s.y0 && y <= s.y1 && cl <= ch;
+    const uint32_t b0 = (uint32_t)y * fx + (row ? (uint32_t)cl : 0u);
+    const uint32_t b1 = row ? (uint32_t)y * fx + (uint32_t)ch + 1u : b0;  // bits [b0, b1)
     for (int j = 0; j < W; ++j) {
       const uint32_t lo_b = max(b0, 64u * j), hi_b = min(b1, 64u * j + 64u);
-      if (lo_b >= hi_b) continue;
-      const uint32_t n = hi_b - lo_b, sh = lo_b - 64u * j;
+      const uint32_t n = hi_b > lo_b ? hi_b - lo_b : 0u, sh = (lo_b - 64u * j) & 63u;
       const uint64_t bits = n >= 64u ? ~0ull : ((1ull << n) - 1ull);
       m[j] |= bits << sh;
     }

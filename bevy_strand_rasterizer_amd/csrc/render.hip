@@ -979,9 +979,12 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           const bool hit = !(item & 0x80000000u) && !(cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) &&
                            (tt >= 0.0f && tt <= 1.0f) && d2 < r * r;
           if (!hit) continue;
-          float cov = wclamp(1.0f - sqrtf(d2) / r, 0.0f, 1.0f);
+          // coverage and the perspective weight from the hardware square root
+          // and reciprocal (within 1 ulp): they scale colour and alpha (1e-3
+          // tolerance); whether the fragment covers the pixel is d2 < r^2, exact
+          float cov = wclamp(1.0f - __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rcpf(r), 0.0f, 1.0f);
           float aw0 = wg.y, aw1 = wg.z;
-          float tw = (tt * aw0) / ((1.0f - tt) * aw1 + tt * aw0);
+          float tw = (tt * aw0) * __builtin_amdgcn_rcpf((1.0f - tt) * aw1 + tt * aw0);
           float4 s0 = b.s0[k], s1 = b.s1[k];
           float alpha = wmix(s0.w, s1.w, tw) * cov;
           float vis = wg.w;
