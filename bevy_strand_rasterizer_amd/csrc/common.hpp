@@ -180,6 +180,7 @@ struct FrameParams {
   uint8_t* slot_hit_count;
   uint32_t hit_k;
   uint32_t mark_queue;  // fine walks a wave queues while marking (<= kMarkQ)
+  uint32_t tile_queue;  // coarse tiles a wave queues per marking round (1 .. kTileQ)
   // shadow raster: span columns of each workgroup's first batches, kept
   // from its first pass for the second (kShadowColBytes per frame)
   unsigned long long* shadow_cols;
@@ -269,7 +270,15 @@ SR_HD float normalize_depth01(float z) {
   return wclamp(z * 0.5f + 0.5f, 0.0f, 1.0f);
 }
 SR_HD uint32_t quantize_depth01(float z) { return f2u(rintf(wclamp(z, 0.0f, 1.0f) * (float)kQuant)); }
-SR_HD float dequantize_depth01(uint32_t q) { return (float)q / (float)kQuant; }
+// q / 16777215 as the product with the rounded reciprocal corrected once by
+// an FMA: the correctly rounded quotient for every q < 2^24
+// (tests/unorm_check.cpp checks them all), and so for every float (float)q,
+// whose quotients are those of a 24-bit significand scaled by a power of two.
+SR_HD float dequantize_depth01(uint32_t q) {
+  const float x = (float)q, r = 1.0f / (float)kQuant;
+  const float p = x * r;
+  return fmaf(fmaf(-p, (float)kQuant, x), r, p);
+}
 
 SR_HD V4 world_to_screen_raw(V4 pos, const float* clip_from_world, float vw, float vh) {
   V4 clip = mat_vec(clip_from_world, pos);

@@ -52,9 +52,12 @@ struct PrepassScratch {
   uint32_t radix_blocks, radix_passes;
 };
 
-// Fine walks each wave of the marking pass can queue (40 B each; four waves
-// per workgroup: 30 KB of LDS); FrameParams::mark_queue may lower it.
-constexpr uint32_t kMarkQ = 192;
+// Fine walks (40 B each) and coarse-tile visits (4 B each) each wave of the
+// marking pass can queue, beside 36 B of segment state per lane: four waves
+// per workgroup hold 38 KB of LDS, so four workgroups fit a CU.
+// FrameParams::mark_queue and tile_queue may lower them.
+constexpr uint32_t kMarkQ = 160;
+constexpr uint32_t kTileQ = 128;
 
 // Bytes of the shadow raster's column cache (FrameParams::shadow_cols).
 constexpr uint64_t kShadowColBytes = 64ull << 20;

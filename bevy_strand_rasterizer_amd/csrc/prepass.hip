@@ -298,22 +298,25 @@ __global__ void k_lut(FrameParams fp) {
 // ------------------------------------------------ binning traversal
 // coarse_depth_slice_for_depth: the first used slice whose range holds the
 // quantised depth, else the first nearest one.  Both endpoint depths are
-// looked up in one pass over the tile's 16 (z_min_q, z_max_q) pairs, loaded
-// together (a used slice has z_min_q <= z_max_q; unused ones are (~0, 0)).
-// The LUT slice of a quantised depth: the first used slice whose range holds
-// it, else the used slice nearest to it (first on ties), else invalid.  The
-// holding slices come from one mask per depth; the nearest-slice scan runs
-// only for a depth no slice holds.
-__device__ __forceinline__ uint32_t nearest_slice(const uint4 (&q)[kSlices / 2], uint32_t zq) {
+// looked up in one pass over the tile's 16 (z_min_q, z_max_q) pairs, two
+// pairs per load (a used slice has z_min_q <= z_max_q; unused ones are
+// (~0, 0)).  The LUT slice of a quantised depth: the first used slice whose
+// range holds it, else the used slice nearest to it (first on ties), else
+// invalid.  The holding slices come from one mask per depth; the
+// nearest-slice scan runs only for a depth no slice holds.
+__device__ uint32_t nearest_slice(const uint4* l4, uint32_t zq) {
   uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
+  for (uint32_t i = 0; i < kSlices; i += 2) {
+    const uint4 q = l4[i / 2];
 #pragma unroll
-  for (uint32_t i = 0; i < kSlices; ++i) {
-    const uint32_t ex = (i & 1u) ? q[i / 2].z : q[i / 2].x, ey = (i & 1u) ? q[i / 2].w : q[i / 2].y;
-    if (ex > ey) continue;
-    const uint32_t d = zq > ey ? zq - ey : ex - zq;
-    if (d < best_d) {
-      best_d = d;
-      best = i;
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint32_t ex = h ? q.z : q.x, ey = h ? q.w : q.y;
+      if (ex > ey) continue;
+      const uint32_t d = zq > ey ? zq - ey : ex - zq;
+      if (d < best_d) {
+        best_d = d;
+        best = i + h;
+      }
     }
   }
   return best;
@@ -323,18 +326,15 @@ __device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, float za,
                                                       uint32_t* sb) {
   const uint32_t qa = quantize_depth01(za), qb = quantize_depth01(zb);
   const uint4* l4 = reinterpret_cast<const uint4*>(lz);
-  uint4 q[kSlices / 2];
-#pragma unroll
-  for (uint32_t i = 0; i < kSlices / 2; ++i) q[i] = l4[i];
   uint32_t ma = 0u, mb = 0u;  // slices holding qa / qb (a used slice has x <= y)
-#pragma unroll
-  for (uint32_t i = 0; i < kSlices; ++i) {
-    const uint32_t ex = (i & 1u) ? q[i / 2].z : q[i / 2].x, ey = (i & 1u) ? q[i / 2].w : q[i / 2].y;
-    ma |= (qa >= ex && qa <= ey) ? (1u << i) : 0u;
-    mb |= (qb >= ex && qb <= ey) ? (1u << i) : 0u;
+#pragma unroll 2
+  for (uint32_t i = 0; i < kSlices; i += 2) {
+    const uint4 q = l4[i / 2];
+    ma |= ((qa >= q.x && qa <= q.y) ? (1u << i) : 0u) | ((qa >= q.z && qa <= q.w) ? (2u << i) : 0u);
+    mb |= ((qb >= q.x && qb <= q.y) ? (1u << i) : 0u) | ((qb >= q.z && qb <= q.w) ? (2u << i) : 0u);
   }
-  *sa = ma ? (uint32_t)(__ffs(ma) - 1) : nearest_slice(q, qa);
-  *sb = mb ? (uint32_t)(__ffs(mb) - 1) : nearest_slice(q, qb);
+  *sa = ma ? (uint32_t)(__ffs(ma) - 1) : nearest_slice(l4, qa);
+  *sb = mb ? (uint32_t)(__ffs(mb) - 1) : nearest_slice(l4, qb);
 }
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
@@ -551,55 +551,96 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
   }
 }
 
-// Coarse 2D DDA over the asset range's coarse tiles, then the per-tile
-// trace (visit_segment_asset_coarse_tiles, strand_prepass.wgsl:1591-1696).
-template <int MODE>
-__device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V3 p0, V3 p1,
-                              const sr_fine_seg_ref& ref, uint32_t task, HitRec& rec) {
+// The coarse 2D DDA of visit_segment_asset_coarse_tiles
+// (strand_prepass.wgsl:1591-1696) over the asset range's coarse tiles,
+// resumable: init clips the segment to the range's tile rectangle and finds
+// the first tile, next() the one after it.  `pending` is the frustum-local
+// tile to visit; tiles outside the frustum's coarse-depth tiles are skipped
+// as the reference skips them.
+struct CoarseWalk {
+  int32_t f[2], f1[2], step[2], lo[2], hi[2];
+  float delta[2], tmax[2];
+  uint32_t left;     // DDA iterations left (the reference's safety bound)
+  uint32_t pending;  // frustum-local coarse tile to visit next
+  bool live;         // pending is valid
+};
+
+__device__ __forceinline__ bool coarse_walk_next(const sr_frustum_desc& fd, CoarseWalk& cw) {
+  while (cw.left != 0u) {
+    --cw.left;
+    if (cw.f[0] < cw.lo[0] || cw.f[0] > cw.hi[0] || cw.f[1] < cw.lo[1] || cw.f[1] > cw.hi[1]) break;
+    const uint32_t local = (uint32_t)cw.f[1] * fd.coarse_tiles_x + (uint32_t)cw.f[0];
+    if (cw.f[0] == cw.f1[0] && cw.f[1] == cw.f1[1]) {
+      cw.left = 0u;  // the last tile
+    } else if (cw.tmax[0] <= cw.tmax[1]) {
+      cw.f[0] += cw.step[0];
+      cw.tmax[0] += cw.delta[0];
+    } else {
+      cw.f[1] += cw.step[1];
+      cw.tmax[1] += cw.delta[1];
+    }
+    if (local < fd.coarse_depth_tile_count) {
+      cw.pending = local;
+      cw.live = true;
+      return true;
+    }
+  }
+  cw.live = false;
+  return false;
+}
+
+__device__ bool coarse_walk_init(const FrameParams& fp, uint32_t fi, uint32_t ii, V3 p0, V3 p1, const SegDir& sd,
+                                 CoarseWalk& cw) {
+  cw.live = false;
+  cw.left = 0u;
   const sr_frustum_desc& fd = fp.fr[fi].d;
   const sr_coarse_asset_range r = fp.ranges[(size_t)ii * fp.F + fi];
-  if (r.geo_id != ii || r.frustum_id != fi) return;
+  if (r.geo_id != ii || r.frustum_id != fi) return false;
   float W = (float)fd.screen_width, H = (float)fd.screen_height;
   uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
   uint32_t min_cx = max(r.min_x, min(f2u(fmaxf(fminf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
   uint32_t max_cx = min(r.max_x, min(f2u(fmaxf(fmaxf(p0.x, p1.x), 0.0f)) / cpx, fd.coarse_tiles_x - 1u));
   uint32_t min_cy = max(r.min_y, min(f2u(fmaxf(fminf(p0.y, p1.y), 0.0f)) / cpy, fd.coarse_tiles_y - 1u));
   uint32_t max_cy = min(r.max_y, min(f2u(fmaxf(fmaxf(p0.y, p1.y), 0.0f)) / cpy, fd.coarse_tiles_y - 1u));
-  if (max_cx < min_cx || max_cy < min_cy) return;
+  if (max_cx < min_cx || max_cy < min_cy) return false;
   V3 bmin{(float)(min_cx * cpx), (float)(min_cy * cpy), 0.0f};
   V3 bmax{fminf((float)((max_cx + 1u) * cpx), W) - 1e-3f, fminf((float)((max_cy + 1u) * cpy), H) - 1e-3f, 1.0f};
-  const SegDir sd = seg_dir(p0, p1);  // shared with every coarse tile's clip
   Clipped c = clip_segment_dir(p0, p1, sd, bmin, bmax);
-  if (!c.ok) return;
+  if (!c.ok) return false;
   float pa[2] = {div_by(c.p0.x, cpx), div_by(c.p0.y, cpy)};
   float pb[2] = {div_by(c.p1.x, cpx), div_by(c.p1.y, cpy)};
-  const int32_t lo[2] = {(int32_t)min_cx, (int32_t)min_cy}, hi[2] = {(int32_t)max_cx, (int32_t)max_cy};
-  int32_t f[2], f1[2], step[2];
-  float delta[2], tmax[2];
+  cw.lo[0] = (int32_t)min_cx; cw.lo[1] = (int32_t)min_cy;
+  cw.hi[0] = (int32_t)max_cx; cw.hi[1] = (int32_t)max_cy;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    f[k] = iclamp(f2i(floorf(pa[k])), lo[k], hi[k]);
-    f1[k] = iclamp(f2i(floorf(pb[k])), lo[k], hi[k]);
+    cw.f[k] = iclamp(f2i(floorf(pa[k])), cw.lo[k], cw.hi[k]);
+    cw.f1[k] = iclamp(f2i(floorf(pb[k])), cw.lo[k], cw.hi[k]);
     float d = pb[k] - pa[k];
-    step[k] = sgn_i32(d);
+    cw.step[k] = sgn_i32(d);
     bool tiny = fabsf(d) < 1e-6f;
     float safe = tiny ? 1e-6f : d;
-    delta[k] = fabsf(1.0f / safe);
+    cw.delta[k] = fabsf(1.0f / safe);
     float fl = floorf(pa[k]);
-    tmax[k] = step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
-    if (tiny) tmax[k] = 1e38f;
+    cw.tmax[k] = cw.step[k] > 0 ? ((fl + 1.0f) - pa[k]) / safe : (fl - pa[k]) / safe;
+    if (tiny) cw.tmax[k] = 1e38f;
   }
-  uint32_t max_steps = (max_cx - min_cx + 1u) + (max_cy - min_cy + 1u) + 2u;
-  for (uint32_t safety = 0; safety <= max_steps; ++safety) {
-    if (f[0] < lo[0] || f[0] > hi[0] || f[1] < lo[1] || f[1] > hi[1]) break;
-    uint32_t local = (uint32_t)f[1] * fd.coarse_tiles_x + (uint32_t)f[0];
-    if (local < fd.coarse_depth_tile_count)
-      trace_coarse_tile<MODE>(fp, p0, p1, sd, fd, fd.coarse_depth_tile_base + local, local, (uint32_t)f[0],
-                              (uint32_t)f[1], ref, task, rec);
-    if (f[0] == f1[0] && f[1] == f1[1]) break;
-    if (tmax[0] <= tmax[1]) { f[0] += step[0]; tmax[0] += delta[0]; }
-    else { f[1] += step[1]; tmax[1] += delta[1]; }
-  }
+  cw.left = (max_cx - min_cx + 1u) + (max_cy - min_cy + 1u) + 3u;
+  return coarse_walk_next(fd, cw);
+}
+
+// Every coarse tile of the segment in walk order, each traced in place.
+template <int MODE>
+__device__ void visit_segment(const FrameParams& fp, uint32_t fi, uint32_t ii, V3 p0, V3 p1,
+                              const sr_fine_seg_ref& ref, uint32_t task, HitRec& rec) {
+  const sr_frustum_desc& fd = fp.fr[fi].d;
+  const SegDir sd = seg_dir(p0, p1);  // shared with every coarse tile's clip
+  CoarseWalk cw;
+  if (!coarse_walk_init(fp, fi, ii, p0, p1, sd, cw)) return;
+  do {
+    const uint32_t local = cw.pending;
+    trace_coarse_tile<MODE>(fp, p0, p1, sd, fd, fd.coarse_depth_tile_base + local, local, local % fd.coarse_tiles_x,
+                            local / fd.coarse_tiles_x, ref, task, rec);
+  } while (coarse_walk_next(fd, cw));
 }
 
 // Endpoints as every binning pass sees them: the packed ProjectedSegment
@@ -640,24 +681,36 @@ __global__ __launch_bounds__(256) void k_visit(FrameParams fp) {
 // Pass 1: per slot validity ballot + packed endpoints in slot layout; valid
 // slots also mark their pages (the LUT is built before projection) and, for
 // the CSR backend, record their fine cells and count page candidates.
+// Each wave balances its 64 segments' work in rounds: the lanes queue their
+// segments' coarse tiles (a few DDA steps each), then every lane traces
+// queued tiles (clip, LUT slices, page marks) and queues their fine walks,
+// then every lane runs queued fine walks.  A segment crossing many tiles no
+// longer holds the other 63 lanes; a lane whose tiles do not fit the queue
+// resumes its walk in the next round.  The wave is converged at every flush.
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_project_mark(FrameParams fp, sr_projected_segment* slot_proj) {
   __shared__ MarkJob jobs[WAVES][kMarkQ];
-  __shared__ uint32_t jn[WAVES], nrec[WAVES][64];
+  __shared__ uint32_t tjobs[WAVES][kTileQ];  // frustum-local coarse tile << 6 | owner lane
+  __shared__ float lseg[WAVES][9][64];       // per lane: endpoints and 1 / direction
+  __shared__ uint32_t jn[WAVES], tn[WAVES], nrec[WAVES][64];
   const uint64_t total = (uint64_t)fp.F * fp.slots;
   const bool csr = fp.st.fine_binning_backend != 0u;
   const bool record = csr && fp.hit_k != 0u;
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (uint64_t)gridDim.x * blockDim.x) {
-    if (lane == 0u) jn[w] = 0u;
+    if (lane == 0u) {
+      jn[w] = 0u;
+      tn[w] = 0u;
+    }
     nrec[w][lane] = 0u;
-    MarkQueue mq{jobs[w], &jn[w], nrec[w], lane};
-    uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);
+    const uint32_t fi = (uint32_t)(s / fp.slots), pos = (uint32_t)(s % fp.slots);  // one frustum per wave
+    const sr_frustum_desc& fd = fp.fr[fi].d;
     uint32_t ii, strand, seg;
     sr_projected_segment ps;
     float4 scr[2];
-    bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps, scr);
-    uint32_t hits = 0;
+    const bool ok = project_slot(fp, fi, pos, &ii, &strand, &seg, &ps, scr);
+    CoarseWalk cw;
+    cw.live = false;
     if (ok) {
       slot_proj[s] = ps;
       fp.slot_screen[2 * s] = scr[0];
@@ -665,16 +718,37 @@ __global__ __launch_bounds__(64 * WAVES) void k_project_mark(FrameParams fp, sr_
       // K10: mark the (coarse tile, LUT slice) pages the segment touches,
       // from the same packed endpoints the count and fill passes unpack
       V3 p0, p1;
-      task_endpoints(fp.fr[fi].d, ps, &p0, &p1);
-      HitRec rec{record ? fp.slot_hits + s : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
-      if (csr) rec.mq = &mq;
-      visit_segment<MODE_MARK>(fp, fi, ii, p0, p1, sr_fine_seg_ref{ii, seg, 0u}, 0u, rec);
+      task_endpoints(fd, ps, &p0, &p1);
+      const SegDir sd = seg_dir(p0, p1);
+      float* ls = &lseg[w][0][lane];
+      ls[0 * 64] = p0.x; ls[1 * 64] = p0.y; ls[2 * 64] = p0.z;
+      ls[3 * 64] = p1.x; ls[4 * 64] = p1.y; ls[5 * 64] = p1.z;
+      ls[6 * 64] = sd.inv.x; ls[7 * 64] = sd.inv.y; ls[8 * 64] = sd.inv.z;
+      coarse_walk_init(fp, fi, ii, p0, p1, sd, cw);
     }
-    if (csr) {  // the wave's queued fine walks, one per lane (the wave is converged here)
-      const sr_frustum_desc& fd = fp.fr[fi].d;
-      const uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
+    const uint32_t cpx = max(1u, fd.froxel_size_x * kExt), cpy = max(1u, fd.froxel_size_y * kExt);
+    while (__ballot(cw.live) != 0ull) {
+      while (cw.live) {  // queue this lane's tiles while they fit
+        const uint32_t j = atomicAdd(&tn[w], 1u);
+        if (j >= fp.tile_queue) break;  // the pending tile waits for the next round
+        tjobs[w][j] = (cw.pending << 6) | lane;
+        coarse_walk_next(fd, cw);
+      }
+      const uint32_t nt = min(tn[w], fp.tile_queue);
+      for (uint32_t j = lane; j < nt; j += 64u) {  // the queued tiles, one per lane
+        const uint32_t job = tjobs[w][j], owner = job & 63u, local = job >> 6;
+        const float* os = &lseg[w][0][owner];
+        const V3 p0{os[0 * 64], os[1 * 64], os[2 * 64]}, p1{os[3 * 64], os[4 * 64], os[5 * 64]};
+        const SegDir sd{V3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z}, V3{os[6 * 64], os[7 * 64], os[8 * 64]}};
+        MarkQueue omq{jobs[w], &jn[w], nrec[w], owner};
+        HitRec orec{record ? fp.slot_hits + (s - lane + owner) : nullptr, 0u, fp.hit_k, 0u, 0u, (uint32_t)total};
+        orec.mq = &omq;
+        trace_coarse_tile<MODE_MARK>(fp, p0, p1, sd, fd, fd.coarse_depth_tile_base + local, local,
+                                     local % fd.coarse_tiles_x, local / fd.coarse_tiles_x, sr_fine_seg_ref{}, 0u,
+                                     orec);
+      }
       const uint32_t nj = min(jn[w], fp.mark_queue);
-      for (uint32_t j = lane; j < nj; j += 64u) {
+      for (uint32_t j = lane; j < nj; j += 64u) {  // their queued fine walks, one per lane
         const MarkJob jb = jobs[w][j];
         const uint32_t owner = jb.lo & 63u, local = jb.lo >> 6;
         MarkQueue omq{nullptr, nullptr, nrec[w], owner};
@@ -684,7 +758,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_project_mark(FrameParams fp, sr_
         fine_dda(V3{jb.q0x, jb.q0y, jb.q0z}, V3{jb.q1x, jb.q1y, jb.q1z}, origin, jb.zmin, jb.zmax, fd,
                  occluder_for(fp, fd, local), [&](uint32_t cell) { mark_cell(orec, jb.pti, cell); });
       }
-      hits = ok ? nrec[w][lane] : 0u;
+      if (lane == 0u) {  // every lane has read both counts
+        tn[w] = 0u;
+        jn[w] = 0u;
+      }
+    }
+    if (csr) {
+      const uint32_t hits = ok ? nrec[w][lane] : 0u;
       if (record && ok) fp.slot_hit_count[s] = (uint8_t)(hits > fp.hit_k ? kHitOverflow : hits);
       fp.slot_hit_total[s] = hits;  // offsets by a scan: no device atomic
     }

@@ -611,9 +611,15 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
 
 // Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)) over the
 // k with |d - 2 pi k| <= r = sqrt(32 var): every term left out is below
-// e^-16 (1e-7 of the largest term).  For r < pi that is at most one term.
+// e^-16 (1e-7 of the largest term).  For r < pi that is at most one term,
+// the k nearest d / (2 pi), evaluated without a loop.
 __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, float r) {
   const float inv_2pi = (float)(1.0 / (2.0 * 3.14159265359));
+  if (r < kPiF) {
+    const float k = fminf(10.0f, fmaxf(-10.0f, rintf(pd * inv_2pi)));
+    const float ang = pd - TWO_PI_F * k;
+    return fabsf(ang) <= r ? fexp(ang * ang * neg_inv_2var) : 0.0f;
+  }
   int k0 = max(-10, (int)ceilf((pd - r) * inv_2pi)), k1 = min(10, (int)floorf((pd + r) * inv_2pi));
   float total = 0.0f;
   for (int k = k0; k <= k1; ++k) {
@@ -660,7 +666,8 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
     const float x = lp.h * ieta;
     float gamma_t = asinf(x);
     float f = fresnel_reflectance(eta, sacos(ctd * lp.cos_gamma_i));
-    float e = 1.0f + fcos(2.0f * gamma_t);
+    // 1 + cos(2 asin x) = 2 - 2 x^2 while x is in asin's domain
+    float e = fabsf(x) <= 1.0f ? 2.0f - 2.0f * (x * x) : 1.0f + fcos(2.0f * gamma_t);
     V3 T{fexp(-2.0f * mup.x * e), fexp(-2.0f * mup.y * e), fexp(-2.0f * mup.z * e)};
     // asin(clamp(x, -1, 1)) is gamma_t whenever x is already in range
     float gt_phi = fabsf(x) <= 1.0f ? gamma_t : asinf(wclamp(x, -1.0f, 1.0f));
@@ -684,9 +691,10 @@ __device__ void np_lobes(float phi, float theta_i, float theta_r, float eta, V3 
       float det;
       if (narrow) det = fabsf(fmodf(pd, TWO_PI_F)) < 1e-3f ? 0.0f : 1e6f;
       else det = norm * wrapped_gaussian(pd, neg_inv_2var, r_win);
-      acc[p].x = acc[p].x + att.x * det * wgt;
-      acc[p].y = acc[p].y + att.y * det * wgt;
-      acc[p].z = acc[p].z + att.z * det * wgt;
+      const float dw = det * wgt;
+      acc[p].x = acc[p].x + att.x * dw;
+      acc[p].y = acc[p].y + att.y * dw;
+      acc[p].z = acc[p].z + att.z * dw;
     }
   }
   V3 total{0.0f, 0.0f, 0.0f};

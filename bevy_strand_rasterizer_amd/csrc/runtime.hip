@@ -1012,6 +1012,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.tel = tel_words ? sl.buf[B_TELEMETRY].as<uint32_t>() : nullptr;
   fp.hit_k = hit_k;
   fp.mark_queue = (ctx->flags & SR_CONFIG_CSR_FEW_HITS) ? 2u : kMarkQ;
+  fp.tile_queue = (ctx->flags & SR_CONFIG_CSR_FEW_HITS) ? 2u : kTileQ;
   fp.slot_hits = sl.buf[B_SLOT_HITS].as<uint32_t>();
   fp.slot_hit_count = sl.buf[B_SLOT_HIT_COUNT].as<uint8_t>();
 

@@ -529,7 +529,9 @@ typedef struct sr_device_config {
 /* sr_device_config.flags: the page CSR backend records only 2 fine cells per
  * segment while marking (instead of up to 24), so most segments take the
  * re-trace path of the later passes, and queues only 2 fine walks per wave,
- * so most walks run in place (exercises both; results are identical). */
+ * so most walks run in place, and 2 coarse tiles per wave and marking round,
+ * so waves walk their segments in many rounds (exercises all three; results
+ * are identical). */
 #define SR_CONFIG_CSR_FEW_HITS 2u
 /* sr_device_config.flags: capture each frame's launch sequence in a HIP graph
  * and replay it for later frames of the same shape (buffers, capacities,

@@ -41,8 +41,9 @@ def test_spans_keep_every_covered_pixel(span_check):
 
 
 def test_unorm16_matches_division(tmp_path):
-    """The kernels' i / 65535 (an FMA-corrected product) equals the correctly
-    rounded quotient for all 65,536 values of a 16-bit unorm."""
+    """The kernels' i / 65535 and q / 16777215 (FMA-corrected products) equal
+    the correctly rounded quotients for all 65,536 values of a 16-bit unorm
+    and all 2^24 quantised depths."""
     if not Path(HIPCC).exists():
         pytest.skip("hipcc not available")
     exe = tmp_path / "unorm_check"
