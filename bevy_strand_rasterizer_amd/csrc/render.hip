@@ -612,7 +612,8 @@ __device__ float Mp(float v, float theta_i_, float theta_r, float shift) {
 // Wrapped Gaussian sum_{k=-10..10} exp(-(d - 2 pi k)^2 / (2 var)) over the
 // k with |d - 2 pi k| <= r = sqrt(32 var): every term left out is below
 // e^-16 (1e-7 of the largest term).  For r < pi that is at most one term,
-// the k nearest d / (2 pi), evaluated without a loop.
+// the k nearest d / (2 pi), and for r < 2 pi at most two; both are
+// evaluated without a loop.
 __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, float r) {
   const float inv_2pi = (float)(1.0 / (2.0 * 3.14159265359));
   if (r < kPiF) {
@@ -621,6 +622,12 @@ __device__ __forceinline__ float wrapped_gaussian(float pd, float neg_inv_2var, 
     return fabsf(ang) <= r ? fexp(ang * ang * neg_inv_2var) : 0.0f;
   }
   int k0 = max(-10, (int)ceilf((pd - r) * inv_2pi)), k1 = min(10, (int)floorf((pd + r) * inv_2pi));
+  if (r < TWO_PI_F) {  // at most two terms, k0 and k0 + 1, summed as the loop below sums them
+    const float a0 = pd - TWO_PI_F * (float)k0, a1 = pd - TWO_PI_F * (float)(k0 + 1);
+    const float t0 = k0 <= k1 ? fexp(a0 * a0 * neg_inv_2var) : 0.0f;
+    const float t1 = k0 + 1 <= k1 ? fexp(a1 * a1 * neg_inv_2var) : 0.0f;
+    return t0 + t1;
+  }
   float total = 0.0f;
   for (int k = k0; k <= k1; ++k) {
     float ang = pd - TWO_PI_F * (float)k;
