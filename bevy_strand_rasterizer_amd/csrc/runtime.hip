@@ -763,12 +763,13 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const bool csr = st.fine_binning_backend != 0u;
   // cells recorded per slot while marking: 24 words per slot (a segment of
   // the benchmark scalps hits 5 cells on average and more than 24 about once
-  // in 10^5), fewer when the table would pass 8 GiB, none when page-table
-  // entries do not fit 24 bits; a slot with more cells is traced again
+  // in 10^5), fewer when the table would pass 48 GiB (the stress frames'
+  // 400M+ slots record all 24), none when page-table entries do not fit 24
+  // bits; a slot with more cells is traced again
   uint32_t hit_k = 0;
   if (csr && ptab < (1u << 24))
-    for (uint32_t k : {24u, 16u, 8u})
-      if (4ull * k * task_cap <= (8ull << 30)) { hit_k = k; break; }
+    for (uint32_t k : {24u, 16u, 12u, 8u, 6u, 4u})
+      if (4ull * k * task_cap <= (48ull << 30)) { hit_k = k; break; }
   if (hit_k && (ctx->flags & SR_CONFIG_CSR_FEW_HITS)) hit_k = 2u;
   const uint64_t hit_bytes = 4ull * hit_k * task_cap;
   const uint32_t hit_cap = csr ? ref_cap : 0u;
