@@ -362,17 +362,22 @@ def run_batched(w, steps: int, warmup: int, plain_fps: float | None, tune: bool 
     return out
 
 
-def run_stress(views: int, views_per_frame: int, first_frame_views: int, backend: int, repeats: int = 2):
+def run_stress(views: int, views_per_frame: int, first_frame_views: int, backend: int, repeats: int = 2,
+               tune: bool = True):
     """The stress configuration as batches of `views` 4K views: the first
     frame of a batch draws `first_frame_views` cameras and builds the 4
     lights' 8-layer shadow maps, every later frame draws `views_per_frame`
-    cameras with those maps (SR_FRAME_REUSE_LIGHT_DOMS).  Every frame shape
-    first renders until its capacities fit (the runtime grows them after a
-    frame reports capacity_exceeded), then `repeats` batches are timed, each
-    building its lights again."""
+    cameras with those maps (SR_FRAME_REUSE_LIGHT_DOMS).  As every batched
+    configuration, the froxel sizes and launch shape come from the autotuner
+    (run on the light-building frame, keeping the binning backend: a segment
+    scatter trial of a 1.85G-reference frame is not worth its time), applied
+    to every frame of the batch.  Every frame shape first renders until its
+    capacities fit (the runtime grows them after a frame reports
+    capacity_exceeded), then `repeats` batches are timed, each building its
+    lights again."""
     import torch
 
-    from bevy_strand_rasterizer_amd import scene
+    from bevy_strand_rasterizer_amd import abi, scene
     from bevy_strand_rasterizer_amd.runtime import StrandRenderer
 
     ws = [scene.retile(w, 0, 0, backend)
@@ -380,6 +385,16 @@ def run_stress(views: int, views_per_frame: int, first_frame_views: int, backend
                                               first_frame_views=first_frame_views)]
     r = StrandRenderer(device=0)
     r.upload(ws[0].assets[0])
+    best, trials = None, []
+    if tune:
+        first = ws[0]
+        saved = int(first.settings.get("frame_flags", 0))
+        first.settings["frame_flags"] = saved | abi.FRAME_TUNE_KEEP_BACKEND
+        res = r.autotune(first, frames_per_trial=1, min_psnr_db=50.0)
+        first.settings["frame_flags"] = saved
+        best, trials = res["best"], res["trials"]
+        ws = [scene.apply_tuning(w, best) for w in ws]
+        print(f"bench: stress tuned {best}", file=sys.stderr, flush=True)
     grow = 0
     per_frame = []
     for w in ws:
@@ -414,7 +429,7 @@ def run_stress(views: int, views_per_frame: int, first_frame_views: int, backend
             "frames_per_s": round(views * repeats / dt, 3), "ms_per_view": round(dt * 1e3 / (views * repeats), 3),
             "ms_per_batch": round(dt * 1e3 / repeats, 2),
             "msegments_per_s": round(segs * views * repeats / dt / 1e6, 2), "warm_growths": grow,
-            "frames_over_capacity": exceeded, "frames": per_frame}
+            "frames_over_capacity": exceeded, "frames": per_frame, "tuned": best, "trials": trials}
 
 
 class PinnedPool:

@@ -30,6 +30,7 @@ FRAME_REUSE_LIGHT_DOMS = 1  # sr_frame_settings.flags
 FRAME_RGBA8_OUTPUTS = 2
 FRAME_VIRTUAL_DOMS = 4
 FRAME_DEPTH_SORTED_BLEND = 8
+FRAME_TUNE_KEEP_BACKEND = 16  # read by sr_autotune only
 BLEND_SORT_FRAGMENTS = 4  # SR_BLEND_SORT_FRAGMENTS
 DOM_PAGE_XY = 256  # SR_DOM_PAGE_XY
 DOM_PREFETCH_PAGE_BORDER = 1

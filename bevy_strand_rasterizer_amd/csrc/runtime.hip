@@ -580,7 +580,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
   if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS | SR_FRAME_VIRTUAL_DOMS |
-                   SR_FRAME_DEPTH_SORTED_BLEND))
+                   SR_FRAME_DEPTH_SORTED_BLEND | SR_FRAME_TUNE_KEEP_BACKEND))
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
   const bool reuse = (st.flags & SR_FRAME_REUSE_LIGHT_DOMS) != 0u;
   const bool rgba8 = (st.flags & SR_FRAME_RGBA8_OUTPUTS) != 0u;
@@ -1866,9 +1866,12 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
   struct Cand { uint32_t backend, cam, light, graphs, mark_waves, raster_waves; };
   std::vector<Cand> cands;
   cands.push_back({fd->settings.fine_binning_backend, cam0, light0, 0u, fd->mark_waves, fd->raster_waves});  // the reference
-  for (uint32_t b : {0u, 1u})
+  const bool keep_backend = (fd->settings.flags & SR_FRAME_TUNE_KEEP_BACKEND) != 0u;
+  for (uint32_t b : {0u, 1u}) {
+    if (keep_backend && b != fd->settings.fine_binning_backend) continue;
     for (uint32_t cf : {8u, 16u})
       for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, 1u, 0u, 0u});
+  }
   const size_t stage1 = cands.size();
   bool stage2 = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -389,6 +389,10 @@ SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 88, "sr_frame_settings is 88 bytes
  * many fragments on a pixel is blended in exact depth order. */
 #define SR_FRAME_DEPTH_SORTED_BLEND 8u
 #define SR_BLEND_SORT_FRAGMENTS 4u
+/* sr_frame_settings.flags, read by sr_autotune only: every candidate keeps
+ * the frame's fine_binning_backend (froxel sizes and launch shape are still
+ * searched).  For frames too large to try the other backend on. */
+#define SR_FRAME_TUNE_KEEP_BACKEND 16u
 #define SR_DOM_PAGE_XY 256u              /* DOM_PAGE_XY, src/plugin.rs:68 */
 #define SR_DOM_PREFETCH_PAGE_BORDER 1u   /* DOM_PREFETCH_PAGE_BORDER, src/plugin.rs:69 */
 
@@ -610,8 +614,9 @@ const char* sr_last_error(sr_context* ctx);
  * binning backend per run, src/resources.rs).  Times one frame under the
  * candidate configurations, replayed from a HIP graph, against the frame as
  * given without graphs (the quality reference), on the context's main slot:
- * first fine-binning backend (segment scatter / page CSR) x camera froxel size
- * {8, 16} x light froxel size {8, 16}; then, for the fastest accepted of
+ * first fine-binning backend (segment scatter / page CSR; only the frame's own
+ * with SR_FRAME_TUNE_KEEP_BACKEND) x camera froxel size {8, 16} x light
+ * froxel size {8, 16}; then, for the fastest accepted of
  * those, the launch shape: marking workgroups of 1, 2 or 4 waves x 8, 16 or
  * 24 persistent raster waves per CU.  A candidate whose composite stays
  * within min_psnr_db of the reference is accepted (tile size changes the

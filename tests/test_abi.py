@@ -114,6 +114,18 @@ def test_version_string():
     assert b"gfx950" in lib.sr_version()
 
 
+def test_flag_constants_match_header():
+    """The SR_CONFIG_*, SR_FRAME_* and SR_REQUEST_* bits of abi.py are the
+    header's, and no two flags of one family share a bit."""
+    text = HEADER.read_text()
+    for family in ("CONFIG", "FRAME", "REQUEST"):
+        defs = {m.group(1): int(m.group(2)) for m in re.finditer(rf"#define SR_{family}_(\w+) (\d+)u\b", text)}
+        assert defs, family
+        for name, value in defs.items():
+            assert getattr(abi, f"{family}_{name}") == value, f"SR_{family}_{name}"
+        assert len(set(defs.values())) == len(defs), family
+
+
 def test_binning_field_packing():
     # task_contract.rs:137-157: level bits 0-1, shadow bit 2, frustum bits 3-31
     packed = (0 & 3) | (1 << 2) | (37 << 3)

@@ -387,6 +387,24 @@ def test_autotune_picks_an_accepted_configuration():
     compare(tuned, ctr, gpu, ref)
 
 
+@pytest.mark.parametrize("backend", [0, 1])
+def test_autotune_can_keep_the_backend(backend):
+    """SR_FRAME_TUNE_KEEP_BACKEND: every trial keeps the frame's binning
+    backend; the froxel sizes and launch shapes are still searched."""
+    w = scene.config_small(froxel=8)
+    w.settings["fine_binning_backend"] = backend
+    w.settings["frame_flags"] = abi.FRAME_TUNE_KEEP_BACKEND
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        res = r.autotune(w, frames_per_trial=1, min_psnr_db=40.0)
+    trials = res["trials"]
+    print(f"[autotune keep backend {backend}] {len(trials)} trials, best {res['best']}", flush=True)
+    assert len(trials) == 13  # reference, 4 froxel candidates, 8 launch shapes of the best
+    assert all(t["fine_binning_backend"] == backend for t in trials)
+    assert {(t["camera_froxel"], t["light_froxel"]) for t in trials[1:5]} == {(c, l) for c in (8, 16) for l in (8, 16)}
+    assert res["best"]["accepted"] == 1 and res["best"]["fine_binning_backend"] == backend
+
+
 def telemetry_from_oracle(w, ref, page_cap):
     """finalize_telemetry's words (strand_prepass.wgsl:2145-2188) and the
     per-frustum words (:736-737, 1085, 1136, 2830-2832), restated from the
