@@ -96,6 +96,7 @@ enum Counter : uint32_t {
   C_DOM_PAGES = 14,   // resident virtual DOM pages
   C_DOM_WANT = 15,    // requested pages within every light's budget (resident with an unbounded pool)
   C_DOM_DROPS = 16,   // of those, pages past the pool capacity (not resident)
+  C_SORT_PAIR = 17,   // page CSR: the (key, slot) buffer pair holding the page-sorted hits
   C_COUNT = 20
 };
 

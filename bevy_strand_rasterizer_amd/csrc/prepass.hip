@@ -1012,7 +1012,19 @@ struct HitBaseDst {
 constexpr uint32_t kRadixRounds = kRadixBlock / 256u;  // rounds of 64 per wave
 constexpr uint32_t kRadixChunk = kRadixRounds * 64u;   // items per wave
 
-__device__ __forceinline__ uint32_t radix_count(const uint32_t* n_dev, uint32_t cap) { return min(*n_dev, cap); }
+__device__ __forceinline__ uint32_t radix_count(const uint32_t* counters, uint32_t cap) {
+  return min(counters[C_CANDS], cap);
+}
+
+// Whether the LSD pass over key bits [shift, shift + 8) reorders anything.
+// Keys are page * 256 + cell; with no dropped page every page id is below
+// C_PAGES, so a pass whose digit is zero for every page would leave the
+// order as it is, and it is skipped (the pass count follows the page
+// capacity, which frames fill to a fraction).  The first pass always runs.
+__device__ __forceinline__ bool radix_pass_needed(const uint32_t* counters, uint32_t shift) {
+  if (shift <= 8u) return true;
+  return counters[C_PAGE_DROPS] != 0u || counters[C_PAGES] > (1u << (shift - 8u));
+}
 
 // Lanes of the wave whose (valid) digit equals this lane's.
 __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) {
@@ -1030,11 +1042,11 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, bool has) 
 // the wave's first two digit groups add their sizes with one LDS atomic each,
 // the other lanes one by one), stored block-major (hist[b * 256 + d], one
 // coalesced row per block).
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* n_dev, uint32_t cap,
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* counters, uint32_t cap,
                                                     uint32_t shift, uint32_t* hist) {
   __shared__ uint32_t cnt[256];
-  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x;
-  if (b * kRadixBlock >= n) return;
+  const uint32_t n = radix_count(counters, cap), b = blockIdx.x;
+  if (b * kRadixBlock >= n || !radix_pass_needed(counters, shift)) return;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   cnt[threadIdx.x] = 0u;
   __syncthreads();
@@ -1076,13 +1088,14 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
 // one workgroup that scans the chunks per digit and adds the digit offsets,
 // then each chunk writes its rows' bases.  Every access is a coalesced row.
 
-__device__ __forceinline__ uint32_t radix_nb(const uint32_t* n_dev, uint32_t cap) {
-  return (radix_count(n_dev, cap) + kRadixBlock - 1u) / kRadixBlock;
+__device__ __forceinline__ uint32_t radix_nb(const uint32_t* counters, uint32_t cap) {
+  return (radix_count(counters, cap) + kRadixBlock - 1u) / kRadixBlock;
 }
 
-__global__ __launch_bounds__(256) void k_digit_chunk_sums(const uint32_t* hist, const uint32_t* n_dev, uint32_t cap,
-                                                         uint32_t* chunk) {
-  const uint32_t nb = radix_nb(n_dev, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
+__global__ __launch_bounds__(256) void k_digit_chunk_sums(const uint32_t* hist, const uint32_t* counters, uint32_t cap,
+                                                         uint32_t shift, uint32_t* chunk) {
+  if (!radix_pass_needed(counters, shift)) return;
+  const uint32_t nb = radix_nb(counters, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
   const uint32_t c = blockIdx.x, d = threadIdx.x, b0 = c * rows, b1 = min(nb, b0 + rows);
   uint32_t sum = 0u;
   for (uint32_t bl = b0; bl < b1; bl += 8u) {  // eight rows' loads in flight
@@ -1095,8 +1108,9 @@ __global__ __launch_bounds__(256) void k_digit_chunk_sums(const uint32_t* hist, 
   chunk[c * 256u + d] = sum;
 }
 
-__global__ __launch_bounds__(256) void k_digit_chunk_scan(uint32_t* chunk) {
+__global__ __launch_bounds__(256) void k_digit_chunk_scan(const uint32_t* counters, uint32_t shift, uint32_t* chunk) {
   __shared__ uint32_t lds4[4];
+  if (!radix_pass_needed(counters, shift)) return;
   const uint32_t d = threadIdx.x;
   uint32_t run = 0u;
   uint32_t ex[kDigitChunks / 16u][16];
@@ -1117,9 +1131,10 @@ __global__ __launch_bounds__(256) void k_digit_chunk_scan(uint32_t* chunk) {
     for (uint32_t j = 0; j < 16u; ++j) chunk[(g * 16u + j) * 256u + d] = ex[g][j] + off;
 }
 
-__global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* hist, const uint32_t* n_dev, uint32_t cap,
-                                                    const uint32_t* chunk, uint32_t* base) {
-  const uint32_t nb = radix_nb(n_dev, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
+__global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* hist, const uint32_t* counters, uint32_t cap,
+                                                    uint32_t shift, const uint32_t* chunk, uint32_t* base) {
+  if (!radix_pass_needed(counters, shift)) return;
+  const uint32_t nb = radix_nb(counters, cap), rows = (nb + kDigitChunks - 1u) / kDigitChunks;
   const uint32_t c = blockIdx.x, d = threadIdx.x, b0 = c * rows, b1 = min(nb, b0 + rows);
   uint32_t run = chunk[c * 256u + d];
   for (uint32_t bl = b0; bl < b1; bl += 8u) {  // eight rows' loads, then their stores
@@ -1138,15 +1153,18 @@ __global__ __launch_bounds__(256) void k_digit_bases(const uint32_t* hist, const
 // LDS and written out in that order, so consecutive lanes write consecutive
 // addresses of one digit's output range.
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals,
-                                                       uint32_t* okeys, uint32_t* ovals, const uint32_t* n_dev,
-                                                       uint32_t cap, uint32_t shift, const uint32_t* bases) {
+                                                       uint32_t* okeys, uint32_t* ovals, uint32_t* counters,
+                                                       uint32_t cap, uint32_t shift, const uint32_t* bases,
+                                                       uint32_t dst_pair) {
   __shared__ uint32_t lkeys[kRadixBlock];
   __shared__ uint32_t lvals[kRadixBlock];
   __shared__ uint32_t wcnt[4][256];  // per wave: running digit count, then the wave's start of each digit
   __shared__ uint32_t dstart[256];   // block-local start of each digit
   __shared__ uint32_t gbase[256];    // output start of each digit for this block
   __shared__ uint32_t lds4[4];
-  const uint32_t n = radix_count(n_dev, cap), b = blockIdx.x, base = b * kRadixBlock;
+  const uint32_t n = radix_count(counters, cap), b = blockIdx.x, base = b * kRadixBlock;
+  if (!radix_pass_needed(counters, shift)) return;
+  if (b == 0u && threadIdx.x == 0u) counters[C_SORT_PAIR] = dst_pair;  // the pair now holding the sort
   if (base >= n) return;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t nvalid = min(kRadixBlock, n - base);
@@ -1204,10 +1222,18 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, con
   }
 }
 
+// The two (key, slot) buffer pairs of the hit sort; counters[C_SORT_PAIR]
+// names the one its last executed pass wrote.
+struct HitPairs {
+  const uint32_t* keys[2];
+  const uint32_t* vals[2];
+};
+
 // Each page's hits [bounds[p], bounds[p + 1]) in the page-sorted keys: the
 // lower bound of p << 8, one thread per page boundary (dropped pages' hits,
 // page >= page_cap, sort after every kept page).
-__global__ __launch_bounds__(256) void k_page_bounds(FrameParams fp, const uint32_t* keys) {
+__global__ __launch_bounds__(256) void k_page_bounds(FrameParams fp, HitPairs hp) {
+  const uint32_t* keys = hp.keys[fp.counters[C_SORT_PAIR] & 1u];
   const uint32_t P = fp.counters[C_PAGES];
   const uint32_t n = min(fp.counters[C_CANDS], fp.ref_cap);
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= P; p += gridDim.x * blockDim.x) {
@@ -1282,7 +1308,10 @@ __global__ __launch_bounds__(256) void k_page_order_scatter(FrameParams fp) {
 constexpr uint32_t kPageRounds = 8u;                    // rounds of 64 per wave and step
 constexpr uint32_t kPageStep = 4u * 64u * kPageRounds;  // hits per step
 
-__global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, const uint32_t* keys, const uint32_t* vals) {
+__global__ __launch_bounds__(256) void k_page_cells(FrameParams fp, HitPairs hp) {
+  const uint32_t pair = fp.counters[C_SORT_PAIR] & 1u;
+  const uint32_t* keys = hp.keys[pair];
+  const uint32_t* vals = hp.vals[pair];
   __shared__ uint32_t cur[256];       // output position of each cell's next reference
   __shared__ uint32_t wcnt[4][256];   // per wave: digit counts, then its start per cell in the step
   __shared__ uint32_t lstart[256], gbase[256];
@@ -1872,29 +1901,31 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   hipLaunchKernelGGL(k_project_write, dim3(grid_for(slots_total, 256, 1u << 20)), dim3(256), 0, s, fp, ps.slot_proj);
   if (csr) {
     timer->mark(s, "sort_hits");
-    // stable LSD passes over the page bits of (key, position) pairs; the last
-    // leaves the page-sorted hits in pair 0
+    // stable LSD passes over the page bits of (key, position) pairs, as many
+    // as the page capacity needs; a pass above the frame's highest page digit
+    // returns at once, and the page kernels read the pair the last executed
+    // pass wrote (all passes executed: pair 0)
     int src = (ps.radix_passes & 1u) ? 1 : 0;  // the pair the hits were written to
     for (uint32_t k = 0; k < ps.radix_passes; ++k, src ^= 1) {
       const uint32_t shift = 8u + 8u * k;
       const int dst = k + 1u == ps.radix_passes ? 0 : (src ^ 1);
-      hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters + C_CANDS,
+      hipLaunchKernelGGL(k_radix_hist, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], fp.counters,
                          fp.ref_cap, shift, ps.radix_hist);
-      hipLaunchKernelGGL(k_digit_chunk_sums, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist,
-                         fp.counters + C_CANDS, fp.ref_cap, ps.partials);
-      hipLaunchKernelGGL(k_digit_chunk_scan, dim3(1), dim3(256), 0, s, ps.partials);
-      hipLaunchKernelGGL(k_digit_bases, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist, fp.counters + C_CANDS,
-                         fp.ref_cap, ps.partials, ps.radix_base);
+      hipLaunchKernelGGL(k_digit_chunk_sums, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist, fp.counters,
+                         fp.ref_cap, shift, ps.partials);
+      hipLaunchKernelGGL(k_digit_chunk_scan, dim3(1), dim3(256), 0, s, fp.counters, shift, ps.partials);
+      hipLaunchKernelGGL(k_digit_bases, dim3(kDigitChunks), dim3(256), 0, s, ps.radix_hist, fp.counters,
+                         fp.ref_cap, shift, ps.partials, ps.radix_base);
       hipLaunchKernelGGL(k_radix_scatter, dim3(ps.radix_blocks), dim3(256), 0, s, ps.hkeys[src], ps.hvals[src],
-                         ps.hkeys[dst], ps.hvals[dst], fp.counters + C_CANDS, fp.ref_cap, shift, ps.radix_base);
+                         ps.hkeys[dst], ps.hvals[dst], fp.counters, fp.ref_cap, shift, ps.radix_base, (uint32_t)dst);
     }
     timer->mark(s, "page_cells");
+    const HitPairs hp{{ps.hkeys[0], ps.hkeys[1]}, {ps.hvals[0], ps.hvals[1]}};
     hipLaunchKernelGGL(k_page_bounds, dim3(grid_for((uint64_t)fp.page_cap + 1u, 256, 1u << 16)), dim3(256), 0, s, fp,
-                       ps.hkeys[0]);
+                       hp);
     hipLaunchKernelGGL(k_page_order_hist, dim3(kOrderBlocks), dim3(256), 0, s, fp);
     hipLaunchKernelGGL(k_page_order_scatter, dim3(kOrderBlocks), dim3(256), 0, s, fp);
-    hipLaunchKernelGGL(k_page_cells, dim3(grid_for(fp.page_cap, 1, 1u << 20)), dim3(256), 0, s, fp, ps.hkeys[0],
-                       ps.hvals[0]);
+    hipLaunchKernelGGL(k_page_cells, dim3(grid_for(fp.page_cap, 1, 1u << 20)), dim3(256), 0, s, fp, hp);
     timer->mark(s, "page_bases");
     launch_scan<uint32_t>(PageTotalSrc{fp.page_totals}, PageBaseDst{fp}, fp.page_cap,
                           fp.counters + C_PAGES, ps.partials, s);
