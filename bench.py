@@ -667,9 +667,10 @@ def main():
     if not args.no_batched:
         batched = {}
         for name, wb in (("64views_4lights_2kx16", scene.config_batched()),
-                         ("named_8views_1light", scene.config_batched_named(8))):
+                         ("named_8views_1light", scene.config_batched_named(8)),
+                         ("named_16views_1light", scene.config_batched_named(16))):
             wb.settings["fine_binning_backend"] = 1 if args.backend == "csr" else 0
-            if name == "named_8views_1light":  # each view alone is the named plain frame
+            if name.startswith("named_"):  # each view alone is the named plain frame
                 plain, plain_what = fps, "the named plain frame (one view, its light rendered per frame)"
             else:
                 plain = plain_views_fps(wb)
