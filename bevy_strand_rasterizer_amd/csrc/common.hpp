@@ -130,6 +130,8 @@ struct FrameParams {
   // binning
   sr_coarse_depth_lut_entry* lut;    // [coarse_total * 16]
   uint2* lut_z;                      // [coarse_total * 16] (z_min_q, z_max_q) per slice; unused: (~0u, 0)
+  uint2* lut_iv;                     // [coarse_total] a tile's one depth interval spanning >= 16 quantised
+                                     // depths (its 16 slices split it evenly), else (1, 0)
   uint32_t* page_table;              // [coarse_total * 16]
   sr_fine_page_meta* page_meta;      // [coarse_total * 16]
   uint32_t* page_totals;             // [coarse_total * 16]
@@ -279,6 +281,23 @@ SR_HD float dequantize_depth01(uint32_t q) {
   const float x = (float)q, r = 1.0f / (float)kQuant;
   const float p = x * r;
   return fmaf(fmaf(-p, (float)kQuant, x), r, p);
+}
+
+// The coarse depth slice of quantised depth q in a tile whose one depth
+// interval [zmin, zmax] (span = zmax - zmin + 1 >= 16) build_depth_warp_lut
+// splits into 16 slices [zmin + span * i / 16, zmin + span * (i + 1) / 16 - 1]:
+// the last i whose start is <= q (zmin <= q <= zmax).  rcp_span is 1 / span
+// within a few ulp; the float estimate of floor((16 d + 15) / span),
+// d = q - zmin, is then within one of it (relative error near 2^-22 on a
+// quotient below 16) and is settled on the exact integer starts
+// (tests/even_slice_check.cpp compares it with the slice scan).
+SR_HD uint32_t even_slice(uint32_t zmin, uint32_t zmax, uint32_t q, float rcp_span) {
+  const uint32_t span = zmax - zmin + 1u, d = q - zmin;
+  uint32_t c = (uint32_t)((float)(16u * d + 15u) * rcp_span);
+  c = c < kSlices - 1u ? c : kSlices - 1u;
+  if (c > 0u && ((span * c) >> 4) > d) --c;
+  else if (c + 1u < kSlices && ((span * (c + 1u)) >> 4) <= d) ++c;
+  return c;
 }
 
 SR_HD V4 world_to_screen_raw(V4 pos, const float* clip_from_world, float vw, float vh) {

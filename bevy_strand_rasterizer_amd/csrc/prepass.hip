@@ -293,6 +293,11 @@ __global__ void k_lut(FrameParams fp) {
     fp.lut_z[(size_t)t * kSlices + z] =
         out[z].virtual_count_q ? make_uint2(out[z].z_min_q, out[z].z_max_q) : make_uint2(~0u, 0u);
   }
+  // one interval of at least 16 depths: its slices are 16 contiguous ranges
+  // [zmin + span * i / 16, zmin + span * (i + 1) / 16 - 1] (finish above),
+  // which the marking pass finds arithmetically
+  const bool single = b.count == 1u && b.maxs[0] - b.mins[0] + 1u >= kSlices;
+  fp.lut_iv[t] = single ? make_uint2(b.mins[0], b.maxs[0]) : make_uint2(1u, 0u);
 }
 
 // ------------------------------------------------ binning traversal
@@ -322,9 +327,15 @@ __device__ uint32_t nearest_slice(const uint4* l4, uint32_t zq) {
   return best;
 }
 
-__device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, float za, float zb, uint32_t* sa,
+__device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, uint2 iv, float za, float zb, uint32_t* sa,
                                                       uint32_t* sb) {
   const uint32_t qa = quantize_depth01(za), qb = quantize_depth01(zb);
+  if (qa >= iv.x && qa <= iv.y && qb >= iv.x && qb <= iv.y) {  // both inside one evenly split interval
+    const float r = __builtin_amdgcn_rcpf((float)(iv.y - iv.x + 1u));  // within 1 ulp
+    *sa = even_slice(iv.x, iv.y, qa, r);
+    *sb = even_slice(iv.x, iv.y, qb, r);
+    return;
+  }
   const uint4* l4 = reinterpret_cast<const uint4*>(lz);
   uint32_t ma = 0u, mb = 0u;  // slices holding qa / qb (a used slice has x <= y)
 #pragma unroll 2
@@ -490,7 +501,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
   if (!c.ok) return;
   const uint2* lz = fp.lut_z + (size_t)tile * kSlices;
   uint32_t z0s, z1s;
-  lut_slices_for_depths(lz, c.p0.z, c.p1.z, &z0s, &z1s);
+  lut_slices_for_depths(lz, fp.lut_iv[tile], c.p0.z, c.p1.z, &z0s, &z1s);
   if (z0s == kInvalid || z1s == kInvalid) return;
   const SegDir cd = seg_dir(c.p0, c.p1);  // the tile's piece, clipped against each slice
   uint32_t zlo = min(z0s, z1s), zhi = max(z0s, z1s);

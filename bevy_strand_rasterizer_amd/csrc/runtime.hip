@@ -808,7 +808,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
       {B_RANGES, 32ull * NI * F + 32}, {B_SLOT_MASKS, 8ull * task_cap / 64}, {B_WAVE_OFF, 4ull * task_cap / 64},
       {B_FR_TASK_BASE, 4ull * (F + 1)}, {B_FR_CAM_BASE, 4ull * F}, {B_SHADED, (uint64_t)std::max(cams, 1u) * slots},
       {B_TASKS, 16ull * task_cap}, {B_PROJ, 12ull * task_cap}, {B_SHADING_Q, 4ull * task_cap},
-      {B_LUT, 16ull * ptab}, {B_LUT_Z, 8ull * ptab}, {B_PAGE_TABLE, 4ull * ptab}, {B_PAGE_META, 32ull * page_cap},
+      {B_LUT, 16ull * ptab}, {B_LUT_Z, 8ull * (ptab + coarse_total)}, {B_PAGE_TABLE, 4ull * ptab}, {B_PAGE_META, 32ull * page_cap},
       {B_PAGE_TOTALS, 4ull * page_cap}, {B_CELL_COUNTS, 4ull * page_cap * kPage},
       {B_CELL_OFFSETS, 4ull * page_cap * kPage}, {B_CELL_CURSORS, 4ull * page_cap * kPage},
       {B_REFS, 12ull * ref_cap}, {B_ACTIVE_FLAGS, 4ull * tiles16}, {B_ACTIVE_Q, 4ull * tiles16},
@@ -943,6 +943,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.shading_queue = sl.buf[B_SHADING_Q].as<uint32_t>();
   fp.lut = sl.buf[B_LUT].as<sr_coarse_depth_lut_entry>();
   fp.lut_z = sl.buf[B_LUT_Z].as<uint2>();
+  fp.lut_iv = fp.lut_z + ptab;  // after the per-slice pairs
   fp.page_table = sl.buf[B_PAGE_TABLE].as<uint32_t>();
   fp.page_meta = sl.buf[B_PAGE_META].as<sr_fine_page_meta>();
   fp.page_totals = sl.buf[B_PAGE_TOTALS].as<uint32_t>();

@@ -51,3 +51,16 @@ def test_unorm16_matches_division(tmp_path):
                    check=True, capture_output=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.split() == ["0"], out.stdout
+
+def test_even_slice_matches_the_slice_scan(tmp_path):
+    """The marking pass's arithmetic LUT slice of a depth in an evenly split
+    single-interval tile equals the first slice of build_depth_warp_lut's 16
+    holding it, for reciprocals of the span within 3 ulp
+    (tests/even_slice_check.cpp)."""
+    if not Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    exe = tmp_path / "even_slice_check"
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", str(HERE / "even_slice_check.cpp"), "-o",
+                    str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.split() == ["0"], out.stdout
