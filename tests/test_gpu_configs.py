@@ -11,6 +11,9 @@
 * Serving mix on the 8 resident serving assets (10k .. 500k strands):
   requests including the 500k asset and a 3840x2160 view equal the same
   requests drawn alone.
+* More than 2^16 pages (the named asset under an 8192^2 light): the page-CSR
+  backend's hit sort then runs three radix passes, and every dump equals the
+  segment-scatter backend's, which sorts no hits.
 """
 import numpy as np
 import pytest
@@ -55,6 +58,25 @@ def test_named_frame_matches_oracle(named, backend):
     assert ctr["fine_refs"] > 30_000_000 and ctr["allocated_pages"] > 10_000
     assert ctr["page_drops"] == 0
     compare(w, ctr, gpu, ref)
+
+
+def test_many_pages_backends_agree():
+    base = scene.config_named()
+    eye = (scene.HEAD_CENTER[0], scene.HEAD_CENTER[1] + 0.05, scene.HEAD_CENTER[2] + 0.6)
+    w = scene.build_workload("scalp-100k-x32-1080p-light8k", base.assets[0], [(eye, 1920, 1080)], 1, 8192, 4)
+    out = {}
+    for backend in (1, 0):
+        print(f"[configs] 8192^2 light, backend {backend}", flush=True)
+        with StrandRenderer() as r:
+            r.upload_workload(w)
+            ctr = render_until_fits(r, scene.retile(w, backend=backend))
+            out[backend] = (ctr, {k: r.dump(k) for k in EXACT + FLOAT})
+    ctr = out[1][0]
+    print(f"[configs] pages {ctr['allocated_pages']}, refs {ctr['fine_refs']}", flush=True)
+    assert ctr["allocated_pages"] > 1 << 16 and ctr["page_drops"] == 0  # the third radix pass runs
+    assert out[0][0]["fine_refs"] == ctr["fine_refs"]
+    for k in EXACT + FLOAT:
+        assert_same(out[1][1][k], out[0][1][k], k)
 
 
 def split_cameras(w, flat, channels):
