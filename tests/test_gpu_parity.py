@@ -100,6 +100,43 @@ def cache_workload():
 CASES["cache-asset"] = cache_workload
 
 
+def edge_workload(res: int = 64, far_away: bool = False):
+    """Strands at the edges of the pipeline's arithmetic, under the identity
+    camera and light of config_two_strands (screen x = (x / 2 + 1 / 2) res,
+    y = (1 / 2 - y / 2) res, depth = z): repeated points (zero-length
+    segments), a one-point strand (dropped when packing, src/plugin.rs:1334),
+    an exactly vertical and an exactly horizontal strand (a zero direction
+    component in both DDAs), a strand pointing at the camera (segments of
+    zero screen length, depth only), one on a fine-tile edge, a diagonal
+    through tile corners, one leaving the screen and one leaving the depth
+    range.  far_away moves every strand off both views: a frame with no
+    binning task at all."""
+    px = lambda sx: 2.0 * sx / res - 1.0  # screen x (pixels) -> clip x
+    py = lambda sy: 1.0 - 2.0 * sy / res
+    strands = [
+        [(px(10.5), py(12.5), 0.7), (px(10.5), py(12.5), 0.7), (px(30.5), py(12.5), 0.7), (px(30.5), py(12.5), 0.7)],
+        [(px(40.0), py(40.0), 0.5)],
+        [(px(20.0), py(4.0), 0.6), (px(20.0), py(60.0), 0.6)],
+        [(px(4.0), py(50.0), 0.45), (px(60.0), py(50.0), 0.45)],
+        [(px(44.3), py(20.7), 0.3), (px(44.3), py(20.7), 0.9), (px(44.3), py(20.7), 0.95)],
+        [(px(16.0), py(2.0), 0.55), (px(16.0), py(30.0), 0.65)],
+        [(px(8.0), py(8.0), 0.2), (px(32.0), py(32.0), 0.6), (px(56.0), py(56.0), 0.95)],
+        [(px(-30.0), py(36.0), 0.5), (px(34.0), py(36.5), 0.5)],
+        [(px(24.0), py(44.0), -0.3), (px(40.0), py(46.0), 0.35), (px(48.0), py(47.0), 1.4)],
+    ]
+    if far_away:
+        strands = [[(x + 5.0, y, z) for (x, y, z) in st] for st in strands]
+    w = scene.config_two_strands(res=res)
+    asset = scene.asset_from_strands(strands, radius_px=1.5)
+    for i in range(asset.materials.shape[0]):
+        asset.materials["absorption_color"][i] = (0.2 + 0.08 * i, 0.5, 0.8 - 0.07 * i, 0.55)
+    return dataclasses.replace(w, name=f"edge-strands-{res}{'-far' if far_away else ''}", assets=[asset])
+
+
+CASES["edge-strands"] = lambda: edge_workload()
+CASES["empty-view"] = lambda: edge_workload(far_away=True)
+
+
 def rect_froxels(w, fx: int, fy: int):
     """The workload with every frustum's froxels fx x fy pixels."""
     cfgs = [(int(f["screen_width"]), int(f["screen_height"]), fx, fy, int(f["depth_slices"])) for f in w.frusta]
@@ -126,6 +163,8 @@ CSR_CASES = {
     "dense-one-tile-csr": lambda: csr(scene.config_tiny(strands=3000, points=8, res=32, light_res=32)),
     # a 4096^2 light: page capacity above 2^16, three radix passes over the page bits
     "wide-light-csr": lambda: csr(scene.config_tiny(res=64, light_res=4096)),
+    "edge-strands-csr": lambda: csr(edge_workload()),
+    "empty-view-csr": lambda: csr(edge_workload(far_away=True)),
 }
 
 
@@ -133,7 +172,7 @@ CSR_FLAGS = {"few-hits": abi.CONFIG_CSR_FEW_HITS, "few-hits+graphs": abi.CONFIG_
 
 
 @pytest.mark.parametrize("flags", list(CSR_FLAGS))
-@pytest.mark.parametrize("case", ["tiny-f8-csr", "small-f8-csr", "dense-one-tile-csr"])
+@pytest.mark.parametrize("case", ["tiny-f8-csr", "small-f8-csr", "dense-one-tile-csr", "edge-strands-csr"])
 def test_parity_csr_paths(case, flags):
     """Most segments through the re-trace path (2 recorded cells per
     segment), eager and replayed from a HIP graph."""
@@ -171,7 +210,9 @@ def test_parity(case):
     ref = oracle.render(w)
     ctr, gpu = gpu_frame(w)
     compare(w, ctr, gpu, ref)
-    if w.settings.get("fine_binning_backend", 0):
+    if w.name.endswith(("-far", "-far-csr")):
+        assert ctr["binning_tasks"] == 0 and ctr["fine_refs"] == 0
+    elif w.settings.get("fine_binning_backend", 0):
         assert ctr["page_candidates"] > 0
 
 
