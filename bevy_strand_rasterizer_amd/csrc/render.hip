@@ -426,7 +426,7 @@ __global__ __launch_bounds__(TPX) void k_shadow_raster(FrameParams fp) {
             float dzp = fmaxf(0.0f, z0 - pz) * inv_span;
             float u = wclamp(dzp, 0.0f, 1.0f);  // pow(u, DOM_GAMMA = 1)
             float tL = u * (float)L;
-            uint32_t si = min(f2u(floorf(tL)), L - 1u);
+            uint32_t si = min((uint32_t)floorf(tL), L - 1u);  // tL in [0, L]: u is clamped, never NaN
             float wgt = tL - floorf(tL);
             const float a = cov * b.c[k].w * 0.5f;
             const float a0 = part_px[si * TPX];
@@ -957,8 +957,7 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
         if (valid) {
           const float4 sg = b.seg[k], ge = b.geo[k];
           x0 = sg.x; y0 = sg.y; ax = sg.z; ay = sg.w;
-          const float minr = fmaxf(ge.w, 1e-4f);
-          rmx = span_radius(minr, fmaxf(b.wgt[k].x, minr), span_depth01_max(ge.y, ge.z));
+          rmx = span_radius(ge.w, b.wgt[k].x, span_depth01_max(ge.y, ge.z));  // radii clamped by the load
         }
         batch_columns<TPX>(valid, x0, y0, ax, ay, rmx, tx * fx, ty * fy, fx, fy, cols);
       }
@@ -985,14 +984,14 @@ __global__ __launch_bounds__(TPX) void k_camera_raster(FrameParams fp) {
           float qx = sg.x + sg.z * tt, qy = sg.y + sg.w * tt;
           float qz = wmix(ge.y, ge.z, tt);
           const float4 wg = b.wgt[k];
-          float minr = fmaxf(ge.w, 1e-4f);
-          float r = wmix(minr, fmaxf(wg.x, minr), wclamp(normalize_depth01(qz), 0.0f, 1.0f));
+          // minr >= 1e-4 and maxr >= minr already (load_color_ref)
+          float r = wmix(ge.w, wg.x, wclamp(normalize_depth01(qz), 0.0f, 1.0f));
           float dx = cx - qx, dy = cy - qy;
           float d2 = dx * dx + dy * dy;
-          // one predicate for the box, projection and radius tests (the
-          // records of an invalid reference are stale but never used)
-          const bool hit = !(item & 0x80000000u) && !(cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) &&
-                           (tt >= 0.0f && tt <= 1.0f) && d2 < r * r;
+          // one predicate for the box, projection and radius tests (an
+          // invalid reference has an empty column: it is never a candidate)
+          const bool hit = !(cx < bx.x || cy < bx.y || cx > bx.z || cy > bx.w) && (tt >= 0.0f && tt <= 1.0f) &&
+                           d2 < r * r;
           if (!hit) continue;
           // coverage and the perspective weight from the hardware square root
           // and reciprocal (within 1 ulp): they scale colour and alpha (1e-3
