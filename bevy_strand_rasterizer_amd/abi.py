@@ -17,6 +17,7 @@ LIB_PATH = PKG / "lib" / "libstrand_rt.so"
 
 COARSE_FINE_TILE_EXTENT = 4
 COARSE_DEPTH_SLICES = 16
+MAX_COARSE_DEPTH_SLICES = 32
 COARSE_COUNT_PAGE_SIZE = 256
 DEPTH_QUANT_MAX = 16777215
 INVALID_PTR = 0xFFFFFFFF
@@ -126,7 +127,7 @@ class FrameSettings(C.Structure):
                 ("ambient_color", C.c_float * 4), ("scene_clear_color", C.c_float * 4),
                 ("scene_clear_depth", C.c_float), ("stamp_target_width", C.c_uint32),
                 ("stamp_target_height", C.c_uint32), ("flags", C.c_uint32),
-                ("dom_resident_pages", C.c_uint32), ("pad", C.c_uint32)]
+                ("dom_resident_pages", C.c_uint32), ("coarse_depth_slices", C.c_uint32)]
 
 
 class FrameDesc(C.Structure):
@@ -169,7 +170,7 @@ class ScalpParams(C.Structure):
 # Every symbol the header declares, with its ctypes signature.
 tune_trial_dtype = np.dtype([("fine_binning_backend", "<u4"), ("camera_froxel", "<u4"), ("light_froxel", "<u4"),
                              ("hip_graphs", "<u4"), ("ms_per_frame", "<f4"), ("psnr_db", "<f4"), ("accepted", "<u4"),
-                             ("mark_waves", "<u4"), ("raster_waves", "<u4"), ("pad", "<u4")])
+                             ("mark_waves", "<u4"), ("raster_waves", "<u4"), ("coarse_depth_slices", "<u4")])
 TUNE_MAX_TRIALS = 24
 tune_result_dtype = np.dtype([("best", tune_trial_dtype), ("trial_count", "<u4"), ("pad", "<u4"),
                               ("trials", tune_trial_dtype, TUNE_MAX_TRIALS)])
@@ -186,7 +187,7 @@ request_dtype = np.dtype([("instance", strand_instance_dtype), ("camera", froxel
                                                  ("ambient_color", "<f4", 4), ("scene_clear_color", "<f4", 4),
                                                  ("scene_clear_depth", "<f4"), ("stamp_target_width", "<u4"),
                                                  ("stamp_target_height", "<u4"), ("flags", "<u4"),
-                                                 ("dom_resident_pages", "<u4"), ("pad", "<u4")])),
+                                                 ("dom_resident_pages", "<u4"), ("coarse_depth_slices", "<u4")])),
                           ("out_rgba", "<u8"), ("out_depth", "<u8"), ("tag", "<u8")], align=True)
 request_result_dtype = np.dtype([("ticket", "<u8"), ("tag", "<u8"), ("status", "<i4"), ("frame_views", "<u4"),
                                  ("submit_ms", "<f8"), ("launch_ms", "<f8"), ("done_ms", "<f8"),

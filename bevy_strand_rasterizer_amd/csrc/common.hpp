@@ -18,7 +18,8 @@
 namespace srk {
 
 constexpr uint32_t kExt = SR_COARSE_FINE_TILE_EXTENT;   // 4 fine tiles per coarse tile edge
-constexpr uint32_t kSlices = SR_COARSE_DEPTH_SLICES;    // 16 coarse and 16 fine depth slices
+constexpr uint32_t kSlices = SR_COARSE_DEPTH_SLICES;    // 16 fine depth slices per page (and default LUT slices)
+constexpr uint32_t kMaxSlices = SR_MAX_COARSE_DEPTH_SLICES;  // 32: most depth-warp LUT slices per coarse tile
 constexpr uint32_t kPage = SR_COARSE_COUNT_PAGE_SIZE;   // 256 cells per count page
 constexpr uint32_t kQuant = SR_DEPTH_QUANT_MAX;
 constexpr uint32_t kInvalid = SR_INVALID_PTR;
@@ -111,6 +112,7 @@ struct FrameParams {
   uint32_t dom_layers, page_cap;
   uint32_t mark_waves;     // waves per workgroup of the projection / marking pass (1, 2 or 4)
   uint32_t raster_waves;   // waves per CU of the persistent rasterisers (0 = 16)
+  uint32_t slices, slice_shift;  // depth-warp LUT slices per coarse tile (8, 16 or 32) and their log2
   sr_frame_settings st;
 
   uint32_t* counters;  // Counter words
@@ -128,13 +130,13 @@ struct FrameParams {
   sr_projected_segment* proj;        // [task_cap]
   uint32_t* shading_queue;           // [task_cap]
   // binning
-  sr_coarse_depth_lut_entry* lut;    // [coarse_total * 16]
-  uint2* lut_z;                      // [coarse_total * 16] (z_min_q, z_max_q) per slice; unused: (~0u, 0)
-  uint2* lut_iv;                     // [coarse_total] a tile's one depth interval spanning >= 16 quantised
-                                     // depths (its 16 slices split it evenly), else (1, 0)
-  uint32_t* page_table;              // [coarse_total * 16]
-  sr_fine_page_meta* page_meta;      // [coarse_total * 16]
-  uint32_t* page_totals;             // [coarse_total * 16]
+  sr_coarse_depth_lut_entry* lut;    // [coarse_total * slices]
+  uint2* lut_z;                      // [coarse_total * slices] (z_min_q, z_max_q) per slice; unused: (~0u, 0)
+  uint2* lut_iv;                     // [coarse_total] a tile's one depth interval spanning >= slices quantised
+                                     // depths (its slices split it evenly), else (1, 0)
+  uint32_t* page_table;              // [coarse_total * slices]
+  sr_fine_page_meta* page_meta;      // [page_cap]
+  uint32_t* page_totals;             // [page_cap]
   uint32_t* cell_counts;             // [pages * 256]
   uint32_t* cell_offsets;            // [pages * 256]
   uint32_t* cell_cursors;            // [pages * 256]
@@ -284,19 +286,20 @@ SR_HD float dequantize_depth01(uint32_t q) {
 }
 
 // The coarse depth slice of quantised depth q in a tile whose one depth
-// interval [zmin, zmax] (span = zmax - zmin + 1 >= 16) build_depth_warp_lut
-// splits into 16 slices [zmin + span * i / 16, zmin + span * (i + 1) / 16 - 1]:
-// the last i whose start is <= q (zmin <= q <= zmax).  rcp_span is 1 / span
-// within a few ulp; the float estimate of floor((16 d + 15) / span),
-// d = q - zmin, is then within one of it (relative error near 2^-22 on a
-// quotient below 16) and is settled on the exact integer starts
+// interval [zmin, zmax] (span = zmax - zmin + 1 >= S, S = 2^shift slices)
+// build_depth_warp_lut splits into S slices
+// [zmin + span * i / S, zmin + span * (i + 1) / S - 1]: the last i whose
+// start is <= q (zmin <= q <= zmax), i.e. floor((S d + S - 1) / span) for
+// d = q - zmin.  rcp_span is 1 / span within a few ulp; the float estimate
+// of that quotient is then within one of it (relative error near 2^-22 on a
+// quotient below S <= 32) and is settled on the exact integer starts
 // (tests/even_slice_check.cpp compares it with the slice scan).
-SR_HD uint32_t even_slice(uint32_t zmin, uint32_t zmax, uint32_t q, float rcp_span) {
-  const uint32_t span = zmax - zmin + 1u, d = q - zmin;
-  uint32_t c = (uint32_t)((float)(16u * d + 15u) * rcp_span);
-  c = c < kSlices - 1u ? c : kSlices - 1u;
-  if (c > 0u && ((span * c) >> 4) > d) --c;
-  else if (c + 1u < kSlices && ((span * (c + 1u)) >> 4) <= d) ++c;
+SR_HD uint32_t even_slice(uint32_t zmin, uint32_t zmax, uint32_t q, float rcp_span, uint32_t shift) {
+  const uint32_t S = 1u << shift, span = zmax - zmin + 1u, d = q - zmin;
+  uint32_t c = (uint32_t)((float)((d << shift) + S - 1u) * rcp_span);
+  c = c < S - 1u ? c : S - 1u;
+  if (c > 0u && ((span * c) >> shift) > d) --c;
+  else if (c + 1u < S && ((span * (c + 1u)) >> shift) <= d) ++c;
   return c;
 }
 

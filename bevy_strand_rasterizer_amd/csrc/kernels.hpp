@@ -41,7 +41,7 @@ struct PrepassScratch {
   sr_projected_segment* slot_proj;  // [F * slots]
   uint32_t* partials;               // scan tile partials
   uint4* partials4;                 // scan tile partials (4-wide)
-  uint32_t* page_xy_mask;           // [coarse_total * 16]
+  uint32_t* page_xy_mask;           // [page_cap]
   // page-CSR hit sort: the two (key, slot) buffer pairs (pair 0 holds the
   // hits as written and, after the last pass, the sorted keys), digit
   // histograms, block and pass counts

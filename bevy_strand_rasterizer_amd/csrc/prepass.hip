@@ -202,9 +202,10 @@ __device__ bool project_slot(const FrameParams& fp, uint32_t fi, uint32_t pos, u
 
 // ------------------------------------------------------- K7 + K8 LUT
 // Interval merge of build_depth_warp_lut, fed one interval at a time in
-// linked-list walk order.
+// linked-list walk order, for S slices per coarse tile.
+template <uint32_t S>
 struct LutBuilder {
-  uint32_t mins[kSlices], maxs[kSlices];
+  uint32_t mins[S], maxs[S];
   uint32_t count;
   __device__ void add(uint32_t a, uint32_t b) {
     uint32_t zmin = min(a, b), zmax = max(a, b);
@@ -217,12 +218,12 @@ struct LutBuilder {
       }
     }
     if (merged) return;
-    if (count < kSlices) {
+    if (count < S) {
       mins[count] = zmin; maxs[count] = zmax; ++count;
       return;
     }
     uint32_t best = 0, best_gap = 0xFFFFFFFFu;
-    for (uint32_t i = 0; i < kSlices; ++i) {
+    for (uint32_t i = 0; i < S; ++i) {
       uint32_t gap = maxs[i] < zmin ? zmin - maxs[i] : (zmax < mins[i] ? mins[i] - zmax : 0u);
       if (gap < best_gap) { best_gap = gap; best = i; }
     }
@@ -231,7 +232,7 @@ struct LutBuilder {
   }
   __device__ void finish(sr_coarse_depth_lut_entry* out) {
     if (count == 0u) {
-      for (uint32_t i = 0; i < kSlices; ++i) out[i] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+      for (uint32_t i = 0; i < S; ++i) out[i] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
       return;
     }
     for (uint32_t i = 1; i < count; ++i) {
@@ -241,17 +242,17 @@ struct LutBuilder {
     }
     uint32_t out_slice = 0;
     for (uint32_t iv = 0; iv < count; ++iv) {
-      uint32_t slice_count = max(1u, kSlices / count) + (iv < kSlices % count ? 1u : 0u);
+      uint32_t slice_count = max(1u, S / count) + (iv < S % count ? 1u : 0u);
       uint32_t zmin = mins[iv], zmax = maxs[iv];
       uint32_t span = max(1u, zmax - zmin + 1u);
-      for (uint32_t ls = 0; ls < slice_count && out_slice < kSlices; ++ls) {
+      for (uint32_t ls = 0; ls < slice_count && out_slice < S; ++ls) {
         uint32_t start = zmin + (span * ls) / slice_count;
         uint32_t ex_end = zmin + (span * (ls + 1u)) / slice_count;
         uint32_t end = min(ex_end > start ? ex_end - 1u : start, zmax);
         out[out_slice++] = sr_coarse_depth_lut_entry{start, end, start, max(1u, end - start + 1u)};
       }
     }
-    while (out_slice < kSlices) out[out_slice++] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+    while (out_slice < S) out[out_slice++] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
   }
 };
 
@@ -265,15 +266,16 @@ __device__ __forceinline__ uint32_t frustum_for_coarse_tile(const FrameParams& f
 
 // One thread per coarse tile: walk the intervals of every asset range that
 // covers it (newest-first over (instance, frustum) insertion order), build
-// the 16-slice depth-warp LUT and clear the tile's page-table and active
+// the S-slice depth-warp LUT and clear the tile's page-table and active
 // words for this frame.
+template <uint32_t S>
 __global__ void k_lut(FrameParams fp) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fp.coarse_total) return;
-  for (uint32_t z = 0; z < kSlices; ++z) fp.page_table[t * kSlices + z] = 0u;
+  for (uint32_t z = 0; z < S; ++z) fp.page_table[t * S + z] = 0u;
   for (uint32_t z = 0; z < kExt * kExt; ++z) fp.active_flags[t * kExt * kExt + z] = 0u;
   uint32_t fi = frustum_for_coarse_tile(fp, t);
-  LutBuilder b;
+  LutBuilder<S> b;
   b.count = 0;
   if (fi != kInvalid) {
     const sr_frustum_desc& d = fp.fr[fi].d;
@@ -286,32 +288,32 @@ __global__ void k_lut(FrameParams fp) {
       b.add(r.z_min_q, r.z_max_q);
     }
   }
-  sr_coarse_depth_lut_entry out[kSlices];
+  sr_coarse_depth_lut_entry out[S];
   b.finish(out);
-  for (uint32_t z = 0; z < kSlices; ++z) {
-    fp.lut[(size_t)t * kSlices + z] = out[z];
-    fp.lut_z[(size_t)t * kSlices + z] =
+  for (uint32_t z = 0; z < S; ++z) {
+    fp.lut[(size_t)t * S + z] = out[z];
+    fp.lut_z[(size_t)t * S + z] =
         out[z].virtual_count_q ? make_uint2(out[z].z_min_q, out[z].z_max_q) : make_uint2(~0u, 0u);
   }
-  // one interval of at least 16 depths: its slices are 16 contiguous ranges
-  // [zmin + span * i / 16, zmin + span * (i + 1) / 16 - 1] (finish above),
+  // one interval of at least S depths: its slices are S contiguous ranges
+  // [zmin + span * i / S, zmin + span * (i + 1) / S - 1] (finish above),
   // which the marking pass finds arithmetically
-  const bool single = b.count == 1u && b.maxs[0] - b.mins[0] + 1u >= kSlices;
+  const bool single = b.count == 1u && b.maxs[0] - b.mins[0] + 1u >= S;
   fp.lut_iv[t] = single ? make_uint2(b.mins[0], b.maxs[0]) : make_uint2(1u, 0u);
 }
 
 // ------------------------------------------------ binning traversal
 // coarse_depth_slice_for_depth: the first used slice whose range holds the
 // quantised depth, else the first nearest one.  Both endpoint depths are
-// looked up in one pass over the tile's 16 (z_min_q, z_max_q) pairs, two
+// looked up in one pass over the tile's S (z_min_q, z_max_q) pairs, two
 // pairs per load (a used slice has z_min_q <= z_max_q; unused ones are
 // (~0, 0)).  The LUT slice of a quantised depth: the first used slice whose
 // range holds it, else the used slice nearest to it (first on ties), else
 // invalid.  The holding slices come from one mask per depth; the
 // nearest-slice scan runs only for a depth no slice holds.
-__device__ uint32_t nearest_slice(const uint4* l4, uint32_t zq) {
+__device__ uint32_t nearest_slice(const uint4* l4, uint32_t zq, uint32_t S) {
   uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
-  for (uint32_t i = 0; i < kSlices; i += 2) {
+  for (uint32_t i = 0; i < S; i += 2) {
     const uint4 q = l4[i / 2];
 #pragma unroll
     for (uint32_t h = 0; h < 2; ++h) {
@@ -328,24 +330,24 @@ __device__ uint32_t nearest_slice(const uint4* l4, uint32_t zq) {
 }
 
 __device__ __forceinline__ void lut_slices_for_depths(const uint2* lz, uint2 iv, float za, float zb, uint32_t* sa,
-                                                      uint32_t* sb) {
+                                                      uint32_t* sb, uint32_t S, uint32_t shift) {
   const uint32_t qa = quantize_depth01(za), qb = quantize_depth01(zb);
   if (qa >= iv.x && qa <= iv.y && qb >= iv.x && qb <= iv.y) {  // both inside one evenly split interval
     const float r = __builtin_amdgcn_rcpf((float)(iv.y - iv.x + 1u));  // within 1 ulp
-    *sa = even_slice(iv.x, iv.y, qa, r);
-    *sb = even_slice(iv.x, iv.y, qb, r);
+    *sa = even_slice(iv.x, iv.y, qa, r, shift);
+    *sb = even_slice(iv.x, iv.y, qb, r, shift);
     return;
   }
   const uint4* l4 = reinterpret_cast<const uint4*>(lz);
-  uint32_t ma = 0u, mb = 0u;  // slices holding qa / qb (a used slice has x <= y)
+  uint32_t ma = 0u, mb = 0u;  // slices holding qa / qb (a used slice has x <= y; S <= 32 bits)
 #pragma unroll 2
-  for (uint32_t i = 0; i < kSlices; i += 2) {
+  for (uint32_t i = 0; i < S; i += 2) {
     const uint4 q = l4[i / 2];
     ma |= ((qa >= q.x && qa <= q.y) ? (1u << i) : 0u) | ((qa >= q.z && qa <= q.w) ? (2u << i) : 0u);
     mb |= ((qb >= q.x && qb <= q.y) ? (1u << i) : 0u) | ((qb >= q.z && qb <= q.w) ? (2u << i) : 0u);
   }
-  *sa = ma ? (uint32_t)(__ffs(ma) - 1) : nearest_slice(l4, qa);
-  *sb = mb ? (uint32_t)(__ffs(mb) - 1) : nearest_slice(l4, qb);
+  *sa = ma ? (uint32_t)(__ffs(ma) - 1) : nearest_slice(l4, qa, S);
+  *sb = mb ? (uint32_t)(__ffs(mb) - 1) : nearest_slice(l4, qb, S);
 }
 
 enum { MODE_MARK = 0, MODE_COUNT = 1, MODE_FILL = 2, MODE_SCATTER = 3 };
@@ -499,9 +501,9 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
   coarse_tile_box(fd, cx, cy, &origin, &bmax);
   Clipped c = clip_segment_dir(p0, p1, sd, V3{origin.x, origin.y, 0.0f}, bmax);
   if (!c.ok) return;
-  const uint2* lz = fp.lut_z + (size_t)tile * kSlices;
+  const uint2* lz = fp.lut_z + (size_t)tile * fp.slices;
   uint32_t z0s, z1s;
-  lut_slices_for_depths(lz, fp.lut_iv[tile], c.p0.z, c.p1.z, &z0s, &z1s);
+  lut_slices_for_depths(lz, fp.lut_iv[tile], c.p0.z, c.p1.z, &z0s, &z1s, fp.slices, fp.slice_shift);
   if (z0s == kInvalid || z1s == kInvalid) return;
   const SegDir cd = seg_dir(c.p0, c.p1);  // the tile's piece, clipped against each slice
   uint32_t zlo = min(z0s, z1s), zhi = max(z0s, z1s);
@@ -511,7 +513,7 @@ __device__ void trace_coarse_tile(const FrameParams& fp, V3 p0, V3 p1, const Seg
     float zmin = dequantize_depth01(e.x), zmax = dequantize_depth01(e.y);
     Clipped dc = clip_segment_dir(c.p0, c.p1, cd, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
     if (!dc.ok) continue;
-    uint32_t pti = tile * kSlices + cz;
+    uint32_t pti = tile * fp.slices + cz;
     if (MODE == MODE_MARK) {
       fp.page_table[pti] = kMarked;
       if (fp.st.fine_binning_backend != 0u) {  // the visible cells: recorded (up to cap) and counted
@@ -839,7 +841,7 @@ __global__ __launch_bounds__(256) void k_project_write(FrameParams fp, const sr_
       uint32_t nh = kHitOverflow, base = 0u, mat = 0u;
       uint32_t hv[kPre], he[kPre];
       const uint32_t* h = fp.slot_hits + s;
-      const uint32_t pt_last = fp.coarse_total * kSlices - 1u;
+      const uint32_t pt_last = fp.coarse_total * fp.slices - 1u;
       if (csr) {
         nh = fp.hit_k ? fp.slot_hit_count[s] : kHitOverflow;
         base = fp.hit_base[s];
@@ -899,6 +901,7 @@ struct PageAllocDst {
   sr_fine_page_meta* meta;
   uint32_t* counters;
   uint32_t page_cap;
+  uint32_t slice_shift;  // page-table entry e is slice e % 2^shift of coarse tile e >> shift
   __device__ void store(uint32_t e, uint32_t ex, uint32_t v) const {
     if (!v) return;
     if (ex >= page_cap) {  // the reference drops pages past its capacity (entry 0)
@@ -907,7 +910,7 @@ struct PageAllocDst {
       return;
     }
     table[e] = ex + 1u;
-    meta[ex] = sr_fine_page_meta{0u, 0u, 0u, 0u, e / kSlices, e % kSlices, 0u, 0u};
+    meta[ex] = sr_fine_page_meta{0u, 0u, 0u, 0u, e >> slice_shift, e & ((1u << slice_shift) - 1u), 0u, 0u};
   }
   __device__ void total(uint32_t t) const {
     counters[C_MARKED] = t;
@@ -1549,17 +1552,18 @@ struct TileLane {
   uint32_t count, base, page, cell, pti, fz;
 };
 
-// A fine tile's 256 depth lanes, four per thread of a wave: lane
-// l = 4 * thread + i serves depth lane 255 - l, so lanes run near (large
-// keys) to far.  A thread's four depth lanes share one page (a page holds
-// the 16 fine slices of a LUT slice), so its page-table and page-meta words
-// are read once and the four cells' counts and offsets are read together.
-constexpr uint32_t kEmitPer = kPage / 64u;
-
+// A fine tile's 16 S depth lanes (S LUT slices of 16 fine slices), PER =
+// S / 4 per thread of a wave: lane l = PER * thread + i serves depth lane
+// 16 S - 1 - l, so lanes run near (large keys) to far.  A thread's PER
+// depth lanes share one page (a page holds the 16 fine slices of a LUT
+// slice), so its page-table and page-meta words are read once and the
+// cells' counts and offsets are read together.
+template <uint32_t PER>
 __device__ __forceinline__ void tile_lanes(const FrameParams& fp, uint32_t tile, uint32_t lx, uint32_t ly,
-                                           TileLane (&tl)[kEmitPer]) {
-  const uint32_t dl0 = (kPage - 1u) - threadIdx.x * kEmitPer;
-  const uint32_t pti = tile * kSlices + dl0 / kSlices;
+                                           TileLane (&tl)[PER]) {
+  constexpr uint32_t S = 4u * PER;
+  const uint32_t dl0 = (S * kSlices - 1u) - threadIdx.x * PER;
+  const uint32_t pti = tile * S + dl0 / kSlices;
   const uint32_t h = fp.page_table[pti];
   const bool paged = h != 0u && h != kMarked;
   const uint32_t page = paged ? h - 1u : 0u;
@@ -1570,31 +1574,32 @@ __device__ __forceinline__ void tile_lanes(const FrameParams& fp, uint32_t tile,
     ref_base = pm.seg_ref_base;
     ok = pm.seg_ref_count > 0u && pm.flags == 0u;
   }
-  uint32_t n[kEmitPer], off[kEmitPer];
+  uint32_t n[PER], off[PER];
 #pragma unroll
-  for (uint32_t i = 0; i < kEmitPer; ++i) {
+  for (uint32_t i = 0; i < PER; ++i) {
     const uint32_t fz = (dl0 - i) % kSlices, cell = (fz * kExt + ly) * kExt + lx;
     tl[i] = TileLane{0u, 0u, paged ? page : kInvalid, paged ? cell : 0u, pti, fz};
     n[i] = ok ? fp.cell_counts[page * kPage + cell] : 0u;
     off[i] = ok ? fp.cell_offsets[page * kPage + cell] : 0u;
   }
 #pragma unroll
-  for (uint32_t i = 0; i < kEmitPer; ++i) {
+  for (uint32_t i = 0; i < PER; ++i) {
     tl[i].count = n[i];
     tl[i].base = ref_base + off[i];
   }
 }
 
+template <uint32_t PER>
 __global__ __launch_bounds__(64) void k_emit_count(FrameParams fp) {
   const uint32_t A = fp.counters[C_ACTIVE];
   for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
     uint32_t at = fp.active_queue[a];
     uint32_t tile = at / (kExt * kExt), lf = at % (kExt * kExt);
-    TileLane tl[kEmitPer];
-    tile_lanes(fp, tile, lf % kExt, lf / kExt, tl);
+    TileLane tl[PER];
+    tile_lanes<PER>(fp, tile, lf % kExt, lf / kExt, tl);
     uint32_t works = 0u, load = 0u;
 #pragma unroll
-    for (uint32_t i = 0; i < kEmitPer; ++i) {
+    for (uint32_t i = 0; i < PER; ++i) {
       works += tl[i].count ? 1u : 0u;
       load += tl[i].count;
     }
@@ -1624,6 +1629,7 @@ struct TileBaseDst {
   }
 };
 
+template <uint32_t PER>
 __global__ __launch_bounds__(64) void k_emit_write(FrameParams fp) {
   const uint32_t A = fp.counters[C_ACTIVE];
   for (uint32_t a = blockIdx.x; a < A; a += gridDim.x) {
@@ -1638,15 +1644,15 @@ __global__ __launch_bounds__(64) void k_emit_write(FrameParams fp) {
     uint32_t lc = tile - fr.d.coarse_depth_tile_base;
     uint32_t cx = lc % fr.d.coarse_tiles_x, cy = lc / fr.d.coarse_tiles_x;
     uint32_t screen_tile = (cy * kExt + ly) * fr.fine_tiles_x + (cx * kExt + lx);
-    TileLane tl[kEmitPer];
-    tile_lanes(fp, tile, lx, ly, tl);
+    TileLane tl[PER];
+    tile_lanes<PER>(fp, tile, lx, ly, tl);
     uint32_t mine = 0u;
 #pragma unroll
-    for (uint32_t i = 0; i < kEmitPer; ++i) mine += tl[i].count ? 1u : 0u;
+    for (uint32_t i = 0; i < PER; ++i) mine += tl[i].count ? 1u : 0u;
     uint32_t w = tb.x + wave_inclusive<uint32_t>(mine) - mine;  // lane order
     const sr_coarse_depth_lut_entry e = fp.lut[tl[0].pti];      // the thread's one page slice
 #pragma unroll
-    for (uint32_t i = 0; i < kEmitPer; ++i) {
+    for (uint32_t i = 0; i < PER; ++i) {
       if (!tl[i].count) continue;
       if (w < fp.work_cap)
         fp.work[w] = sr_raster_work_item{tl[i].base, tl[i].count, fi, screen_tile,
@@ -1865,10 +1871,21 @@ __global__ __launch_bounds__(256) void k_telemetry(FrameParams fp) {
   }
 }
 
+// Raster work emission: one wave per active fine tile, PER = S / 4 depth
+// lanes per thread, then the run bases, then the work items.
+template <uint32_t PER>
+static void launch_emit(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, uint32_t max_grid) {
+  const dim3 g(grid_for(fp.coarse_total * kExt * kExt, 1, max_grid));
+  hipLaunchKernelGGL(k_emit_count<PER>, g, dim3(64), 0, s, fp);
+  launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
+                  fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
+  hipLaunchKernelGGL(k_emit_write<PER>, g, dim3(64), 0, s, fp);
+}
+
 void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t s, StageTimer* timer) {
   const uint32_t kMaxGrid = 256u * 16u;
   const uint64_t slots_total = (uint64_t)fp.F * fp.slots;
-  const uint32_t ptab = fp.coarse_total * kSlices;
+  const uint32_t ptab = fp.coarse_total * fp.slices;
   const bool csr = fp.st.fine_binning_backend != 0u;
 
   if (fp.opaque_tiles) {
@@ -1878,7 +1895,12 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   timer->mark(s, "broad");
   hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 64, 1024)), dim3(64), 0, s, fp);
   timer->mark(s, "depth_lut");
-  hipLaunchKernelGGL(k_lut, dim3(grid_for(fp.coarse_total, 64, 1u << 20)), dim3(64), 0, s, fp);
+  {
+    const dim3 g(grid_for(fp.coarse_total, 64, 1u << 20));
+    if (fp.slices == 8u) hipLaunchKernelGGL(k_lut<8>, g, dim3(64), 0, s, fp);
+    else if (fp.slices == 32u) hipLaunchKernelGGL(k_lut<32>, g, dim3(64), 0, s, fp);
+    else hipLaunchKernelGGL(k_lut<16>, g, dim3(64), 0, s, fp);
+  }
   timer->mark(s, "project_mark");
   // one to four waves per workgroup, the same number of threads in flight
   switch (fp.mark_waves) {
@@ -1899,7 +1921,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
                         (uint32_t)(slots_total / 64u), nullptr, ps.partials, s);
   hipLaunchKernelGGL(k_frustum_bases, dim3(1), dim3(64), 0, s, fp);
   timer->mark(s, "allocate");
-  launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap},
+  launch_scan<uint32_t>(PageMarkSrc{fp.page_table}, PageAllocDst{fp.page_table, fp.page_meta, fp.counters, fp.page_cap, fp.slice_shift},
                         ptab, nullptr, ps.partials, s);
   if (csr) {
     timer->mark(s, "hit_offsets");
@@ -1959,10 +1981,9 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
   launch_scan<uint32_t>(FlagSrc{fp.active_flags}, ActiveDst{fp.active_queue, fp.counters},
                         fp.coarse_total * kExt * kExt, nullptr, ps.partials, s);
   timer->mark(s, "emit");
-  hipLaunchKernelGGL(k_emit_count, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, 4u * kMaxGrid)), dim3(64), 0, s, fp);
-  launch_scan<U4>(TileWorkSrc{fp.tile_work}, TileBaseDst{fp.tile_bases, fp.counters}, fp.coarse_total * kExt * kExt,
-                  fp.counters + C_ACTIVE, reinterpret_cast<U4*>(ps.partials4), s);
-  hipLaunchKernelGGL(k_emit_write, dim3(grid_for(fp.coarse_total * kExt * kExt, 1, 4u * kMaxGrid)), dim3(64), 0, s, fp);
+  if (fp.slices == 8u) launch_emit<2>(fp, ps, s, 4u * kMaxGrid);
+  else if (fp.slices == 32u) launch_emit<8>(fp, ps, s, 4u * kMaxGrid);
+  else launch_emit<4>(fp, ps, s, 4u * kMaxGrid);
   hipLaunchKernelGGL(k_run_hist, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
   hipLaunchKernelGGL(k_run_scatter, dim3(2 * kOrderBlocks), dim3(256), 0, s, fp);
   if (fp.tel) {

@@ -71,13 +71,15 @@ __device__ __forceinline__ V3 world_v3(const InstRec& in, uint32_t vi) {
 // ------------------------------------------------ tile-run work tables
 // A workgroup owns one tile run at a time, taken from a device queue head
 // (dynamic balance: run loads vary by orders of magnitude).  The run's work
-// items (<= 256 cells, near to far) are staged as an exclusive prefix of
-// their reference counts, so reference batches of TPX entries span work-item
+// items (one per non-empty cell of the fine tile's 16 x S depth lanes: at
+// most 512, near to far) are staged as an exclusive prefix of their
+// reference counts, so reference batches of TPX entries span work-item
 // boundaries: one barrier pair per TPX references instead of per cell.
+constexpr uint32_t kRunItems = 16u * kMaxSlices;
 template <int TPX>
 struct RunTable {
-  uint32_t pref[257];  // exclusive prefix of counts; pref[n] = total
-  uint32_t base[256];  // seg_ref_base per work item
+  uint32_t pref[kRunItems + 1];  // exclusive prefix of counts; pref[n] = total
+  uint32_t base[kRunItems];      // seg_ref_base per work item
   uint32_t n;
   uint32_t run;
 };
@@ -100,11 +102,12 @@ __device__ bool next_run(uint32_t* head, uint32_t count, const uint32_t* order, 
 template <int TPX, bool CHECK>
 __device__ void stage_run(const FrameParams& fp, const sr_raster_tile_run& run, uint32_t fi, RunTable<TPX>& t) {
   __shared__ uint32_t lds4[4];
-  const uint32_t n = min(run.work_count, 256u);
-  uint32_t cnt[256 / TPX];
+  const uint32_t n = min(run.work_count, kRunItems);
+  constexpr uint32_t kPer = kRunItems / TPX;
+  uint32_t cnt[kPer];
 #pragma unroll
-  for (int k = 0; k < 256 / TPX; ++k) {
-    uint32_t w = threadIdx.x * (256 / TPX) + k;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    uint32_t w = threadIdx.x * kPer + k;
     cnt[k] = 0u;
     if (w < n) {
       const sr_raster_work_item wi = fp.work[run.work_base + w];
@@ -115,7 +118,7 @@ __device__ void stage_run(const FrameParams& fp, const sr_raster_tile_run& run, 
   }
   uint32_t local = 0;
 #pragma unroll
-  for (int k = 0; k < 256 / TPX; ++k) local += cnt[k];
+  for (uint32_t k = 0; k < kPer; ++k) local += cnt[k];
   // exclusive scan of per-thread sums across the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t inc = local;
@@ -133,8 +136,8 @@ __device__ void stage_run(const FrameParams& fp, const sr_raster_tile_run& run, 
   }
   uint32_t ex = inc - local;
 #pragma unroll
-  for (int k = 0; k < 256 / TPX; ++k) {
-    uint32_t w = threadIdx.x * (256 / TPX) + k;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    uint32_t w = threadIdx.x * kPer + k;
     if (w <= n) t.pref[w] = ex;
     ex += cnt[k];
   }

@@ -180,7 +180,7 @@ enum BufId {
 // Shape of the last submitted frame, for dumps.
 struct FrameShape {
   bool valid = false;
-  uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0;
+  uint32_t F = 0, NI = 0, NL = 0, cams = 0, slots = 0, coarse_total = 0, slices = 16;
   uint32_t task_cap = 0, ref_cap = 0, work_cap = 0, page_cap = 0, hit_cap = 0, atlas_texels = 0, dom_layers = 0;
   uint64_t dom_px = 0, img_px = 0, stamp_px = 0, tel_words = 0, opaque_tiles = 0;
   int hist_cur = 0;
@@ -579,6 +579,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   if (st.dom_layers == 0 || st.dom_layers > SR_MAX_DOM_LAYERS)
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: dom_layers must be in 1..16");
   if (st.fine_binning_backend > 1) return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown binning backend");
+  if (!(st.coarse_depth_slices == 0u || st.coarse_depth_slices == 8u || st.coarse_depth_slices == 16u ||
+        st.coarse_depth_slices == 32u))
+    return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: coarse_depth_slices must be 8, 16 or 32 (0 = 16)");
+  const uint32_t slices = st.coarse_depth_slices ? st.coarse_depth_slices : kSlices;
   if (st.flags & ~(SR_FRAME_REUSE_LIGHT_DOMS | SR_FRAME_RGBA8_OUTPUTS | SR_FRAME_VIRTUAL_DOMS |
                    SR_FRAME_DEPTH_SORTED_BLEND | SR_FRAME_TUNE_KEEP_BACKEND))
     return efail(err, SR_ERR_INVALID_ARGUMENT, "sr_frame_submit: unknown frame flags");
@@ -740,7 +744,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   const uint32_t slots = (uint32_t)align_up(std::max<uint64_t>(pos, 64), 64);
   if ((uint64_t)F * slots > 0x7FFFFFFFull) return efail(err, SR_ERR_CAPACITY, "frame exceeds 2^31 segment slots");
   const uint32_t task_cap = F * slots;
-  const uint32_t ptab = coarse_total * kSlices;
+  const uint32_t ptab = coarse_total * slices;
   const bool tight = (ctx->flags & SR_CONFIG_TIGHT_CAPACITY) != 0u;
   uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, tight ? 16u : std::max(4096u, ptab / 2)),
                                          sl.page_cap_hint);
@@ -927,6 +931,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   fp.dom_layers = st.dom_layers;
   fp.mark_waves = fd->mark_waves ? fd->mark_waves : 4u;
   fp.raster_waves = fd->raster_waves;  // 0: the kernel's default
+  fp.slices = slices;
+  fp.slice_shift = slices == 8u ? 3u : (slices == 32u ? 5u : 4u);
   fp.st = st;
   fp.st.shadow_history_valid = hist_valid ? 1u : 0u;
   fp.counters = sl.buf[B_COUNTERS].as<uint32_t>();
@@ -1121,6 +1127,7 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   FrameShape& sh = sl.shape;
   sh.valid = true;
   sh.F = F; sh.NI = NI; sh.NL = NL; sh.cams = cams; sh.slots = slots; sh.coarse_total = coarse_total;
+  sh.slices = slices;
   sh.task_cap = task_cap; sh.ref_cap = ref_cap; sh.work_cap = work_cap; sh.page_cap = page_cap;
   sh.hit_cap = hit_cap;
   sh.tel_words = tel_words;
@@ -1224,8 +1231,8 @@ int sr_frame_dump(sr_context* ctx, int32_t kind, void* dst, uint64_t capacity, u
     case SR_DUMP_TASKS: src = buf[B_TASKS].p; bytes = 16ull * std::min(fc.binning_tasks, sh.task_cap); break;
     case SR_DUMP_PROJECTED: src = buf[B_PROJ].p; bytes = 12ull * std::min(fc.binning_tasks, sh.task_cap); break;
     case SR_DUMP_SHADING_QUEUE: src = buf[B_SHADING_Q].p; bytes = 4ull * fc.camera_tasks; break;
-    case SR_DUMP_LUT: src = buf[B_LUT].p; bytes = 16ull * sh.coarse_total * kSlices; break;
-    case SR_DUMP_PAGE_TABLE: src = buf[B_PAGE_TABLE].p; bytes = 4ull * sh.coarse_total * kSlices; break;
+    case SR_DUMP_LUT: src = buf[B_LUT].p; bytes = 16ull * sh.coarse_total * sh.slices; break;
+    case SR_DUMP_PAGE_TABLE: src = buf[B_PAGE_TABLE].p; bytes = 4ull * sh.coarse_total * sh.slices; break;
     case SR_DUMP_PAGE_META: src = buf[B_PAGE_META].p; bytes = 32ull * P; break;
     case SR_DUMP_CELL_COUNTS: src = buf[B_CELL_COUNTS].p; bytes = 4ull * P * kPage; break;
     case SR_DUMP_CELL_OFFSETS: src = buf[B_CELL_OFFSETS].p; bytes = 4ull * P * kPage; break;
@@ -1812,11 +1819,12 @@ int sr_server_stop(sr_server* srv) {
 // with the launch sequence in a HIP graph, against the frame as given
 // without graphs (the quality reference): fine-binning backend x camera
 // froxel size x light froxel size, then for the fastest accepted of those
-// the launch shape (marking waves per workgroup x raster waves per CU).  A
-// candidate is accepted when its composite is within min_psnr_db of the
-// reference (tile size changes the order in which overlapping fragments
-// blend; the launch shape changes no result); the fastest accepted one is
-// `best`.
+// the other depth-warp slice counts (8, 16, 32 per coarse tile), then for the
+// fastest accepted so far the launch shape (marking waves per workgroup x
+// raster waves per CU).  A candidate is accepted when its composite is
+// within min_psnr_db of the reference (tile size and slice count change the
+// order in which overlapping fragments blend; the launch shape changes no
+// result); the fastest accepted one is `best`.
 namespace {
 
 struct TuneFrame {
@@ -1824,8 +1832,8 @@ struct TuneFrame {
   sr_frame_desc fd;
 };
 
-int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint32_t light_fx, uint32_t mark_waves,
-               uint32_t raster_waves, TuneFrame& out) {
+int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint32_t light_fx, uint32_t slices,
+               uint32_t mark_waves, uint32_t raster_waves, TuneFrame& out) {
   const uint32_t F = src->frustum_count;
   std::vector<sr_froxel_config> cfg(F);
   std::vector<uint32_t> is_light(F), cascade(F);
@@ -1843,6 +1851,7 @@ int tune_frame(const sr_frame_desc* src, uint32_t backend, uint32_t cam_fx, uint
   out.fd = *src;
   out.fd.frusta = out.frusta.data();
   out.fd.settings.fine_binning_backend = backend;
+  out.fd.settings.coarse_depth_slices = slices;
   out.fd.mark_waves = mark_waves;
   out.fd.raster_waves = raster_waves;
   return SR_OK;
@@ -1864,17 +1873,19 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
     if (fd->frusta[i].kind == 0u && !cam0) cam0 = fd->frusta[i].froxel_size_x;
     if (fd->frusta[i].kind == 1u && !light0) light0 = fd->frusta[i].froxel_size_x;
   }
-  struct Cand { uint32_t backend, cam, light, graphs, mark_waves, raster_waves; };
+  struct Cand { uint32_t backend, cam, light, slices, graphs, mark_waves, raster_waves; };
   std::vector<Cand> cands;
-  cands.push_back({fd->settings.fine_binning_backend, cam0, light0, 0u, fd->mark_waves, fd->raster_waves});  // the reference
+  const uint32_t slices0 = fd->settings.coarse_depth_slices ? fd->settings.coarse_depth_slices : kSlices;
+  cands.push_back({fd->settings.fine_binning_backend, cam0, light0, slices0, 0u, fd->mark_waves,
+                   fd->raster_waves});  // the reference
   const bool keep_backend = (fd->settings.flags & SR_FRAME_TUNE_KEEP_BACKEND) != 0u;
   for (uint32_t b : {0u, 1u}) {
     if (keep_backend && b != fd->settings.fine_binning_backend) continue;
     for (uint32_t cf : {8u, 16u})
-      for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, 1u, 0u, 0u});
+      for (uint32_t lf : {8u, 16u}) cands.push_back({b, cam0 ? cf : 0u, light0 ? lf : 0u, slices0, 1u, 0u, 0u});
   }
-  const size_t stage1 = cands.size();
-  bool stage2 = false;
+  size_t stage_end = cands.size();  // the next stage's candidates follow the best of those before
+  int stage = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<float4> ref_img, img;
   int rc = SR_OK;
@@ -1883,18 +1894,26 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
   const uint32_t frames = frames_per_trial ? frames_per_trial : 3;
   float best_ms = 0.0f;
   for (size_t c = 0; rc == SR_OK && out->trial_count < SR_TUNE_MAX_TRIALS; ++c) {
-    if (c == stage1 && !stage2 && best_ms > 0.0f) {  // launch shapes of the best configuration so far
-      stage2 = true;
+    if (c == stage_end && stage < 3 && best_ms > 0.0f) {
       const sr_tune_trial b = out->best;
-      for (uint32_t mw : {1u, 2u, 4u})
-        for (uint32_t rw : {8u, 16u, 24u})
-          if (!(mw == 4u && rw == 16u))  // the defaults, timed in the first stage
-            cands.push_back({b.fine_binning_backend, b.camera_froxel, b.light_froxel, 1u, mw, rw});
+      if (stage == 1) {  // the other depth-warp slice counts of the best configuration so far
+        for (uint32_t sc : {8u, 16u, 32u})
+          if (sc != b.coarse_depth_slices)
+            cands.push_back({b.fine_binning_backend, b.camera_froxel, b.light_froxel, sc, 1u, 0u, 0u});
+      } else {  // launch shapes of the best configuration so far
+        for (uint32_t mw : {1u, 2u, 4u})
+          for (uint32_t rw : {8u, 16u, 24u})
+            if (!(mw == 4u && rw == 16u))  // the defaults, timed in the earlier stages
+              cands.push_back({b.fine_binning_backend, b.camera_froxel, b.light_froxel, b.coarse_depth_slices, 1u, mw,
+                               rw});
+      }
+      ++stage;
+      stage_end = cands.size();
     }
     if (c >= cands.size()) break;
     const Cand k = cands[c];
     TuneFrame tf;
-    rc = tune_frame(fd, k.backend, k.cam, k.light, k.mark_waves, k.raster_waves, tf);
+    rc = tune_frame(fd, k.backend, k.cam, k.light, k.slices, k.mark_waves, k.raster_waves, tf);
     if (rc != SR_OK) break;
     ctx->flags = k.graphs ? (saved_flags | SR_CONFIG_HIP_GRAPHS) : (saved_flags & ~SR_CONFIG_HIP_GRAPHS);
     sr_frame_counters fc;
@@ -1936,6 +1955,7 @@ extern "C" int sr_autotune(sr_context* ctx, const sr_frame_desc* fd, uint32_t fr
     t.hip_graphs = k.graphs;
     t.mark_waves = k.mark_waves;
     t.raster_waves = k.raster_waves;
+    t.coarse_depth_slices = k.slices;
     t.ms_per_frame = ms / (float)frames;
     t.psnr_db = mse > 0.0 ? (float)(10.0 * std::log10(1.0 / mse)) : 999.0f;
     t.accepted = t.psnr_db >= min_psnr_db ? 1u : 0u;

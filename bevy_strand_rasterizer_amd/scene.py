@@ -90,6 +90,7 @@ class Workload:
         st.stamp_target_height = int(d["stamp_target_height"])
         st.flags = int(d["frame_flags"])
         st.dom_resident_pages = int(d["dom_resident_pages"])
+        st.coarse_depth_slices = int(d.get("coarse_depth_slices", 0))
         return st
 
     def frame_desc(self, keep: list | None = None) -> abi.FrameDesc:
@@ -134,6 +135,7 @@ def default_settings() -> dict:
         "stamp_target_height": 0,
         "frame_flags": 0,  # SR_FRAME_* bits
         "dom_resident_pages": 0,  # SR_FRAME_VIRTUAL_DOMS: resident pages per light, 0 = all requested
+        "coarse_depth_slices": 0,  # LUT slices per coarse tile: 8, 16 or 32 (0 = 16)
     }
 
 
@@ -377,9 +379,10 @@ def retile(w: Workload, camera_froxel: int = 0, light_froxel: int = 0, backend: 
 
 def apply_tuning(w: Workload, best: dict) -> Workload:
     """The workload with an autotuner result applied: froxel sizes and
-    binning backend (retile) and the launch shape (marking waves per
-    workgroup, raster waves per CU)."""
+    binning backend (retile), depth-warp slices per coarse tile and the
+    launch shape (marking waves per workgroup, raster waves per CU)."""
     t = retile(w, best["camera_froxel"], best["light_froxel"], best["fine_binning_backend"])
+    t.settings["coarse_depth_slices"] = int(best.get("coarse_depth_slices", 0))
     t.settings["mark_waves"] = int(best.get("mark_waves", 0))
     t.settings["raster_waves"] = int(best.get("raster_waves", 0))
     return t

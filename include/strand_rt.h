@@ -37,7 +37,15 @@ extern "C" {
 /* Constants (src/plugin.rs:57-70, src/shaders/strand_prepass.wgsl:65-69)    */
 /* ------------------------------------------------------------------------ */
 #define SR_COARSE_FINE_TILE_EXTENT 4u
+/* COARSE_DEPTH_SLICES is one shader constant in the reference: the slices of
+ * a coarse tile's depth-warp LUT and page table, and the fine depth slices of
+ * a count page (4 x 4 x 16 = SR_COARSE_COUNT_PAGE_SIZE cells).  Here a frame
+ * may give its coarse tiles 8, 16 or 32 LUT slices
+ * (sr_frame_settings.coarse_depth_slices, an autotuner dimension); a page
+ * keeps its 16 fine slices, so it spans 1/8, 1/16 or 1/32 of the tile's
+ * depth intervals.  At 16 every layout is the reference's. */
 #define SR_COARSE_DEPTH_SLICES 16u
+#define SR_MAX_COARSE_DEPTH_SLICES 32u
 #define SR_COARSE_COUNT_PAGE_SIZE 256u
 #define SR_DEPTH_QUANT_MAX 16777215u
 #define SR_INVALID_PTR 0xFFFFFFFFu
@@ -338,7 +346,7 @@ typedef struct sr_frame_settings {
   uint32_t stamp_target_height;
   uint32_t flags;                /* SR_FRAME_* bits */
   uint32_t dom_resident_pages;   /* SR_FRAME_VIRTUAL_DOMS: resident DOM pages per light, 0 = every requested page */
-  uint32_t pad;
+  uint32_t coarse_depth_slices;  /* depth-warp slices per coarse tile: 8, 16 or 32; 0 = 16 (SR_COARSE_DEPTH_SLICES) */
 } sr_frame_settings;
 SR_STATIC_ASSERT(sizeof(sr_frame_settings) == 88, "sr_frame_settings is 88 bytes");
 
@@ -616,14 +624,16 @@ const char* sr_last_error(sr_context* ctx);
  * given without graphs (the quality reference), on the context's main slot:
  * first fine-binning backend (segment scatter / page CSR; only the frame's own
  * with SR_FRAME_TUNE_KEEP_BACKEND) x camera froxel size {8, 16} x light
- * froxel size {8, 16}; then, for the fastest accepted of
- * those, the launch shape: marking workgroups of 1, 2 or 4 waves x 8, 16 or
+ * froxel size {8, 16}; then, for the fastest accepted of those, the other
+ * depth-warp slice counts of {8, 16, 32}; then, for the fastest accepted so
+ * far, the launch shape: marking workgroups of 1, 2 or 4 waves x 8, 16 or
  * 24 persistent raster waves per CU.  A candidate whose composite stays
- * within min_psnr_db of the reference is accepted (tile size changes the
- * order in which overlapping fragments blend; the launch shape changes no
- * result); `best` is the fastest accepted one.  The caller rebuilds its
- * frustum table with best's froxel sizes (sr_build_frustum_table), sets the
- * backend, SR_CONFIG_HIP_GRAPHS and the frame's mark_waves / raster_waves. */
+ * within min_psnr_db of the reference is accepted (tile size and slice count
+ * change the order in which overlapping fragments blend; the launch shape
+ * changes no result); `best` is the fastest accepted one.  The caller
+ * rebuilds its frustum table with best's froxel sizes
+ * (sr_build_frustum_table), sets the backend, the slice count,
+ * SR_CONFIG_HIP_GRAPHS and the frame's mark_waves / raster_waves. */
 typedef struct sr_tune_trial {
   uint32_t fine_binning_backend;
   uint32_t camera_froxel;       /* 0 when the frame has no camera */
@@ -634,7 +644,7 @@ typedef struct sr_tune_trial {
   uint32_t accepted;
   uint32_t mark_waves;          /* sr_frame_desc.mark_waves of the trial (0 = default) */
   uint32_t raster_waves;        /* sr_frame_desc.raster_waves of the trial (0 = default) */
-  uint32_t pad;
+  uint32_t coarse_depth_slices; /* sr_frame_settings.coarse_depth_slices of the trial (8, 16 or 32) */
 } sr_tune_trial;
 
 #define SR_TUNE_MAX_TRIALS 24u

@@ -47,7 +47,7 @@ def load() -> C.CDLL:
         ("oracle_pack2x16unorm", C.c_uint32, [C.c_float, C.c_float]),
         ("oracle_quantize_depth01", C.c_uint32, [C.c_float]),
         ("oracle_normalize_depth01", C.c_float, [C.c_float]),
-        ("oracle_build_lut", None, [C.c_void_p, C.c_uint32, C.c_void_p]),
+        ("oracle_build_lut", None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
         ("oracle_clip_segment", C.c_int, [C.c_void_p] * 5),
         ("oracle_gaussian_detector", C.c_float, [C.c_float, C.c_float, C.c_uint32]),
         ("oracle_fresnel", C.c_float, [C.c_float, C.c_float, C.c_float]),

@@ -43,7 +43,8 @@
 namespace {
 
 constexpr uint32_t kExt = SR_COARSE_FINE_TILE_EXTENT;      // 4
-constexpr uint32_t kSlices = SR_COARSE_DEPTH_SLICES;       // 16
+constexpr uint32_t kSlices = SR_COARSE_DEPTH_SLICES;       // 16: fine depth slices of a page, default LUT slices
+constexpr uint32_t kMaxSlices = SR_MAX_COARSE_DEPTH_SLICES;  // 32
 constexpr uint32_t kPage = SR_COARSE_COUNT_PAGE_SIZE;      // 256
 constexpr uint32_t kQuant = SR_DEPTH_QUANT_MAX;            // 16777215
 constexpr uint32_t kInvalid = SR_INVALID_PTR;
@@ -200,6 +201,8 @@ struct Frame {
   uint32_t F = 0, NI = 0;
   uint32_t coarse_total = 0;
   uint32_t ref_capacity = 0;
+  // depth-warp LUT slices per coarse tile (sr_frame_settings.coarse_depth_slices)
+  uint32_t slices() const { return st.coarse_depth_slices ? st.coarse_depth_slices : kSlices; }
   // the previous frame's shadow history (float2 per atlas texel, .x = the
   // filtered visibility read as shadow_history_prev), or none
   const float* prev_history = nullptr;
@@ -513,13 +516,14 @@ inline uint32_t disjoint_interval_gap(uint32_t amin, uint32_t amax, uint32_t bmi
 
 // strand_prepass.wgsl:884-1014 over one coarse tile's interval list, given
 // newest-first (the linked-list walk order).
-void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals, sr_coarse_depth_lut_entry* out) {
-  uint32_t mins[kSlices], maxs[kSlices];
+void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals, sr_coarse_depth_lut_entry* out,
+                        uint32_t S) {
+  uint32_t mins[kMaxSlices], maxs[kMaxSlices];
   uint32_t count = 0;
   for (auto& iv : ivals) {
     uint32_t zmin = std::min(iv.first, iv.second), zmax = std::max(iv.first, iv.second);
     bool merged = false;
-    for (uint32_t i = 0; i < kSlices; ++i) {
+    for (uint32_t i = 0; i < S; ++i) {
       if (i >= count) break;
       if (zmin <= maxs[i] + 1u && zmax + 1u >= mins[i]) {
         mins[i] = std::min(mins[i], zmin);
@@ -528,11 +532,11 @@ void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals,
       }
     }
     if (!merged) {
-      if (count < kSlices) {
+      if (count < S) {
         mins[count] = zmin; maxs[count] = zmax; ++count;
       } else {
         uint32_t best = 0, best_gap = 0xFFFFFFFFu;
-        for (uint32_t i = 0; i < kSlices; ++i) {
+        for (uint32_t i = 0; i < S; ++i) {
           uint32_t gap = disjoint_interval_gap(mins[i], maxs[i], zmin, zmax);
           if (gap < best_gap) { best_gap = gap; best = i; }
         }
@@ -542,10 +546,10 @@ void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals,
     }
   }
   if (count == 0) {
-    for (uint32_t i = 0; i < kSlices; ++i) out[i] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+    for (uint32_t i = 0; i < S; ++i) out[i] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
     return;
   }
-  for (uint32_t i = 1; i < kSlices; ++i) {  // insertion sort by min
+  for (uint32_t i = 1; i < S; ++i) {  // insertion sort by min
     if (i >= count) break;
     uint32_t j = i, kmin = mins[i], kmax = maxs[i];
     while (!(j == 0u || mins[j - 1u] <= kmin)) {
@@ -554,15 +558,15 @@ void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals,
     mins[j] = kmin; maxs[j] = kmax;
   }
   uint32_t out_slice = 0;
-  for (uint32_t iv = 0; iv < kSlices; ++iv) {
+  for (uint32_t iv = 0; iv < S; ++iv) {
     if (iv >= count) break;
-    uint32_t base_cnt = std::max(1u, kSlices / count);
-    uint32_t rem = kSlices % count;
+    uint32_t base_cnt = std::max(1u, S / count);
+    uint32_t rem = S % count;
     uint32_t slice_count = base_cnt + (iv < rem ? 1u : 0u);
     uint32_t zmin = mins[iv], zmax = maxs[iv];
     uint32_t span = std::max(1u, zmax - zmin + 1u);
-    for (uint32_t ls = 0; ls < kSlices; ++ls) {
-      if (ls >= slice_count || out_slice >= kSlices) break;
+    for (uint32_t ls = 0; ls < S; ++ls) {
+      if (ls >= slice_count || out_slice >= S) break;
       uint32_t start = zmin + (span * ls) / slice_count;
       uint32_t ex_end = zmin + (span * (ls + 1u)) / slice_count;
       uint32_t end = std::min(ex_end > start ? ex_end - 1u : start, zmax);
@@ -570,12 +574,13 @@ void build_lut_for_tile(const std::vector<std::pair<uint32_t, uint32_t>>& ivals,
       ++out_slice;
     }
   }
-  while (out_slice < kSlices) out[out_slice++] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
+  while (out_slice < S) out[out_slice++] = sr_coarse_depth_lut_entry{0, 0, 0, 0};
 }
 
 // strand_prepass.wgsl:835-882 (intervals) + 884-1014 (LUT)
 void build_luts(Frame& F) {
-  F.lut.assign((size_t)F.coarse_total * kSlices, sr_coarse_depth_lut_entry{0, 0, 0, 0});
+  const uint32_t S = F.slices();
+  F.lut.assign((size_t)F.coarse_total * S, sr_coarse_depth_lut_entry{0, 0, 0, 0});
   std::vector<std::vector<std::pair<uint32_t, uint32_t>>> lists(F.coarse_total);
   // insertion in slot order (instance, frustum); a list walk is newest-first
   for (uint32_t ii = 0; ii < F.NI; ++ii)
@@ -593,16 +598,17 @@ void build_luts(Frame& F) {
     }
   for (uint32_t t = 0; t < F.coarse_total; ++t) {
     std::vector<std::pair<uint32_t, uint32_t>> newest_first(lists[t].rbegin(), lists[t].rend());
-    build_lut_for_tile(newest_first, &F.lut[(size_t)t * kSlices]);
+    build_lut_for_tile(newest_first, &F.lut[(size_t)t * S], S);
   }
 }
 
 // strand_prepass.wgsl:1251-1274
 inline uint32_t lut_slice_for_depth(const Frame& F, uint32_t tile, float z) {
   uint32_t zq = quantize_depth01(z);
-  const sr_coarse_depth_lut_entry* lut = &F.lut[(size_t)tile * kSlices];
+  const uint32_t S = F.slices();
+  const sr_coarse_depth_lut_entry* lut = &F.lut[(size_t)tile * S];
   uint32_t best = kInvalid, best_d = 0xFFFFFFFFu;
-  for (uint32_t i = 0; i < kSlices; ++i) {
+  for (uint32_t i = 0; i < S; ++i) {
     const sr_coarse_depth_lut_entry& e = lut[i];
     if (e.virtual_count_q == 0u) continue;
     if (zq >= e.z_min_q && zq <= e.z_max_q) return i;
@@ -694,12 +700,12 @@ void trace_coarse_tile(Frame& F, V3 p0, V3 p1, const sr_frustum_desc& fd, uint32
   if (z0s == kInvalid || z1s == kInvalid) return;
   uint32_t zlo = std::min(z0s, z1s), zhi = std::max(z0s, z1s);
   for (uint32_t cz = zlo; cz <= zhi; ++cz) {
-    const sr_coarse_depth_lut_entry& e = F.lut[(size_t)tile * kSlices + cz];
+    const sr_coarse_depth_lut_entry& e = F.lut[(size_t)tile * F.slices() + cz];
     if (e.virtual_count_q == 0u) continue;
     float zmin = dequantize_depth01(e.z_min_q), zmax = dequantize_depth01(e.z_max_q);
     Clipped dc = clip_segment_to_box(c.p0, c.p1, V3{origin.x, origin.y, zmin}, V3{bmax.x, bmax.y, zmax});
     if (!dc.ok) continue;
-    uint32_t pti = tile * kSlices + cz;
+    uint32_t pti = tile * F.slices() + cz;
     if (mode == Mode::Mark) {
       F.page_table[pti] = kMarked;
       continue;
@@ -823,7 +829,8 @@ inline void task_endpoints(const Frame& F, uint32_t t, uint32_t* fi, V3* p0, V3*
 
 // K10 mark (:1928-1932) + K11 allocate (:1934-1969) + K13 count (:1986-1990)
 void bin_count(Frame& F) {
-  size_t ptab = (size_t)F.coarse_total * kSlices;
+  const uint32_t S = F.slices();
+  size_t ptab = (size_t)F.coarse_total * S;
   F.page_table.assign(ptab, 0u);
   for (uint32_t t = 0; t < F.tasks.size(); ++t) {  // mark: idempotent stores
     uint32_t fi; V3 p0, p1;
@@ -835,8 +842,8 @@ void bin_count(Frame& F) {
   for (size_t e = 0; e < ptab; ++e) {
     if (F.page_table[e] != kMarked) continue;
     sr_fine_page_meta pm{};
-    pm.coarse_tile_id = (uint32_t)(e / kSlices);
-    pm.coarse_z = (uint32_t)(e % kSlices);
+    pm.coarse_tile_id = (uint32_t)(e / S);
+    pm.coarse_z = (uint32_t)(e % S);
     F.page_meta.push_back(pm);
     F.page_table[e] = ++P;  // page + 1
   }
@@ -958,10 +965,11 @@ void emit_work(Frame& F) {
     uint32_t screen_tile = fyy * ftx + fxx;
     uint32_t base = (uint32_t)F.work.size();
     uint32_t load = 0, count = 0;
-    for (uint32_t lane = 0; lane < kPage; ++lane) {
-      uint32_t dl = (kPage - 1u) - lane;  // near (high keys) first
+    const uint32_t S = F.slices(), lanes = S * kSlices;  // (LUT slice, fine slice) depth lanes
+    for (uint32_t lane = 0; lane < lanes; ++lane) {
+      uint32_t dl = (lanes - 1u) - lane;  // near (high keys) first
       uint32_t cz = dl / kSlices, fz = dl % kSlices;
-      uint32_t pti = tile * kSlices + cz;
+      uint32_t pti = tile * S + cz;
       uint32_t h = F.page_table[pti];
       if (h == 0u || h == kMarked) continue;
       uint32_t page = h - 1u;
@@ -1675,6 +1683,10 @@ int oracle_render_history(const sr_asset_slabs* assets, uint32_t asset_count, co
   if (fd->light_count) F.lights.assign(fd->lights, fd->lights + fd->light_count);
   F.st = fd->settings;
   if (F.st.dom_layers == 0u || F.st.dom_layers > SR_MAX_DOM_LAYERS) { delete of; return -1; }
+  {
+    uint32_t S = F.st.coarse_depth_slices;
+    if (S != 0u && S != 8u && S != 16u && S != 32u) { delete of; return -1; }
+  }
   F.threads = std::max(1, threads);
   F.F = fd->frustum_count; F.NI = fd->instance_count;
   F.coarse_total = 0;
@@ -1768,10 +1780,11 @@ float oracle_hash_to_unit_float(uint32_t x) { return hash_to_unit_float(x); }
 uint32_t oracle_pack2x16unorm(float a, float b) { return pack2x16unorm(a, b); }
 uint32_t oracle_quantize_depth01(float z) { return quantize_depth01(z); }
 float oracle_normalize_depth01(float z) { return normalize_depth01(z); }
-void oracle_build_lut(const uint32_t* zmin_zmax_newest_first, uint32_t n, sr_coarse_depth_lut_entry* out16) {
+void oracle_build_lut(const uint32_t* zmin_zmax_newest_first, uint32_t n, sr_coarse_depth_lut_entry* out,
+                      uint32_t slices) {
   std::vector<std::pair<uint32_t, uint32_t>> iv;
   for (uint32_t i = 0; i < n; ++i) iv.push_back({zmin_zmax_newest_first[2 * i], zmin_zmax_newest_first[2 * i + 1]});
-  build_lut_for_tile(iv, out16);
+  build_lut_for_tile(iv, out, slices ? slices : kSlices);
 }
 int oracle_clip_segment(const float* p0, const float* p1, const float* bmin, const float* bmax, float* out6) {
   Clipped c = clip_segment_to_box(V3{p0[0], p0[1], p0[2]}, V3{p1[0], p1[1], p1[2]}, V3{bmin[0], bmin[1], bmin[2]},

@@ -59,6 +59,8 @@ def invalid_frames(w):
     with_settings("unknown-backend", "SR_ERR_INVALID_ARGUMENT", fine_binning_backend=2)
     with_settings("3-marking-waves", "SR_ERR_INVALID_ARGUMENT", mark_waves=3)
     with_settings("33-raster-waves", "SR_ERR_INVALID_ARGUMENT", raster_waves=33)
+    with_settings("12-depth-slices", "SR_ERR_INVALID_ARGUMENT", coarse_depth_slices=12)
+    with_settings("64-depth-slices", "SR_ERR_INVALID_ARGUMENT", coarse_depth_slices=64)
     big = scene.retile(w, 32, 0)  # 32 x 32 px raster tiles: 1024 pixels
     out.append(("froxel-tiles-above-256-px", big, "SR_ERR_UNSUPPORTED"))
     inst = w.instances.copy()

@@ -100,6 +100,24 @@ def cache_workload():
 CASES["cache-asset"] = cache_workload
 
 
+def depth_stack(n: int = 10, step: float = 0.5):
+    """n copies of the tiny scalp, each step metres further along the view
+    axis: n disjoint depth intervals over the same coarse tiles (more than 8,
+    so an 8-slice LUT merges intervals into their nearest slices)."""
+    w = scene.config_tiny(strands=30, points=6, res=64, light_res=64)
+    extra = []
+    for k in range(1, n):
+        inst = scene.identity_instance(0)
+        m = np.eye(4)
+        m[:3, 3] = (0.0, 0.0, -step * k)
+        inst["world_from_local"] = m.T.reshape(-1).astype(np.float32)
+        extra.append(inst)
+    return dataclasses.replace(w, name=f"tiny-stack-{n}", instances=np.concatenate([w.instances] + extra))
+
+
+CASES["depth-stack"] = depth_stack
+
+
 def edge_workload(res: int = 64, far_away: bool = False):
     """Strands at the edges of the pipeline's arithmetic, under the identity
     camera and light of config_two_strands (screen x = (x / 2 + 1 / 2) res,
@@ -165,6 +183,7 @@ CSR_CASES = {
     "wide-light-csr": lambda: csr(scene.config_tiny(res=64, light_res=4096)),
     "edge-strands-csr": lambda: csr(edge_workload()),
     "empty-view-csr": lambda: csr(edge_workload(far_away=True)),
+    "depth-stack-csr": lambda: csr(depth_stack()),
 }
 
 
@@ -178,6 +197,36 @@ def test_parity_csr_paths(case, flags):
     segment), eager and replayed from a HIP graph."""
     w = CSR_CASES[case]()
     print(f"[parity] {case} ({flags}): {w.name}", flush=True)
+    ref = oracle.render(w)
+    ctr, gpu = gpu_frame(w, flags=CSR_FLAGS[flags])
+    compare(w, ctr, gpu, ref)
+
+
+@pytest.mark.parametrize("slices", [8, 32])
+@pytest.mark.parametrize("case", ["tiny-f8", "small-f8", "depth-stack", "tiny-f8-csr", "small-f8-csr", "depth-stack-csr",
+                                  "dense-one-tile-csr", "edge-strands-csr", "tiny-2lights-8layers-csr"])
+def test_parity_depth_slices(case, slices):
+    """8 and 32 depth-warp LUT slices per coarse tile (the oracle bins with
+    the same count): LUT, page table, pages, references, work and images.
+    At 32 slices a fine tile has 512 depth lanes, and the tiny and stacked
+    scenes have tile runs of more than 256 work items."""
+    w = {**CASES, **CSR_CASES}[case]()
+    w.settings["coarse_depth_slices"] = slices
+    print(f"[parity] {case} at {slices} slices: {w.name}", flush=True)
+    ref = oracle.render(w)
+    assert ref.get("lut").size == slices * int(w.frusta["coarse_depth_tile_count"].sum())
+    if slices == 32 and case.startswith(("tiny-f8", "depth-stack")):
+        runs = np.concatenate([ref.get("camera_runs"), ref.get("shadow_runs")])
+        assert runs["work_count"].max() > 256
+    ctr, gpu = gpu_frame(w)
+    compare(w, ctr, gpu, ref)
+
+
+@pytest.mark.parametrize("flags", list(CSR_FLAGS))
+def test_parity_depth_slices_csr_paths(flags):
+    """32 slices through the re-trace path, eager and from a HIP graph."""
+    w = csr(depth_stack())
+    w.settings["coarse_depth_slices"] = 32
     ref = oracle.render(w)
     ctr, gpu = gpu_frame(w, flags=CSR_FLAGS[flags])
     compare(w, ctr, gpu, ref)
@@ -405,7 +454,8 @@ def test_autotune_picks_an_accepted_configuration():
         res = r.autotune(w, frames_per_trial=2, min_psnr_db=40.0)
     print(f"[autotune] best {res['best']}", flush=True)
     trials = res["trials"]
-    assert len(trials) == 17  # reference, 8 backend x froxel candidates, 8 launch shapes of the best
+    # reference, 8 backend x froxel candidates, 2 other slice counts, 8 launch shapes of the best
+    assert len(trials) == 19
     assert trials[0]["hip_graphs"] == 0 and trials[0]["psnr_db"] > 900  # the reference itself
     assert res["best"]["accepted"] == 1 and res["best"]["ms_per_frame"] > 0
     assert res["best"]["ms_per_frame"] == min(t["ms_per_frame"] for t in trials if t["accepted"])
@@ -413,15 +463,20 @@ def test_autotune_picks_an_accepted_configuration():
     same = [t for t in trials[1:9] if t["camera_froxel"] == 8 and t["light_froxel"] == 8
             and t["fine_binning_backend"] == trials[0]["fine_binning_backend"]]
     assert same and same[0]["psnr_db"] > 900
-    # the launch shapes keep the stage-one winner's configuration and change no result
+    assert all(t["coarse_depth_slices"] == 16 for t in trials[:9])
+    # the slice counts 8 and 32 of the stage-one winner's configuration
     first = min((t for t in trials[1:9] if t["accepted"]), key=lambda t: t["ms_per_frame"])
-    shapes = trials[9:]
+    key = lambda t: (t["fine_binning_backend"], t["camera_froxel"], t["light_froxel"])
+    assert sorted(t["coarse_depth_slices"] for t in trials[9:11]) == [8, 32]
+    assert all(key(t) == key(first) for t in trials[9:11])
+    # the launch shapes keep the stage-two winner's configuration and change no result
+    second = min((t for t in [first] + trials[9:11] if t["accepted"]), key=lambda t: t["ms_per_frame"])
+    shapes = trials[11:]
     assert {(t["mark_waves"], t["raster_waves"]) for t in shapes} == {
         (m, r) for m in (1, 2, 4) for r in (8, 16, 24)} - {(4, 16)}
     for t in shapes:
-        assert (t["fine_binning_backend"], t["camera_froxel"], t["light_froxel"]) == (
-            first["fine_binning_backend"], first["camera_froxel"], first["light_froxel"])
-        assert t["psnr_db"] == first["psnr_db"]
+        assert key(t) == key(second) and t["coarse_depth_slices"] == second["coarse_depth_slices"]
+        assert t["psnr_db"] == second["psnr_db"]
     tuned = scene.apply_tuning(w, res["best"])
     ref = oracle.render(tuned)
     ctr, gpu = gpu_frame(tuned)
@@ -431,7 +486,8 @@ def test_autotune_picks_an_accepted_configuration():
 @pytest.mark.parametrize("backend", [0, 1])
 def test_autotune_can_keep_the_backend(backend):
     """SR_FRAME_TUNE_KEEP_BACKEND: every trial keeps the frame's binning
-    backend; the froxel sizes and launch shapes are still searched."""
+    backend; the froxel sizes, slice counts and launch shapes are still
+    searched."""
     w = scene.config_small(froxel=8)
     w.settings["fine_binning_backend"] = backend
     w.settings["frame_flags"] = abi.FRAME_TUNE_KEEP_BACKEND
@@ -440,7 +496,7 @@ def test_autotune_can_keep_the_backend(backend):
         res = r.autotune(w, frames_per_trial=1, min_psnr_db=40.0)
     trials = res["trials"]
     print(f"[autotune keep backend {backend}] {len(trials)} trials, best {res['best']}", flush=True)
-    assert len(trials) == 13  # reference, 4 froxel candidates, 8 launch shapes of the best
+    assert len(trials) == 15  # reference, 4 froxel candidates, 2 other slice counts, 8 launch shapes of the best
     assert all(t["fine_binning_backend"] == backend for t in trials)
     assert {(t["camera_froxel"], t["light_froxel"]) for t in trials[1:5]} == {(c, l) for c in (8, 16) for l in (8, 16)}
     assert res["best"]["accepted"] == 1 and res["best"]["fine_binning_backend"] == backend

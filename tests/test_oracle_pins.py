@@ -14,6 +14,7 @@ restatement's code:
   analytic distance-to-segment formula.
 """
 import ctypes as C
+import dataclasses
 import math
 
 import numpy as np
@@ -95,10 +96,10 @@ def test_depth_quantisation(lib):
     assert lib.oracle_normalize_depth01(-3.0) == 0.0
 
 
-def lut(lib, intervals):
+def lut(lib, intervals, slices=16):
     arr = np.array(intervals, np.uint32).reshape(-1)
-    out = np.zeros(16, abi.coarse_depth_lut_entry_dtype)
-    lib.oracle_build_lut(abi.ptr(arr) if arr.size else None, len(intervals), abi.ptr(out))
+    out = np.zeros(slices, abi.coarse_depth_lut_entry_dtype)
+    lib.oracle_build_lut(abi.ptr(arr) if arr.size else None, len(intervals), abi.ptr(out), slices)
     return out
 
 
@@ -126,6 +127,29 @@ def test_depth_warp_lut_closed_forms(lib):
     e = lut(lib, ivs)
     for a, b in ivs:
         assert any(lo <= a and b <= hi for lo, hi in zip(e["z_min_q"], e["z_max_q"]))
+
+
+@pytest.mark.parametrize("S", [8, 32])
+def test_depth_warp_lut_other_slice_counts(lib, S):
+    """The LUT builder with S slices per coarse tile: one interval splits into
+    S floor-divided slices, k disjoint intervals into S // k (+1 for the first
+    S % k), and more than S disjoint intervals merge into the nearest."""
+    e = lut(lib, [(0, 159)], S)
+    assert list(e["z_min_q"]) == [(160 * k) // S for k in range(S)]
+    assert list(e["z_max_q"]) == [(160 * (k + 1)) // S - 1 for k in range(S)]
+    assert int(e["virtual_count_q"].sum()) == 160
+    e = lut(lib, [(0, 29), (100, 129), (200, 229)], S)
+    first = S // 3 + (1 if S % 3 > 0 else 0)
+    second = S // 3 + (1 if S % 3 > 1 else 0)
+    assert list(e["z_min_q"][:first]) == [(30 * k) // first for k in range(first)]
+    assert e["z_min_q"][first] == 100 and e["z_min_q"][first + second] == 200
+    assert e["z_max_q"][S - 1] == 229
+    ivs = [(1000 * i, 1000 * i + 10) for i in range(S + 4)]
+    e = lut(lib, ivs, S)
+    for a, b in ivs:
+        assert any(lo <= a and b <= hi for lo, hi in zip(e["z_min_q"], e["z_max_q"]))
+    # a span of 160 gives 8 slices 20 wide, or 32 slices 5 wide
+    assert (lut(lib, [(0, 159)], S)["virtual_count_q"] == 160 // S).all()
 
 
 def clip(lib, p0, p1, bmin, bmax):
@@ -254,16 +278,27 @@ def test_scans_and_consistency(small16):
         assert np.all(np.diff(dk) <= 0)
 
 
-def test_dda_cells_match_sampled_intersections(tiny):
+@pytest.fixture(scope="module", params=[16, 8, 32])
+def tiny_slices(request, tiny):
+    """The tiny workload binned with 16 (default), 8 or 32 depth-warp slices
+    per coarse tile."""
+    if request.param == 16:
+        return tiny + (16,)
+    w = dataclasses.replace(tiny[0], settings={**tiny[0].settings, "coarse_depth_slices": request.param})
+    return w, oracle.render(w, threads=2), request.param
+
+
+def test_dda_cells_match_sampled_intersections(tiny_slices):
     """Every cell a densely sampled projected segment passes through (away from
     cell boundaries) is among the cells the 3D DDA assigned to it, and the
-    DDA assigns no cell the segment does not come close to."""
-    w, f = tiny
+    DDA assigns no cell the segment does not come close to, at every
+    depth-warp slice count."""
+    w, f, S = tiny_slices
     tasks, proj, lut_e = f.get("tasks"), f.get("projected"), f.get("lut")
     fr = w.frusta
     got = cells_by_segment(w, f)
     pt = f.get("page_table")
-    checked = 0
+    checked = covered = 0
     for t, (tk, ps) in enumerate(zip(tasks, proj)):
         fi = int(tk["packed_field"]) >> 3
         d = fr[fi]
@@ -277,30 +312,27 @@ def test_dda_cells_match_sampled_intersections(tiny):
         fx, fy = int(d["froxel_size_x"]), int(d["froxel_size_y"])
         cpx, cpy = 4 * fx, 4 * fy
         sampled, near = set(), set()
-        for s in np.linspace(0.0, 1.0, 3001):
-            p = p0 + (p1 - p0) * s
-            cx, cy = int(p[0] // cpx), int(p[1] // cpy)
-            if not (0 <= cx < d["coarse_tiles_x"] and 0 <= cy < d["coarse_tiles_y"]):
-                continue
-            tile = int(d["coarse_depth_tile_base"]) + cy * int(d["coarse_tiles_x"]) + cx
-            zq = p[2] * Q
-            for cz in range(16):
-                e = lut_e[tile * 16 + cz]
-                if e["virtual_count_q"] == 0 or not (e["z_min_q"] - 0.5 <= zq <= e["z_max_q"] + 0.5):
-                    continue
-                zmin, zmax = e["z_min_q"] / Q, e["z_max_q"] / Q
-                span = max(zmax - zmin, 1.0 / Q)
-                lx = (p[0] - cx * cpx) / fx
-                ly = (p[1] - cy * cpy) / fy
-                lz = min(max((p[2] - zmin) / span, 0.0), 0.999999) * 16
-                page = int(pt[tile * 16 + cz]) - 1
-                if page < 0:
-                    continue
-                cell = (int(lz) * 4 + min(int(ly), 3)) * 4 + min(int(lx), 3)
-                margin = min(abs(lx - round(lx)), abs(ly - round(ly)), abs(lz - round(lz)))
-                near.add((page, cell))
-                if margin > 0.02:
-                    sampled.add((page, cell))
+        p = p0 + (p1 - p0) * np.linspace(0.0, 1.0, 3001)[:, None]
+        cx, cy = np.floor_divide(p[:, 0], cpx).astype(np.int64), np.floor_divide(p[:, 1], cpy).astype(np.int64)
+        inside = (cx >= 0) & (cx < int(d["coarse_tiles_x"])) & (cy >= 0) & (cy < int(d["coarse_tiles_y"]))
+        p, cx, cy = p[inside], cx[inside], cy[inside]
+        tile = int(d["coarse_depth_tile_base"]) + cy * int(d["coarse_tiles_x"]) + cx
+        zq = p[:, 2] * Q
+        lx = (p[:, 0] - cx * cpx) / fx
+        ly = (p[:, 1] - cy * cpy) / fy
+        for cz in range(S):
+            e = lut_e[tile * S + cz]
+            lo, hi = e["z_min_q"].astype(np.float64), e["z_max_q"].astype(np.float64)
+            page = pt[tile * S + cz].astype(np.int64) - 1
+            ok = (e["virtual_count_q"] != 0) & (lo - 0.5 <= zq) & (zq <= hi + 0.5) & (page >= 0)
+            zmin, zmax = lo / Q, hi / Q
+            span = np.maximum(zmax - zmin, 1.0 / Q)
+            lz = np.clip((p[:, 2] - zmin) / span, 0.0, 0.999999) * 16
+            cell = (lz.astype(np.int64) * 4 + np.minimum(ly.astype(np.int64), 3)) * 4 + np.minimum(lx.astype(np.int64), 3)
+            margin = np.minimum(np.minimum(np.abs(lx - np.round(lx)), np.abs(ly - np.round(ly))), np.abs(lz - np.round(lz)))
+            near.update(zip(page[ok].tolist(), cell[ok].tolist()))
+            far = ok & (margin > 0.02)
+            sampled.update(zip(page[far].tolist(), cell[far].tolist()))
         dda = got.get((fi, int(tk["id_info"]), int(tk["seg_idx"])), set())
         missing = sampled - dda
         assert not missing, f"task {t}: DDA misses sampled cells {sorted(missing)[:4]}"
@@ -308,7 +340,9 @@ def test_dda_cells_match_sampled_intersections(tiny):
         extra = dda - near
         assert len(extra) <= 2, f"task {t}: DDA visits cells the segment does not touch: {sorted(extra)[:4]}"
         checked += 1
+        covered += len(sampled)
     assert checked == len(tasks) > 50
+    assert covered >= checked  # the samples reach cells, so the checks are not vacuous
 
 
 # ------------------------------------------------------------ render pins

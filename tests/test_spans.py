@@ -54,7 +54,8 @@ def test_unorm16_matches_division(tmp_path):
 
 def test_even_slice_matches_the_slice_scan(tmp_path):
     """The marking pass's arithmetic LUT slice of a depth in an evenly split
-    single-interval tile equals the first slice of build_depth_warp_lut's 16
+    single-interval tile equals the first of build_depth_warp_lut's 8, 16 or 32
+    slices
     holding it, for reciprocals of the span within 3 ulp
     (tests/even_slice_check.cpp)."""
     if not Path(HIPCC).exists():
