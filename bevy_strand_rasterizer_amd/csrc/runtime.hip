@@ -1630,7 +1630,9 @@ struct sr_server {
         }
         // a multi-view frame stays within one slot's share of the budget, so
         // every slot can hold a frame at once and none gives its buffers
-        // back: larger groups are halved, the tail goes back to the queue
+        // back: a larger group keeps the views its share holds at its
+        // measured bytes per view (the frame's fixed part makes that an
+        // underestimate), the tail goes back to the queue
         if (budget && g.size() > 1) {
           uint64_t assets;
           {
@@ -1640,7 +1642,8 @@ struct sr_server {
           const uint64_t share = budget > assets ? (budget - assets) / slots.size() : 0u;
           if (own > share) {
             std::lock_guard<std::mutex> lk(mu);
-            const size_t keep = g.size() / 2;
+            const size_t fit = (size_t)((double)g.size() * (double)share / (double)own);
+            const size_t keep = std::min(std::max<size_t>(fit, 1), g.size() - 1);
             for (size_t k = g.size(); k-- > keep;) queue.push_front(g[k]);
             g.resize(keep);
             ++stats.groups_split;

@@ -526,3 +526,12 @@ def test_opaque_occlusion_culls_hidden_cells():
     for t in list(covered)[:50]:
         tx, ty = t % ftx, t // ftx
         assert (img[ty * 8:(ty + 1) * 8, tx * 8:(tx + 1) * 8, 3] == 0).all()
+
+
+@pytest.mark.parametrize("slices", [4, 12, 64])
+def test_oracle_rejects_other_slice_counts(slices):
+    """Only 8, 16 or 32 depth-warp slices per coarse tile (0 = 16) bin."""
+    w = scene.config_tiny()
+    w.settings["coarse_depth_slices"] = slices
+    with pytest.raises(RuntimeError):
+        oracle.render(w, threads=1)
