@@ -11,10 +11,13 @@ Prints one JSON line.  value = camera frames per second over K pipelined
 frames (whole job, one GPU); Msegments/s, p50/p99 frame latency, per-stage
 times, the roofline of the dominant kernel and the CPU baseline ride along,
 and so do the two system patterns:
+  plain_tuned -- the named frame at the autotuned configuration (froxel
+             sizes, depth-warp slices, launch shape), replayed from a graph;
   batched -- many views and lights in one frame replayed from a HIP graph
              with the autotuned configuration: the 2,000 x 16 scalp from 64
              cameras over {720p, 1080p, 1440p} with 4 lights of 2048^2, and
-             the named asset from 8 1080p cameras sharing its light;
+             the named asset from 8 and from 16 1080p cameras sharing its
+             light;
   serving -- the request loop over 8 resident assets (10k..500k strands x
              32 points) under a Poisson stream, p50/p99 latency and frames/s;
   stress  -- 1M strands x 64 points from 16 4K views with 4 lights of
@@ -663,8 +666,17 @@ def main():
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom, "avg_ms": round(stage_acc[dom], 4)}
 
-    batched = None
+    batched = plain_tuned = None
     if not args.no_batched:
+        # the named plain frame at the autotuned configuration (froxel sizes,
+        # depth-warp slices, launch shape; the image within 50 dB of the
+        # reference configuration's): what the tuner alone buys, the fair
+        # denominator for what batching adds
+        wt = scene.config_named() if args.config == "named" else scene.config_small()
+        wt.settings["fine_binning_backend"] = 1 if args.backend == "csr" else 0
+        plain_tuned = run_batched(wt, steps=args.steps, warmup=2, plain_fps=fps)
+        plain_tuned["plain"] = "the named plain frame at the reference configuration (value)"
+        print(f"bench: plain tuned {plain_tuned['frames_per_s']} frames/s", file=sys.stderr, flush=True)
         batched = {}
         for name, wb in (("64views_4lights_2kx16", scene.config_batched()),
                          ("named_8views_1light", scene.config_batched_named(8)),
@@ -678,6 +690,8 @@ def main():
             batched[name] = run_batched(wb, steps=max(3, args.steps // 4), warmup=2, plain_fps=plain)
             batched[name]["plain_frames_per_s"] = round(plain, 3)
             batched[name]["plain"] = plain_what
+            if name.startswith("named_"):
+                batched[name]["vs_plain_tuned"] = round(batched[name]["frames_per_s"] / plain_tuned["frames_per_s"], 3)
             print(f"bench: batched {name} {batched[name]['frames_per_s']} frames/s", file=sys.stderr, flush=True)
 
     vdoms = None
@@ -742,6 +756,7 @@ def main():
         "binning_roofline": binning,
         "binning_kernel_traffic": binning_kernels,
         "cpu_baseline": cpu,
+        "plain_tuned": plain_tuned,
         "batched": batched,
         "virtual_doms": vdoms,
         "depth_sorted_blend": sorted_blend,

@@ -46,17 +46,19 @@ def plain(r, w):
     return r.dump("composite").reshape(-1), r.dump("depth"), ctr
 
 
-@pytest.mark.parametrize("backend", [0, 1])
-def test_noalloc_matches_frame_submit(backend):
-    """Two calls in a row, the second with a smaller frame."""
+@pytest.mark.parametrize("backend,slices", [(0, (16, 16)), (1, (16, 16)), (1, (32, 8))])
+def test_noalloc_matches_frame_submit(backend, slices):
+    """Two calls in a row, the second with a smaller frame (and, in the last
+    case, 32 and then 8 depth-warp slices per coarse tile)."""
     frames = [scene.config_small(froxel=8), scene.config_tiny(strands=60, res=80, light_res=72)]
     with StrandRenderer() as r:
-        for w in frames:
+        for w, sc in zip(frames, slices):
             w.settings["fine_binning_backend"] = backend
+            w.settings["coarse_depth_slices"] = sc
         ids = [r.upload(w.assets[0]).geo_id for w in frames]
         frames = [scene.with_asset_id(w, i) for w, i in zip(frames, ids)]
         for k, w in enumerate(frames):
-            print(f"[noalloc] call {k}: {w.name} backend {backend}", flush=True)
+            print(f"[noalloc] call {k}: {w.name} backend {backend} slices {slices[k]}", flush=True)
             rc, comp, depth, ctr = noalloc(r, w)
             assert rc == 0, r.lib.sr_last_error(r.ctx)
             want_comp, want_depth, want_ctr = plain(r, w)
