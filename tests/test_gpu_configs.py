@@ -8,6 +8,8 @@
   and 1440p, 4 lights of 2048^2 x 4 layers) with the autotuner's
   configuration: every view of the batch equals that view drawn alone, and
   two views (one 1440p) match the oracle.
+* The named asset from 16 views at the tuned configuration (8 depth-warp
+  slices): every view equals that view drawn alone, one matches the oracle.
 * Serving mix on the 8 resident serving assets (10k .. 500k strands):
   requests including the 500k asset and a 3840x2160 view equal the same
   requests drawn alone.
@@ -82,6 +84,36 @@ def test_many_pages_backends_agree():
 def split_cameras(w, flat, channels):
     """Per camera slices of a concatenated per-pixel output."""
     return [flat[off * channels:(off + cw * ch) * channels] for off, cw, ch in scene.camera_pixel_offsets(w)]
+
+
+def test_named_views_at_tuned_configuration():
+    """The named asset from 16 1080p views at the configuration the tuner
+    picks for it (16-px camera and 8-px light froxels, 8 depth-warp slices,
+    page CSR, graph replay): every view of the batch equals that view drawn
+    alone, and the first matches the oracle at the same configuration."""
+    w = scene.retile(scene.config_batched_named(16), 16, 8, 1)
+    w.settings["coarse_depth_slices"] = 8
+    with StrandRenderer(flags=abi.CONFIG_HIP_GRAPHS) as r:
+        r.upload_workload(w)
+        render_until_fits(r, w)
+        ctr = r.render(w)
+        assert ctr["capacity_exceeded"] == 0 and ctr["page_drops"] == 0
+        comp = split_cameras(w, r.dump("composite"), 4)
+        depth = split_cameras(w, r.dump("depth"), 1)
+    cams = [i for i in range(len(w.frusta)) if int(w.frusta[i]["kind"]) == 0]
+    assert len(cams) == 16
+    with StrandRenderer() as r:
+        r.upload_workload(w)
+        for k, fi in enumerate(cams):
+            one = scene.select_views(w, [fi])
+            ctr = render_until_fits(r, one)
+            assert_same(r.dump("composite"), comp[k], f"view {k}: composite differs from alone")
+            assert_same(r.dump("depth"), depth[k], f"view {k}: depth differs from alone")
+            if k == 0:
+                print(f"[configs] oracle named view 0 at 8 slices", flush=True)
+                gpu = {n: r.dump(n) for n in EXACT + FLOAT}
+                compare(one, ctr, gpu, oracle.render(one))
+    print("[configs] 16 named views equal alone", flush=True)
 
 
 def test_batched_views_equal_alone_and_oracle():
