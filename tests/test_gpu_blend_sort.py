@@ -6,7 +6,8 @@ the HIP path.
   nearest first.
 * Scalp frames (8x8 and 16x16 tiles, and the named 100k x 32 frame, where
   many pixels hold more fragments per cell than the sorted list and take the
-  list's overflow rule) against the oracle at the parity bar.
+  list's overflow rule) and ten stacked scalps at 32 depth-warp slices (tile
+  runs above 256 work items) against the oracle at the parity bar.
 """
 import numpy as np
 import pytest
@@ -41,11 +42,22 @@ def test_pair_sorted_matches_oracle_and_nearest_first_default():
         assert_same(gpu[k], nearest_first[k], f"sorted vs nearest-first default: {k}")
 
 
-@pytest.mark.parametrize("which", ["small8", "small16", "named"])
+def depth_stack_32():
+    """Ten stacked scalps at 32 depth-warp slices: tile runs of more than
+    256 work items."""
+    from test_gpu_parity import depth_stack
+
+    w = scene.retile(depth_stack(), backend=1)
+    w.settings["coarse_depth_slices"] = 32
+    return w
+
+
+@pytest.mark.parametrize("which", ["small8", "small16", "named", "stack32"])
 def test_sorted_blend_matches_oracle(which):
     w = {"small8": lambda: scene.retile(scene.config_small(froxel=8), backend=1),
          "small16": lambda: scene.config_small(froxel=16),
-         "named": lambda: scene.retile(scene.config_named(), backend=1)}[which]()
+         "named": lambda: scene.retile(scene.config_named(), backend=1),
+         "stack32": depth_stack_32}[which]()
     w = sorted_blend(w)
     print(f"[sorted] {w.name}", flush=True)
     ctr, gpu = render(w)

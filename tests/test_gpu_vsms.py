@@ -65,10 +65,11 @@ def test_every_requested_page_resident_equals_dense(which):
         assert ctr["dom_pages"] == 20
 
 
-@pytest.mark.parametrize("budget", [0, 7])
-def test_virtual_doms_match_oracle(budget):
+@pytest.mark.parametrize("budget,slices", [(0, 16), (7, 16), (7, 8), (7, 32)])
+def test_virtual_doms_match_oracle(budget, slices):
     w = scene.config_virtual_doms(budget=budget)
-    print(f"[vsms] oracle {w.name}", flush=True)
+    w.settings["coarse_depth_slices"] = slices
+    print(f"[vsms] oracle {w.name} at {slices} depth slices", flush=True)
     ref = oracle.render(w)
     with StrandRenderer() as r:
         r.upload_workload(w)
