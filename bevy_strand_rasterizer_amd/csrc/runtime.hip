@@ -749,6 +749,10 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
   uint32_t page_cap = std::max<uint32_t>(std::min<uint32_t>(ptab, tight ? 16u : std::max(4096u, ptab / 2)),
                                          sl.page_cap_hint);
   page_cap = std::min(page_cap, ptab);
+  // page-CSR hit keys are page * 256 + cell in 32 bits, the dropped-page key
+  // page_cap included: at most 2^24 - 1 pages (further pages are dropped and
+  // counted, as past any capacity)
+  if (st.fine_binning_backend != 0u) page_cap = std::min(page_cap, (1u << 24) - 1u);
   // reference slab: 8 references per task slot (the reference sizes it at 4x
   // its binning queue, src/plugin.rs:540-546); work items: half of that.
   // Both grow when a frame reports an overflow.
