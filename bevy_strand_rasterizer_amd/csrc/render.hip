@@ -265,20 +265,61 @@ __device__ __forceinline__ bool shadow_cover(const ShadowBatch<TPX>& b, uint32_t
   return in && c > 0.0f;
 }
 
+// The value lane l ^ SH holds, with every lane of the wave active (DPP
+// within a row for SH < 16; the gfx950 permlane swaps across rows): no LDS
+// round trip.  quad_perm [1,0,3,2] and [2,3,0,1] exchange within a quad;
+// row_half_mirror (lane i of an 8-lane half takes 7 - i) then quad_perm
+// [3,2,1,0] (i takes i ^ 3) gives i ^ 4; row_ror:8 gives i ^ 8 in a 16-lane
+// row.  permlane16_swap(v, v) swaps the odd 16-lane rows of its first
+// operand with the even rows of its second, so the first result holds, in
+// the odd rows, the rows below them and the second, in the even rows, the
+// rows above (permlane32_swap likewise with halves of the wave).
+template <uint32_t SH>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
+  if constexpr (SH == 1u) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+  } else if constexpr (SH == 2u) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+  } else if constexpr (SH == 4u) {
+    const int m = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false);
+  } else if constexpr (SH == 8u) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
+  } else if constexpr (SH == 16u) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16u) ? r[0] : r[1];
+  } else {
+    static_assert(SH == 32u, "a lane distance of the wave");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32u) ? r[0] : r[1];
+  }
+}
+
+// One butterfly stage of the transpose: lanes l and l ^ SH swap the
+// off-diagonal SH-bit blocks (selected by KEEP in the lower lane) of their
+// 64-bit rows, without branches.
+template <uint32_t SH, unsigned long long KEEP>
+__device__ __forceinline__ unsigned long long transpose_stage(unsigned long long x, uint32_t lane) {
+  const uint32_t ylo = lane_xor<SH>((uint32_t)x, lane), yhi = lane_xor<SH>((uint32_t)(x >> 32), lane);
+  const unsigned long long y = ((unsigned long long)yhi << 32) | ylo;
+  const unsigned long long upper = 0ull - (unsigned long long)((lane >> __builtin_ctz(SH)) & 1u);  // all ones in the upper lane
+  const unsigned long long mk = KEEP ^ upper;  // x's bits that stay: KEEP (lower lane) or ~KEEP (upper lane)
+  const unsigned long long t = ((y >> SH) & upper) | ((y << SH) & ~upper);
+  return (x & mk) | (t & ~mk);
+}
+
 // 64 x 64 bit-matrix transpose across the wave: lane l holds row l on entry
 // and column l on return (bit j = bit l of lane j's row).  Six butterfly
-// stages swap the off-diagonal blocks of halving size.
+// stages swap the off-diagonal blocks of halving size.  Every lane of the
+// wave must be active.
 __device__ __forceinline__ unsigned long long wave_transpose64(unsigned long long x) {
   const uint32_t l = threadIdx.x & 63u;
-  const unsigned long long keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                                      0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const uint32_t sh = 32u >> i;
-    const unsigned long long y = __shfl_xor(x, (int)sh);
-    x = (l & sh) ? ((x & ~keep[i]) | ((y & ~keep[i]) >> sh)) : ((x & keep[i]) | ((y & keep[i]) << sh));
-  }
-  return x;
+  x = transpose_stage<32u, 0x00000000FFFFFFFFull>(x, l);
+  x = transpose_stage<16u, 0x0000FFFF0000FFFFull>(x, l);
+  x = transpose_stage<8u, 0x00FF00FF00FF00FFull>(x, l);
+  x = transpose_stage<4u, 0x0F0F0F0F0F0F0F0Full>(x, l);
+  x = transpose_stage<2u, 0x3333333333333333ull>(x, l);
+  return transpose_stage<1u, 0x5555555555555555ull>(x, l);
 }
 
 // The hardware reciprocal and square root (within 1 ulp) for the span setup.
