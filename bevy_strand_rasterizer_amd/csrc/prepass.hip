@@ -1083,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
         const unsigned long long act = __ballot(pending);
         if (act == 0ull) break;
         const int first = __ffsll((long long)act) - 1;
-        const uint32_t dl = (uint32_t)__shfl((int)d, first);
+        const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane((int)d, first);  // first is wave-uniform
         const bool same = pending && d == dl;
         const unsigned long long grp = __ballot(same);
         if ((int)lane == first) atomicAdd(&cnt[dl], (uint32_t)__popcll(grp));
@@ -1603,8 +1603,8 @@ __global__ __launch_bounds__(64) void k_emit_count(FrameParams fp) {
       works += tl[i].count ? 1u : 0u;
       load += tl[i].count;
     }
-    works = (uint32_t)__shfl((int)wave_inclusive<uint32_t>(works), 63);
-    load = (uint32_t)__shfl((int)wave_inclusive<uint32_t>(load), 63);
+    works = (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive<uint32_t>(works), 63);
+    load = (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive<uint32_t>(load), 63);
     if (threadIdx.x == 0) {
       uint32_t fi = frustum_for_coarse_tile(fp, tile);
       uint32_t kind = fi == kInvalid ? 2u : fp.fr[fi].d.kind;
@@ -1706,12 +1706,12 @@ __device__ __forceinline__ void run_bucket_add(const RunChunk& c, uint32_t i0, u
   uint64_t todo = __ballot(valid);
   while (todo) {
     const int first = __ffsll((long long)todo) - 1;
-    const uint32_t fb = (uint32_t)__shfl((int)b, first);
+    const uint32_t fb = (uint32_t)__builtin_amdgcn_readlane((int)b, first);  // first is wave-uniform
     const uint64_t m = __ballot(valid && b == fb) & todo;
     uint32_t at = 0u;
     if ((int)lane == first) at = atomicAdd(&cnt[fb], (uint32_t)__popcll(m));
     if (place) {
-      at = (uint32_t)__shfl((int)at, first);
+      at = (uint32_t)__builtin_amdgcn_readlane((int)at, first);
       if ((m >> lane) & 1ull) c.order[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
     }
     todo &= ~m;
