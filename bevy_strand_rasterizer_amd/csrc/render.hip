@@ -121,12 +121,7 @@ __device__ void stage_run(const FrameParams& fp, const sr_raster_tile_run& run, 
   for (uint32_t k = 0; k < kPer; ++k) local += cnt[k];
   // exclusive scan of per-thread sums across the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t inc = local;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t o = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += o;
-  }
+  uint32_t inc = wave_inclusive<uint32_t>(local);
   if (TPX > 64) {
     if (lane == 63) lds4[wave] = inc;
     __syncthreads();

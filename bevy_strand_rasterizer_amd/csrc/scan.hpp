@@ -23,22 +23,27 @@ struct U4 {
   uint32_t x, y, z, w;
 };
 __device__ __forceinline__ U4 operator+(U4 a, U4 b) { return U4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
-__device__ __forceinline__ uint32_t shfl_up_v(uint32_t v, int d) { return __shfl_up(v, d, 64); }
-__device__ __forceinline__ U4 shfl_up_v(U4 v, int d) {
-  return U4{__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64), __shfl_up(v.z, d, 64), __shfl_up(v.w, d, 64)};
-}
+__device__ __forceinline__ U4 operator-(U4 a, U4 b) { return U4{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
 template <class T> __device__ __forceinline__ T zero_v() { return T{}; }
 
-// Inclusive scan across one wave64.
-template <class T>
-__device__ __forceinline__ T wave_inclusive(T v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    T o = shfl_up_v(v, d);
-    if (lane >= d) v = v + o;
-  }
+// Inclusive scan across one wave64 with every lane active, in the VALU: DPP
+// row_shr:1, 2, 4, 8 scan each 16-lane row (a lane with no source lane in its
+// row adds 0), then row_bcast:15 adds row r's last lane to row r + 1 (rows 1
+// and 3) and row_bcast:31 adds lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
   return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_inclusive(T v);
+template <> __device__ __forceinline__ uint32_t wave_inclusive<uint32_t>(uint32_t v) { return wave_inclusive_u32(v); }
+template <> __device__ __forceinline__ U4 wave_inclusive<U4>(U4 v) {
+  return U4{wave_inclusive_u32(v.x), wave_inclusive_u32(v.y), wave_inclusive_u32(v.z), wave_inclusive_u32(v.w)};
 }
 
 // Exclusive scan across the 256-thread block; returns the block total too.
@@ -57,11 +62,7 @@ __device__ __forceinline__ T block_exclusive(T v, T* total, T* lds4) {
   }
   __syncthreads();
   *total = tot;
-  T exc = inc;
-  // exclusive = inclusive - v, expressed without subtraction for U4 too
-  exc = shfl_up_v(inc, 1);
-  if (lane == 0) exc = zero_v<T>();
-  return wave_base + exc;
+  return wave_base + (inc - v);  // exclusive = inclusive - v (modular, exact)
 }
 
 __device__ __forceinline__ uint32_t scan_count(uint32_t cap, const uint32_t* n_dev) {
