@@ -65,81 +65,104 @@ __device__ float screen_area(const FrustumRec& fr, const CornerBounds& cb) {
   return ex * ey;
 }
 
-__global__ void k_broad(FrameParams fp) {
-  uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ii >= fp.NI) return;
-  const InstRec& in = fp.inst[ii];
-  for (uint32_t fi = 0; fi < fp.F; ++fi) {
-    sr_coarse_asset_range none{};
-    none.geo_id = kInvalid;
-    fp.ranges[(size_t)ii * fp.F + fi] = none;
-    fp.keep[(size_t)ii * fp.F + fi] = 0.0f;
-  }
-  if (!in.valid || in.meta_count == 0u || in.geo_strand_count == 0u) {
-    fp.broad[ii] = sr_broad_instance_meta{0u, 0u, 0.0f, 0.0f};
-    return;
-  }
+// The instance's coarse asset range and keep probability in frustum fi
+// (emit_coarse_asset_range_for_aabb + the stochastic keep); returns whether
+// the instance is visible there.
+__device__ bool broad_frustum(const FrameParams& fp, uint32_t ii, const InstRec& in, uint32_t strand_count,
+                              uint32_t fi, float* keep) {
+  const FrustumRec& fr = fp.fr[fi];
   const sr_frame_settings& st = fp.st;
-  uint32_t strand_count = min(in.meta_count, in.geo_strand_count);
-  bool visible = false;
-  float cam = 0.0f, shd = 0.0f;
-  for (uint32_t fi = 0; fi < fp.F; ++fi) {
-    const FrustumRec& fr = fp.fr[fi];
-    float k = 0.0f;
-    if (fr.d.kind > 1u) { fp.keep[(size_t)ii * fp.F + fi] = 0.0f; continue; }
-    // emit_coarse_asset_range_for_aabb
-    bool vis = false;
-    CornerBounds cb = project_aabb(fr, in.aabb, in.world_from_local);
-    float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
-    if (fr.d.coarse_depth_tile_count != 0u && !(fr.d.kind == 1u && fr.light_layer == kInvalid)) {
-      CornerBounds r = cb;
-      if (r.near_clip) {
-        r.smin = V2{0.0f, 0.0f};
-        r.smax = V2{vw - 1.0f, vh - 1.0f};
-        r.zmax = 1.0f;
-        if (!r.any) r.zmin = 0.0f;
-        r.any = true;
-      }
-      if (r.any && !(r.smax.x < 0.0f || r.smax.y < 0.0f || r.smin.x >= vw || r.smin.y >= vh) &&
-          !(r.zmax < 0.0f || r.zmin > 1.0f)) {
-        uint32_t cpx = max(1u, fr.d.froxel_size_x * kExt), cpy = max(1u, fr.d.froxel_size_y * kExt);
-        sr_coarse_asset_range rg;
-        rg.geo_id = ii;
-        rg.frustum_id = fi;
-        rg.min_x = min(f2u(wclamp(r.smin.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
-        rg.max_x = min(f2u(wclamp(r.smax.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
-        rg.min_y = min(f2u(wclamp(r.smin.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
-        rg.max_y = min(f2u(wclamp(r.smax.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
-        rg.z_min_q = quantize_depth01(r.zmin);
-        rg.z_max_q = quantize_depth01(r.zmax);
-        fp.ranges[(size_t)ii * fp.F + fi] = rg;
-        vis = true;
-      }
+  *keep = 0.0f;
+  if (fr.d.kind > 1u) return false;
+  bool vis = false;
+  CornerBounds cb = project_aabb(fr, in.aabb, in.world_from_local);
+  float vw = (float)fr.d.screen_width, vh = (float)fr.d.screen_height;
+  if (fr.d.coarse_depth_tile_count != 0u && !(fr.d.kind == 1u && fr.light_layer == kInvalid)) {
+    CornerBounds r = cb;
+    if (r.near_clip) {
+      r.smin = V2{0.0f, 0.0f};
+      r.smax = V2{vw - 1.0f, vh - 1.0f};
+      r.zmax = 1.0f;
+      if (!r.any) r.zmin = 0.0f;
+      r.any = true;
     }
-    visible = vis || visible;
-    if (vis) {
-      float area = screen_area(fr, cb);
-      float target = fmaxf(st.target_strands_per_pixel, 1e-5f);
-      float min_keep = wclamp(st.min_keep_probability, 0.0f, 1.0f);
-      if (st.stochastic_cull_enabled == 0u) {
-        k = 1.0f;
-      } else if (area < 0.0f) {
-        k = 0.0f;
-      } else {
-        float area_keep = wclamp(area * target / fmaxf((float)strand_count, 1.0f), min_keep, 1.0f);
-        if (fr.d.kind == 0u) {
-          k = area_keep;
-        } else {
-          float cap = wclamp(st.shadow_keep_probability, 0.0f, 1.0f);
-          float scale = 1.0f / exp2f((float)fr.d.cascade_index);
-          k = wclamp(area_keep * area_keep * scale * cap, min_keep, 1.0f);
-        }
-      }
-      if (fr.d.kind == 0u) cam = fmaxf(cam, k); else shd = fmaxf(shd, k);
+    if (r.any && !(r.smax.x < 0.0f || r.smax.y < 0.0f || r.smin.x >= vw || r.smin.y >= vh) &&
+        !(r.zmax < 0.0f || r.zmin > 1.0f)) {
+      uint32_t cpx = max(1u, fr.d.froxel_size_x * kExt), cpy = max(1u, fr.d.froxel_size_y * kExt);
+      sr_coarse_asset_range rg;
+      rg.geo_id = ii;
+      rg.frustum_id = fi;
+      rg.min_x = min(f2u(wclamp(r.smin.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
+      rg.max_x = min(f2u(wclamp(r.smax.x, 0.0f, vw - 1.0f)) / cpx, fr.d.coarse_tiles_x - 1u);
+      rg.min_y = min(f2u(wclamp(r.smin.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
+      rg.max_y = min(f2u(wclamp(r.smax.y, 0.0f, vh - 1.0f)) / cpy, fr.d.coarse_tiles_y - 1u);
+      rg.z_min_q = quantize_depth01(r.zmin);
+      rg.z_max_q = quantize_depth01(r.zmax);
+      fp.ranges[(size_t)ii * fp.F + fi] = rg;
+      vis = true;
     }
-    fp.keep[(size_t)ii * fp.F + fi] = k;
   }
-  fp.broad[ii] = sr_broad_instance_meta{strand_count, visible ? 1u : 0u, cam, shd};
+  if (vis) {
+    float area = screen_area(fr, cb);
+    float target = fmaxf(st.target_strands_per_pixel, 1e-5f);
+    float min_keep = wclamp(st.min_keep_probability, 0.0f, 1.0f);
+    float k;
+    if (st.stochastic_cull_enabled == 0u) {
+      k = 1.0f;
+    } else if (area < 0.0f) {
+      k = 0.0f;
+    } else {
+      float area_keep = wclamp(area * target / fmaxf((float)strand_count, 1.0f), min_keep, 1.0f);
+      if (fr.d.kind == 0u) {
+        k = area_keep;
+      } else {
+        float cap = wclamp(st.shadow_keep_probability, 0.0f, 1.0f);
+        float scale = 1.0f / exp2f((float)fr.d.cascade_index);
+        k = wclamp(area_keep * area_keep * scale * cap, min_keep, 1.0f);
+      }
+    }
+    *keep = k;
+  }
+  return vis;
+}
+
+// One wave per instance, its lanes over the frusta: every (instance,
+// frustum) range and keep probability, then the instance's broad record
+// (visible anywhere; the largest camera and shadow keep, a max that does not
+// depend on the order).
+__global__ __launch_bounds__(64) void k_broad(FrameParams fp) {
+  __shared__ uint32_t kmax[2];  // largest camera / shadow keep, as the bits of a float >= 0
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t ii = blockIdx.x; ii < fp.NI; ii += gridDim.x) {
+    const InstRec& in = fp.inst[ii];
+    for (uint32_t fi = lane; fi < fp.F; fi += 64u) {
+      sr_coarse_asset_range none{};
+      none.geo_id = kInvalid;
+      fp.ranges[(size_t)ii * fp.F + fi] = none;
+      fp.keep[(size_t)ii * fp.F + fi] = 0.0f;
+    }
+    if (!in.valid || in.meta_count == 0u || in.geo_strand_count == 0u) {
+      if (lane == 0u) fp.broad[ii] = sr_broad_instance_meta{0u, 0u, 0.0f, 0.0f};
+      continue;
+    }
+    if (lane < 2u) kmax[lane] = 0u;
+    __syncthreads();
+    const uint32_t strand_count = min(in.meta_count, in.geo_strand_count);
+    bool visible = false;
+    for (uint32_t fi = lane; fi < fp.F; fi += 64u) {
+      float k;
+      const bool vis = broad_frustum(fp, ii, in, strand_count, fi, &k);
+      visible = visible || vis;
+      fp.keep[(size_t)ii * fp.F + fi] = k;
+      if (vis && k > 0.0f) atomicMax(&kmax[fp.fr[fi].d.kind == 0u ? 0 : 1], __float_as_uint(k));
+    }
+    const bool any = __ballot(visible) != 0ull;
+    __syncthreads();
+    if (lane == 0u)
+      fp.broad[ii] = sr_broad_instance_meta{strand_count, any ? 1u : 0u, __uint_as_float(kmax[0]),
+                                            __uint_as_float(kmax[1])};
+    __syncthreads();
+  }
 }
 
 // -------------------------------------------------- projection (K3 + K5)
@@ -1893,7 +1916,7 @@ void launch_prepass(const FrameParams& fp, const PrepassScratch& ps, hipStream_t
     hipLaunchKernelGGL(k_opaque_reduce, dim3(grid_for(fp.fine_total, 256, kMaxGrid)), dim3(256), 0, s, fp);
   }
   timer->mark(s, "broad");
-  hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 64, 1024)), dim3(64), 0, s, fp);
+  hipLaunchKernelGGL(k_broad, dim3(grid_for(fp.NI, 1, 4096)), dim3(64), 0, s, fp);
   timer->mark(s, "depth_lut");
   {
     const dim3 g(grid_for(fp.coarse_total, 64, 1u << 20));
