@@ -1124,7 +1124,34 @@ void launch_dom_pages(const FrameParams& fp, hipStream_t s) {
   hipLaunchKernelGGL(k_dom_residency, dim3(1), dim3(256), 0, s, fp);
 }
 
+// Atlas and shadow-history texels that no camera segment start shades this
+// frame read as zero (the reference leaves them stale; dumps must be
+// deterministic): a texel of layer (camera, instance) whose slot is not a
+// shaded segment start is cleared.  Shading writes every texel of a shaded
+// start, and the texel after a strand's last one, afterwards on the same
+// stream, so clearing that last one too changes nothing; only the texels it
+// leaves are written here, instead of the whole atlas.
+__global__ __launch_bounds__(256) void k_clear_unshaded(FrameParams fp) {
+  const uint32_t T = fp.atlas_texels, layers = fp.cams * fp.NI;
+  for (uint32_t layer = 0; layer < layers; ++layer) {
+    const uint32_t cam = layer / fp.NI, ii = layer - cam * fp.NI;
+    const InstRec& in = fp.inst[ii];
+    const uint32_t n = min(in.index_count, T);
+    const uint8_t* sh = fp.shaded + (size_t)cam * fp.slots + in.pos_base;
+    float4* atlas = fp.atlas + (size_t)layer * T;
+    float2* hist = fp.history + (size_t)layer * T;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < T; t += gridDim.x * 256u) {
+      if (t < n && sh[t] != 0u) continue;
+      atlas[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      hist[t] = make_float2(0.0f, 0.0f);
+    }
+  }
+}
+
 void launch_shading(const FrameParams& fp, hipStream_t s) {
+  const uint32_t T = fp.atlas_texels;
+  if (T && fp.cams && fp.NI)
+    hipLaunchKernelGGL(k_clear_unshaded, dim3(std::min<uint32_t>(4096u, (T + 255u) / 256u)), dim3(256), 0, s, fp);
   // 16k workgroups: each thread shades a few segments, so the uneven per-segment
   // cost (lights, lobe windows) leaves a short tail
   hipLaunchKernelGGL(k_shading, dim3(256u * 64u), dim3(128), 0, s, fp);

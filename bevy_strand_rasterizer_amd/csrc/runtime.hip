@@ -1041,10 +1041,8 @@ int frame_submit_slot(sr_context* ctx, FrameSlot& sl, const sr_frame_desc* fd, h
     }
     clear.add(reinterpret_cast<uint32_t*>(fp.image), img_px * 4);
     clear.add(reinterpret_cast<uint32_t*>(fp.depth), img_px);
-    // texels no segment writes this frame read as zero (the reference leaves
-    // them stale; dumps must be deterministic)
-    clear.add(reinterpret_cast<uint32_t*>(fp.atlas), atlas_n * 4);
-    clear.add(reinterpret_cast<uint32_t*>(fp.history), atlas_n * 2);
+    // (atlas and history texels no segment shades are cleared before the
+    // shading pass, from the marking pass's shaded flags)
     launch_fill_spans(clear, s);
     launch_prepass(fp, ps, s, &sl.timer);
     if (vdom && !reuse) {
