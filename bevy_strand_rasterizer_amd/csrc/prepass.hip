@@ -951,7 +951,7 @@ __global__ __launch_bounds__(256) void k_zero_cells(FrameParams fp, uint32_t* ce
 
 // ------------------------------------------------- K14 page prefix
 // One 256-lane workgroup per page: page-local exclusive scan of the cell
-// counts (wave64 shuffles + one LDS hop across the four waves), cursor reset,
+// counts (wave64 DPP scans + one LDS hop across the four waves), cursor reset,
 // page total and the 16-bit mask of fine tiles with work.
 __global__ __launch_bounds__(256) void k_page_prefix(FrameParams fp, uint32_t* page_xy_mask) {
   __shared__ uint32_t lds4[4];
